@@ -1,0 +1,170 @@
+"""Benchmark: Mrays/s of one 1080p primary + sun-shadow frame on the 2048x256x2048 world.
+
+One step = one frame: 1920x1080 pinhole primary rays (tmax 700, brick LOD past 400 and tile
+LOD past 520) through the seeded benchmark world, then one any-hit shadow ray toward the sun
+(tmax 480) from every primary hit, all in one launch of the engine's frame kernel. The world
+is generated on the host, uploaded once, and resident in HBM before the timed region.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints ONE JSON line on stdout; details go to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(world, wl, cam, sun, budget_s=12.0):
+    """The oracle's R0 (src/raycast.rs restated, per-voxel DDA) on a bounded sample of the
+    same frame: evenly spaced image rows, every host thread, primary + shadow rays."""
+    import numpy as np
+
+    import oracle
+    ow = oracle.OracleWorld.view(world)
+    threads = min(os.cpu_count() or 1, 256)
+    # calibrate on 2 rows, then size the sample to about budget_s of CPU time
+    t0 = time.perf_counter()
+    h, s = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, wl.shadow_tmax,
+                       rows=(wl.height // 2, wl.height // 2 + 2), threads=threads)
+    dt = time.perf_counter() - t0
+    n_rows = int(max(4, min(wl.height, 2 * budget_s / max(dt, 1e-3))))
+    step = max(1, wl.height // n_rows)
+    rows = list(range(step // 2, wl.height, step))[:n_rows]
+    rays = 0
+    t0 = time.perf_counter()
+    for r in rows:
+        h, s = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, wl.shadow_tmax,
+                           rows=(r, r + 1), threads=threads)
+        rays += wl.width + int(h["hit"].sum())
+    dt = time.perf_counter() - t0
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{len(rows)} of {wl.height} rows (every {step}th), {rays} primary+shadow "
+                      f"rays, R0 = per-voxel DDA of src/raycast.rs, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build runs the ray-query path on a single "
+              "MI355X (rays are independent; see DESIGN.md for how frames would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from voxelg_amd import workloads
+    from voxelg_amd._lib import HIT_RECORD_BYTES, MATERIAL_BYTES, algorithmic_bytes
+    from voxelg_amd.device import Context, decode_hits, device_count
+
+    assert device_count() > 0, "bench.py needs a HIP device"
+    wl = workloads.LARGE
+    t0 = time.perf_counter()
+    world = wl.world()
+    log(f"world {wl.dims} generated in {time.perf_counter() - t0:.1f} s, "
+        f"solid {world.solid_fraction():.3f}")
+    _, cam = wl.camera()
+    sun = wl.sun()
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    dw = ctx.world(wl.dims).upload(world)
+    log(f"uploaded in {time.perf_counter() - t0:.2f} s")
+
+    W, H = wl.width, wl.height
+    hits = ctx.alloc(W * H * 20)
+    occ = ctx.alloc(W * H)
+    stream = ctx.stream()
+
+    def frame():
+        dw.trace_frame(cam, W, H, wl.primary_tmax, wl.primary_flags, sun, wl.shadow_tmax, hits,
+                       occ, stream)
+
+    # algorithmic accounting (outside the timed region): iteration classes of this frame
+    _, _, counts = dw.trace_frame_counts(cam, W, H, wl.primary_tmax, wl.primary_flags, sun,
+                                         wl.shadow_tmax)
+    frame()
+    stream.synchronize()
+    hh = decode_hits(hits)
+    n_hit = int(hh["hit"].sum())
+    n_shadowed = int(occ.numpy(np.uint8).sum())
+    rays_per_frame = W * H + n_hit
+    bytes_per_frame = (algorithmic_bytes(counts) + W * H * HIT_RECORD_BYTES +
+                       n_hit * MATERIAL_BYTES + W * H)
+    log(f"frame: {W}x{H} primary, {n_hit} hits -> {n_hit} shadow rays ({n_shadowed} occluded)")
+    log(f"iteration classes: {counts}")
+
+    for _ in range(args.warmup):
+        frame()
+    stream.synchronize()
+    starts = [ctx.event() for _ in range(args.steps)]
+    ends = [ctx.event() for _ in range(args.steps)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        frame()
+        ends[i].record(stream)
+    stream.synchronize()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [s.elapsed_ms(e) for s, e in zip(starts, ends)]
+    avg_kern_s = float(np.mean(kern_ms)) / 1e3
+    ms_per_step = wall / args.steps * 1e3
+    value = rays_per_frame * args.steps / wall / 1e6
+    achieved = bytes_per_frame / avg_kern_s / 1e9
+    log(f"wall {ms_per_step:.3f} ms/frame, kernel {np.mean(kern_ms):.3f} ms "
+        f"(min {np.min(kern_ms):.3f}), {value:.0f} Mrays/s")
+
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(world, wl, cam, sun)
+        log(f"cpu baseline: {base}")
+
+    out = {
+        "metric": "Mrays/s at 1080p primary+shadow",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm terrain with caves, generated on the host)",
+        "config": {
+            "workload": "large: 2048x256x2048 world (32x4x32 chunks), 1920x1080 primary rays "
+                        "tmax 700 with brick/tile LOD past 400/520, one sun shadow ray per hit "
+                        "tmax 480",
+            "world": list(wl.dims), "resolution": [W, H], "seed": wl.seed,
+            "primary_rays": W * H, "shadow_rays": n_hit,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_trace_frame<false,1>", "kernel_ms": round(avg_kern_s * 1e3, 4),
+            "algorithmic_bytes": int(bytes_per_frame),
+        },
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+/*
+ * vg.h — C ABI of libvoxelg_amd.so, the MI355X (gfx950) voxel ray-query engine.
+ *
+ * The reference (marcstampfli1/voxelG) has no FFI layer; its ray query sits behind three
+ * de-facto boundaries, each replaced here by plain-pointer entry points:
+ *   1. the CPU ray query      pub fn raycast(camera_world, dir, &World, world_origin)
+ *                             -> Option<PickHit>                       src/raycast.rs:21
+ *   2. the batched GPU query  Ray[] -> Hit[] storage buffers           shaders/accel_probe.wgsl:21-126
+ *                             (host side probe_with_accel,            src/accel.rs:417-513)
+ *   3. the frame kernels      cs_beam + cs_main's trace + trace_any    shaders/beam.wgsl:31-113,
+ *                             over the uploaded world                 shaders/raymarch.wgsl:576-762
+ *                             (Renderer::upload_world / render,       src/renderer.rs:1248, 1533)
+ * plus the host-side world builders the upload path feeds on (src/voxel.rs:233-272, 950-1000).
+ *
+ * Conventions
+ *   - Every entry point returns an int status (VG_OK or a negative VG_E_*); nothing throws
+ *     across the ABI. vg_last_error() returns a thread-local message for the last failure.
+ *   - Device pointers (rays, hits, occlusion bytes) are plain device addresses the caller
+ *     allocated on the world's device; host pointers are marked _host.
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream. Kernels are
+ *     enqueued asynchronously; calls on one world must not overlap with vg_world_upload.
+ *   - World dimensions are voxels per axis, each a multiple of 64; an axis whose window
+ *     origin is not a multiple of its dimension (toroidal fold) must be a multiple of 256.
+ *   - Worlds of one context may be used from several host threads for tracing; creation,
+ *     upload and destruction are not thread-safe against tracing on the same world.
+ */
+#ifndef VG_H
+#define VG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VG_ABI_VERSION 1
+
+enum {
+    VG_OK = 0,
+    VG_E_INVALID = -1,     /* bad argument (null pointer, bad dims, size mismatch) */
+    VG_E_NOMEM = -2,       /* host or device allocation failed */
+    VG_E_DEVICE = -3,      /* HIP runtime / launch error */
+    VG_E_UNSUPPORTED = -4, /* valid request this build does not serve */
+    VG_E_STATE = -5        /* world not uploaded, or wrong context */
+};
+
+/* Ray flags (vg_ray.flags and the per-call flags are OR-ed). */
+#define VG_RAY_LOD 1u              /* brick LOD past 400, tile LOD past 520 (raymarch.wgsl:4489-4494) */
+#define VG_RAY_SKIP_TRANSPARENT 2u /* pass water and glass (trace_no_water, raymarch.wgsl:3450) */
+#define VG_RAY_ANY_HIT 4u          /* occlusion semantics (trace_any, raymarch.wgsl:4345) */
+
+/* Face codes (encode_face_normal, shaders/raymarch.wgsl:482-486): the hit face's outward
+ * normal is +x 0, -x 1, +y 2, -y 3, +z 4, -z 5; 6 = no face (miss, or no entry plane). */
+#define VG_FACE_NONE 6
+
+/* voxelG's Brick (src/voxel.rs:296-300): 72 bytes, voxel bit / material index x + 4z + 16y. */
+typedef struct vg_brick {
+    uint64_t occupancy;
+    uint8_t materials[64];
+} vg_brick;
+
+/* A ray of the batched query: accel_probe's Ray (shaders/accel_probe.wgsl:21-26) with the
+ * two pad words carrying a per-ray tmax and flags. d need not be normalized (it is
+ * normalized like glam's normalize_or_zero; a zero d misses). World coordinates. */
+typedef struct vg_ray {
+    float o[3];
+    float tmax;
+    float d[3];
+    uint32_t flags;
+} vg_ray;
+
+/* A hit record (accel_probe's Hit, shaders/accel_probe.wgsl:28-39, packed to 20 bytes):
+ * world voxel, entry-t of that voxel, face code, material, hit flag. A miss has hit 0,
+ * t -1, face VG_FACE_NONE and zeros elsewhere. */
+typedef struct vg_hit {
+    int32_t voxel[3];
+    float t;
+    uint8_t face;
+    uint8_t material;
+    uint8_t hit;
+    uint8_t pad;
+} vg_hit;
+
+/* CameraUniform (src/camera.rs:61-113), 176 bytes, same field order. The traversal reads
+ * origin, forward, right, up, tan_half_fov, resolution and jitter. */
+typedef struct vg_camera {
+    float origin[3];
+    float _pad0;
+    float forward[3];
+    float _pad1;
+    float right[3];
+    float _pad2;
+    float up[3];
+    float tan_half_fov;
+    float resolution[2];
+    float time;
+    float wind_x;
+    int32_t world_origin[3];
+    int32_t gi_round;
+    float jitter[2];
+    float taa_blend;
+    float reproject_lighting;
+    float prev_origin[3];
+    float wind_z;
+    float prev_forward[3];
+    float sun_time;
+    float prev_right[3];
+    float prev_valid;
+    float prev_up[3];
+    float _pad9;
+} vg_camera;
+
+/* Iteration classes of the hierarchical walk, reported by the *_counts entry points. */
+enum {
+    VG_IT_L4_SKIP = 0,
+    VG_IT_CHUNK_SKIP,
+    VG_IT_TILE_SKIP,
+    VG_IT_TILE_LOD,
+    VG_IT_TILE_UNIFORM,
+    VG_IT_BRICK_SKIP,
+    VG_IT_BRICK_UNIFORM,
+    VG_IT_BRICK_LOD,
+    VG_IT_VOXEL,
+    VG_IT_COUNT
+};
+
+typedef struct vg_ctx vg_ctx;
+typedef struct vg_world vg_world;
+
+/* ------------------------------------------------ device memory / timing -- */
+/* Plain device buffers, streams and events for callers that have no allocator of their
+ * own (the engine needs nothing else from its host). Copies are synchronous. */
+int vg_device_count(int *out);
+int vg_alloc(vg_ctx *ctx, size_t bytes, void **dev_out);
+int vg_free(vg_ctx *ctx, void *dev);
+int vg_copy_to_device(vg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int vg_copy_to_host(vg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+int vg_memset(vg_ctx *ctx, void *dev, int value, size_t bytes);
+int vg_synchronize(vg_ctx *ctx);
+int vg_stream_create(vg_ctx *ctx, void **stream_out); /* default priority */
+int vg_stream_destroy(vg_ctx *ctx, void *stream);
+int vg_stream_synchronize(vg_ctx *ctx, void *stream);
+int vg_event_create(vg_ctx *ctx, void **event_out);
+int vg_event_destroy(vg_ctx *ctx, void *event);
+int vg_event_record(vg_ctx *ctx, void *event, void *stream);
+/* milliseconds between two recorded events (waits for `end`) */
+int vg_event_elapsed_ms(vg_ctx *ctx, void *start, void *end, float *ms_out);
+
+/* ----------------------------------------------------------------- misc -- */
+int vg_abi_version(void);
+const char *vg_last_error(void);
+const char *vg_status_string(int status);
+
+/* ---------------------------------------------- host-side world building -- */
+/* Bricks, tiles, chunks, L4 cells and voxels of a window (src/world_dims.rs:9-37). */
+int vg_world_counts(const int32_t dims[3], int64_t out[5]);
+
+/* Benchmark terrain into caller-owned bricks (counts[0] entries, reference x-fastest order).
+ * World voxel origin+local is stored at its toroidal slot. Replaces World::fill_demo_terrain
+ * (src/voxel.rs:1005) with the benchmark generator: fBm heightmap, 3D-noise caves,
+ * grass/dirt/stone/sand, water below y=56. threads <= 0 uses all hardware threads. */
+int vg_generate_terrain(const int32_t dims[3], const int32_t origin[3], uint64_t seed,
+                        vg_brick *bricks_host, int threads);
+/* Terrain surface height of world column (x, z) for the benchmark generator. */
+int vg_terrain_height(const int32_t dims[3], uint64_t seed, int32_t x, int32_t z, int32_t *h_out);
+
+/* World::rebuild_all_masks + rebuild_all_uniform (src/voxel.rs:950-1000) over caller-owned
+ * arrays in the reference's x-fastest order. */
+int vg_build_pyramid(const int32_t dims[3], const vg_brick *bricks_host, uint64_t *tile_mask,
+                     uint64_t *chunk_mask, uint64_t *l4_mask, uint8_t *brick_uniform,
+                     uint8_t *tile_uniform, int threads);
+
+/* CameraUniform::from_camera (src/camera.rs:130-166) for a fly camera at pos with yaw,
+ * pitch and vertical fov (radians); basis per Camera::forward/right/up (:25-42). */
+int vg_camera_from_pose(const float pos[3], float yaw, float pitch, float fov_y, int32_t width,
+                        int32_t height, const int32_t world_origin[3], vg_camera *out);
+
+/* ------------------------------------------------------------- device -- */
+/* A context owns one HIP device. */
+int vg_ctx_create(int device, vg_ctx **out);
+int vg_ctx_destroy(vg_ctx *ctx);
+
+/* Allocate a world's device storage (pyramid, LUTs, occupancy, materials). */
+int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out);
+int vg_world_destroy(vg_world *world);
+
+/* Renderer::upload_world (src/renderer.rs:1248-1395), full upload: copies the caller's
+ * reference-layout arrays (counts from vg_world_counts) into the engine's own HBM layout.
+ * Synchronous; the caller keeps ownership of the host arrays. */
+int vg_world_upload(vg_world *world, const vg_brick *bricks, const uint64_t *tile_mask,
+                    const uint64_t *chunk_mask, const uint64_t *l4_mask,
+                    const uint8_t *brick_uniform, const uint8_t *tile_uniform,
+                    const int32_t origin[3]);
+
+/* Batched ray query (shaders/accel_probe.wgsl:66-126): n rays in, n hits out, both device
+ * arrays. First solid voxel along each ray within its tmax, water included unless
+ * VG_RAY_SKIP_TRANSPARENT. */
+int vg_trace_rays(vg_world *world, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
+                  void *stream);
+
+/* cs_main's primary trace (shaders/raymarch.wgsl:576-626 + trace :4600): one pinhole ray
+ * per pixel from the camera, hits[y * width + x]. */
+int vg_trace_primary(vg_world *world, const vg_camera *camera_host, int32_t width,
+                     int32_t height, float tmax, uint32_t flags, vg_hit *hits, void *stream);
+
+/* shade()'s sun shadow ray (shaders/raymarch.wgsl:3953-3995 + trace_any :4345): for every
+ * primary hit, an any-hit ray from p_hit + n * 0.001 toward sun; occluded[i] = 0/1. */
+int vg_trace_shadow(vg_world *world, const vg_camera *camera_host, int32_t width,
+                    int32_t height, const vg_hit *primary_hits, const float sun[3], float tmax,
+                    uint8_t *occluded, void *stream);
+
+/* Primary + shadow for one frame in one launch. */
+int vg_trace_frame(vg_world *world, const vg_camera *camera_host, int32_t width, int32_t height,
+                   float primary_tmax, uint32_t primary_flags, const float sun[3],
+                   float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream);
+
+/* The same frame, also reporting the walk's iteration-class totals (VG_IT_*) into
+ * counts_host; synchronous. Used for algorithmic-byte accounting, not for timing. */
+int vg_trace_frame_counts(vg_world *world, const vg_camera *camera_host, int32_t width,
+                          int32_t height, float primary_tmax, uint32_t primary_flags,
+                          const float sun[3], float shadow_tmax, vg_hit *hits,
+                          uint8_t *occluded, uint64_t counts_host[VG_IT_COUNT]);
+int vg_trace_rays_counts(vg_world *world, const vg_ray *rays, size_t n, uint32_t flags,
+                         vg_hit *hits, uint64_t counts_host[VG_IT_COUNT]);
+
+/* raycast (src/raycast.rs:21) over a world on the device: one ray, host in and out.
+ * Returns VG_OK with hit_host->hit = 0/1. Synchronous. */
+int vg_raycast(vg_world *world, const float camera_world[3], const float dir[3],
+               vg_hit *hit_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

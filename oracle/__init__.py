@@ -1,0 +1,307 @@
+"""CPU ORACLE — test infrastructure only.
+
+ctypes wrapper over oracle/liboracle.so (oracle.c: a plain-C restatement of voxelG's ray
+query, see oracle.h for what each function follows and what pins it). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this package, and only as
+the checker or the timed CPU baseline; the product (voxelg_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+BRICK_DTYPE = np.dtype([("occupancy", "<u8"), ("materials", "u1", (64,))])
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("tmax", "<f4"), ("d", "<f4", (3,)), ("flags", "<u4")])
+HIT_DTYPE = np.dtype(
+    [("voxel", "<i4", (3,)), ("t", "<f4"), ("face", "u1"), ("material", "u1"), ("hit", "u1"),
+     ("pad", "u1")]
+)
+LOD, SKIP_TRANSPARENT, ANY_HIT = 1, 2, 4
+FACE_NONE = 6
+IT_COUNT = 9
+IT_NAMES = ("l4_skip", "chunk_skip", "tile_skip", "tile_lod", "tile_uniform", "brick_skip",
+            "brick_uniform", "brick_lod", "voxel")
+# Algorithmic bytes per iteration class: the pyramid words the walk's test chain reads in
+# the reference's order (L4 word 8, chunk word 8, tile-uniform byte 1, tile word 8,
+# brick-uniform byte 1, occupancy word 8); a tile-LOD test adds its tile word, the first
+# child brick's occupancy and one material byte (17) to the class that follows it; a brick
+# LOD reads the occupancy and one material byte past the brick-uniform byte.
+IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34)
+
+
+class OrcWorld(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_int32 * 3), ("bricks", C.c_void_p),
+                ("tile_mask", C.c_void_p), ("chunk_mask", C.c_void_p), ("l4_mask", C.c_void_p),
+                ("brick_uniform", C.c_void_p), ("tile_uniform", C.c_void_p)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P, I32P, U64P, F32P = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(C.c_float)
+        W = C.POINTER(OrcWorld)
+        sig = {
+            "orc_counts": (None, [I32P, C.POINTER(C.c_int64)]),
+            "orc_brick_uniform_of": (C.c_uint8, [P]),
+            "orc_rebuild_all_masks": (None, [W]),
+            "orc_rebuild_all_uniform": (None, [W]),
+            "orc_set_voxel": (None, [W, C.c_int32, C.c_int32, C.c_int32, C.c_uint8]),
+            "orc_material_at_world": (C.c_uint8, [W, C.c_int32, C.c_int32, C.c_int32]),
+            "orc_hash2": (C.c_float, [C.c_int32, C.c_int32]),
+            "orc_hash3": (C.c_float, [C.c_int32, C.c_int32, C.c_int32]),
+            "orc_value_noise_2d": (C.c_float, [C.c_float, C.c_float]),
+            "orc_value_noise_3d": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "orc_fbm_2d": (C.c_float, [C.c_float, C.c_float, C.c_uint32]),
+            "orc_seed_offset_xz": (None, [C.c_uint64, F32P]),
+            "orc_terrain_height": (C.c_int32, [I32P, C.c_uint64, C.c_int32, C.c_int32]),
+            "orc_generate": (None, [W, C.c_uint64]),
+            "orc_raycast_r0": (None, [W, F32P, F32P, C.c_float, P]),
+            "orc_trace_r1": (None, [W, P, C.c_uint32, P, U64P]),
+            "orc_trace_flat": (None, [W, P, C.c_uint32, P]),
+            "orc_trace_brute": (None, [W, P, P, I32P]),
+            "orc_trace_r1_batch": (None, [W, P, C.c_size_t, C.c_uint32, P, U64P, C.c_int]),
+            "orc_raycast_r0_batch": (None, [W, P, C.c_size_t, P, C.c_int]),
+            "orc_primary_dir": (None, [P, C.c_int32, C.c_int32, F32P]),
+            "orc_primary_rays": (None, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, P]),
+            "orc_frame": (None, [W, P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P,
+                                 C.c_float, C.c_int32, C.c_int32, P, P, C.c_int, U64P]),
+            "orc_frame_r0": (None, [W, P, C.c_int32, C.c_int32, C.c_float, F32P, C.c_float,
+                                    C.c_int32, C.c_int32, P, P, C.c_int]),
+            "orc_shadow_origin": (None, [F32P, F32P, P, F32P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(np.float32(x)) for x in v])
+
+
+def _i3(v):
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def counts(dims):
+    out = (C.c_int64 * 5)()
+    lib().orc_counts(_i3(dims), out)
+    return tuple(int(x) for x in out)
+
+
+class OracleWorld:
+    """A world in the reference layout whose arrays the oracle reads (and may write).
+
+    `OracleWorld.view(w)` wraps any object with the World attributes (dims, origin, bricks,
+    tile_mask, chunk_mask, l4_mask, brick_uniform, tile_uniform) without copying."""
+
+    def __init__(self, dims, origin=(0, 0, 0)):
+        nb, nt, nc, nl4, _ = counts(dims)
+        self.dims = tuple(int(d) for d in dims)
+        self.origin = tuple(int(o) for o in origin)
+        self.bricks = np.zeros(nb, BRICK_DTYPE)
+        self.tile_mask = np.zeros(nt, np.uint64)
+        self.chunk_mask = np.zeros(nc, np.uint64)
+        self.l4_mask = np.zeros(max(nl4, 1), np.uint64)
+        self.brick_uniform = np.zeros(nb, np.uint8)
+        self.tile_uniform = np.zeros(nt, np.uint8)
+
+    @classmethod
+    def view(cls, w) -> "OracleWorld":
+        o = cls.__new__(cls)
+        for a in ("dims", "origin", "bricks", "tile_mask", "chunk_mask", "l4_mask",
+                  "brick_uniform", "tile_uniform"):
+            setattr(o, a, getattr(w, a))
+        return o
+
+    def cstruct(self) -> OrcWorld:
+        s = OrcWorld()
+        s.dims = _i3(self.dims)
+        s.origin = _i3(self.origin)
+        s.bricks = _p(self.bricks)
+        s.tile_mask = _p(self.tile_mask)
+        s.chunk_mask = _p(self.chunk_mask)
+        s.l4_mask = _p(self.l4_mask)
+        s.brick_uniform = _p(self.brick_uniform)
+        s.tile_uniform = _p(self.tile_uniform)
+        self._c = s  # keep alive
+        return s
+
+    # -------------------------------------------------------------- build --
+    def generate(self, seed: int) -> "OracleWorld":
+        lib().orc_generate(C.byref(self.cstruct()), C.c_uint64(seed))
+        return self
+
+    def rebuild(self) -> "OracleWorld":
+        s = C.byref(self.cstruct())
+        lib().orc_rebuild_all_masks(s)
+        lib().orc_rebuild_all_uniform(s)
+        return self
+
+    def set_voxel(self, x, y, z, mat) -> None:
+        """Reference set_voxel on WINDOW-LOCAL slot coordinates (incremental masks)."""
+        lib().orc_set_voxel(C.byref(self.cstruct()), int(x), int(y), int(z), int(mat))
+
+    def apply_edit(self, wx, wy, wz, mat) -> None:
+        """World coordinates -> toroidal slot, then set_voxel (src/voxel.rs:450-474)."""
+        rel = [wx - self.origin[0], wy - self.origin[1], wz - self.origin[2]]
+        if any(r < 0 or r >= d for r, d in zip(rel, self.dims)):
+            return
+        s = [int(np.mod(v, d)) for v, d in zip((wx, wy, wz), self.dims)]
+        self.set_voxel(s[0], s[1], s[2], mat)
+
+    def material_at(self, wx, wy, wz) -> int:
+        return int(lib().orc_material_at_world(C.byref(self.cstruct()), int(wx), int(wy), int(wz)))
+
+    # ------------------------------------------------------------- traces --
+    def raycast_r0(self, cam, d, tmax=np.inf) -> np.void:
+        out = np.zeros(1, HIT_DTYPE)
+        lib().orc_raycast_r0(C.byref(self.cstruct()), _f3(cam), _f3(d), float(tmax), _p(out))
+        return out[0]
+
+    def trace_r1(self, o, d, tmax=np.inf, flags=0):
+        ray = make_rays([o], [d], tmax, 0)
+        out = np.zeros(1, HIT_DTYPE)
+        cnt = (C.c_uint64 * IT_COUNT)()
+        lib().orc_trace_r1(C.byref(self.cstruct()), _p(ray), flags, _p(out), cnt)
+        return out[0], dict(zip(IT_NAMES, (int(c) for c in cnt)))
+
+    def trace_flat(self, rays: np.ndarray, flags=0) -> np.ndarray:
+        out = np.zeros(len(rays), HIT_DTYPE)
+        s = C.byref(self.cstruct())
+        for i in range(len(rays)):
+            lib().orc_trace_flat(s, rays[i:i + 1].ctypes.data, flags, out[i:i + 1].ctypes.data)
+        return out
+
+    def trace_brute(self, rays: np.ndarray):
+        out = np.zeros(len(rays), HIT_DTYPE)
+        ties = np.zeros(len(rays), np.int32)
+        s = C.byref(self.cstruct())
+        t = C.c_int32()
+        for i in range(len(rays)):
+            lib().orc_trace_brute(s, rays[i:i + 1].ctypes.data, out[i:i + 1].ctypes.data,
+                                  C.byref(t))
+            ties[i] = t.value
+        return out, ties
+
+    def trace_rays(self, rays: np.ndarray, flags=0, threads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(len(rays), HIT_DTYPE)
+        cnt = (C.c_uint64 * IT_COUNT)()
+        lib().orc_trace_r1_batch(C.byref(self.cstruct()), _p(rays), len(rays), flags, _p(out),
+                                 cnt, threads or os.cpu_count())
+        return out, dict(zip(IT_NAMES, (int(c) for c in cnt)))
+
+    def raycast_rays(self, rays: np.ndarray, threads=0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(len(rays), HIT_DTYPE)
+        lib().orc_raycast_r0_batch(C.byref(self.cstruct()), _p(rays), len(rays), _p(out),
+                                   threads or os.cpu_count())
+        return out
+
+    def frame(self, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
+              rows=None, threads=0):
+        """R1 frame: hits (+ shadow bytes) for rows [r0, r1) of the image."""
+        r0, r1 = rows if rows is not None else (0, height)
+        n = (r1 - r0) * width
+        hits = np.zeros(n, HIT_DTYPE)
+        shadow = np.zeros(n, np.uint8)
+        cnt = (C.c_uint64 * IT_COUNT)()
+        cam = np.ascontiguousarray(camera, np.float32)
+        lib().orc_frame(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
+                        primary_flags, _f3(sun if sun is not None else (0, 0, 0)),
+                        shadow_tmax, r0, r1, _p(hits), _p(shadow), threads or os.cpu_count(), cnt)
+        return hits, shadow, dict(zip(IT_NAMES, (int(c) for c in cnt)))
+
+    def frame_r0(self, camera, width, height, primary_tmax, sun, shadow_tmax, rows=None,
+                 threads=0):
+        r0, r1 = rows if rows is not None else (0, height)
+        n = (r1 - r0) * width
+        hits = np.zeros(n, HIT_DTYPE)
+        shadow = np.zeros(n, np.uint8)
+        cam = np.ascontiguousarray(camera, np.float32)
+        lib().orc_frame_r0(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
+                           _f3(sun if sun is not None else (0, 0, 0)), shadow_tmax, r0, r1,
+                           _p(hits), _p(shadow), threads or os.cpu_count())
+        return hits, shadow
+
+
+def make_rays(o, d, tmax=np.inf, flags=0) -> np.ndarray:
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    r = np.zeros(len(o), RAY_DTYPE)
+    r["o"] = o
+    r["d"] = d
+    r["tmax"] = tmax
+    r["flags"] = flags
+    return r
+
+
+def primary_dirs(camera, width, height) -> np.ndarray:
+    rays = np.zeros(width * height, RAY_DTYPE)
+    cam = np.ascontiguousarray(camera, np.float32)
+    lib().orc_primary_rays(_p(cam), width, height, np.inf, 0, _p(rays))
+    return rays["d"].copy()
+
+
+def algorithmic_bytes(it_counts: dict) -> int:
+    return int(sum(it_counts[n] * b for n, b in zip(IT_NAMES, IT_BYTES)))
+
+
+# noise / generator helpers
+def hash2(x, z):
+    return lib().orc_hash2(int(x), int(z))
+
+
+def hash3(x, y, z):
+    return lib().orc_hash3(int(x), int(y), int(z))
+
+
+def value_noise_2d(x, z):
+    return lib().orc_value_noise_2d(float(x), float(z))
+
+
+def value_noise_3d(x, y, z):
+    return lib().orc_value_noise_3d(float(x), float(y), float(z))
+
+
+def fbm_2d(x, z, octaves):
+    return lib().orc_fbm_2d(float(x), float(z), int(octaves))
+
+
+def seed_offset_xz(seed):
+    out = (C.c_float * 2)()
+    lib().orc_seed_offset_xz(C.c_uint64(seed), out)
+    return float(out[0]), float(out[1])
+
+
+def terrain_height(dims, seed, x, z):
+    return int(lib().orc_terrain_height(_i3(dims), C.c_uint64(seed), int(x), int(z)))
+
+
+def brick_uniform_of(brick: np.ndarray) -> int:
+    b = np.ascontiguousarray(np.asarray(brick, BRICK_DTYPE).reshape(1))
+    return int(lib().orc_brick_uniform_of(_p(b)))

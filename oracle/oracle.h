@@ -1,0 +1,155 @@
+/*
+ * CPU ORACLE — test infrastructure only.
+ *
+ * A plain-C restatement of voxelG's voxel ray query and the data it walks.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this library, and only as the checker (or as the timed CPU baseline). The
+ * product path (voxelg_amd, libvoxelg_amd.so) never links, loads or calls it.
+ *
+ * Contents and what each follows in the reference:
+ *   - world storage and index conventions    src/voxel.rs:189-326, src/world_dims.rs:9-37
+ *   - pyramid / uniform-LUT builders          src/voxel.rs:233-272, 879-1000
+ *   - value noise, fBm, seed offsets          src/voxel.rs:1136-1232, 1999-2021
+ *   - the benchmark terrain generator         (new: fBm heightmap + 3D-noise caves)
+ *   - R0: literal CPU ray query               src/raycast.rs:21-141
+ *   - R1: exact hierarchical walk             shaders/raymarch.wgsl:3369-3448, 4600-4864
+ *   - pinhole primary rays                    shaders/raymarch.wgsl:430-438, 600; src/camera.rs:25-42
+ *   - shadow rays                             shaders/raymarch.wgsl:3953-3995, 4345-4409
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   index/bit conventions, builders  closed forms + brute-force rebuild + incremental==full
+ *   noise / generator                lattice closed forms, continuity, bounds, determinism
+ *   R0                               the reference's KAT scenes (src/accel.rs:515-942)
+ *   R1 (LOD off)                     R1 == flat per-voxel walk; == brute-force slab test on
+ *                                    tiny worlds; R1 == R0 on clean hits
+ *   R1 LOD (t > 400 / 520)           hand-built bricks with known topmost material; parity
+ *                                    vs the reference is UNPINNED (its shader cannot run here)
+ *   primary ray generation           closed forms (centre pixel == forward, unit length)
+ *   shadow                           roof-shadow KAT; any-hit == (first hit t <= tmax)
+ */
+#ifndef VG_ORACLE_H
+#define VG_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* voxelG's 72-byte Brick (src/voxel.rs:296-300): occupancy bit per voxel, voxel
+ * index x + 4z + 16y, one material byte per voxel. */
+typedef struct orc_brick {
+    uint64_t occupancy;
+    uint8_t materials[64];
+} orc_brick;
+
+/* World in the reference's own layout: x-fastest linear indices at every level. */
+typedef struct orc_world {
+    int32_t dims[3];      /* voxels */
+    int32_t origin[3];    /* world-voxel coordinate of the window corner */
+    orc_brick *bricks;    /* bx + by*BX + bz*BX*BY */
+    uint64_t *tile_mask;  /* one u64 per 16^3 tile, bit per child brick */
+    uint64_t *chunk_mask; /* one u64 per 64^3 chunk, bit per child tile */
+    uint64_t *l4_mask;    /* one u64 per 256^3 cell, bit per child chunk */
+    uint8_t *brick_uniform;
+    uint8_t *tile_uniform;
+} orc_world;
+
+/* Ray and hit records: the same bytes as vg_ray / vg_hit in include/vg.h. */
+typedef struct orc_ray {
+    float o[3];
+    float tmax;
+    float d[3];
+    uint32_t flags;
+} orc_ray;
+
+typedef struct orc_hit {
+    int32_t voxel[3];
+    float t;
+    uint8_t face;
+    uint8_t material;
+    uint8_t hit;
+    uint8_t pad;
+} orc_hit;
+
+#define ORC_LOD 1u
+#define ORC_SKIP_TRANSPARENT 2u
+#define ORC_ANY_HIT 4u
+
+#define ORC_FACE_NONE 6
+
+/* Iteration classes of the hierarchical walk, counted per ray. */
+enum {
+    ORC_IT_L4_SKIP = 0,
+    ORC_IT_CHUNK_SKIP,
+    ORC_IT_TILE_SKIP,
+    ORC_IT_TILE_LOD,
+    ORC_IT_TILE_UNIFORM,
+    ORC_IT_BRICK_SKIP,
+    ORC_IT_BRICK_UNIFORM,
+    ORC_IT_BRICK_LOD,
+    ORC_IT_VOXEL,
+    ORC_IT_COUNT
+};
+
+/* dims / counts helpers */
+void orc_counts(const int32_t dims[3], int64_t out[5]); /* bricks, tiles, chunks, l4, voxels */
+
+/* storage */
+int orc_world_alloc(orc_world *w, const int32_t dims[3]);
+void orc_world_free(orc_world *w);
+void orc_world_clear(orc_world *w);
+
+/* builders (reference semantics) */
+uint8_t orc_brick_uniform_of(const orc_brick *b);
+void orc_rebuild_all_masks(orc_world *w);
+void orc_rebuild_all_uniform(orc_world *w);
+void orc_set_voxel(orc_world *w, int32_t x, int32_t y, int32_t z, uint8_t mat);
+uint8_t orc_material_at_world(const orc_world *w, int32_t x, int32_t y, int32_t z);
+
+/* noise + generator */
+float orc_hash2(int32_t x, int32_t z);
+float orc_hash3(int32_t x, int32_t y, int32_t z);
+float orc_value_noise_2d(float x, float z);
+float orc_value_noise_3d(float x, float y, float z);
+float orc_fbm_2d(float x, float z, uint32_t octaves);
+void orc_seed_offset_xz(uint64_t seed, float out[2]);
+int32_t orc_terrain_height(const int32_t dims[3], uint64_t seed, int32_t x, int32_t z);
+void orc_generate(orc_world *w, uint64_t seed);
+
+/* R0: literal transliteration of src/raycast.rs, extended with tmax, t and material. */
+void orc_raycast_r0(const orc_world *w, const float cam[3], const float dir[3], float tmax,
+                    orc_hit *out);
+/* R1: the exact hierarchical walk (counts may be NULL). */
+void orc_trace_r1(const orc_world *w, const orc_ray *r, uint32_t flags, orc_hit *out,
+                  uint64_t counts[ORC_IT_COUNT]);
+/* R1 without any hierarchy or uniform shortcut: one voxel per iteration. */
+void orc_trace_flat(const orc_world *w, const orc_ray *r, uint32_t flags, orc_hit *out);
+/* Brute force over every solid voxel (tiny worlds only): nearest slab entry. */
+void orc_trace_brute(const orc_world *w, const orc_ray *r, orc_hit *out, int32_t *n_ties);
+
+void orc_trace_r1_batch(const orc_world *w, const orc_ray *rays, size_t n, uint32_t flags,
+                        orc_hit *out, uint64_t counts[ORC_IT_COUNT], int threads);
+void orc_raycast_r0_batch(const orc_world *w, const orc_ray *rays, size_t n, orc_hit *out,
+                          int threads);
+
+/* primary rays + frame (CameraUniform-compatible 176-byte camera, see src/camera.rs:61-113) */
+void orc_primary_dir(const float *camera176, int32_t px, int32_t py, float out[3]);
+void orc_primary_rays(const float *camera176, int32_t w, int32_t h, float tmax, uint32_t flags,
+                      orc_ray *out);
+/* One frame: primary (R1, flags) then a sun shadow ray per hit. */
+void orc_frame(const orc_world *w, const float *camera176, int32_t width, int32_t height,
+               float primary_tmax, uint32_t primary_flags, const float sun[3], float shadow_tmax,
+               int32_t row0, int32_t row1, orc_hit *hits, uint8_t *shadow, int threads,
+               uint64_t counts[ORC_IT_COUNT]);
+/* Same frame with R0 as the traversal (the CPU baseline). */
+void orc_frame_r0(const orc_world *w, const float *camera176, int32_t width, int32_t height,
+                  float primary_tmax, const float sun[3], float shadow_tmax, int32_t row0,
+                  int32_t row1, orc_hit *hits, uint8_t *shadow, int threads);
+void orc_shadow_origin(const float o[3], const float d[3], const orc_hit *h, float out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

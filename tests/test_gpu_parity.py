@@ -1,0 +1,217 @@
+"""GPU parity: every result of the HIP path equals the oracle's R1 bit for bit (voxel, t bits,
+face, material, hit flag, shadow byte), and GPU-vs-R0 disagreements are edge ties only."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+import voxelg_amd as vg
+from voxelg_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from voxelg_amd.device import Context, device_count
+    assert device_count() > 0, "GPU tests need a HIP device"
+    return Context(0)
+
+
+@pytest.fixture(scope="module")
+def small(ctx, small_world):
+    return ctx.world(small_world.dims).upload(small_world)
+
+
+def gpu_hits(h):
+    from voxelg_amd.device import decode_hits
+    return decode_hits(h)
+
+
+def assert_same(got, ref, what):
+    gb = got.view(np.uint8).reshape(len(got), -1)
+    rb = ref.view(np.uint8).reshape(len(ref), -1)
+    bad = np.nonzero((gb != rb).any(1))[0]
+    assert len(bad) == 0, f"{what}: {len(bad)} of {len(got)} differ, first {bad[:4]}: " \
+                          f"gpu {got[bad[:2]]} oracle {ref[bad[:2]]}"
+
+
+def test_small_primary_frame_bit_exact(ctx, small, small_world):
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    print(f"case: small primary {wl.width}x{wl.height}", flush=True)
+    hits = small.trace_primary(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags)
+    ctx.synchronize()
+    ref, _, _ = oracle.OracleWorld.view(small_world).frame(
+        cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags, None, 0.0, threads=THREADS)
+    got = gpu_hits(hits)
+    assert_same(got, ref, "small primary")
+    assert 0.3 < got["hit"].mean() < 1.0
+
+
+def test_small_frame_lod_shadow_bit_exact(ctx, small, small_world):
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    print("case: small frame LOD+shadow", flush=True)
+    hits, occ = small.trace_frame(cam, wl.width, wl.height, 700.0, vg.VG_RAY_LOD, sun, 480.0)
+    ctx.synchronize()
+    ref, ref_occ, _ = oracle.OracleWorld.view(small_world).frame(
+        cam, wl.width, wl.height, 700.0, vg.VG_RAY_LOD, sun, 480.0, threads=THREADS)
+    assert_same(gpu_hits(hits), ref, "small frame hits")
+    occ = occ.numpy(np.uint8)
+    assert np.array_equal(occ, ref_occ)
+    assert occ[ref["hit"] == 0].sum() == 0
+    assert 0 < occ.sum() < ref["hit"].sum()
+    # the split entry points give the same frame
+    h2 = small.trace_primary(cam, wl.width, wl.height, 700.0, vg.VG_RAY_LOD)
+    o2 = small.trace_shadow(cam, wl.width, wl.height, h2, sun, 480.0)
+    ctx.synchronize()
+    assert_same(gpu_hits(h2), ref, "split primary")
+    assert np.array_equal(o2.numpy(np.uint8), ref_occ)
+
+
+def mixed_rays(rng, n, dims):
+    d = np.asarray(dims, np.float32)
+    o = (rng.uniform(-0.2, 1.2, (n, 3)) * d).astype(np.float32)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32)
+    k = rng.integers(0, 3, n)
+    sel = rng.random(n) < 0.1
+    dirs[sel, k[sel]] = 0.0
+    sel = rng.random(n) < 0.02
+    dirs[sel, k[sel]] = np.float32(5e-9)
+    sel = rng.random(n) < 0.01
+    dirs[sel] = 0.0
+    tmax = rng.choice(np.array([32, 128, 512, 1024, np.inf], np.float32), n)
+    flags = rng.choice(np.array([0, 1, 2, 4, 5], np.uint32), n)
+    return oracle.make_rays(o, dirs, tmax, flags)
+
+
+def test_batched_rays_bit_exact(ctx, small, small_world):
+    rng = np.random.default_rng(11)
+    rays = mixed_rays(rng, 1 << 18, small_world.dims)
+    print(f"case: batched {len(rays)} mixed rays", flush=True)
+    hits = small.trace_rays(ctx.rays(rays))
+    ctx.synchronize()
+    ref, _ = oracle.OracleWorld.view(small_world).trace_rays(rays, 0, THREADS)
+    assert_same(gpu_hits(hits), ref, "batched")
+    # a call-level flag ORs into every ray
+    hits = small.trace_rays(ctx.rays(rays), flags=vg.VG_RAY_SKIP_TRANSPARENT)
+    ctx.synchronize()
+    ref, _ = oracle.OracleWorld.view(small_world).trace_rays(rays, oracle.SKIP_TRANSPARENT, THREADS)
+    assert_same(gpu_hits(hits), ref, "batched skip-transparent")
+
+
+def test_iteration_counts_match_oracle(ctx, small, small_world):
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    print("case: counts", flush=True)
+    _, _, c = small.trace_frame_counts(cam, wl.width, wl.height, 700.0, vg.VG_RAY_LOD, sun, 480.0)
+    _, _, rc = oracle.OracleWorld.view(small_world).frame(
+        cam, wl.width, wl.height, 700.0, vg.VG_RAY_LOD, sun, 480.0, threads=THREADS)
+    assert c == rc
+    rng = np.random.default_rng(5)
+    rays = mixed_rays(rng, 1 << 15, small_world.dims)
+    _, c = small.trace_rays_counts(ctx.rays(rays))
+    _, rc = oracle.OracleWorld.view(small_world).trace_rays(rays, 0, THREADS)
+    assert c == rc
+
+
+@pytest.mark.parametrize("origin", [(0, 0, 0), (96, 0, 160)])
+def test_kat_floor_pillars(ctx, origin):
+    ow = scenes.floor_pillars(origin)
+    eyes, dirs = scenes.floor_pillar_rays(origin)
+    rays = oracle.make_rays(eyes, dirs)
+    print(f"case: floor+pillars origin {origin}", flush=True)
+    dw = ctx.world(ow.dims).upload(scenes.to_world(ow))
+    got = gpu_hits(dw.trace_rays(ctx.rays(rays)))
+    ctx.synchronize()
+    ref, _ = ow.trace_rays(rays)
+    assert_same(got, ref, "pillars vs R1")
+    r0 = np.array([ow.raycast_r0(e, d) for e, d in zip(eyes, dirs)])
+    clean, floor, pillar, grazing, bad = scenes.classify(eyes, dirs, got, r0)
+    assert not bad and clean > len(eyes) / 3 and floor > 0 and pillar > 0 and grazing < clean
+    # the drop-in raycast returns PickHit with world voxel and normal
+    from voxelg_amd.device import FACE_NORMALS, raycast
+    for i in range(0, len(eyes), 37):
+        p = raycast(eyes[i], dirs[i], dw, origin)
+        if r0[i]["hit"]:
+            assert p is not None and p.voxel == tuple(int(v) for v in ref[i]["voxel"])
+            assert p.normal == FACE_NORMALS[int(ref[i]["face"])]
+        else:
+            assert p is None
+    dw.close()
+
+
+def test_kat_roof_shadow_and_empty(ctx):
+    ow = scenes.roof_scene()
+    origins, sun, under, open_ = scenes.roof_samples()
+    rays = oracle.make_rays(origins, np.repeat(sun[None], len(origins), 0))
+    print("case: roof shadow", flush=True)
+    dw = ctx.world(ow.dims).upload(scenes.to_world(ow))
+    got = gpu_hits(dw.trace_rays(ctx.rays(rays), flags=vg.VG_RAY_ANY_HIT))
+    ctx.synchronize()
+    ref, _ = ow.trace_rays(rays, oracle.ANY_HIT)
+    assert_same(got, ref, "roof shadow")
+    occ = got["hit"] == 1
+    cpu = np.array([ow.raycast_r0(o, sun)["hit"] for o in origins]) == 1
+    assert np.sum((occ != cpu) & (under | open_)) == 0
+    assert np.sum(under & occ) > 20 and np.sum(open_ & ~occ) > 20
+    dw.close()
+    empty = vg.World(scenes.REF_DIMS)
+    dw = ctx.world(empty.dims).upload(empty)
+    got = gpu_hits(dw.trace_rays(ctx.rays(rays)))
+    ctx.synchronize()
+    assert not got["hit"].any() and np.all(got["t"] == -1.0) and np.all(got["face"] == 6)
+    dw.close()
+
+
+def test_edge_cases_and_errors(ctx, small, small_world):
+    from voxelg_amd._lib import VGError
+    # zero rays is a no-op
+    small.trace_rays(ctx.alloc(32), n=0)
+    # a world that was never uploaded refuses to trace
+    dw = ctx.world((64, 64, 64))
+    with pytest.raises(VGError):
+        dw.trace_rays(ctx.alloc(4 * 32), n=4)
+    # a folding origin needs dims that are multiples of 256
+    w = vg.World((64, 64, 64), origin=(32, 0, 0))
+    with pytest.raises(VGError):
+        dw.upload(w)
+    dw.close()
+    # camera resolution must match the launch
+    _, cam = workloads.SMALL.camera()
+    with pytest.raises(VGError):
+        small.trace_primary(cam, 640, 360, 512.0)
+
+
+def test_large_frame_sampled_rows_bit_exact(ctx):
+    wl = workloads.LARGE
+    print("case: large world generate", flush=True)
+    world = wl.world()
+    _, cam = wl.camera()
+    sun = wl.sun()
+    dw = ctx.world(world.dims).upload(world)
+    print("case: large frame", flush=True)
+    hits, occ = dw.trace_frame(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags, sun,
+                               wl.shadow_tmax)
+    ctx.synchronize()
+    got = gpu_hits(hits).reshape(wl.height, wl.width)
+    occ = occ.numpy(np.uint8).reshape(wl.height, wl.width)
+    ow = oracle.OracleWorld.view(world)
+    for r0 in range(0, wl.height, wl.height // 12):
+        ref, ref_occ, _ = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags,
+                                   sun, wl.shadow_tmax, rows=(r0, r0 + 2), threads=THREADS)
+        assert_same(got[r0:r0 + 2].reshape(-1), ref, f"large rows {r0}")
+        assert np.array_equal(occ[r0:r0 + 2].reshape(-1), ref_occ)
+    flat = got.reshape(-1)
+    h = flat["hit"] == 1
+    assert h.mean() > 0.3
+    assert np.all(flat["t"][h] <= wl.primary_tmax) and np.all(flat["t"][h] >= 0)
+    assert np.all(occ.reshape(-1)[~h] == 0)
+    dw.close()

@@ -1,0 +1,140 @@
+"""ctypes binding of libvoxelg_amd.so (include/vg.h).
+
+The library is the product: the HIP kernels for gfx950 and the host-side world builders.
+Loading fails loudly when it has not been built; nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvoxelg_amd.so")
+
+BRICK_DTYPE = np.dtype([("occupancy", "<u8"), ("materials", "u1", (64,))])
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("tmax", "<f4"), ("d", "<f4", (3,)), ("flags", "<u4")])
+HIT_DTYPE = np.dtype(
+    [("voxel", "<i4", (3,)), ("t", "<f4"), ("face", "u1"), ("material", "u1"), ("hit", "u1"),
+     ("pad", "u1")]
+)
+assert BRICK_DTYPE.itemsize == 72 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 20
+
+VG_OK = 0
+VG_RAY_LOD = 1
+VG_RAY_SKIP_TRANSPARENT = 2
+VG_RAY_ANY_HIT = 4
+VG_FACE_NONE = 6
+VG_IT_COUNT = 9
+IT_NAMES = ("l4_skip", "chunk_skip", "tile_skip", "tile_lod", "tile_uniform", "brick_skip",
+            "brick_uniform", "brick_lod", "voxel")
+# Algorithmic bytes per iteration class: the pyramid words the walk's test chain reads in
+# the reference's order (shaders/raymarch.wgsl:4626-4775): L4 word 8, chunk word 8,
+# tile-uniform byte 1, tile word 8, brick-uniform byte 1, occupancy word 8. A tile-LOD test
+# adds its tile word, the first child brick's occupancy and one material byte (17); a brick
+# LOD reads the occupancy and one material byte past the brick-uniform byte (35).
+IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34)
+HIT_RECORD_BYTES = 20   # one vg_hit written per ray
+MATERIAL_BYTES = 1      # material byte read at a voxel hit
+RAY_INPUT_BYTES = 32    # one vg_ray read per batched ray
+
+
+def algorithmic_bytes(it_counts: dict) -> int:
+    return int(sum(int(it_counts[n]) * b for n, b in zip(IT_NAMES, IT_BYTES)))
+
+
+class VGError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+_lock = threading.Lock()
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_U64P = C.POINTER(C.c_uint64)
+_F32P = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "vg_abi_version": (C.c_int, []),
+    "vg_last_error": (C.c_char_p, []),
+    "vg_status_string": (C.c_char_p, [C.c_int]),
+    "vg_world_counts": (C.c_int, [_I32P, _I64P]),
+    "vg_generate_terrain": (C.c_int, [_I32P, _I32P, C.c_uint64, _P, C.c_int]),
+    "vg_terrain_height": (C.c_int, [_I32P, C.c_uint64, C.c_int32, C.c_int32, _I32P]),
+    "vg_build_pyramid": (C.c_int, [_I32P, _P, _P, _P, _P, _P, _P, C.c_int]),
+    "vg_camera_from_pose": (C.c_int, [_F32P, C.c_float, C.c_float, C.c_float, C.c_int32,
+                                      C.c_int32, _I32P, _P]),
+    "vg_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "vg_ctx_destroy": (C.c_int, [_P]),
+    "vg_world_create": (C.c_int, [_P, _I32P, C.POINTER(_P)]),
+    "vg_world_destroy": (C.c_int, [_P]),
+    "vg_world_upload": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _I32P]),
+    "vg_trace_rays": (C.c_int, [_P, _P, C.c_size_t, C.c_uint32, _P, _P]),
+    "vg_trace_primary": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _P, _P]),
+    "vg_trace_shadow": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _F32P, C.c_float, _P, _P]),
+    "vg_trace_frame": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _F32P,
+                                 C.c_float, _P, _P, _P]),
+    "vg_trace_frame_counts": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
+                                        _F32P, C.c_float, _P, _P, _U64P]),
+    "vg_trace_rays_counts": (C.c_int, [_P, _P, C.c_size_t, C.c_uint32, _P, _U64P]),
+    "vg_raycast": (C.c_int, [_P, _F32P, _F32P, _P]),
+    "vg_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "vg_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "vg_free": (C.c_int, [_P, _P]),
+    "vg_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "vg_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "vg_memset": (C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    "vg_synchronize": (C.c_int, [_P]),
+    "vg_stream_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "vg_stream_destroy": (C.c_int, [_P, _P]),
+    "vg_stream_synchronize": (C.c_int, [_P, _P]),
+    "vg_event_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "vg_event_destroy": (C.c_int, [_P, _P]),
+    "vg_event_record": (C.c_int, [_P, _P, _P]),
+    "vg_event_elapsed_ms": (C.c_int, [_P, _P, _P, C.POINTER(C.c_float)]),
+}
+
+
+def lib():
+    """Load libvoxelg_amd.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                    "g.build()'` (or make -C voxelg_amd/csrc)"
+                )
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(status: int, where: str) -> None:
+    if status != VG_OK:
+        msg = lib().vg_last_error().decode(errors="replace")
+        raise VGError(status, where, msg)
+
+
+def i32x3(v) -> C.Array:
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+def f32x3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(np.float32(x)) for x in v])
+
+
+def ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data

@@ -1,0 +1,1053 @@
+// Device engine for gfx950 (MI355X, CDNA4, wave64): world storage in HBM, the
+// relayout kernels of the upload path, and the exact hierarchical voxel walk used
+// by every trace entry point.
+//
+// HBM layout (all arrays owned by vg_world):
+//   l4[n_l4]          u64  reference order (x fastest)             one bit per child chunk
+//   chunk[n_chunks]   u64  reference order                         one bit per child tile
+//   tile[c*64 + tb]   u64  nested: chunk index, then tile bit      one bit per child brick
+//   tile_uni[same]    u8
+//   occ[t*64 + bb]    u64  nested: nested tile, then brick bit     one bit per voxel
+//   brick_uni[same]   u8
+//   mat[b*64 + vb]    u8   nested brick, then voxel bit
+// so the 64 child words of any tile (or chunk) are one contiguous 512-byte run that a
+// wave reads in one coalesced pass, and a brick's 64 materials are one 64-byte line.
+//
+// Exactness: the walk uses only IEEE +, -, *, / and floor in fp32 (contraction off,
+// correctly rounded divide and sqrt), recomputes every plane-crossing t from integer
+// coordinates, and skips empty cells with an integer-exact exit computation, so its
+// result equals a one-voxel-at-a-time walk bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "vg_internal.h"
+
+namespace vg {
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(VG_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+struct DevWorld {
+    const uint64_t *l4;
+    const uint64_t *chunk;
+    const uint64_t *tile;
+    const uint64_t *occ;
+    const uint8_t *tile_uni;
+    const uint8_t *brick_uni;
+    const uint8_t *mat;
+    int32_t dims[3];
+    int32_t ro_mod[3];
+    int32_t origin[3];
+    int32_t cdim[3];
+    int32_t l4dim[3];
+};
+
+struct CamParams {
+    float origin[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float tan_half_fov;
+    float res[2];
+    float jitter[2];
+};
+
+// ------------------------------------------------------------ material rules --
+// shaders/raymarch.wgsl:891-920 predicates; src/raycast.rs:88-89 fringe exclusion.
+__device__ __forceinline__ bool is_transparent(uint32_t m) { return (m >= 5u && m <= 12u) || m == 18u; }
+__device__ __forceinline__ bool is_foliage(uint32_t m) {
+    return m == 14u || m == 25u || m == 26u || m == 27u || m == 30u || m == 31u || m == 33u ||
+           m == 34u || m == 35u || m == 36u || m == 37u;
+}
+__device__ __forceinline__ bool uniform_optimisable(uint32_t m) {
+    return !is_foliage(m) && !is_transparent(m);
+}
+__device__ __forceinline__ bool solid_mat(uint32_t m, uint32_t flags) {
+    if (m == 0u || m == 33u) return false;
+    if ((flags & VG_RAY_SKIP_TRANSPARENT) && is_transparent(m)) return false;
+    if ((flags & VG_RAY_ANY_HIT) && m == 35u) return false;
+    return true;
+}
+
+__device__ __forceinline__ float fsel_min(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fsel_max(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float safe_inv(float x) { return fabsf(x) < 1e-8f ? 1e30f : 1.0f / x; }
+__device__ __forceinline__ int cbit(int32_t x, int32_t y, int32_t z) { return x + z * 4 + y * 16; }
+
+// --------------------------------------------------------------- walk state --
+struct Walk {
+    float org[3], dir[3], inv[3];
+    int32_t step[3];
+    int32_t l[3];
+    float t_cur;
+    int32_t last_axis;
+    int32_t entry_face;
+    int32_t face_pack; // 4 bits per axis: face code of an entry through that axis
+};
+
+struct HitRec {
+    int32_t v[3];
+    float t;
+    uint32_t face, mat, hit;
+};
+
+__device__ __forceinline__ void hit_miss(HitRec &h) {
+    h.v[0] = h.v[1] = h.v[2] = 0;
+    h.t = -1.0f;
+    h.face = VG_FACE_NONE;
+    h.mat = 0;
+    h.hit = 0;
+}
+
+// Ray setup: window-local origin, slab entry, 1e-3 bias clamped into the window
+// (shaders/raymarch.wgsl:3369-3411, src/raycast.rs:34-50). dir is already normalized.
+__device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const float o[3],
+                                          const float d[3]) {
+    float tmn[3], tmx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.org[k] = o[k] - (float)w.origin[k];
+        r.dir[k] = d[k];
+        r.inv[k] = safe_inv(d[k]);
+        float t0 = (0.0f - r.org[k]) * r.inv[k];
+        float t1 = ((float)w.dims[k] - r.org[k]) * r.inv[k];
+        tmn[k] = fsel_min(t0, t1);
+        tmx[k] = fsel_max(t0, t1);
+        r.step[k] = fabsf(d[k]) < 1e-8f ? 0 : (d[k] > 0.0f ? 1 : -1);
+    }
+    if (r.step[0] == 0 && r.step[1] == 0 && r.step[2] == 0) return false; // cannot advance
+    float t_enter = fsel_max(fsel_max(tmn[0], fsel_max(tmn[1], tmn[2])), 0.0f);
+    float t_exit = fsel_min(tmx[0], fsel_min(tmx[1], tmx[2]));
+    if (t_enter >= t_exit || t_exit < 0.0f) return false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float p = r.org[k] + d[k] * (t_enter + 1e-3f);
+        p = fsel_min(fsel_max(p, 1e-3f), (float)w.dims[k] - 1e-3f);
+        r.l[k] = (int32_t)floorf(p);
+    }
+    r.t_cur = t_enter;
+    r.last_axis = -1;
+    r.face_pack = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int f = r.step[k] > 0 ? 2 * k + 1 : (r.step[k] < 0 ? 2 * k : VG_FACE_NONE);
+        r.face_pack |= f << (4 * k);
+    }
+    int la = (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) ? 0 : (tmn[1] >= tmn[2] ? 1 : 2);
+    r.entry_face = (r.face_pack >> (4 * la)) & 15;
+    return true;
+}
+
+// t of the plane crossed to enter voxel e along axis k.
+__device__ __forceinline__ float t_plane(const Walk &r, int k, int32_t e) {
+    int32_t p = r.step[k] > 0 ? e : e + 1;
+    return ((float)p - r.org[k]) * r.inv[k];
+}
+
+// One voxel advance: nearest plane, ties to the lowest axis.
+__device__ __forceinline__ void walk_step(Walk &r) {
+    int ax = -1;
+    float m = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (r.step[k] != 0) {
+            float t = t_plane(r, k, r.l[k] + r.step[k]);
+            if (ax < 0 || t < m) {
+                ax = k;
+                m = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k == ax) r.l[k] += r.step[k];
+    r.t_cur = m;
+    r.last_axis = ax;
+}
+
+// Leave the aligned empty cell of size S containing the current voxel at exactly the
+// state a one-voxel walk reaches at its first cell-face crossing (T*, ax*): the crossed
+// axis lands just outside the face, every other axis on the last in-cell plane whose
+// (t, axis) orders before (T*, ax*), found from a floor estimate and corrected by direct t
+// comparisons (t along an axis is monotone in the plane coordinate).
+__device__ __forceinline__ void walk_skip(Walk &r, const int32_t sl[3], int32_t S) {
+    int32_t lo[3];
+    int ax = -1;
+    float ts = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = r.l[k] - (sl[k] & (S - 1));
+        if (r.step[k] != 0) {
+            int32_t plane = r.step[k] > 0 ? lo[k] + S : lo[k];
+            float t = ((float)plane - r.org[k]) * r.inv[k];
+            if (ax < 0 || t < ts) {
+                ts = t;
+                ax = k;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k == ax) {
+            r.l[k] = r.step[k] > 0 ? lo[k] + S : lo[k] - 1;
+        } else if (r.step[k] != 0) {
+            const int32_t st = r.step[k];
+            const int32_t first = r.l[k];
+            const int32_t last = st > 0 ? lo[k] + S - 1 : lo[k];
+            int32_t e = (int32_t)floorf(r.org[k] + r.dir[k] * ts);
+            if (st > 0) {
+                e = e < first ? first : e;
+                e = e > last ? last : e;
+            } else {
+                e = e > first ? first : e;
+                e = e < last ? last : e;
+            }
+            // bounded by the cell width; normally 0-2 iterations
+            for (int g = 0; g < 258 && e != first; ++g) {
+                float t = t_plane(r, k, e);
+                if (t < ts || (t == ts && k < ax)) break;
+                e -= st;
+            }
+            for (int g = 0; g < 258 && e != last; ++g) {
+                float t = t_plane(r, k, e + st);
+                if (!(t < ts || (t == ts && k < ax))) break;
+                e += st;
+            }
+            r.l[k] = e;
+        }
+    }
+    r.t_cur = ts;
+    r.last_axis = ax;
+}
+
+__device__ __forceinline__ void emit(const Walk &r, const DevWorld &w, uint32_t m, HitRec &h) {
+    h.hit = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h.v[k] = r.l[k] + w.origin[k];
+    h.t = r.t_cur;
+    h.mat = m;
+    h.face = r.last_axis < 0 ? (uint32_t)r.entry_face
+                             : (uint32_t)((r.face_pack >> (4 * r.last_axis)) & 15);
+}
+
+// shaders/raymarch.wgsl:253-295: topmost non-turf voxel's material, layer y = 3 first.
+__device__ __forceinline__ uint32_t topmost_material(const DevWorld &w, uint32_t bn, uint64_t occ) {
+#pragma unroll
+    for (int y = 3; y >= 0; --y) {
+        uint32_t layer = (uint32_t)(occ >> (16 * y)) & 0xFFFFu;
+        if (layer) {
+            uint32_t m = w.mat[(uint64_t)bn * 64u + 16u * y + __builtin_ctz(layer)];
+            if (m != 35u) return m;
+        }
+    }
+    return occ ? 35u : 0u;
+}
+
+#define LOD_BRICK_T 400.0f
+#define TILE_LOD_T 520.0f
+
+// The walk: phase 1 leaves solid voxels the ray starts in (src/raycast.rs:108-115),
+// phase 2 returns the first solid voxel. Each iteration tests the coarsest level and
+// skips the largest empty aligned cell (shaders/raymarch.wgsl:4626-4775). The words of
+// the current L4 cell, chunk, tile and brick stay in registers until the walk leaves
+// them. COUNT tallies iteration classes for algorithmic-byte accounting.
+template <bool COUNT>
+__device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax, uint32_t flags,
+                                         HitRec &h, uint32_t *cnt) {
+    const bool lod = (flags & VG_RAY_LOD) && !(flags & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
+    bool phase1 = true;
+    uint32_t cur_l4 = ~0u, cur_c = ~0u, cur_t = ~0u, cur_b = ~0u;
+    uint64_t l4w = 0, cw = 0, tw = 0, ow = 0;
+    uint32_t tu = 0, bu = 0;
+    // every iteration crosses at least one voxel plane inside the window
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    for (int it = 0; it < max_iter; ++it) {
+        if (r.t_cur > tmax) return;
+        if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
+            (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
+            return;
+        int32_t sl[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t s = r.l[k] + w.ro_mod[k];
+            sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
+        }
+        int32_t S = 0; // size of the empty (or phase-1 solid) cell to leave; 0 = one voxel
+        do {
+            const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
+                                 (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
+            if (l4i != cur_l4) {
+                cur_l4 = l4i;
+                l4w = w.l4[l4i];
+            }
+            if (l4w == 0) {
+                if (COUNT) cnt[VG_IT_L4_SKIP]++;
+                phase1 = false;
+                S = 256;
+                break;
+            }
+            if (!((l4w >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
+                if (COUNT) cnt[VG_IT_CHUNK_SKIP]++;
+                phase1 = false;
+                S = 64;
+                break;
+            }
+            const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                                (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+            if (ci != cur_c) {
+                cur_c = ci;
+                cw = w.chunk[ci];
+            }
+            const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
+            if (!((cw >> tb) & 1)) {
+                if (COUNT) cnt[VG_IT_TILE_SKIP]++;
+                phase1 = false;
+                S = 16;
+                break;
+            }
+            const uint32_t tn = ci * 64u + (uint32_t)tb;
+            if (tn != cur_t) {
+                cur_t = tn;
+                tw = w.tile[tn];
+                tu = w.tile_uni[tn];
+            }
+            if (lod && !phase1 && r.t_cur > TILE_LOD_T) {
+                if (COUNT) cnt[VG_IT_TILE_LOD]++;
+                if (tw) {
+                    const uint32_t fb = tn * 64u + (uint32_t)__builtin_ctzll(tw);
+                    uint32_t lm = topmost_material(w, fb, w.occ[fb]);
+                    if (lm != 0u && uniform_optimisable(lm)) {
+                        emit(r, w, lm, h);
+                        return;
+                    }
+                }
+            }
+            if (tu != 0u) {
+                if (COUNT) cnt[VG_IT_TILE_UNIFORM]++;
+                if (solid_mat(tu, flags)) {
+                    if (!phase1) {
+                        emit(r, w, tu, h);
+                        return;
+                    }
+                } else {
+                    phase1 = false;
+                }
+                S = 16;
+                break;
+            }
+            const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
+            if (!((tw >> bb) & 1)) {
+                if (COUNT) cnt[VG_IT_BRICK_SKIP]++;
+                phase1 = false;
+                S = 4;
+                break;
+            }
+            const uint32_t bn = tn * 64u + (uint32_t)bb;
+            if (bn != cur_b) {
+                cur_b = bn;
+                ow = w.occ[bn];
+                bu = w.brick_uni[bn];
+            }
+            if (bu != 0u) {
+                if (COUNT) cnt[VG_IT_BRICK_UNIFORM]++;
+                if (solid_mat(bu, flags)) {
+                    if (!phase1) {
+                        emit(r, w, bu, h);
+                        return;
+                    }
+                } else {
+                    phase1 = false;
+                }
+                S = 4;
+                break;
+            }
+            if (lod && !phase1 && r.t_cur > LOD_BRICK_T) {
+                if (COUNT) cnt[VG_IT_BRICK_LOD]++;
+                uint32_t m = topmost_material(w, bn, ow);
+                if (!is_foliage(m)) {
+                    emit(r, w, m, h);
+                    return;
+                }
+            }
+            if (COUNT) cnt[VG_IT_VOXEL]++;
+            const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
+            if ((ow >> vb) & 1) {
+                uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+                if (solid_mat(m, flags)) {
+                    if (!phase1) {
+                        emit(r, w, m, h);
+                        return;
+                    }
+                    break; // phase 1: step out of the solid voxel
+                }
+            }
+            phase1 = false;
+        } while (false);
+        if (S) walk_skip(r, sl, S);
+        else walk_step(r);
+    }
+}
+
+// glam normalize_or_zero: d * (1 / sqrt(dot)); false when that is not a finite positive
+// reciprocal or the result is zero.
+__device__ __forceinline__ bool normalize_or_zero(const float in[3], float out[3]) {
+    float len = sqrtf(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+    float rcp = 1.0f / len;
+    if (!(isfinite(rcp) && rcp > 0.0f)) return false;
+    out[0] = in[0] * rcp;
+    out[1] = in[1] * rcp;
+    out[2] = in[2] * rcp;
+    return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
+}
+
+// Pinhole direction (shaders/raymarch.wgsl:430-438, 600).
+__device__ __forceinline__ void primary_dir(const CamParams &c, int32_t px, int32_t py,
+                                            float out[3]) {
+    float u = ((float)px + 0.5f + c.jitter[0]) / c.res[0];
+    float v = ((float)py + 0.5f + c.jitter[1]) / c.res[1];
+    float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
+    float aspect = c.res[0] / c.res[1];
+    float a = nx * c.tan_half_fov * aspect, b = ny * c.tan_half_fov;
+    float r[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = c.forward[k] + c.right[k] * a + c.up[k] * b;
+    float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    float rcp = 1.0f / len;
+    out[0] = r[0] * rcp;
+    out[1] = r[1] * rcp;
+    out[2] = r[2] * rcp;
+}
+
+// p_hit + n * 0.001 (shaders/raymarch.wgsl:3955).
+__device__ __forceinline__ void shadow_origin(const float o[3], const float d[3], const HitRec &h,
+                                              float out[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float n = 0.0f;
+        if (h.face < 6u && (int)(h.face >> 1) == k) n = (h.face & 1u) ? -1.0f : 1.0f;
+        float p = o[k] + d[k] * h.t;
+        out[k] = p + n * 0.001f;
+    }
+}
+
+__device__ __forceinline__ void store_hit(vg_hit *dst, const HitRec &h) {
+    uint32_t *p = reinterpret_cast<uint32_t *>(dst);
+    p[0] = (uint32_t)h.v[0];
+    p[1] = (uint32_t)h.v[1];
+    p[2] = (uint32_t)h.v[2];
+    p[3] = __float_as_uint(h.t);
+    p[4] = (h.face & 0xFFu) | ((h.mat & 0xFFu) << 8) | ((h.hit & 0xFFu) << 16);
+}
+
+__device__ __forceinline__ void load_hit(const vg_hit *src, HitRec &h) {
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(src);
+    h.v[0] = (int32_t)p[0];
+    h.v[1] = (int32_t)p[1];
+    h.v[2] = (int32_t)p[2];
+    h.t = __uint_as_float(p[3]);
+    h.face = p[4] & 0xFFu;
+    h.mat = (p[4] >> 8) & 0xFFu;
+    h.hit = (p[4] >> 16) & 0xFFu;
+}
+
+// Wave-reduce the per-lane counters and add them to the global totals.
+__device__ __forceinline__ void flush_counts(const uint32_t *cnt, unsigned long long *counts) {
+#pragma unroll
+    for (int k = 0; k < VG_IT_COUNT; ++k) {
+        unsigned long long v = cnt[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counts[k], v);
+    }
+}
+
+// ------------------------------------------------------------------ kernels --
+constexpr int kBlock = 256;
+
+// Spread consecutive launch-order blocks round-robin over the 8 XCDs back into
+// contiguous ranges per XCD, so neighbouring screen blocks share one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nb) {
+    uint32_t full = nb & ~7u;
+    if (bid >= full) return bid;
+    return (bid & 7u) * (full >> 3) + (bid >> 3);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevWorld w, const vg_ray *__restrict__ rays,
+                                                       int64_t n, uint32_t flags,
+                                                       vg_hit *__restrict__ hits,
+                                                       unsigned long long *counts) {
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t cnt[VG_IT_COUNT];
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    if (i < n) {
+        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+        float4 a = rp[0], b = rp[1];
+        float o[3] = {a.x, a.y, a.z}, din[3] = {b.x, b.y, b.z}, d[3];
+        uint32_t f = flags | __float_as_uint(b.w);
+        HitRec h;
+        hit_miss(h);
+        if (normalize_or_zero(din, d)) {
+            Walk r;
+            if (walk_init(r, w, o, d)) walk_run<COUNT>(r, w, a.w, f, h, cnt);
+        }
+        store_hit(hits + i, h);
+    }
+    if (COUNT) flush_counts(cnt, counts);
+}
+
+// One thread per pixel, a wave per 8x8 pixel block, 16x16 pixels per workgroup.
+template <bool COUNT, int MODE> // MODE 0 primary, 1 primary+shadow, 2 shadow from hits
+__global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams cam, int32_t width,
+                                                        int32_t height, uint32_t nbx,
+                                                        float ptmax, uint32_t pflags, float sx,
+                                                        float sy, float sz, float stmax,
+                                                        vg_hit *__restrict__ hits,
+                                                        uint8_t *__restrict__ occluded,
+                                                        unsigned long long *counts) {
+    const uint32_t nb = gridDim.x;
+    const uint32_t b = xcd_block(blockIdx.x, nb);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t px = (int32_t)((b % nbx) * 16 + (wave & 1) * 8 + (lane & 7));
+    const int32_t py = (int32_t)((b / nbx) * 16 + (wave >> 1) * 8 + (lane >> 3));
+    uint32_t cnt[VG_IT_COUNT];
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    if (px < width && py < height) {
+        const int64_t i = (int64_t)py * width + px;
+        float d[3];
+        primary_dir(cam, px, py, d);
+        HitRec h;
+        if (MODE == 2) {
+            load_hit(hits + i, h);
+        } else {
+            hit_miss(h);
+            Walk r;
+            if (walk_init(r, w, cam.origin, d)) walk_run<COUNT>(r, w, ptmax, pflags, h, cnt);
+            store_hit(hits + i, h);
+        }
+        if (MODE != 0) {
+            uint32_t occ = 0;
+            if (h.hit) {
+                float so[3], sd[3] = {sx, sy, sz};
+                shadow_origin(cam.origin, d, h, so);
+                HitRec hs;
+                hit_miss(hs);
+                Walk r2;
+                if (walk_init(r2, w, so, sd)) walk_run<COUNT>(r2, w, stmax, VG_RAY_ANY_HIT, hs, cnt);
+                occ = hs.hit;
+            }
+            occluded[i] = (uint8_t)occ;
+        }
+    }
+    if (COUNT) flush_counts(cnt, counts);
+}
+
+// Upload relayout: reference-order bricks (72 B AoS) -> nested occupancy + materials.
+__global__ void k_relayout_bricks(const vg_brick *__restrict__ staged,
+                                  const uint8_t *__restrict__ bu_ref, int64_t first, int64_t count,
+                                  int32_t bx_n, int32_t by_n, int32_t cx_n, int32_t cy_n,
+                                  uint64_t *__restrict__ occ, uint8_t *__restrict__ mat,
+                                  uint8_t *__restrict__ bu) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t bi = first + i;
+    int32_t bx = (int32_t)(bi % bx_n);
+    int32_t by = (int32_t)((bi / bx_n) % by_n);
+    int32_t bz = (int32_t)(bi / ((int64_t)bx_n * by_n));
+    int64_t ci = (bx >> 4) + (int64_t)(by >> 4) * cx_n + (int64_t)(bz >> 4) * cx_n * cy_n;
+    int64_t tn = ci * 64 + cbit((bx >> 2) & 3, (by >> 2) & 3, (bz >> 2) & 3);
+    int64_t bn = tn * 64 + cbit(bx & 3, by & 3, bz & 3);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(staged + i);
+    occ[bn] = src[0];
+    uint64_t *dst = reinterpret_cast<uint64_t *>(mat + bn * 64);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[k] = src[1 + k];
+    bu[bn] = bu_ref[i]; // the staging buffer holds this slab only
+}
+
+__global__ void k_relayout_tiles(const uint64_t *__restrict__ tile_ref,
+                                 const uint8_t *__restrict__ tu_ref, int64_t n, int32_t tx_n,
+                                 int32_t ty_n, int32_t cx_n, int32_t cy_n,
+                                 uint64_t *__restrict__ tile, uint8_t *__restrict__ tu) {
+    int64_t ti = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= n) return;
+    int32_t tx = (int32_t)(ti % tx_n);
+    int32_t ty = (int32_t)((ti / tx_n) % ty_n);
+    int32_t tz = (int32_t)(ti / ((int64_t)tx_n * ty_n));
+    int64_t ci = (tx >> 2) + (int64_t)(ty >> 2) * cx_n + (int64_t)(tz >> 2) * cx_n * cy_n;
+    int64_t tn = ci * 64 + cbit(tx & 3, ty & 3, tz & 3);
+    tile[tn] = tile_ref[ti];
+    tu[tn] = tu_ref[ti];
+}
+
+} // namespace vg
+
+using namespace vg;
+
+struct vg_ctx {
+    int device;
+};
+
+struct vg_world {
+    vg_ctx *ctx;
+    Dims d;
+    DevWorld dw;
+    bool uploaded;
+    uint64_t *l4, *chunk, *tile, *occ;
+    uint8_t *tile_uni, *brick_uni, *mat;
+    void *scratch;                   // one ray + one hit for vg_raycast
+    unsigned long long *counts_dev;  // VG_IT_COUNT totals for the *_counts entry points
+};
+
+namespace vg {
+
+static int set_device(const vg_world *w) {
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    return VG_OK;
+}
+
+static int check_world(const vg_world *w) {
+    if (!w) return fail(VG_E_INVALID, "null world");
+    if (!w->uploaded) return fail(VG_E_STATE, "world has not been uploaded");
+    return set_device(w);
+}
+
+static CamParams cam_params(const vg_camera *c) {
+    CamParams p;
+    for (int k = 0; k < 3; ++k) {
+        p.origin[k] = c->origin[k];
+        p.forward[k] = c->forward[k];
+        p.right[k] = c->right[k];
+        p.up[k] = c->up[k];
+    }
+    p.tan_half_fov = c->tan_half_fov;
+    p.res[0] = c->resolution[0];
+    p.res[1] = c->resolution[1];
+    p.jitter[0] = c->jitter[0];
+    p.jitter[1] = c->jitter[1];
+    return p;
+}
+
+// Host-side normalize_or_zero for the sun direction (same expression as the device).
+static bool host_normalize(const float in[3], float out[3]) {
+    float len = sqrtf(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+    float rcp = 1.0f / len;
+    if (!(std::isfinite(rcp) && rcp > 0.0f)) return false;
+    for (int k = 0; k < 3; ++k) out[k] = in[k] * rcp;
+    return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
+}
+
+static int launch_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                        float ptmax, uint32_t pflags, const float sun[3], float stmax,
+                        vg_hit *hits, uint8_t *occluded, int mode, bool count,
+                        hipStream_t stream) {
+    int st = check_world(w);
+    if (st) return st;
+    if (!camera || width <= 0 || height <= 0 || !hits) return fail(VG_E_INVALID, "bad frame args");
+    if (camera->resolution[0] != (float)width || camera->resolution[1] != (float)height)
+        return fail(VG_E_INVALID, "camera resolution %gx%g does not match %dx%d",
+                    camera->resolution[0], camera->resolution[1], width, height);
+    float s[3] = {0.0f, 0.0f, 0.0f};
+    if (mode != 0) {
+        if (!sun || !occluded) return fail(VG_E_INVALID, "null sun/occluded");
+        if (!host_normalize(sun, s)) s[0] = s[1] = s[2] = 0.0f;
+    }
+    const uint32_t nbx = (uint32_t)((width + 15) / 16), nby = (uint32_t)((height + 15) / 16);
+    const uint32_t nb = nbx * nby;
+    CamParams cp = cam_params(camera);
+    dim3 grid(nb), block(kBlock);
+    unsigned long long *cd = count ? w->counts_dev : nullptr;
+#define LAUNCH_F(C, M)                                                                        \
+    hipLaunchKernelGGL((k_trace_frame<C, M>), grid, block, 0, stream, w->dw, cp, width, height, \
+                       nbx, ptmax, pflags, s[0], s[1], s[2], stmax, hits, occluded, cd)
+    if (count) {
+        if (mode == 0) LAUNCH_F(true, 0);
+        else if (mode == 1) LAUNCH_F(true, 1);
+        else LAUNCH_F(true, 2);
+    } else {
+        if (mode == 0) LAUNCH_F(false, 0);
+        else if (mode == 1) LAUNCH_F(false, 1);
+        else LAUNCH_F(false, 2);
+    }
+#undef LAUNCH_F
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
+}
+
+static int launch_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
+                       bool count, hipStream_t stream) {
+    int st = check_world(w);
+    if (st) return st;
+    if (n == 0) return VG_OK;
+    if (!rays || !hits) return fail(VG_E_INVALID, "null rays/hits");
+    if (((uintptr_t)rays & 15u) != 0) return fail(VG_E_INVALID, "rays must be 16-byte aligned");
+    if (((uintptr_t)hits & 3u) != 0) return fail(VG_E_INVALID, "hits must be 4-byte aligned");
+    int64_t nb = ((int64_t)n + kBlock - 1) / kBlock;
+    if (nb > 0x7FFFFFFF) return fail(VG_E_INVALID, "too many rays");
+    if (count)
+        hipLaunchKernelGGL((k_trace_rays<true>), dim3((uint32_t)nb), dim3(kBlock), 0, stream,
+                           w->dw, rays, (int64_t)n, flags, hits, w->counts_dev);
+    else
+        hipLaunchKernelGGL((k_trace_rays<false>), dim3((uint32_t)nb), dim3(kBlock), 0, stream,
+                           w->dw, rays, (int64_t)n, flags, hits, nullptr);
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
+}
+
+static void free_world(vg_world *w) {
+    void *ptrs[] = {w->l4, w->chunk, w->tile, w->occ, w->tile_uni, w->brick_uni, w->mat, w->scratch,
+                    w->counts_dev};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+} // namespace vg
+
+extern "C" {
+
+int vg_ctx_create(int device, vg_ctx **out) {
+    if (!out) return fail(VG_E_INVALID, "null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(VG_E_DEVICE, "no HIP device (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= n) return fail(VG_E_INVALID, "device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    vg_ctx *c = new (std::nothrow) vg_ctx;
+    if (!c) return fail(VG_E_NOMEM, "ctx");
+    c->device = device;
+    *out = c;
+    return VG_OK;
+}
+
+int vg_ctx_destroy(vg_ctx *ctx) {
+    delete ctx;
+    return VG_OK;
+}
+
+int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
+    if (!ctx || !out) return fail(VG_E_INVALID, "null ctx/out");
+    Dims d;
+    int st = make_dims(dims, &d);
+    if (st) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    vg_world *w = new (std::nothrow) vg_world();
+    if (!w) return fail(VG_E_NOMEM, "world");
+    std::memset(w, 0, sizeof(*w));
+    w->ctx = ctx;
+    w->d = d;
+    struct {
+        void **p;
+        size_t bytes;
+    } allocs[] = {
+        {(void **)&w->l4, (size_t)d.n_l4 * 8},
+        {(void **)&w->chunk, (size_t)d.n_chunks * 8},
+        {(void **)&w->tile, (size_t)d.n_tiles * 8},
+        {(void **)&w->tile_uni, (size_t)d.n_tiles},
+        {(void **)&w->occ, (size_t)d.n_bricks * 8},
+        {(void **)&w->brick_uni, (size_t)d.n_bricks},
+        {(void **)&w->mat, (size_t)d.n_bricks * 64},
+        {(void **)&w->scratch, 256},
+        {(void **)&w->counts_dev, VG_IT_COUNT * sizeof(unsigned long long)},
+    };
+    for (auto &a : allocs) {
+        if (hipMalloc(a.p, a.bytes) != hipSuccess) {
+            free_world(w);
+            delete w;
+            return fail(VG_E_NOMEM, "hipMalloc(%zu) failed", a.bytes);
+        }
+    }
+    DevWorld &dw = w->dw;
+    dw.l4 = w->l4;
+    dw.chunk = w->chunk;
+    dw.tile = w->tile;
+    dw.occ = w->occ;
+    dw.tile_uni = w->tile_uni;
+    dw.brick_uni = w->brick_uni;
+    dw.mat = w->mat;
+    for (int k = 0; k < 3; ++k) {
+        dw.dims[k] = d.v[k];
+        dw.cdim[k] = d.c[k];
+        dw.l4dim[k] = d.l4[k];
+        dw.origin[k] = 0;
+        dw.ro_mod[k] = 0;
+    }
+    *out = w;
+    return VG_OK;
+}
+
+int vg_world_destroy(vg_world *w) {
+    if (!w) return VG_OK;
+    (void)hipSetDevice(w->ctx->device);
+    free_world(w);
+    delete w;
+    return VG_OK;
+}
+
+int vg_world_upload(vg_world *w, const vg_brick *bricks, const uint64_t *tile_mask,
+                    const uint64_t *chunk_mask, const uint64_t *l4_mask,
+                    const uint8_t *brick_uniform, const uint8_t *tile_uniform,
+                    const int32_t origin[3]) {
+    if (!w) return fail(VG_E_INVALID, "null world");
+    if (!bricks || !tile_mask || !chunk_mask || !l4_mask || !brick_uniform || !tile_uniform ||
+        !origin)
+        return fail(VG_E_INVALID, "null upload array");
+    const Dims &d = w->d;
+    for (int k = 0; k < 3; ++k)
+        if (pos_mod(origin[k], d.v[k]) != 0 && d.v[k] % 256 != 0)
+            return fail(VG_E_INVALID,
+                        "origin[%d]=%d folds the window but dims[%d]=%d is not a multiple of 256", k,
+                        origin[k], k, d.v[k]);
+    int st = set_device(w);
+    if (st) return st;
+    w->uploaded = false;
+    HIP_TRY(hipMemcpy(w->l4, l4_mask, (size_t)d.n_l4 * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->chunk, chunk_mask, (size_t)d.n_chunks * 8, hipMemcpyHostToDevice));
+    // tiles
+    {
+        void *tmp = nullptr;
+        size_t tbytes = (size_t)d.n_tiles * 9;
+        HIP_TRY(hipMalloc(&tmp, tbytes));
+        uint64_t *tref = (uint64_t *)tmp;
+        uint8_t *turef = (uint8_t *)tmp + (size_t)d.n_tiles * 8;
+        hipError_t e = hipMemcpy(tref, tile_mask, (size_t)d.n_tiles * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(turef, tile_uniform, (size_t)d.n_tiles, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            int64_t nb = (d.n_tiles + 255) / 256;
+            hipLaunchKernelGGL(k_relayout_tiles, dim3((uint32_t)nb), dim3(256), 0, 0, tref, turef,
+                               d.n_tiles, d.t[0], d.t[1], d.c[0], d.c[1], w->tile, w->tile_uni);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+        }
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(VG_E_DEVICE, "tile upload: %s", hipGetErrorString(e));
+    }
+    // bricks, staged in slabs of at most 4 Mi bricks (288 MiB + 4 MiB)
+    {
+        const int64_t slab = std::min<int64_t>(d.n_bricks, (int64_t)1 << 22);
+        void *tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, (size_t)slab * (sizeof(vg_brick) + 1)));
+        vg_brick *sb = (vg_brick *)tmp;
+        uint8_t *sbu = (uint8_t *)tmp + (size_t)slab * sizeof(vg_brick);
+        hipError_t e = hipSuccess;
+        for (int64_t first = 0; first < d.n_bricks && e == hipSuccess; first += slab) {
+            int64_t cnt = std::min(slab, d.n_bricks - first);
+            e = hipMemcpy(sb, bricks + first, (size_t)cnt * sizeof(vg_brick), hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(sbu, brick_uniform + first, (size_t)cnt, hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                int64_t nb = (cnt + 255) / 256;
+                hipLaunchKernelGGL(k_relayout_bricks, dim3((uint32_t)nb), dim3(256), 0, 0, sb, sbu,
+                                   first, cnt, d.b[0], d.b[1], d.c[0], d.c[1], w->occ, w->mat,
+                                   w->brick_uni);
+                e = hipGetLastError();
+                if (e == hipSuccess) e = hipDeviceSynchronize();
+            }
+        }
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(VG_E_DEVICE, "brick upload: %s", hipGetErrorString(e));
+    }
+    for (int k = 0; k < 3; ++k) {
+        w->dw.origin[k] = origin[k];
+        w->dw.ro_mod[k] = pos_mod(origin[k], d.v[k]);
+    }
+    w->uploaded = true;
+    return VG_OK;
+}
+
+int vg_trace_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
+                  void *stream) {
+    return launch_rays(w, rays, n, flags, hits, false, (hipStream_t)stream);
+}
+
+int vg_trace_rays_counts(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
+                         uint64_t counts_host[VG_IT_COUNT]) {
+    if (!counts_host) return fail(VG_E_INVALID, "null counts");
+    int st = check_world(w);
+    if (st) return st;
+    HIP_TRY(hipMemset(w->counts_dev, 0, VG_IT_COUNT * sizeof(unsigned long long)));
+    st = launch_rays(w, rays, n, flags, hits, true, 0);
+    if (st) return st;
+    HIP_TRY(hipMemcpy(counts_host, w->counts_dev, VG_IT_COUNT * 8, hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_trace_primary(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                     float tmax, uint32_t flags, vg_hit *hits, void *stream) {
+    return launch_frame(w, camera, width, height, tmax, flags, nullptr, 0.0f, hits, nullptr, 0,
+                        false, (hipStream_t)stream);
+}
+
+int vg_trace_shadow(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                    const vg_hit *primary_hits, const float sun[3], float tmax, uint8_t *occluded,
+                    void *stream) {
+    return launch_frame(w, camera, width, height, 0.0f, 0u, sun, tmax,
+                        const_cast<vg_hit *>(primary_hits), occluded, 2, false,
+                        (hipStream_t)stream);
+}
+
+int vg_trace_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                   float primary_tmax, uint32_t primary_flags, const float sun[3],
+                   float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream) {
+    return launch_frame(w, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
+                        hits, occluded, 1, false, (hipStream_t)stream);
+}
+
+int vg_trace_frame_counts(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                          float primary_tmax, uint32_t primary_flags, const float sun[3],
+                          float shadow_tmax, vg_hit *hits, uint8_t *occluded,
+                          uint64_t counts_host[VG_IT_COUNT]) {
+    if (!counts_host) return fail(VG_E_INVALID, "null counts");
+    int st = check_world(w);
+    if (st) return st;
+    HIP_TRY(hipMemset(w->counts_dev, 0, VG_IT_COUNT * sizeof(unsigned long long)));
+    st = launch_frame(w, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax, hits,
+                      occluded, 1, true, 0);
+    if (st) return st;
+    HIP_TRY(hipMemcpy(counts_host, w->counts_dev, VG_IT_COUNT * 8, hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_raycast(vg_world *w, const float cam[3], const float dir[3], vg_hit *hit_host) {
+    if (!cam || !dir || !hit_host) return fail(VG_E_INVALID, "null raycast argument");
+    int st = check_world(w);
+    if (st) return st;
+    vg_ray r;
+    std::memset(&r, 0, sizeof(r));
+    for (int k = 0; k < 3; ++k) {
+        r.o[k] = cam[k];
+        r.d[k] = dir[k];
+    }
+    r.tmax = INFINITY; // the reference pick is bounded only by the window
+    vg_ray *dr = (vg_ray *)w->scratch;
+    vg_hit *dh = (vg_hit *)((char *)w->scratch + 64);
+    HIP_TRY(hipMemcpy(dr, &r, sizeof(r), hipMemcpyHostToDevice));
+    st = launch_rays(w, dr, 1, 0u, dh, false, 0);
+    if (st) return st;
+    HIP_TRY(hipMemcpy(hit_host, dh, sizeof(vg_hit), hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_device_count(int *out) {
+    if (!out) return fail(VG_E_INVALID, "null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *out = n;
+    return VG_OK;
+}
+
+#define CTX_DEVICE(ctx)                                                                       \
+    do {                                                                                      \
+        if (!(ctx)) return fail(VG_E_INVALID, "null ctx");                                    \
+        HIP_TRY(hipSetDevice((ctx)->device));                                                 \
+    } while (0)
+
+int vg_alloc(vg_ctx *ctx, size_t bytes, void **dev_out) {
+    CTX_DEVICE(ctx);
+    if (!dev_out) return fail(VG_E_INVALID, "null out");
+    *dev_out = nullptr;
+    if (bytes == 0) return VG_OK;
+    if (hipMalloc(dev_out, bytes) != hipSuccess) return fail(VG_E_NOMEM, "hipMalloc(%zu)", bytes);
+    return VG_OK;
+}
+
+int vg_free(vg_ctx *ctx, void *dev) {
+    CTX_DEVICE(ctx);
+    if (dev) HIP_TRY(hipFree(dev));
+    return VG_OK;
+}
+
+int vg_copy_to_device(vg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    CTX_DEVICE(ctx);
+    if (bytes == 0) return VG_OK;
+    if (!dst || !src) return fail(VG_E_INVALID, "null copy pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return VG_OK;
+}
+
+int vg_copy_to_host(vg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    CTX_DEVICE(ctx);
+    if (bytes == 0) return VG_OK;
+    if (!dst || !src) return fail(VG_E_INVALID, "null copy pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_memset(vg_ctx *ctx, void *dev, int value, size_t bytes) {
+    CTX_DEVICE(ctx);
+    if (bytes == 0) return VG_OK;
+    if (!dev) return fail(VG_E_INVALID, "null pointer");
+    HIP_TRY(hipMemset(dev, value, bytes));
+    return VG_OK;
+}
+
+int vg_synchronize(vg_ctx *ctx) {
+    CTX_DEVICE(ctx);
+    HIP_TRY(hipDeviceSynchronize());
+    return VG_OK;
+}
+
+int vg_stream_create(vg_ctx *ctx, void **out) {
+    CTX_DEVICE(ctx);
+    if (!out) return fail(VG_E_INVALID, "null out");
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = (void *)s;
+    return VG_OK;
+}
+
+int vg_stream_destroy(vg_ctx *ctx, void *stream) {
+    CTX_DEVICE(ctx);
+    if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return VG_OK;
+}
+
+int vg_stream_synchronize(vg_ctx *ctx, void *stream) {
+    CTX_DEVICE(ctx);
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return VG_OK;
+}
+
+int vg_event_create(vg_ctx *ctx, void **out) {
+    CTX_DEVICE(ctx);
+    if (!out) return fail(VG_E_INVALID, "null out");
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    *out = (void *)e;
+    return VG_OK;
+}
+
+int vg_event_destroy(vg_ctx *ctx, void *event) {
+    CTX_DEVICE(ctx);
+    if (event) HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return VG_OK;
+}
+
+int vg_event_record(vg_ctx *ctx, void *event, void *stream) {
+    CTX_DEVICE(ctx);
+    if (!event) return fail(VG_E_INVALID, "null event");
+    HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return VG_OK;
+}
+
+int vg_event_elapsed_ms(vg_ctx *ctx, void *start, void *end, float *ms) {
+    CTX_DEVICE(ctx);
+    if (!start || !end || !ms) return fail(VG_E_INVALID, "null event/out");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)end));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return VG_OK;
+}
+
+} // extern "C"
