@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (a plain copy kernel "
+                         "of known bytes, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs the ray-query path on a single "
@@ -129,6 +132,16 @@ def main():
     achieved = bytes_per_frame / avg_kern_s / 1e9
     log(f"wall {ms_per_step:.3f} ms/frame, kernel {np.mean(kern_ms):.3f} ms "
         f"(min {np.min(kern_ms):.3f}), {value:.0f} Mrays/s")
+
+    if args.copy_calibration_mib:
+        import ctypes
+        from voxelg_amd._lib import check, lib
+        nbytes = args.copy_calibration_mib << 20
+        a, b = ctx.alloc(nbytes).zero(), ctx.alloc(nbytes)
+        check(lib().vg_copy_device(ctx._h, b.data_ptr(), a.data_ptr(), ctypes.c_size_t(nbytes)),
+              "vg_copy_device")
+        ctx.synchronize()
+        log(f"copy calibration: {nbytes} bytes device-to-device")
 
     base = None
     if not args.no_cpu_baseline:

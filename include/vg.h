@@ -137,6 +137,7 @@ int vg_free(vg_ctx *ctx, void *dev);
 int vg_copy_to_device(vg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int vg_copy_to_host(vg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
 int vg_memset(vg_ctx *ctx, void *dev, int value, size_t bytes);
+int vg_copy_device(vg_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 int vg_synchronize(vg_ctx *ctx);
 int vg_stream_create(vg_ctx *ctx, void **stream_out); /* default priority */
 int vg_stream_destroy(vg_ctx *ctx, void *stream);

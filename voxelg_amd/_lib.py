@@ -91,6 +91,7 @@ SIGNATURES = {
     "vg_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "vg_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "vg_memset": (C.c_int, [_P, _P, C.c_int, C.c_size_t]),
+    "vg_copy_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "vg_synchronize": (C.c_int, [_P]),
     "vg_stream_create": (C.c_int, [_P, C.POINTER(_P)]),
     "vg_stream_destroy": (C.c_int, [_P, _P]),

@@ -993,6 +993,14 @@ int vg_memset(vg_ctx *ctx, void *dev, int value, size_t bytes) {
     return VG_OK;
 }
 
+int vg_copy_device(vg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    CTX_DEVICE(ctx);
+    if (bytes == 0) return VG_OK;
+    if (!dst || !src) return fail(VG_E_INVALID, "null copy pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return VG_OK;
+}
+
 int vg_synchronize(vg_ctx *ctx) {
     CTX_DEVICE(ctx);
     HIP_TRY(hipDeviceSynchronize());
