@@ -80,7 +80,7 @@ def lib():
             "orc_primary_dir": (None, [P, C.c_int32, C.c_int32, F32P]),
             "orc_primary_rays": (None, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, P]),
             "orc_frame": (None, [W, P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P,
-                                 C.c_float, C.c_int32, C.c_int32, P, P, C.c_int, U64P]),
+                                 C.c_float, C.c_int32, C.c_int32, P, P, C.c_int, U64P, P]),
             "orc_frame_r0": (None, [W, P, C.c_int32, C.c_int32, C.c_float, F32P, C.c_float,
                                     C.c_int32, C.c_int32, P, P, C.c_int]),
             "orc_shadow_origin": (None, [F32P, F32P, P, F32P]),
@@ -223,18 +223,22 @@ class OracleWorld:
         return out
 
     def frame(self, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
-              rows=None, threads=0):
-        """R1 frame: hits (+ shadow bytes) for rows [r0, r1) of the image."""
+              rows=None, threads=0, iters=False):
+        """R1 frame: hits (+ shadow bytes) for rows [r0, r1) of the image; with iters=True
+        also the per-pixel (primary, shadow) iteration counts."""
         r0, r1 = rows if rows is not None else (0, height)
         n = (r1 - r0) * width
         hits = np.zeros(n, HIT_DTYPE)
         shadow = np.zeros(n, np.uint8)
+        it = np.zeros((n, 2), np.uint32) if iters else None
         cnt = (C.c_uint64 * IT_COUNT)()
         cam = np.ascontiguousarray(camera, np.float32)
         lib().orc_frame(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
                         primary_flags, _f3(sun if sun is not None else (0, 0, 0)),
-                        shadow_tmax, r0, r1, _p(hits), _p(shadow), threads or os.cpu_count(), cnt)
-        return hits, shadow, dict(zip(IT_NAMES, (int(c) for c in cnt)))
+                        shadow_tmax, r0, r1, _p(hits), _p(shadow), threads or os.cpu_count(), cnt,
+                        _p(it) if iters else None)
+        out = (hits, shadow, dict(zip(IT_NAMES, (int(c) for c in cnt))))
+        return out + (it,) if iters else out
 
     def frame_r0(self, camera, width, height, primary_tmax, sun, shadow_tmax, rows=None,
                  threads=0):

@@ -915,6 +915,7 @@ typedef struct job_t {
     int32_t width, height, row0, row1;
     float ptmax, stmax, sun[3];
     uint8_t *shadow;
+    uint32_t *iters; /* optional: per pixel (primary, shadow) iteration counts */
 } job_t;
 
 static void frame_rows(job_t *j);
@@ -1031,19 +1032,30 @@ static void frame_rows(job_t *j) {
             float dir[3];
             orc_primary_dir(cam, x, y, dir);
             orc_hit *h = &j->out[i];
-            if (j->kind == 2) trace_core(j->w, cam, dir, j->ptmax, j->flags, h, j->counts);
+            uint64_t c[ORC_IT_COUNT] = {0};
+            if (j->kind == 2) trace_core(j->w, cam, dir, j->ptmax, j->flags, h, c);
             else orc_raycast_r0(j->w, cam, dir, j->ptmax, h);
+            uint32_t it_primary = 0;
+            for (int k = 0; k < ORC_IT_COUNT; ++k) it_primary += (uint32_t)c[k];
             uint8_t occ = 0;
             if (h->hit) {
                 float so[3];
                 orc_shadow_origin(cam, dir, h, so);
                 orc_hit sh;
-                if (j->kind == 2)
-                    trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT, &sh, j->counts);
+                if (j->kind == 2) trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT, &sh, c);
                 else orc_raycast_r0(j->w, so, j->sun, j->stmax, &sh);
                 occ = sh.hit;
             }
             j->shadow[i] = occ;
+            uint32_t it_all = 0;
+            for (int k = 0; k < ORC_IT_COUNT; ++k) {
+                it_all += (uint32_t)c[k];
+                j->counts[k] += c[k];
+            }
+            if (j->iters) {
+                j->iters[2 * i] = it_primary;
+                j->iters[2 * i + 1] = it_all - it_primary;
+            }
         }
     }
 }
@@ -1051,7 +1063,7 @@ static void frame_rows(job_t *j) {
 void orc_frame(const orc_world *w, const float *cam, int32_t width, int32_t height,
                float primary_tmax, uint32_t primary_flags, const float sun[3], float shadow_tmax,
                int32_t row0, int32_t row1, orc_hit *hits, uint8_t *shadow, int threads,
-               uint64_t counts[ORC_IT_COUNT]) {
+               uint64_t counts[ORC_IT_COUNT], uint32_t *iters) {
     job_t p;
     memset(&p, 0, sizeof(p));
     p.kind = 2;
@@ -1067,6 +1079,7 @@ void orc_frame(const orc_world *w, const float *cam, int32_t width, int32_t heig
     if (!normalize_or_zero(sun, p.sun)) p.sun[0] = p.sun[1] = p.sun[2] = 0.0f;
     p.out = hits;
     p.shadow = shadow;
+    p.iters = iters;
     run_jobs(&p, threads, counts);
 }
 

@@ -142,7 +142,7 @@ void orc_primary_rays(const float *camera176, int32_t w, int32_t h, float tmax, 
 void orc_frame(const orc_world *w, const float *camera176, int32_t width, int32_t height,
                float primary_tmax, uint32_t primary_flags, const float sun[3], float shadow_tmax,
                int32_t row0, int32_t row1, orc_hit *hits, uint8_t *shadow, int threads,
-               uint64_t counts[ORC_IT_COUNT]);
+               uint64_t counts[ORC_IT_COUNT], uint32_t *iters /* may be NULL */);
 /* Same frame with R0 as the traversal (the CPU baseline). */
 void orc_frame_r0(const orc_world *w, const float *camera176, int32_t width, int32_t height,
                   float primary_tmax, const float sun[3], float shadow_tmax, int32_t row0,

@@ -46,6 +46,7 @@ struct DevWorld {
     int32_t origin[3];
     int32_t cdim[3];
     int32_t l4dim[3];
+    int32_t fold; // 1 when the window origin folds any axis (toroidal slots)
 };
 
 struct CamParams {
@@ -83,9 +84,10 @@ __device__ __forceinline__ int cbit(int32_t x, int32_t y, int32_t z) { return x 
 // --------------------------------------------------------------- walk state --
 struct Walk {
     float org[3], dir[3], inv[3];
+    float tn[3];     // t of the next plane crossing on each axis (+inf for an axis that never steps)
     int32_t step[3];
-    int32_t l[3];
-    float t_cur;
+    int32_t l[3];    // current voxel, window-local
+    float t_cur;     // entry t of the current voxel
     int32_t last_axis;
     int32_t entry_face;
     int32_t face_pack; // 4 bits per axis: face code of an entry through that axis
@@ -103,6 +105,17 @@ __device__ __forceinline__ void hit_miss(HitRec &h) {
     h.face = VG_FACE_NONE;
     h.mat = 0;
     h.hit = 0;
+}
+
+// t of the plane crossed to enter voxel e along axis k (plane e when stepping up, e + 1
+// when stepping down), always from the integer coordinate.
+__device__ __forceinline__ float t_plane(const Walk &r, int k, int32_t e) {
+    int32_t p = r.step[k] > 0 ? e : e + 1;
+    return ((float)p - r.org[k]) * r.inv[k];
+}
+
+__device__ __forceinline__ float t_next(const Walk &r, int k) {
+    return r.step[k] != 0 ? t_plane(r, k, r.l[k] + r.step[k]) : INFINITY;
 }
 
 // Ray setup: window-local origin, slab entry, 1e-3 bias clamped into the window
@@ -131,6 +144,8 @@ __device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const floa
         p = fsel_min(fsel_max(p, 1e-3f), (float)w.dims[k] - 1e-3f);
         r.l[k] = (int32_t)floorf(p);
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.tn[k] = t_next(r, k);
     r.t_cur = t_enter;
     r.last_axis = -1;
     r.face_pack = 0;
@@ -144,59 +159,42 @@ __device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const floa
     return true;
 }
 
-// t of the plane crossed to enter voxel e along axis k.
-__device__ __forceinline__ float t_plane(const Walk &r, int k, int32_t e) {
-    int32_t p = r.step[k] > 0 ? e : e + 1;
-    return ((float)p - r.org[k]) * r.inv[k];
-}
-
-// One voxel advance: nearest plane, ties to the lowest axis.
-__device__ __forceinline__ void walk_step(Walk &r) {
-    int ax = -1;
-    float m = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (r.step[k] != 0) {
-            float t = t_plane(r, k, r.l[k] + r.step[k]);
-            if (ax < 0 || t < m) {
-                ax = k;
-                m = t;
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (k == ax) r.l[k] += r.step[k];
-    r.t_cur = m;
-    r.last_axis = ax;
-}
-
-// Leave the aligned empty cell of size S containing the current voxel at exactly the
-// state a one-voxel walk reaches at its first cell-face crossing (T*, ax*): the crossed
-// axis lands just outside the face, every other axis on the last in-cell plane whose
-// (t, axis) orders before (T*, ax*), found from a floor estimate and corrected by direct t
-// comparisons (t along an axis is monotone in the plane coordinate).
-__device__ __forceinline__ void walk_skip(Walk &r, const int32_t sl[3], int32_t S) {
+// Advance out of the aligned cell of size S (1 = just the current voxel) that holds the
+// current voxel, to exactly the state a one-voxel-at-a-time walk reaches at its first
+// crossing of the cell's faces (T*, ax*): crossings are processed in (t, axis) order, ties
+// to the lowest axis (src/raycast.rs:92-103), every t from its integer plane. The crossed
+// axis lands just outside the face; another axis moves only if its cached next crossing
+// orders before (T*, ax*) - then it lands on the last in-cell plane that does, found from a
+// floor estimate corrected by direct t comparisons (t along an axis is monotone in the
+// plane coordinate), and the first rejected plane's t becomes its next crossing.
+// For S = 1 this is the plain voxel step; for S > 1 it replaces skip_to_cell
+// (shaders/raymarch.wgsl:4807-4864) with an integer-exact exit.
+__device__ __forceinline__ void walk_advance(Walk &r, const int32_t sl[3], int32_t S) {
     int32_t lo[3];
-    int ax = -1;
-    float ts = 0.0f;
+    float tf[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         lo[k] = r.l[k] - (sl[k] & (S - 1));
-        if (r.step[k] != 0) {
-            int32_t plane = r.step[k] > 0 ? lo[k] + S : lo[k];
-            float t = ((float)plane - r.org[k]) * r.inv[k];
-            if (ax < 0 || t < ts) {
-                ts = t;
-                ax = k;
-            }
-        }
+        int32_t plane = r.step[k] > 0 ? lo[k] + S : lo[k];
+        tf[k] = r.step[k] != 0 ? ((float)plane - r.org[k]) * r.inv[k] : INFINITY;
+    }
+    int ax = 0;
+    float ts = tf[0];
+    if (tf[1] < ts) {
+        ts = tf[1];
+        ax = 1;
+    }
+    if (tf[2] < ts) {
+        ts = tf[2];
+        ax = 2;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (k == ax) {
             r.l[k] = r.step[k] > 0 ? lo[k] + S : lo[k] - 1;
-        } else if (r.step[k] != 0) {
+            r.tn[k] = t_plane(r, k, r.l[k] + r.step[k]);
+        } else if (r.tn[k] < ts || (r.tn[k] == ts && k < ax)) {
+            // this axis crosses at least one in-cell plane before the exit (so S > 1)
             const int32_t st = r.step[k];
             const int32_t first = r.l[k];
             const int32_t last = st > 0 ? lo[k] + S - 1 : lo[k];
@@ -208,18 +206,23 @@ __device__ __forceinline__ void walk_skip(Walk &r, const int32_t sl[3], int32_t 
                 e = e > first ? first : e;
                 e = e < last ? last : e;
             }
-            // bounded by the cell width; normally 0-2 iterations
+            // bounded by the cell width; normally one test each way
             for (int g = 0; g < 258 && e != first; ++g) {
                 float t = t_plane(r, k, e);
                 if (t < ts || (t == ts && k < ax)) break;
                 e -= st;
             }
+            float nxt = tf[k]; // the cell face, if e ends on the last in-cell voxel
             for (int g = 0; g < 258 && e != last; ++g) {
                 float t = t_plane(r, k, e + st);
-                if (!(t < ts || (t == ts && k < ax))) break;
+                if (!(t < ts || (t == ts && k < ax))) {
+                    nxt = t;
+                    break;
+                }
                 e += st;
             }
             r.l[k] = e;
+            r.tn[k] = nxt;
         }
     }
     r.t_cur = ts;
@@ -272,13 +275,15 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
         if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
             (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
             return;
-        int32_t sl[3];
+        int32_t sl[3] = {r.l[0], r.l[1], r.l[2]};
+        if (w.fold) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            int32_t s = r.l[k] + w.ro_mod[k];
-            sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
+            for (int k = 0; k < 3; ++k) {
+                int32_t s = r.l[k] + w.ro_mod[k];
+                sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
+            }
         }
-        int32_t S = 0; // size of the empty (or phase-1 solid) cell to leave; 0 = one voxel
+        int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
         do {
             const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
                                  (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
@@ -389,8 +394,7 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
             }
             phase1 = false;
         } while (false);
-        if (S) walk_skip(r, sl, S);
-        else walk_step(r);
+        walk_advance(r, sl, S);
     }
 }
 
@@ -782,6 +786,7 @@ int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
         dw.origin[k] = 0;
         dw.ro_mod[k] = 0;
     }
+    dw.fold = 0;
     *out = w;
     return VG_OK;
 }
@@ -858,9 +863,11 @@ int vg_world_upload(vg_world *w, const vg_brick *bricks, const uint64_t *tile_ma
         (void)hipFree(tmp);
         if (e != hipSuccess) return fail(VG_E_DEVICE, "brick upload: %s", hipGetErrorString(e));
     }
+    w->dw.fold = 0;
     for (int k = 0; k < 3; ++k) {
         w->dw.origin[k] = origin[k];
         w->dw.ro_mod[k] = pos_mod(origin[k], d.v[k]);
+        if (w->dw.ro_mod[k]) w->dw.fold = 1;
     }
     w->uploaded = true;
     return VG_OK;
