@@ -677,6 +677,15 @@ static void r1_emit(const r1_t *s, uint32_t mat, orc_hit *out) {
     out->pad = 0;
 }
 
+/* The 8 child bits of the 2x2x2 group of children holding slot coordinate sl inside a
+ * parent u64 whose children are 2^(sh-1) voxels wide: bits lx + 4 lz + 16 ly for lx, ly, lz
+ * in the group's octant. An empty group is an aligned empty cube twice the child size,
+ * skipped exactly like any other empty cell. */
+static inline uint64_t group_mask(const int32_t sl[3], int sh) {
+    int ox = (sl[0] >> sh) & 1, oy = (sl[1] >> sh) & 1, oz = (sl[2] >> sh) & 1;
+    return (uint64_t)0x330033u << (2 * ox + 8 * oz + 32 * oy);
+}
+
 #define LOD_BRICK_T 400.0f /* shaders/raymarch.wgsl:4489 */
 #define TILE_LOD_T 520.0f  /* shaders/raymarch.wgsl:4494 */
 
@@ -689,6 +698,7 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
     const orc_world *w = s->w;
     const dims_t *d = &s->d;
     const int lod = (flags & ORC_LOD) && !(flags & (ORC_SKIP_TRANSPARENT | ORC_ANY_HIT));
+    const int groups = !(flags & ORC_NO_GROUPS);
     int phase1 = 1;
     for (;;) {
         if (s->t_cur > tmax) return;
@@ -708,16 +718,26 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
             continue;
         }
         if (!((l4w >> child_bit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
-            if (cnt) cnt[ORC_IT_CHUNK_SKIP]++;
             phase1 = 0;
-            r1_skip(s, sl, 64);
+            if (groups && !(l4w & group_mask(sl, 7))) {
+                if (cnt) cnt[ORC_IT_CHUNK_GROUP_SKIP]++;
+                r1_skip(s, sl, 128);
+            } else {
+                if (cnt) cnt[ORC_IT_CHUNK_SKIP]++;
+                r1_skip(s, sl, 64);
+            }
             continue;
         }
         uint64_t cw = w->chunk_mask[lin3(d->c, sl[0] >> 6, sl[1] >> 6, sl[2] >> 6)];
         if (!((cw >> child_bit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3)) & 1)) {
-            if (cnt) cnt[ORC_IT_TILE_SKIP]++;
             phase1 = 0;
-            r1_skip(s, sl, 16);
+            if (groups && !(cw & group_mask(sl, 5))) {
+                if (cnt) cnt[ORC_IT_TILE_GROUP_SKIP]++;
+                r1_skip(s, sl, 32);
+            } else {
+                if (cnt) cnt[ORC_IT_TILE_SKIP]++;
+                r1_skip(s, sl, 16);
+            }
             continue;
         }
         int64_t ti = lin3(d->t, sl[0] >> 4, sl[1] >> 4, sl[2] >> 4);
@@ -752,9 +772,18 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
             continue;
         }
         if (!((tw >> child_bit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3)) & 1)) {
-            if (cnt) cnt[ORC_IT_BRICK_SKIP]++;
             phase1 = 0;
-            r1_skip(s, sl, 4);
+            /* inside an occupied tile a tile-LOD test may fire at any visited voxel past
+             * t 520, so a group skip there must end before t 520: its exit lies within the
+             * 8^3 cell's diagonal (< 14) of the current voxel's entry */
+            if (groups && !(tw & group_mask(sl, 3)) &&
+                (!lod || s->t_cur + 14.0f <= TILE_LOD_T)) {
+                if (cnt) cnt[ORC_IT_BRICK_GROUP_SKIP]++;
+                r1_skip(s, sl, 8);
+            } else {
+                if (cnt) cnt[ORC_IT_BRICK_SKIP]++;
+                r1_skip(s, sl, 4);
+            }
             continue;
         }
         int64_t bi = lin3(d->b, sl[0] >> 2, sl[1] >> 2, sl[2] >> 2);
@@ -1042,7 +1071,9 @@ static void frame_rows(job_t *j) {
                 float so[3];
                 orc_shadow_origin(cam, dir, h, so);
                 orc_hit sh;
-                if (j->kind == 2) trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT, &sh, c);
+                if (j->kind == 2)
+                    trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT | (j->flags & ORC_NO_GROUPS),
+                               &sh, c);
                 else orc_raycast_r0(j->w, so, j->sun, j->stmax, &sh);
                 occ = sh.hit;
             }

@@ -90,8 +90,14 @@ enum {
     ORC_IT_BRICK_UNIFORM,
     ORC_IT_BRICK_LOD,
     ORC_IT_VOXEL,
+    ORC_IT_CHUNK_GROUP_SKIP, /* 2x2x2 empty chunks (128^3) */
+    ORC_IT_TILE_GROUP_SKIP,  /* 2x2x2 empty tiles (32^3) */
+    ORC_IT_BRICK_GROUP_SKIP, /* 2x2x2 empty bricks (8^3) */
     ORC_IT_COUNT
 };
+
+/* Oracle-only flag: walk the reference's levels only (no 2x2x2 group skips). */
+#define ORC_NO_GROUPS 0x100u
 
 /* dims / counts helpers */
 void orc_counts(const int32_t dims[3], int64_t out[5]); /* bricks, tiles, chunks, l4, voxels */

@@ -182,3 +182,34 @@ def test_lod_topmost_material():
     h, c = ow2.trace_r1(o, d, flags=oracle.LOD)
     assert h["hit"] and h["material"] == 3 and tuple(h["voxel"]) == (504, 100, 100)
     assert c["tile_lod"] == 0 and c["brick_lod"] == 1
+
+
+@pytest.mark.parametrize("flags", [0, oracle.LOD])
+def test_group_skips_keep_results(small_world, flags):
+    # 2x2x2 empty-group skips (and their LOD guard) change how the walk moves, never
+    # what it returns: same frame bit for bit with and without them
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    ow = oracle.OracleWorld.view(small_world)
+    a, sa, ca = ow.frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+    b, sb, cb = ow.frame(cam, wl.width, wl.height, 700.0, flags | oracle.NO_GROUPS, sun, 480.0)
+    assert a.tobytes() == b.tobytes() and np.array_equal(sa, sb)
+    assert cb["brick_group_skip"] == 0 and cb["tile_group_skip"] == 0
+    assert ca["brick_group_skip"] > 0 and ca["tile_group_skip"] > 0
+    assert sum(ca.values()) < sum(cb.values())
+
+
+def test_group_skips_keep_lod_results_far():
+    # far LOD region of the large world: every group-skip guard boundary is exercised
+    wl = workloads.LARGE
+    world = wl.world()
+    _, cam = wl.camera()
+    sun = wl.sun()
+    ow = oracle.OracleWorld.view(world)
+    rows = (240, 330)  # rows around the horizon reach past t 400 / 520
+    a, sa, ca = ow.frame(cam, wl.width, wl.height, 700.0, oracle.LOD, sun, 480.0, rows)
+    b, sb, _ = ow.frame(cam, wl.width, wl.height, 700.0, oracle.LOD | oracle.NO_GROUPS, sun,
+                        480.0, rows)
+    assert a.tobytes() == b.tobytes() and np.array_equal(sa, sb)
+    assert ca["tile_lod"] > 0 and ca["brick_lod"] > 0

@@ -252,6 +252,15 @@ __device__ __forceinline__ uint32_t topmost_material(const DevWorld &w, uint32_t
     return occ ? 35u : 0u;
 }
 
+// The 8 child bits of the 2x2x2 group of children holding slot coordinate sl inside a
+// parent u64 whose children are 2^(sh-1) voxels wide (bits lx + 4 lz + 16 ly over the
+// group's octant). An empty group is an aligned empty cube twice the child size, left
+// exactly like any other empty cell; it is not a level of the reference's pyramid.
+__device__ __forceinline__ uint64_t group_mask(const int32_t sl[3], int sh) {
+    const int ox = (sl[0] >> sh) & 1, oy = (sl[1] >> sh) & 1, oz = (sl[2] >> sh) & 1;
+    return (uint64_t)0x330033u << (2 * ox + 8 * oz + 32 * oy);
+}
+
 #define LOD_BRICK_T 400.0f
 #define TILE_LOD_T 520.0f
 
@@ -298,9 +307,10 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
                 break;
             }
             if (!((l4w >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
-                if (COUNT) cnt[VG_IT_CHUNK_SKIP]++;
                 phase1 = false;
-                S = 64;
+                const bool grp = !(l4w & group_mask(sl, 7));
+                if (COUNT) cnt[grp ? VG_IT_CHUNK_GROUP_SKIP : VG_IT_CHUNK_SKIP]++;
+                S = grp ? 128 : 64;
                 break;
             }
             const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
@@ -311,9 +321,10 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
             }
             const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
             if (!((cw >> tb) & 1)) {
-                if (COUNT) cnt[VG_IT_TILE_SKIP]++;
                 phase1 = false;
-                S = 16;
+                const bool grp = !(cw & group_mask(sl, 5));
+                if (COUNT) cnt[grp ? VG_IT_TILE_GROUP_SKIP : VG_IT_TILE_SKIP]++;
+                S = grp ? 32 : 16;
                 break;
             }
             const uint32_t tn = ci * 64u + (uint32_t)tb;
@@ -348,9 +359,13 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
             }
             const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
             if (!((tw >> bb) & 1)) {
-                if (COUNT) cnt[VG_IT_BRICK_SKIP]++;
                 phase1 = false;
-                S = 4;
+                // inside an occupied tile a tile-LOD test may fire at any visited voxel past
+                // t 520, so a group skip there must end before t 520: its exit lies within
+                // the 8^3 cell's diagonal (< 14) of the current voxel's entry
+                const bool grp = !(tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T);
+                if (COUNT) cnt[grp ? VG_IT_BRICK_GROUP_SKIP : VG_IT_BRICK_SKIP]++;
+                S = grp ? 8 : 4;
                 break;
             }
             const uint32_t bn = tn * 64u + (uint32_t)bb;
