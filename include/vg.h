@@ -125,6 +125,7 @@ enum {
     VG_IT_CHUNK_GROUP_SKIP, /* empty 2x2x2 chunk group (128^3) left in one step */
     VG_IT_TILE_GROUP_SKIP,  /* empty 2x2x2 tile group (32^3) */
     VG_IT_BRICK_GROUP_SKIP, /* empty 2x2x2 brick group (8^3) */
+    VG_IT_VOXEL_GROUP_SKIP, /* empty 2x2x2 voxel octant of a brick (2^3) */
     VG_IT_COUNT
 };
 

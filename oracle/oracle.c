@@ -810,9 +810,9 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
                 return;
             }
         }
-        if (cnt) cnt[ORC_IT_VOXEL]++;
         int vi = child_bit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
         if ((b->occupancy >> vi) & 1) {
+            if (cnt) cnt[ORC_IT_VOXEL]++;
             uint32_t m = b->materials[vi];
             if (solid_mat(m, flags)) {
                 if (!phase1) {
@@ -822,9 +822,22 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
                 r1_step(s);
                 continue;
             }
+            phase1 = 0;
+            r1_step(s);
+            continue;
         }
         phase1 = 0;
-        r1_step(s);
+        /* an empty 2x2x2 voxel octant is left whole; inside an occupied brick a brick-LOD
+         * test may fire at any visited voxel past t 400, so the octant (diagonal < 4) must
+         * be left before it */
+        if (groups && !(b->occupancy & group_mask(sl, 1)) &&
+            (!lod || s->t_cur + 4.0f <= LOD_BRICK_T)) {
+            if (cnt) cnt[ORC_IT_VOXEL_GROUP_SKIP]++;
+            r1_skip(s, sl, 2);
+        } else {
+            if (cnt) cnt[ORC_IT_VOXEL]++;
+            r1_step(s);
+        }
     }
 }
 

@@ -93,6 +93,7 @@ enum {
     ORC_IT_CHUNK_GROUP_SKIP, /* 2x2x2 empty chunks (128^3) */
     ORC_IT_TILE_GROUP_SKIP,  /* 2x2x2 empty tiles (32^3) */
     ORC_IT_BRICK_GROUP_SKIP, /* 2x2x2 empty bricks (8^3) */
+    ORC_IT_VOXEL_GROUP_SKIP, /* 2x2x2 empty voxels (2^3) */
     ORC_IT_COUNT
 };
 

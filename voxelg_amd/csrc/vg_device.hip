@@ -395,9 +395,9 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
                     return;
                 }
             }
-            if (COUNT) cnt[VG_IT_VOXEL]++;
             const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
             if ((ow >> vb) & 1) {
+                if (COUNT) cnt[VG_IT_VOXEL]++;
                 uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
                 if (solid_mat(m, flags)) {
                     if (!phase1) {
@@ -406,8 +406,16 @@ __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax,
                     }
                     break; // phase 1: step out of the solid voxel
                 }
+                phase1 = false;
+                break;
             }
             phase1 = false;
+            // an empty 2x2x2 voxel octant is left whole; inside an occupied brick a brick-LOD
+            // test may fire at any visited voxel past t 400, so the octant (diagonal < 4)
+            // must be left before it
+            const bool grp = !(ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T);
+            if (COUNT) cnt[grp ? VG_IT_VOXEL_GROUP_SKIP : VG_IT_VOXEL]++;
+            S = grp ? 2 : 1;
         } while (false);
         walk_advance(r, sl, S);
     }
