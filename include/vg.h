@@ -49,6 +49,9 @@ enum {
 #define VG_RAY_LOD 1u              /* brick LOD past 400, tile LOD past 520 (raymarch.wgsl:4489-4494) */
 #define VG_RAY_SKIP_TRANSPARENT 2u /* pass water and glass (trace_no_water, raymarch.wgsl:3450) */
 #define VG_RAY_ANY_HIT 4u          /* occlusion semantics (trace_any, raymarch.wgsl:4345) */
+#define VG_RAY_BEAM 8u             /* frame primary rays: fused conservative beam prologue per
+                                      8x8 pixel block (cs_beam, shaders/beam.wgsl:31-113); it
+                                      never changes a result. Ignored by vg_trace_rays. */
 
 /* Face codes (encode_face_normal, shaders/raymarch.wgsl:482-486): the hit face's outward
  * normal is +x 0, -x 1, +y 2, -y 3, +z 4, -z 5; 6 = no face (miss, or no entry plane). */
@@ -126,6 +129,7 @@ enum {
     VG_IT_TILE_GROUP_SKIP,  /* empty 2x2x2 tile group (32^3) */
     VG_IT_BRICK_GROUP_SKIP, /* empty 2x2x2 brick group (8^3) */
     VG_IT_VOXEL_GROUP_SKIP, /* empty 2x2x2 voxel octant of a brick (2^3) */
+    VG_IT_BEAM_TEST,        /* beam prologue: one tile tested by one lane */
     VG_IT_COUNT
 };
 

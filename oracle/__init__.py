@@ -22,19 +22,19 @@ HIT_DTYPE = np.dtype(
     [("voxel", "<i4", (3,)), ("t", "<f4"), ("face", "u1"), ("material", "u1"), ("hit", "u1"),
      ("pad", "u1")]
 )
-LOD, SKIP_TRANSPARENT, ANY_HIT = 1, 2, 4
+LOD, SKIP_TRANSPARENT, ANY_HIT, BEAM = 1, 2, 4, 8
 NO_GROUPS = 0x100  # oracle only: the reference's levels without 2x2x2 group skips
 FACE_NONE = 6
-IT_COUNT = 13
+IT_COUNT = 14
 IT_NAMES = ("l4_skip", "chunk_skip", "tile_skip", "tile_lod", "tile_uniform", "brick_skip",
             "brick_uniform", "brick_lod", "voxel", "chunk_group_skip", "tile_group_skip",
-            "brick_group_skip", "voxel_group_skip")
+            "brick_group_skip", "voxel_group_skip", "beam_test")
 # Algorithmic bytes per iteration class: the pyramid words the walk's test chain reads in
 # the reference's order (L4 word 8, chunk word 8, tile-uniform byte 1, tile word 8,
 # brick-uniform byte 1, occupancy word 8); a tile-LOD test adds its tile word, the first
 # child brick's occupancy and one material byte (17) to the class that follows it; a brick
 # LOD reads the occupancy and one material byte past the brick-uniform byte.
-IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34, 8, 16, 25, 34)
+IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34, 8, 16, 25, 34, 8)
 
 
 class OrcWorld(C.Structure):

@@ -660,6 +660,35 @@ static void r1_skip(r1_t *s, const int32_t slot[3], int32_t S) {
     s->last_axis = ax;
 }
 
+/* Jump the walk to the state reached once every plane crossing with t < tb has been
+ * processed (a prefix of the (t, axis) order): each axis lands on its last plane before tb
+ * (floor estimate corrected by direct t comparisons), the entry t is the latest crossing,
+ * ties to the higher axis (the one processed last). Used after the beam prologue proved
+ * every voxel before tb empty, so the walk continues exactly as if it had stepped there. */
+static void r1_advance_to(r1_t *s, float tb) {
+    float tl = -INFINITY;
+    int la = -1;
+    for (int k = 0; k < 3; ++k) {
+        int32_t st = s->step[k];
+        if (st == 0 || !(t_enter_voxel(s, k, s->l[k] + st) < tb)) continue;
+        int32_t nearest = s->l[k] + st;
+        int32_t e = (int32_t)floorf(s->org[k] + s->dir[k] * tb);
+        if (st > 0 ? e < nearest : e > nearest) e = nearest;
+        while (e != nearest && !(t_enter_voxel(s, k, e) < tb)) e -= st;
+        while (t_enter_voxel(s, k, e + st) < tb) e += st;
+        float tk = t_enter_voxel(s, k, e);
+        s->l[k] = e;
+        if (tk > tl || (tk == tl && k > la)) {
+            tl = tk;
+            la = k;
+        }
+    }
+    if (la >= 0) {
+        s->t_cur = tl;
+        s->last_axis = la;
+    }
+}
+
 static void r1_emit(const r1_t *s, uint32_t mat, orc_hit *out) {
     out->hit = 1;
     for (int k = 0; k < 3; ++k) out->voxel[k] = s->l[k] + s->w->origin[k];
@@ -841,12 +870,18 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
     }
 }
 
-static void trace_core(const orc_world *w, const float o[3], const float dir[3], float tmax,
-                       uint32_t flags, orc_hit *out, uint64_t *cnt) {
+static void trace_core_from(const orc_world *w, const float o[3], const float dir[3], float tmax,
+                            uint32_t flags, float t_start, orc_hit *out, uint64_t *cnt) {
     miss(out);
     r1_t s;
     if (!r1_init(&s, w, o, dir)) return;
+    if (t_start > 0.0f && s.t_cur <= tmax) r1_advance_to(&s, t_start);
     r1_walk(&s, tmax, flags, out, cnt);
+}
+
+static void trace_core(const orc_world *w, const float o[3], const float dir[3], float tmax,
+                       uint32_t flags, orc_hit *out, uint64_t *cnt) {
+    trace_core_from(w, o, dir, tmax, flags, 0.0f, out, cnt);
 }
 
 void orc_trace_r1(const orc_world *w, const orc_ray *r, uint32_t flags, orc_hit *out,
@@ -1066,6 +1101,71 @@ void orc_shadow_origin(const float o[3], const float d[3], const orc_hit *h, flo
     }
 }
 
+#define BEAM_STEP 16.0f
+#define BEAM_MAX_STEPS 64
+
+/* Beam prologue of one 8x8 pixel block (the role of cs_beam, shaders/beam.wgsl:31-113, made
+ * conservative): the 64 rays share the camera origin O, so on t in [t0, t1] they all lie in
+ * the box O + [min d, max d] * [t0, t1] (per axis, over the block's directions). March that
+ * box in 16-voxel steps, grown by one voxel, and test every world tile it touches inside the
+ * window (at most 64, one per lane of a wave) against the chunk masks; the block's start
+ * t is the first step whose box touches an occupied tile. Every voxel a block ray visits
+ * before it is empty, so the walk may jump there exactly (r1_advance_to). */
+float orc_beam_t(const orc_world *w, const float *cam, int32_t bx, int32_t by, float tmax,
+                 uint64_t *tests) {
+    dims_t d = mkdims(w->dims);
+    for (int k = 0; k < 3; ++k) {
+        float rel = cam[k] - (float)w->origin[k];
+        if (!(rel >= 0.0f && rel < (float)w->dims[k])) return 0.0f; /* camera outside */
+    }
+    float dmin[3] = {INFINITY, INFINITY, INFINITY}, dmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t ly = 0; ly < 8; ++ly)
+        for (int32_t lx = 0; lx < 8; ++lx) {
+            float dd[3];
+            orc_primary_dir(cam, bx * 8 + lx, by * 8 + ly, dd);
+            for (int k = 0; k < 3; ++k) {
+                dmin[k] = fmin_(dmin[k], dd[k]);
+                dmax[k] = fmax_(dmax[k], dd[k]);
+            }
+        }
+    float t0 = 0.0f;
+    for (int st = 0; st < BEAM_MAX_STEPS && t0 < tmax; ++st) {
+        float t1 = (float)(st + 1) * BEAM_STEP;
+        int32_t lo[3], hi[3];
+        int64_t n = 1;
+        for (int k = 0; k < 3; ++k) {
+            float a = cam[k] + dmin[k] * t0, b = cam[k] + dmin[k] * t1;
+            float c = cam[k] + dmax[k] * t0, e = cam[k] + dmax[k] * t1;
+            float mn = fmin_(fmin_(a, b), fmin_(c, e)) - 1.0f;
+            float mx = fmax_(fmax_(a, b), fmax_(c, e)) + 1.0f;
+            int32_t tl = (int32_t)floorf(mn * 0.0625f), th = (int32_t)floorf(mx * 0.0625f);
+            /* world tiles of the window */
+            int32_t wl = w->origin[k] >> 4, wh = (w->origin[k] + w->dims[k] - 1) >> 4;
+            if (tl < wl) tl = wl;
+            if (th > wh) th = wh;
+            lo[k] = tl;
+            hi[k] = th;
+            n *= th >= tl ? (int64_t)(th - tl + 1) : 0;
+        }
+        if (n > 64) break;
+        int occupied = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            int32_t nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+            int32_t tw[3] = {lo[0] + (int32_t)(j % nx), lo[1] + (int32_t)((j / nx) % ny),
+                             lo[2] + (int32_t)(j / ((int64_t)nx * ny))};
+            int32_t sl[3];
+            for (int k = 0; k < 3; ++k) sl[k] = pos_mod(tw[k] * 16, w->dims[k]);
+            uint64_t cw = w->chunk_mask[lin3(d.c, sl[0] >> 6, sl[1] >> 6, sl[2] >> 6)];
+            if ((cw >> child_bit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3)) & 1)
+                occupied = 1;
+        }
+        if (tests) *tests += (uint64_t)n;
+        if (occupied) break;
+        t0 = t1;
+    }
+    return t0;
+}
+
 static void frame_rows(job_t *j) {
     const float *cam = j->cam;
     for (int32_t y = j->row0 + j->tid; y < j->row1; y += j->nthreads) {
@@ -1075,7 +1175,13 @@ static void frame_rows(job_t *j) {
             orc_primary_dir(cam, x, y, dir);
             orc_hit *h = &j->out[i];
             uint64_t c[ORC_IT_COUNT] = {0};
-            if (j->kind == 2) trace_core(j->w, cam, dir, j->ptmax, j->flags, h, c);
+            float tb = 0.0f;
+            if (j->kind == 2 && (j->flags & ORC_BEAM)) {
+                /* the block's beam start; its tile tests are charged to the block's first pixel */
+                uint64_t *tc = ((x & 7) == 0 && (y & 7) == 0) ? &c[ORC_IT_BEAM_TEST] : NULL;
+                tb = orc_beam_t(j->w, cam, x >> 3, y >> 3, j->ptmax, tc);
+            }
+            if (j->kind == 2) trace_core_from(j->w, cam, dir, j->ptmax, j->flags, tb, h, c);
             else orc_raycast_r0(j->w, cam, dir, j->ptmax, h);
             uint32_t it_primary = 0;
             for (int k = 0; k < ORC_IT_COUNT; ++k) it_primary += (uint32_t)c[k];

@@ -76,6 +76,7 @@ typedef struct orc_hit {
 #define ORC_LOD 1u
 #define ORC_SKIP_TRANSPARENT 2u
 #define ORC_ANY_HIT 4u
+#define ORC_BEAM 8u /* frame primary rays: fused beam prologue per 8x8 pixel block */
 
 #define ORC_FACE_NONE 6
 
@@ -94,6 +95,7 @@ enum {
     ORC_IT_TILE_GROUP_SKIP,  /* 2x2x2 empty tiles (32^3) */
     ORC_IT_BRICK_GROUP_SKIP, /* 2x2x2 empty bricks (8^3) */
     ORC_IT_VOXEL_GROUP_SKIP, /* 2x2x2 empty voxels (2^3) */
+    ORC_IT_BEAM_TEST,        /* beam prologue: one tile tested by one lane */
     ORC_IT_COUNT
 };
 
@@ -155,6 +157,9 @@ void orc_frame_r0(const orc_world *w, const float *camera176, int32_t width, int
                   float primary_tmax, const float sun[3], float shadow_tmax, int32_t row0,
                   int32_t row1, orc_hit *hits, uint8_t *shadow, int threads);
 void orc_shadow_origin(const float o[3], const float d[3], const orc_hit *h, float out[3]);
+/* Beam prologue start t of the 8x8 pixel block (bx, by); tests counts tested tiles. */
+float orc_beam_t(const orc_world *w, const float *camera176, int32_t bx, int32_t by, float tmax,
+                 uint64_t *tests);
 
 #ifdef __cplusplus
 }

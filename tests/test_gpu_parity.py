@@ -75,6 +75,27 @@ def test_small_frame_lod_shadow_bit_exact(ctx, small, small_world):
     assert np.array_equal(o2.numpy(np.uint8), ref_occ)
 
 
+@pytest.mark.parametrize("flags", [vg.VG_RAY_BEAM, vg.VG_RAY_LOD | vg.VG_RAY_BEAM])
+def test_small_frame_with_beam_bit_exact(ctx, small, small_world, flags):
+    # the fused beam prologue never changes a result: GPU beam frame == oracle beam frame ==
+    # GPU frame without the beam; iteration counts (beam tile tests included) == oracle
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    print(f"case: small frame beam flags {flags}", flush=True)
+    hits, occ, c = small.trace_frame_counts(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+    plain, plain_occ = small.trace_frame(cam, wl.width, wl.height, 700.0,
+                                         flags & ~vg.VG_RAY_BEAM, sun, 480.0)
+    ctx.synchronize()
+    ref, ref_occ, rc = oracle.OracleWorld.view(small_world).frame(
+        cam, wl.width, wl.height, 700.0, flags, sun, 480.0, threads=THREADS)
+    assert_same(gpu_hits(hits), ref, "beam frame vs oracle")
+    assert_same(gpu_hits(plain), ref, "beam frame vs plain frame")
+    assert np.array_equal(occ.numpy(np.uint8), ref_occ)
+    assert np.array_equal(plain_occ.numpy(np.uint8), ref_occ)
+    assert c == rc and c["beam_test"] > 0
+
+
 def mixed_rays(rng, n, dims):
     d = np.asarray(dims, np.float32)
     o = (rng.uniform(-0.2, 1.2, (n, 3)) * d).astype(np.float32)
@@ -209,6 +230,13 @@ def test_large_frame_sampled_rows_bit_exact(ctx):
                                    sun, wl.shadow_tmax, rows=(r0, r0 + 2), threads=THREADS)
         assert_same(got[r0:r0 + 2].reshape(-1), ref, f"large rows {r0}")
         assert np.array_equal(occ[r0:r0 + 2].reshape(-1), ref_occ)
+    # the beam prologue is a pure acceleration: the whole frame without it is identical
+    assert wl.primary_flags & vg.VG_RAY_BEAM
+    h2, o2 = dw.trace_frame(cam, wl.width, wl.height, wl.primary_tmax,
+                            wl.primary_flags & ~vg.VG_RAY_BEAM, sun, wl.shadow_tmax)
+    ctx.synchronize()
+    assert gpu_hits(h2).tobytes() == got.tobytes()
+    assert np.array_equal(o2.numpy(np.uint8).reshape(wl.height, wl.width), occ)
     flat = got.reshape(-1)
     h = flat["hit"] == 1
     assert h.mean() > 0.3

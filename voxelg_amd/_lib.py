@@ -28,18 +28,19 @@ VG_OK = 0
 VG_RAY_LOD = 1
 VG_RAY_SKIP_TRANSPARENT = 2
 VG_RAY_ANY_HIT = 4
+VG_RAY_BEAM = 8
 VG_FACE_NONE = 6
-VG_IT_COUNT = 13
+VG_IT_COUNT = 14
 IT_NAMES = ("l4_skip", "chunk_skip", "tile_skip", "tile_lod", "tile_uniform", "brick_skip",
             "brick_uniform", "brick_lod", "voxel", "chunk_group_skip", "tile_group_skip",
-            "brick_group_skip", "voxel_group_skip")
+            "brick_group_skip", "voxel_group_skip", "beam_test")
 # Algorithmic bytes per iteration class: the pyramid words the walk's test chain reads in
 # the reference's order (shaders/raymarch.wgsl:4626-4775): L4 word 8, chunk word 8,
 # tile-uniform byte 1, tile word 8, brick-uniform byte 1, occupancy word 8. A tile-LOD test
 # adds its tile word, the first child brick's occupancy and one material byte (17); a brick
 # LOD reads the occupancy and one material byte past the brick-uniform byte (35). A 2x2x2
 # group skip reads the same word as the single-cell skip of its level.
-IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34, 8, 16, 25, 34)
+IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34, 8, 16, 25, 34, 8)
 HIT_RECORD_BYTES = 20   # one vg_hit written per ray
 MATERIAL_BYTES = 1      # material byte read at a voxel hit
 RAY_INPUT_BYTES = 32    # one vg_ray read per batched ray

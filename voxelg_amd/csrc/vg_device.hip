@@ -229,6 +229,38 @@ __device__ __forceinline__ void walk_advance(Walk &r, const int32_t sl[3], int32
     r.last_axis = ax;
 }
 
+// Jump to the state reached once every plane crossing with t < tb has been processed (a
+// prefix of the (t, axis) order): each axis lands on its last plane before tb (floor
+// estimate corrected by direct t comparisons), the entry t is the latest crossing, ties to
+// the higher axis (processed last). Used after the beam prologue proved every voxel the
+// ray visits before tb empty, so the walk continues exactly as if it had stepped there.
+__device__ __forceinline__ void walk_advance_to(Walk &r, float tb) {
+    float tl = -INFINITY;
+    int la = -1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (r.step[k] != 0 && r.tn[k] < tb) {
+            const int32_t st = r.step[k];
+            const int32_t nearest = r.l[k] + st;
+            int32_t e = (int32_t)floorf(r.org[k] + r.dir[k] * tb);
+            if (st > 0 ? e < nearest : e > nearest) e = nearest;
+            for (int g = 0; g < 4096 && e != nearest && !(t_plane(r, k, e) < tb); ++g) e -= st;
+            for (int g = 0; g < 4096 && t_plane(r, k, e + st) < tb; ++g) e += st;
+            const float tk = t_plane(r, k, e);
+            r.l[k] = e;
+            r.tn[k] = t_plane(r, k, e + st);
+            if (tk > tl || (tk == tl && k > la)) {
+                tl = tk;
+                la = k;
+            }
+        }
+    }
+    if (la >= 0) {
+        r.t_cur = tl;
+        r.last_axis = la;
+    }
+}
+
 __device__ __forceinline__ void emit(const Walk &r, const DevWorld &w, uint32_t m, HitRec &h) {
     h.hit = 1;
 #pragma unroll
@@ -494,6 +526,82 @@ __device__ __forceinline__ void flush_counts(const uint32_t *cnt, unsigned long 
     }
 }
 
+// Fused beam prologue (the role of cs_beam, shaders/beam.wgsl:31-113, made conservative).
+// A wave is one 8x8 pixel block whose 64 rays share the camera origin O, so on [t0, t1]
+// they all lie in O + [min d, max d] * [t0, t1] (per axis, min / max over the wave). The
+// wave marches that box in 16-voxel steps, grown by one voxel, each lane testing one world
+// tile the box touches inside the window (at most 64) against the chunk masks; the block
+// starts at the first step whose box touches an occupied tile. Every voxel a block ray
+// visits before that t is empty, so walk_advance_to may jump there exactly. Every lane of
+// the wave must call this (it reduces and votes across the wave).
+constexpr float kBeamStep = 16.0f;
+constexpr int kBeamMaxSteps = 64;
+
+template <bool COUNT>
+__device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &c,
+                                            const float d[3], float tmax, uint32_t *cnt) {
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float rel = c.origin[k] - (float)w.origin[k];
+        inside = inside && rel >= 0.0f && rel < (float)w.dims[k];
+    }
+    if (!inside) return 0.0f; // wave-uniform: the camera is shared
+    float dmin[3], dmax[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lo = d[k], hi = d[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = fsel_min(lo, __shfl_xor(lo, off, 64));
+            hi = fsel_max(hi, __shfl_xor(hi, off, 64));
+        }
+        dmin[k] = lo;
+        dmax[k] = hi;
+    }
+    const int lane = threadIdx.x & 63;
+    float t0 = 0.0f;
+    for (int st = 0; st < kBeamMaxSteps && t0 < tmax; ++st) {
+        const float t1 = (float)(st + 1) * kBeamStep;
+        int32_t lo[3], n[3];
+        int32_t total = 1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float a = c.origin[k] + dmin[k] * t0, b = c.origin[k] + dmin[k] * t1;
+            float cc = c.origin[k] + dmax[k] * t0, e = c.origin[k] + dmax[k] * t1;
+            float mn = fsel_min(fsel_min(a, b), fsel_min(cc, e)) - 1.0f;
+            float mx = fsel_max(fsel_max(a, b), fsel_max(cc, e)) + 1.0f;
+            int32_t tl = (int32_t)floorf(mn * 0.0625f), th = (int32_t)floorf(mx * 0.0625f);
+            const int32_t wl = w.origin[k] >> 4, wh = (w.origin[k] + w.dims[k] - 1) >> 4;
+            tl = tl < wl ? wl : tl;
+            th = th > wh ? wh : th;
+            lo[k] = tl;
+            n[k] = th >= tl ? th - tl + 1 : 0;
+            total *= n[k];
+        }
+        if (total > 64) break;
+        bool occ = false;
+        if (lane < total) {
+            if (COUNT) cnt[VG_IT_BEAM_TEST]++;
+            const int32_t tx = lo[0] + lane % n[0];
+            const int32_t ty = lo[1] + (lane / n[0]) % n[1];
+            const int32_t tz = lo[2] + lane / (n[0] * n[1]);
+            int32_t sl[3] = {tx * 16, ty * 16, tz * 16};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                int32_t m = sl[k] % w.dims[k];
+                sl[k] = m < 0 ? m + w.dims[k] : m;
+            }
+            const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                                (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+            occ = (w.chunk[ci] >> cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3)) & 1;
+        }
+        if (__ballot(occ) != 0ull) break;
+        t0 = t1;
+    }
+    return t0;
+}
+
 // ------------------------------------------------------------------ kernels --
 constexpr int kBlock = 256;
 
@@ -532,7 +640,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevWorld w, const vg_ray 
 }
 
 // One thread per pixel, a wave per 8x8 pixel block, 16x16 pixels per workgroup.
-template <bool COUNT, int MODE> // MODE 0 primary, 1 primary+shadow, 2 shadow from hits
+template <bool COUNT, int MODE, bool BEAM> // MODE 0 primary, 1 primary+shadow, 2 shadow from hits
 __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams cam, int32_t width,
                                                         int32_t height, uint32_t nbx,
                                                         float ptmax, uint32_t pflags, float sx,
@@ -549,17 +657,22 @@ __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams ca
     if (COUNT)
 #pragma unroll
         for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    float d[3];
+    primary_dir(cam, px, py, d); // every lane, also past the image edge (the beam reads all)
+    float tb = 0.0f;
+    if (BEAM && MODE != 2) tb = beam_start<COUNT>(w, cam, d, ptmax, cnt);
     if (px < width && py < height) {
         const int64_t i = (int64_t)py * width + px;
-        float d[3];
-        primary_dir(cam, px, py, d);
         HitRec h;
         if (MODE == 2) {
             load_hit(hits + i, h);
         } else {
             hit_miss(h);
             Walk r;
-            if (walk_init(r, w, cam.origin, d)) walk_run<COUNT>(r, w, ptmax, pflags, h, cnt);
+            if (walk_init(r, w, cam.origin, d)) {
+                if (BEAM && tb > 0.0f && r.t_cur <= ptmax) walk_advance_to(r, tb);
+                walk_run<COUNT>(r, w, ptmax, pflags, h, cnt);
+            }
             store_hit(hits + i, h);
         }
         if (MODE != 0) {
@@ -694,17 +807,30 @@ static int launch_frame(vg_world *w, const vg_camera *camera, int32_t width, int
     CamParams cp = cam_params(camera);
     dim3 grid(nb), block(kBlock);
     unsigned long long *cd = count ? w->counts_dev : nullptr;
-#define LAUNCH_F(C, M)                                                                        \
-    hipLaunchKernelGGL((k_trace_frame<C, M>), grid, block, 0, stream, w->dw, cp, width, height, \
-                       nbx, ptmax, pflags, s[0], s[1], s[2], stmax, hits, occluded, cd)
+    const bool beam = (pflags & VG_RAY_BEAM) != 0;
+#define LAUNCH_F(C, M, B)                                                                     \
+    hipLaunchKernelGGL((k_trace_frame<C, M, B>), grid, block, 0, stream, w->dw, cp, width,      \
+                       height, nbx, ptmax, pflags, s[0], s[1], s[2], stmax, hits, occluded, cd)
     if (count) {
-        if (mode == 0) LAUNCH_F(true, 0);
-        else if (mode == 1) LAUNCH_F(true, 1);
-        else LAUNCH_F(true, 2);
+        if (mode == 0) {
+            if (beam) LAUNCH_F(true, 0, true);
+            else LAUNCH_F(true, 0, false);
+        } else if (mode == 1) {
+            if (beam) LAUNCH_F(true, 1, true);
+            else LAUNCH_F(true, 1, false);
+        } else {
+            LAUNCH_F(true, 2, false);
+        }
     } else {
-        if (mode == 0) LAUNCH_F(false, 0);
-        else if (mode == 1) LAUNCH_F(false, 1);
-        else LAUNCH_F(false, 2);
+        if (mode == 0) {
+            if (beam) LAUNCH_F(false, 0, true);
+            else LAUNCH_F(false, 0, false);
+        } else if (mode == 1) {
+            if (beam) LAUNCH_F(false, 1, true);
+            else LAUNCH_F(false, 1, false);
+        } else {
+            LAUNCH_F(false, 2, false);
+        }
     }
 #undef LAUNCH_F
     HIP_TRY(hipGetLastError());

@@ -2,7 +2,8 @@
 
 small  256x128x256 world (4x2x4 chunks of 64^3), 1280x720 primary rays, tmax 512, LOD off.
 large  2048x256x2048 world (32x4x32 chunks), 1920x1080 primary rays (tmax 700 = MAX_RAY_DIST,
-       shaders/raymarch.wgsl:4501) with brick/tile LOD past 400/520, plus one sun shadow ray
+       shaders/raymarch.wgsl:4501) with brick/tile LOD past 400/520 and the fused beam
+       prologue, plus one sun shadow ray
        per hit (tmax 480 = SHADOW_MAX_DIST, :4517; sun elevation 35 deg, seeded azimuth).
 Cameras: seeded (x, z) over open ground, height 90, random yaw, pitch -0.35
 (src/renderer.rs:3539), vertical fov 70 deg (src/camera.rs:19).
@@ -14,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import VG_RAY_LOD
+from ._lib import VG_RAY_BEAM, VG_RAY_LOD
 from .world import Camera, World, sun_direction, terrain_height
 
 MAX_RAY_DIST = 700.0
@@ -74,4 +75,5 @@ class FrameWorkload:
 
 
 SMALL = FrameWorkload("small", (256, 128, 256), 1280, 720, 512.0, 0, False)
-LARGE = FrameWorkload("large", (2048, 256, 2048), 1920, 1080, MAX_RAY_DIST, VG_RAY_LOD, True)
+LARGE = FrameWorkload("large", (2048, 256, 2048), 1920, 1080, MAX_RAY_DIST,
+                      VG_RAY_LOD | VG_RAY_BEAM, True)
