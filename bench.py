@@ -97,6 +97,16 @@ def main():
         dw.trace_frame(cam, W, H, wl.primary_tmax, wl.primary_flags, sun, wl.shadow_tmax, hits,
                        occ, stream)
 
+    # schedule autotuning (outside the timed region): waves per CU x refill threshold of the
+    # persistent frame kernel, timed on this very frame
+    from voxelg_amd import _lib as L
+    tuned_ms = dw.autotune_frame(cam, W, H, wl.primary_tmax, wl.primary_flags, sun,
+                                 wl.shadow_tmax)
+    sched = {"waves_per_cu": ctx.get_option(L.VG_OPT_FRAME_WAVES_PER_CU),
+             "refill_below": ctx.get_option(L.VG_OPT_FRAME_REFILL),
+             "cus": ctx.get_option(L.VG_OPT_CU_COUNT)}
+    log(f"autotuned persistent frame kernel: {sched} ({tuned_ms:.3f} ms)")
+
     # algorithmic accounting (outside the timed region): iteration classes of this frame
     _, _, counts = dw.trace_frame_counts(cam, W, H, wl.primary_tmax, wl.primary_flags, sun,
                                          wl.shadow_tmax)
@@ -167,11 +177,12 @@ def main():
                         "tmax 480",
             "world": list(wl.dims), "resolution": [W, H], "seed": wl.seed,
             "primary_rays": W * H, "shadow_rays": n_hit,
+            "schedule": dict(sched, kernel="persistent ray-compacting + beam pre-pass"),
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_trace_frame<false,1>", "kernel_ms": round(avg_kern_s * 1e3, 4),
+            "kernel": "k_beam + k_frame_persistent", "kernel_ms": round(avg_kern_s * 1e3, 4),
             "algorithmic_bytes": int(bytes_per_frame),
         },
         "cpu_baseline": base,

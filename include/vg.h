@@ -18,11 +18,12 @@
  *   - Device pointers (rays, hits, occlusion bytes) are plain device addresses the caller
  *     allocated on the world's device; host pointers are marked _host.
  *   - `stream` is a hipStream_t passed as void*; NULL is the null stream. Kernels are
- *     enqueued asynchronously; calls on one world must not overlap with vg_world_upload.
+ *     enqueued asynchronously; launches on one world must not run concurrently with each
+ *     other or with vg_world_upload (they share the world's scratch counter).
  *   - World dimensions are voxels per axis, each a multiple of 64; an axis whose window
  *     origin is not a multiple of its dimension (toroidal fold) must be a multiple of 256.
- *   - Worlds of one context may be used from several host threads for tracing; creation,
- *     upload and destruction are not thread-safe against tracing on the same world.
+ *   - Distinct worlds may be traced from distinct host threads; creation, upload and
+ *     destruction are not thread-safe against tracing on the same world.
  */
 #ifndef VG_H
 #define VG_H
@@ -232,6 +233,33 @@ int vg_trace_frame_counts(vg_world *world, const vg_camera *camera_host, int32_t
                           uint8_t *occluded, uint64_t counts_host[VG_IT_COUNT]);
 int vg_trace_rays_counts(vg_world *world, const vg_ray *rays, size_t n, uint32_t flags,
                          vg_hit *hits, uint64_t counts_host[VG_IT_COUNT]);
+
+/* ------------------------------------------------------------- scheduling -- */
+/* Traces run as persistent, ray-compacting kernels by default: a machine-sized grid whose
+ * waves refill idle lanes from a global ray counter whenever fewer than `refill` of their
+ * 64 lanes hold a live ray (a primary hit turns into that lane's shadow ray at once).
+ * VG_SCHEDULE_STATIC launches one thread per ray instead. The options live in the context;
+ * the autotuners time every (waves per CU, refill) pair on the caller's workload and keep
+ * the fastest. A world's persistent launches share one counter: launches on one world must
+ * not run concurrently (use one stream per world, or synchronize). */
+#define VG_SCHEDULE_PERSISTENT 0
+#define VG_SCHEDULE_STATIC 1
+enum {
+    VG_OPT_SCHEDULE = 1,
+    VG_OPT_FRAME_WAVES_PER_CU = 2,
+    VG_OPT_FRAME_REFILL = 3,
+    VG_OPT_RAYS_WAVES_PER_CU = 4,
+    VG_OPT_RAYS_REFILL = 5,
+    VG_OPT_CU_COUNT = 6 /* read only */
+};
+int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
+int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);
+int vg_autotune_frame(vg_world *world, const vg_camera *camera_host, int32_t width,
+                      int32_t height, float primary_tmax, uint32_t primary_flags,
+                      const float sun[3], float shadow_tmax, vg_hit *hits, uint8_t *occluded,
+                      float *best_ms);
+int vg_autotune_rays(vg_world *world, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
+                     float *best_ms);
 
 /* raycast (src/raycast.rs:21) over a world on the device: one ray, host in and out.
  * Returns VG_OK with hit_host->hit = 0/1. Synchronous. */

@@ -1108,7 +1108,7 @@ void orc_shadow_origin(const float o[3], const float d[3], const orc_hit *h, flo
  * conservative): the 64 rays share the camera origin O, so on t in [t0, t1] they all lie in
  * the box O + [min d, max d] * [t0, t1] (per axis, over the block's directions). March that
  * box in 16-voxel steps, grown by one voxel, and test every world tile it touches inside the
- * window (at most 64, one per lane of a wave) against the chunk masks; the block's start
+ * window (a 4x4x4 cube at most, one tile per lane of a wave) against the chunk masks; the block's start
  * t is the first step whose box touches an occupied tile. Every voxel a block ray visits
  * before it is empty, so the walk may jump there exactly (r1_advance_to). */
 float orc_beam_t(const orc_world *w, const float *cam, int32_t bx, int32_t by, float tmax,
@@ -1147,7 +1147,11 @@ float orc_beam_t(const orc_world *w, const float *cam, int32_t bx, int32_t by, f
             hi[k] = th;
             n *= th >= tl ? (int64_t)(th - tl + 1) : 0;
         }
-        if (n > 64) break;
+        if (n == 0) { /* the box left the window: every block ray has exited it for good */
+            t0 = t1;
+            break;
+        }
+        if (hi[0] - lo[0] >= 4 || hi[1] - lo[1] >= 4 || hi[2] - lo[2] >= 4) break; /* > 4x4x4 */
         int occupied = 0;
         for (int64_t j = 0; j < n; ++j) {
             int32_t nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;

@@ -192,6 +192,63 @@ def test_kat_roof_shadow_and_empty(ctx):
     dw.close()
 
 
+@pytest.mark.parametrize("schedule", ["static", (8, 1), (16, 32), (32, 64), (4, 16)])
+def test_schedules_agree(ctx, small, small_world, schedule):
+    # the one-thread-per-ray kernels and the persistent ray-compacting kernels at several
+    # (waves per CU, refill threshold) settings return the same bytes as the oracle
+    from voxelg_amd import _lib as L
+    if schedule == "static":
+        ctx.set_option(L.VG_OPT_SCHEDULE, L.VG_SCHEDULE_STATIC)
+    else:
+        ctx.set_option(L.VG_OPT_SCHEDULE, L.VG_SCHEDULE_PERSISTENT)
+        for o, v in ((L.VG_OPT_FRAME_WAVES_PER_CU, schedule[0]), (L.VG_OPT_FRAME_REFILL, schedule[1]),
+                     (L.VG_OPT_RAYS_WAVES_PER_CU, schedule[0]), (L.VG_OPT_RAYS_REFILL, schedule[1])):
+            ctx.set_option(o, v)
+    try:
+        wl = workloads.SMALL
+        _, cam = wl.camera()
+        sun = workloads.LARGE.sun()
+        print(f"case: schedule {schedule}", flush=True)
+        flags = vg.VG_RAY_LOD | vg.VG_RAY_BEAM
+        hits, occ, c = small.trace_frame_counts(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+        ref, ref_occ, rc = oracle.OracleWorld.view(small_world).frame(
+            cam, wl.width, wl.height, 700.0, flags, sun, 480.0, threads=THREADS)
+        assert_same(gpu_hits(hits), ref, f"frame {schedule}")
+        assert np.array_equal(occ.numpy(np.uint8), ref_occ)
+        assert c == rc
+        rays = mixed_rays(np.random.default_rng(9), 100_000, small_world.dims)
+        got, c = small.trace_rays_counts(ctx.rays(rays))
+        ref, rc = oracle.OracleWorld.view(small_world).trace_rays(rays, 0, THREADS)
+        assert_same(gpu_hits(got), ref, f"rays {schedule}")
+        assert c == rc
+    finally:
+        ctx.set_option(L.VG_OPT_SCHEDULE, L.VG_SCHEDULE_PERSISTENT)
+        for o in (L.VG_OPT_FRAME_WAVES_PER_CU, L.VG_OPT_RAYS_WAVES_PER_CU):
+            ctx.set_option(o, 16)
+        for o in (L.VG_OPT_FRAME_REFILL, L.VG_OPT_RAYS_REFILL):
+            ctx.set_option(o, 32)
+
+
+def test_autotune_keeps_results(ctx, small, small_world):
+    from voxelg_amd import _lib as L
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    flags = vg.VG_RAY_LOD | vg.VG_RAY_BEAM
+    ms = small.autotune_frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+    assert 0.0 < ms < 100.0
+    assert 1 <= ctx.get_option(L.VG_OPT_FRAME_WAVES_PER_CU) <= 64
+    assert 1 <= ctx.get_option(L.VG_OPT_FRAME_REFILL) <= 64
+    assert ctx.get_option(L.VG_OPT_CU_COUNT) >= 1
+    hits, occ = small.trace_frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+    ref, ref_occ, _ = oracle.OracleWorld.view(small_world).frame(
+        cam, wl.width, wl.height, 700.0, flags, sun, 480.0, threads=THREADS)
+    ctx.synchronize()
+    assert_same(gpu_hits(hits), ref, "frame after autotune")
+    with pytest.raises(Exception):
+        ctx.set_option(L.VG_OPT_FRAME_REFILL, 0)
+
+
 def test_edge_cases_and_errors(ctx, small, small_world):
     from voxelg_amd._lib import VGError
     # zero rays is a no-op

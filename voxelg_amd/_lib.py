@@ -25,6 +25,14 @@ HIT_DTYPE = np.dtype(
 assert BRICK_DTYPE.itemsize == 72 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 20
 
 VG_OK = 0
+VG_SCHEDULE_PERSISTENT = 0
+VG_SCHEDULE_STATIC = 1
+VG_OPT_SCHEDULE = 1
+VG_OPT_FRAME_WAVES_PER_CU = 2
+VG_OPT_FRAME_REFILL = 3
+VG_OPT_RAYS_WAVES_PER_CU = 4
+VG_OPT_RAYS_REFILL = 5
+VG_OPT_CU_COUNT = 6
 VG_RAY_LOD = 1
 VG_RAY_SKIP_TRANSPARENT = 2
 VG_RAY_ANY_HIT = 4
@@ -105,6 +113,11 @@ SIGNATURES = {
     "vg_event_destroy": (C.c_int, [_P, _P]),
     "vg_event_record": (C.c_int, [_P, _P, _P]),
     "vg_event_elapsed_ms": (C.c_int, [_P, _P, _P, C.POINTER(C.c_float)]),
+    "vg_ctx_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
+    "vg_ctx_get_option": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "vg_autotune_frame": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _F32P,
+                                    C.c_float, _P, _P, C.POINTER(C.c_float)]),
+    "vg_autotune_rays": (C.c_int, [_P, _P, C.c_size_t, C.c_uint32, _P, C.POINTER(C.c_float)]),
 }
 
 

@@ -301,156 +301,182 @@ __device__ __forceinline__ uint64_t group_mask(const int32_t sl[3], int sh) {
 // skips the largest empty aligned cell (shaders/raymarch.wgsl:4626-4775). The words of
 // the current L4 cell, chunk, tile and brick stay in registers until the walk leaves
 // them. COUNT tallies iteration classes for algorithmic-byte accounting.
+// Register-cached pyramid words of the cells holding the current voxel, and the phase.
+struct WalkCache {
+    uint32_t cur_l4, cur_c, cur_t, cur_b;
+    uint64_t l4w, cw, tw, ow;
+    uint32_t tu, bu;
+    bool phase1;
+};
+
+__device__ __forceinline__ void cache_reset(WalkCache &q) {
+    q.cur_l4 = q.cur_c = q.cur_t = q.cur_b = ~0u;
+    q.l4w = q.cw = q.tw = q.ow = 0;
+    q.tu = q.bu = 0;
+    q.phase1 = true;
+}
+
+// One iteration of the walk: phase 1 leaves solid voxels the ray starts in
+// (src/raycast.rs:108-115), phase 2 returns the first solid voxel. The iteration tests the
+// coarsest level first and leaves the largest empty aligned cell (the reference's levels,
+// shaders/raymarch.wgsl:4626-4775, plus exact 2x2x2 groups of empty children). The words of
+// the current L4 cell, chunk, tile and brick stay in registers until the walk leaves them.
+// Returns true when the walk has ended (h then holds the hit, or stays a miss). COUNT
+// tallies iteration classes for algorithmic-byte accounting.
+template <bool COUNT>
+__device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld &w, float tmax,
+                                          uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
+    if (r.t_cur > tmax) return true;
+    if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
+        (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
+        return true;
+    int32_t sl[3] = {r.l[0], r.l[1], r.l[2]};
+    if (w.fold) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t s = r.l[k] + w.ro_mod[k];
+            sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
+        }
+    }
+    int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
+    do {
+        const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
+                             (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
+        if (l4i != q.cur_l4) {
+            q.cur_l4 = l4i;
+            q.l4w = w.l4[l4i];
+        }
+        if (q.l4w == 0) {
+            if (COUNT) cnt[VG_IT_L4_SKIP]++;
+            q.phase1 = false;
+            S = 256;
+            break;
+        }
+        if (!((q.l4w >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
+            q.phase1 = false;
+            const bool grp = !(q.l4w & group_mask(sl, 7));
+            if (COUNT) cnt[grp ? VG_IT_CHUNK_GROUP_SKIP : VG_IT_CHUNK_SKIP]++;
+            S = grp ? 128 : 64;
+            break;
+        }
+        const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                            (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+        if (ci != q.cur_c) {
+            q.cur_c = ci;
+            q.cw = w.chunk[ci];
+        }
+        const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
+        if (!((q.cw >> tb) & 1)) {
+            q.phase1 = false;
+            const bool grp = !(q.cw & group_mask(sl, 5));
+            if (COUNT) cnt[grp ? VG_IT_TILE_GROUP_SKIP : VG_IT_TILE_SKIP]++;
+            S = grp ? 32 : 16;
+            break;
+        }
+        const uint32_t tn = ci * 64u + (uint32_t)tb;
+        if (tn != q.cur_t) {
+            q.cur_t = tn;
+            q.tw = w.tile[tn];
+            q.tu = w.tile_uni[tn];
+        }
+        if (lod && !q.phase1 && r.t_cur > TILE_LOD_T) {
+            if (COUNT) cnt[VG_IT_TILE_LOD]++;
+            if (q.tw) {
+                const uint32_t fb = tn * 64u + (uint32_t)__builtin_ctzll(q.tw);
+                uint32_t lm = topmost_material(w, fb, w.occ[fb]);
+                if (lm != 0u && uniform_optimisable(lm)) {
+                    emit(r, w, lm, h);
+                    return true;
+                }
+            }
+        }
+        if (q.tu != 0u) {
+            if (COUNT) cnt[VG_IT_TILE_UNIFORM]++;
+            if (solid_mat(q.tu, flags)) {
+                if (!q.phase1) {
+                    emit(r, w, q.tu, h);
+                    return true;
+                }
+            } else {
+                q.phase1 = false;
+            }
+            S = 16;
+            break;
+        }
+        const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
+        if (!((q.tw >> bb) & 1)) {
+            q.phase1 = false;
+            // inside an occupied tile a tile-LOD test may fire at any visited voxel past
+            // t 520, so a group skip there must end before t 520: its exit lies within
+            // the 8^3 cell's diagonal (< 14) of the current voxel's entry
+            const bool grp = !(q.tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T);
+            if (COUNT) cnt[grp ? VG_IT_BRICK_GROUP_SKIP : VG_IT_BRICK_SKIP]++;
+            S = grp ? 8 : 4;
+            break;
+        }
+        const uint32_t bn = tn * 64u + (uint32_t)bb;
+        if (bn != q.cur_b) {
+            q.cur_b = bn;
+            q.ow = w.occ[bn];
+            q.bu = w.brick_uni[bn];
+        }
+        if (q.bu != 0u) {
+            if (COUNT) cnt[VG_IT_BRICK_UNIFORM]++;
+            if (solid_mat(q.bu, flags)) {
+                if (!q.phase1) {
+                    emit(r, w, q.bu, h);
+                    return true;
+                }
+            } else {
+                q.phase1 = false;
+            }
+            S = 4;
+            break;
+        }
+        if (lod && !q.phase1 && r.t_cur > LOD_BRICK_T) {
+            if (COUNT) cnt[VG_IT_BRICK_LOD]++;
+            uint32_t m = topmost_material(w, bn, q.ow);
+            if (!is_foliage(m)) {
+                emit(r, w, m, h);
+                return true;
+            }
+        }
+        const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
+        if ((q.ow >> vb) & 1) {
+            if (COUNT) cnt[VG_IT_VOXEL]++;
+            uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+            if (solid_mat(m, flags)) {
+                if (!q.phase1) {
+                    emit(r, w, m, h);
+                    return true;
+                }
+                break; // phase 1: step out of the solid voxel
+            }
+            q.phase1 = false;
+            break;
+        }
+        q.phase1 = false;
+        // an empty 2x2x2 voxel octant is left whole; inside an occupied brick a brick-LOD
+        // test may fire at any visited voxel past t 400, so the octant (diagonal < 4)
+        // must be left before it
+        const bool grp = !(q.ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T);
+        if (COUNT) cnt[grp ? VG_IT_VOXEL_GROUP_SKIP : VG_IT_VOXEL]++;
+        S = grp ? 2 : 1;
+    } while (false);
+    walk_advance(r, sl, S);
+    return false;
+}
+
 template <bool COUNT>
 __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax, uint32_t flags,
                                          HitRec &h, uint32_t *cnt) {
     const bool lod = (flags & VG_RAY_LOD) && !(flags & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
-    bool phase1 = true;
-    uint32_t cur_l4 = ~0u, cur_c = ~0u, cur_t = ~0u, cur_b = ~0u;
-    uint64_t l4w = 0, cw = 0, tw = 0, ow = 0;
-    uint32_t tu = 0, bu = 0;
+    WalkCache q;
+    cache_reset(q);
     // every iteration crosses at least one voxel plane inside the window
     const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
-    for (int it = 0; it < max_iter; ++it) {
-        if (r.t_cur > tmax) return;
-        if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
-            (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
-            return;
-        int32_t sl[3] = {r.l[0], r.l[1], r.l[2]};
-        if (w.fold) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                int32_t s = r.l[k] + w.ro_mod[k];
-                sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
-            }
-        }
-        int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
-        do {
-            const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
-                                 (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
-            if (l4i != cur_l4) {
-                cur_l4 = l4i;
-                l4w = w.l4[l4i];
-            }
-            if (l4w == 0) {
-                if (COUNT) cnt[VG_IT_L4_SKIP]++;
-                phase1 = false;
-                S = 256;
-                break;
-            }
-            if (!((l4w >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
-                phase1 = false;
-                const bool grp = !(l4w & group_mask(sl, 7));
-                if (COUNT) cnt[grp ? VG_IT_CHUNK_GROUP_SKIP : VG_IT_CHUNK_SKIP]++;
-                S = grp ? 128 : 64;
-                break;
-            }
-            const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
-                                (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
-            if (ci != cur_c) {
-                cur_c = ci;
-                cw = w.chunk[ci];
-            }
-            const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
-            if (!((cw >> tb) & 1)) {
-                phase1 = false;
-                const bool grp = !(cw & group_mask(sl, 5));
-                if (COUNT) cnt[grp ? VG_IT_TILE_GROUP_SKIP : VG_IT_TILE_SKIP]++;
-                S = grp ? 32 : 16;
-                break;
-            }
-            const uint32_t tn = ci * 64u + (uint32_t)tb;
-            if (tn != cur_t) {
-                cur_t = tn;
-                tw = w.tile[tn];
-                tu = w.tile_uni[tn];
-            }
-            if (lod && !phase1 && r.t_cur > TILE_LOD_T) {
-                if (COUNT) cnt[VG_IT_TILE_LOD]++;
-                if (tw) {
-                    const uint32_t fb = tn * 64u + (uint32_t)__builtin_ctzll(tw);
-                    uint32_t lm = topmost_material(w, fb, w.occ[fb]);
-                    if (lm != 0u && uniform_optimisable(lm)) {
-                        emit(r, w, lm, h);
-                        return;
-                    }
-                }
-            }
-            if (tu != 0u) {
-                if (COUNT) cnt[VG_IT_TILE_UNIFORM]++;
-                if (solid_mat(tu, flags)) {
-                    if (!phase1) {
-                        emit(r, w, tu, h);
-                        return;
-                    }
-                } else {
-                    phase1 = false;
-                }
-                S = 16;
-                break;
-            }
-            const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
-            if (!((tw >> bb) & 1)) {
-                phase1 = false;
-                // inside an occupied tile a tile-LOD test may fire at any visited voxel past
-                // t 520, so a group skip there must end before t 520: its exit lies within
-                // the 8^3 cell's diagonal (< 14) of the current voxel's entry
-                const bool grp = !(tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T);
-                if (COUNT) cnt[grp ? VG_IT_BRICK_GROUP_SKIP : VG_IT_BRICK_SKIP]++;
-                S = grp ? 8 : 4;
-                break;
-            }
-            const uint32_t bn = tn * 64u + (uint32_t)bb;
-            if (bn != cur_b) {
-                cur_b = bn;
-                ow = w.occ[bn];
-                bu = w.brick_uni[bn];
-            }
-            if (bu != 0u) {
-                if (COUNT) cnt[VG_IT_BRICK_UNIFORM]++;
-                if (solid_mat(bu, flags)) {
-                    if (!phase1) {
-                        emit(r, w, bu, h);
-                        return;
-                    }
-                } else {
-                    phase1 = false;
-                }
-                S = 4;
-                break;
-            }
-            if (lod && !phase1 && r.t_cur > LOD_BRICK_T) {
-                if (COUNT) cnt[VG_IT_BRICK_LOD]++;
-                uint32_t m = topmost_material(w, bn, ow);
-                if (!is_foliage(m)) {
-                    emit(r, w, m, h);
-                    return;
-                }
-            }
-            const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
-            if ((ow >> vb) & 1) {
-                if (COUNT) cnt[VG_IT_VOXEL]++;
-                uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
-                if (solid_mat(m, flags)) {
-                    if (!phase1) {
-                        emit(r, w, m, h);
-                        return;
-                    }
-                    break; // phase 1: step out of the solid voxel
-                }
-                phase1 = false;
-                break;
-            }
-            phase1 = false;
-            // an empty 2x2x2 voxel octant is left whole; inside an occupied brick a brick-LOD
-            // test may fire at any visited voxel past t 400, so the octant (diagonal < 4)
-            // must be left before it
-            const bool grp = !(ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T);
-            if (COUNT) cnt[grp ? VG_IT_VOXEL_GROUP_SKIP : VG_IT_VOXEL]++;
-            S = grp ? 2 : 1;
-        } while (false);
-        walk_advance(r, sl, S);
-    }
+    for (int it = 0; it < max_iter; ++it)
+        if (walk_iter<COUNT>(r, q, w, tmax, flags, lod, h, cnt)) return;
 }
 
 // glam normalize_or_zero: d * (1 / sqrt(dot)); false when that is not a finite positive
@@ -530,11 +556,12 @@ __device__ __forceinline__ void flush_counts(const uint32_t *cnt, unsigned long 
 // A wave is one 8x8 pixel block whose 64 rays share the camera origin O, so on [t0, t1]
 // they all lie in O + [min d, max d] * [t0, t1] (per axis, min / max over the wave). The
 // wave marches that box in 16-voxel steps, grown by one voxel, each lane testing one world
-// tile the box touches inside the window (at most 64) against the chunk masks; the block
+// tile the box touches inside the window (a 4x4x4 cube at most) against the chunk masks; the block
 // starts at the first step whose box touches an occupied tile. Every voxel a block ray
 // visits before that t is empty, so walk_advance_to may jump there exactly. Every lane of
 // the wave must call this (it reduces and votes across the wave).
 constexpr float kBeamStep = 16.0f;
+constexpr size_t kBeamDefaultBlocks = 512 * 512; // 8x8 blocks of a 4096x4096 image
 constexpr int kBeamMaxSteps = 64;
 
 template <bool COUNT>
@@ -579,18 +606,26 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
             n[k] = th >= tl ? th - tl + 1 : 0;
             total *= n[k];
         }
-        if (total > 64) break;
+        if (total == 0) {
+            // the box left the window: every block ray has exited it for good (the window is
+            // convex), so nothing of the window remains to visit
+            t0 = t1;
+            break;
+        }
+        if (n[0] > 4 || n[1] > 4 || n[2] > 4) break; // wider than the wave's 4x4x4 tile cube
+        // lane -> one tile of a 4x4x4 cube at lo
+        const int32_t ix = lane & 3, iz = (lane >> 2) & 3, iy = lane >> 4;
         bool occ = false;
-        if (lane < total) {
+        if (ix < n[0] && iy < n[1] && iz < n[2]) {
             if (COUNT) cnt[VG_IT_BEAM_TEST]++;
-            const int32_t tx = lo[0] + lane % n[0];
-            const int32_t ty = lo[1] + (lane / n[0]) % n[1];
-            const int32_t tz = lo[2] + lane / (n[0] * n[1]);
-            int32_t sl[3] = {tx * 16, ty * 16, tz * 16};
+            const int32_t tw[3] = {lo[0] + ix, lo[1] + iy, lo[2] + iz};
+            int32_t sl[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                int32_t m = sl[k] % w.dims[k];
-                sl[k] = m < 0 ? m + w.dims[k] : m;
+                // tile start, window-local (>= -15), folded into its slot
+                int32_t v = tw[k] * 16 - w.origin[k] + w.ro_mod[k];
+                v = v < 0 ? v + w.dims[k] : v;
+                sl[k] = v >= w.dims[k] ? v - w.dims[k] : v;
             }
             const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
                                 (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
@@ -692,6 +727,220 @@ __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams ca
     if (COUNT) flush_counts(cnt, counts);
 }
 
+// Lanes of `mask` below this lane (the lane's rank among the mask's lanes).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Beam pre-pass of the persistent frame: one wave per 8x8 pixel block writes the block's
+// exact start t (beam_start). Its first thread also zeroes the frame's work counter, which
+// the persistent kernel enqueued after it on the same stream consumes.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint32_t nbx,
+                                                 uint32_t nblocks, float ptmax,
+                                                 float *__restrict__ beam_t, uint32_t *work,
+                                                 unsigned long long *counts) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0u;
+    const uint32_t blk = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (blk >= nblocks) return; // whole waves leave together
+    const uint32_t lane = threadIdx.x & 63;
+    const int32_t px = (int32_t)((blk % nbx) * 8 + (lane & 7));
+    const int32_t py = (int32_t)((blk / nbx) * 8 + (lane >> 3));
+    uint32_t cnt[VG_IT_COUNT];
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    float d[3];
+    primary_dir(cam, px, py, d);
+    const float tb = beam_start<COUNT>(w, cam, d, ptmax, cnt);
+    if (lane == 0) beam_t[blk] = tb;
+    if (COUNT) flush_counts(cnt, counts);
+}
+
+__global__ void k_reset_work(uint32_t *work) { *work = 0u; }
+
+// Persistent, ray-compacting frame kernel. Every wave keeps its 64 lanes busy: whenever
+// fewer than `threshold` lanes hold a live ray, the wave takes as many new primary rays as
+// it has idle lanes from one global counter (one atomic per wave) and hands them out by
+// lane rank. Ray ids run over 8x8 pixel blocks, so a refill stays spatially coherent. A
+// primary hit becomes that lane's shadow ray at once. The grid is sized to the machine
+// (waves per CU chosen by the autotuner), not to the image.
+template <bool COUNT, bool SHADOW, bool BEAM>
+__global__ __launch_bounds__(kBlock) void k_frame_persistent(
+    DevWorld w, CamParams cam, int32_t width, int32_t height, uint32_t nbx, uint32_t total_ids,
+    float ptmax, uint32_t pflags, float sx, float sy, float sz, float stmax,
+    const float *__restrict__ beam_t, vg_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
+    uint32_t *work, int threshold, unsigned long long *counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cnt[VG_IT_COUNT];
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    const bool plod =
+        (pflags & VG_RAY_LOD) && !(pflags & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    int kind = 0; // 0 idle, 1 primary ray, 2 shadow ray
+    int64_t pix = 0;
+    int32_t px = 0, py = 0;
+    int it = 0;
+    Walk r;
+    WalkCache q;
+    HitRec h;
+    hit_miss(h);
+    cache_reset(q);
+    bool exhausted = false;
+    for (;;) {
+        uint64_t act = __ballot(kind != 0);
+        int nact = __popcll(act);
+        if (!exhausted && nact < threshold) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
+            base = __shfl(base, 0, 64);
+            if (base >= total_ids) {
+                exhausted = true;
+            } else if (kind == 0) {
+                const uint32_t id = base + lanes_below(~act);
+                if (id < total_ids) {
+                    const uint32_t blk = id >> 6, qd = id & 63u;
+                    px = (int32_t)((blk % nbx) * 8 + (qd & 7));
+                    py = (int32_t)((blk / nbx) * 8 + (qd >> 3));
+                    if (px < width && py < height) {
+                        pix = (int64_t)py * width + px;
+                        float d[3];
+                        primary_dir(cam, px, py, d);
+                        hit_miss(h);
+                        if (walk_init(r, w, cam.origin, d)) {
+                            if (BEAM) {
+                                const float tb = beam_t[blk];
+                                if (tb > 0.0f && r.t_cur <= ptmax) walk_advance_to(r, tb);
+                            }
+                            cache_reset(q);
+                            it = 0;
+                            kind = 1;
+                        } else {
+                            store_hit(hits + pix, h);
+                            if (SHADOW) occluded[pix] = 0;
+                        }
+                    }
+                }
+            }
+            nact = __popcll(__ballot(kind != 0));
+        }
+        if (nact == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        do {
+            if (kind != 0) {
+                bool done = kind == 1 ? walk_iter<COUNT>(r, q, w, ptmax, pflags, plod, h, cnt)
+                                      : walk_iter<COUNT>(r, q, w, stmax, VG_RAY_ANY_HIT, false, h, cnt);
+                done = done || ++it >= max_iter;
+                if (done) {
+                    if (kind == 1) {
+                        store_hit(hits + pix, h);
+                        kind = 0;
+                        if (SHADOW) {
+                            bool started = false;
+                            if (h.hit) {
+                                float d[3], so[3], sd[3] = {sx, sy, sz};
+                                primary_dir(cam, px, py, d);
+                                shadow_origin(cam.origin, d, h, so);
+                                hit_miss(h);
+                                if (walk_init(r, w, so, sd)) {
+                                    cache_reset(q);
+                                    it = 0;
+                                    kind = 2;
+                                    started = true;
+                                }
+                            }
+                            if (!started) occluded[pix] = 0;
+                        }
+                    } else {
+                        occluded[pix] = (uint8_t)h.hit;
+                        kind = 0;
+                    }
+                }
+            }
+        } while (__popcll(__ballot(kind != 0)) >= threshold);
+    }
+    if (COUNT) flush_counts(cnt, counts);
+}
+
+// Persistent, ray-compacting batched query: the same refill discipline over a vg_ray array.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
+                                                            const vg_ray *__restrict__ rays,
+                                                            uint32_t n, uint32_t flags,
+                                                            vg_hit *__restrict__ hits,
+                                                            uint32_t *work, int threshold,
+                                                            unsigned long long *counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cnt[VG_IT_COUNT];
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    bool busy = false;
+    uint32_t id = 0, f = 0;
+    float tmax = 0.0f;
+    bool lod = false;
+    int it = 0;
+    Walk r;
+    WalkCache q;
+    HitRec h;
+    hit_miss(h);
+    cache_reset(q);
+    bool exhausted = false;
+    for (;;) {
+        uint64_t act = __ballot(busy);
+        int nact = __popcll(act);
+        if (!exhausted && nact < threshold) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
+            base = __shfl(base, 0, 64);
+            if (base >= n) {
+                exhausted = true;
+            } else if (!busy) {
+                id = base + lanes_below(~act);
+                if (id < n) {
+                    const float4 *rp = reinterpret_cast<const float4 *>(rays + id);
+                    const float4 a = rp[0], b = rp[1];
+                    const float o[3] = {a.x, a.y, a.z}, din[3] = {b.x, b.y, b.z};
+                    float d[3];
+                    f = flags | __float_as_uint(b.w);
+                    tmax = a.w;
+                    lod = (f & VG_RAY_LOD) && !(f & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
+                    hit_miss(h);
+                    if (normalize_or_zero(din, d) && walk_init(r, w, o, d)) {
+                        cache_reset(q);
+                        it = 0;
+                        busy = true;
+                    } else {
+                        store_hit(hits + id, h);
+                    }
+                }
+            }
+            nact = __popcll(__ballot(busy));
+        }
+        if (nact == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        do {
+            if (busy) {
+                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
+                done = done || ++it >= max_iter;
+                if (done) {
+                    store_hit(hits + id, h);
+                    busy = false;
+                }
+            }
+        } while (__popcll(__ballot(busy)) >= threshold);
+    }
+    if (COUNT) flush_counts(cnt, counts);
+}
+
 // Upload relayout: reference-order bricks (72 B AoS) -> nested occupancy + materials.
 __global__ void k_relayout_bricks(const vg_brick *__restrict__ staged,
                                   const uint8_t *__restrict__ bu_ref, int64_t first, int64_t count,
@@ -736,6 +985,12 @@ using namespace vg;
 
 struct vg_ctx {
     int device;
+    int cus;             // compute units of the device
+    int schedule;        // VG_SCHEDULE_*
+    int frame_wpc;       // persistent frame kernel: waves per CU
+    int frame_refill;    // ... refill when fewer lanes of a wave are busy
+    int rays_wpc;        // persistent batched kernel: waves per CU
+    int rays_refill;
 };
 
 struct vg_world {
@@ -747,6 +1002,9 @@ struct vg_world {
     uint8_t *tile_uni, *brick_uni, *mat;
     void *scratch;                   // one ray + one hit for vg_raycast
     unsigned long long *counts_dev;  // VG_IT_COUNT totals for the *_counts entry points
+    uint32_t *work;                  // persistent kernels' ray counter
+    float *beam_t;                   // beam start t per 8x8 pixel block
+    size_t beam_cap;                 // entries of beam_t
 };
 
 namespace vg {
@@ -787,6 +1045,75 @@ static bool host_normalize(const float in[3], float out[3]) {
     return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
 }
 
+static int persistent_grid(const vg_ctx *c, int wpc) {
+    int waves = c->cus * (wpc > 0 ? wpc : 1);
+    return waves / (kBlock / 64) > 0 ? waves / (kBlock / 64) : 1;
+}
+
+static int ensure_beam_capacity(vg_world *w, size_t blocks) {
+    if (blocks <= w->beam_cap) return VG_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    if (w->beam_t) HIP_TRY(hipFree(w->beam_t));
+    w->beam_t = nullptr;
+    w->beam_cap = 0;
+    if (hipMalloc((void **)&w->beam_t, blocks * sizeof(float)) != hipSuccess)
+        return fail(VG_E_NOMEM, "beam buffer of %zu blocks", blocks);
+    w->beam_cap = blocks;
+    return VG_OK;
+}
+
+static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t width, int32_t height,
+                                   float ptmax, uint32_t pflags, const float s[3], float stmax,
+                                   vg_hit *hits, uint8_t *occluded, bool shadow, bool beam,
+                                   bool count, hipStream_t stream) {
+    const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
+    const uint64_t nblocks = (uint64_t)nbx * nby;
+    // the counter may overshoot the ray count by 64 per wave after it runs out
+    if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
+    unsigned long long *cd = count ? w->counts_dev : nullptr;
+    if (beam) {
+        int st = ensure_beam_capacity(w, (size_t)nblocks);
+        if (st) return st;
+        dim3 g((uint32_t)((nblocks + 3) / 4));
+        if (count)
+            hipLaunchKernelGGL((k_beam<true>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
+                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, cd);
+        else
+            hipLaunchKernelGGL((k_beam<false>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
+                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, cd);
+    } else {
+        hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, w->work);
+    }
+    HIP_TRY(hipGetLastError());
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
+    const uint32_t total = (uint32_t)(nblocks * 64);
+    const int thr = w->ctx->frame_refill;
+#define LAUNCH_P(C, SH, B)                                                                     \
+    hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, cp, width, \
+                       height, nbx, total, ptmax, pflags, s[0], s[1], s[2], stmax, w->beam_t,  \
+                       hits, occluded, w->work, thr, cd)
+    if (count) {
+        if (shadow) {
+            if (beam) LAUNCH_P(true, true, true);
+            else LAUNCH_P(true, true, false);
+        } else {
+            if (beam) LAUNCH_P(true, false, true);
+            else LAUNCH_P(true, false, false);
+        }
+    } else {
+        if (shadow) {
+            if (beam) LAUNCH_P(false, true, true);
+            else LAUNCH_P(false, true, false);
+        } else {
+            if (beam) LAUNCH_P(false, false, true);
+            else LAUNCH_P(false, false, false);
+        }
+    }
+#undef LAUNCH_P
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
+}
+
 static int launch_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
                         float ptmax, uint32_t pflags, const float sun[3], float stmax,
                         vg_hit *hits, uint8_t *occluded, int mode, bool count,
@@ -802,12 +1129,15 @@ static int launch_frame(vg_world *w, const vg_camera *camera, int32_t width, int
         if (!sun || !occluded) return fail(VG_E_INVALID, "null sun/occluded");
         if (!host_normalize(sun, s)) s[0] = s[1] = s[2] = 0.0f;
     }
-    const uint32_t nbx = (uint32_t)((width + 15) / 16), nby = (uint32_t)((height + 15) / 16);
-    const uint32_t nb = nbx * nby;
     CamParams cp = cam_params(camera);
-    dim3 grid(nb), block(kBlock);
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     const bool beam = (pflags & VG_RAY_BEAM) != 0;
+    if (w->ctx->schedule == VG_SCHEDULE_PERSISTENT && mode != 2)
+        return launch_frame_persistent(w, cp, width, height, ptmax, pflags, s, stmax, hits,
+                                       occluded, mode == 1, beam, count, stream);
+    const uint32_t nbx = (uint32_t)((width + 15) / 16), nby = (uint32_t)((height + 15) / 16);
+    const uint32_t nb = nbx * nby;
+    dim3 grid(nb), block(kBlock);
 #define LAUNCH_F(C, M, B)                                                                     \
     hipLaunchKernelGGL((k_trace_frame<C, M, B>), grid, block, 0, stream, w->dw, cp, width,      \
                        height, nbx, ptmax, pflags, s[0], s[1], s[2], stmax, hits, occluded, cd)
@@ -845,8 +1175,22 @@ static int launch_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags
     if (!rays || !hits) return fail(VG_E_INVALID, "null rays/hits");
     if (((uintptr_t)rays & 15u) != 0) return fail(VG_E_INVALID, "rays must be 16-byte aligned");
     if (((uintptr_t)hits & 3u) != 0) return fail(VG_E_INVALID, "hits must be 4-byte aligned");
+    // the counter may overshoot the ray count by 64 per wave after it runs out
+    if (n > 0xF0000000ull) return fail(VG_E_INVALID, "too many rays");
+    if (w->ctx->schedule == VG_SCHEDULE_PERSISTENT) {
+        hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, w->work);
+        const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
+        if (count)
+            hipLaunchKernelGGL((k_rays_persistent<true>), grid, block, 0, stream, w->dw, rays,
+                               (uint32_t)n, flags, hits, w->work, w->ctx->rays_refill,
+                               w->counts_dev);
+        else
+            hipLaunchKernelGGL((k_rays_persistent<false>), grid, block, 0, stream, w->dw, rays,
+                               (uint32_t)n, flags, hits, w->work, w->ctx->rays_refill, nullptr);
+        HIP_TRY(hipGetLastError());
+        return VG_OK;
+    }
     int64_t nb = ((int64_t)n + kBlock - 1) / kBlock;
-    if (nb > 0x7FFFFFFF) return fail(VG_E_INVALID, "too many rays");
     if (count)
         hipLaunchKernelGGL((k_trace_rays<true>), dim3((uint32_t)nb), dim3(kBlock), 0, stream,
                            w->dw, rays, (int64_t)n, flags, hits, w->counts_dev);
@@ -858,8 +1202,8 @@ static int launch_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags
 }
 
 static void free_world(vg_world *w) {
-    void *ptrs[] = {w->l4, w->chunk, w->tile, w->occ, w->tile_uni, w->brick_uni, w->mat, w->scratch,
-                    w->counts_dev};
+    void *ptrs[] = {w->l4,      w->chunk,   w->tile,       w->occ,  w->tile_uni, w->brick_uni,
+                    w->mat,     w->scratch, w->counts_dev, w->work, w->beam_t};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -876,9 +1220,17 @@ int vg_ctx_create(int device, vg_ctx **out) {
         return fail(VG_E_DEVICE, "no HIP device (%s)", hipGetErrorString(e));
     if (device < 0 || device >= n) return fail(VG_E_INVALID, "device %d of %d", device, n);
     HIP_TRY(hipSetDevice(device));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     vg_ctx *c = new (std::nothrow) vg_ctx;
     if (!c) return fail(VG_E_NOMEM, "ctx");
     c->device = device;
+    c->cus = cus > 0 ? cus : 1;
+    c->schedule = VG_SCHEDULE_PERSISTENT;
+    c->frame_wpc = 16;
+    c->frame_refill = 32;
+    c->rays_wpc = 16;
+    c->rays_refill = 32;
     *out = c;
     return VG_OK;
 }
@@ -912,6 +1264,8 @@ int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
         {(void **)&w->mat, (size_t)d.n_bricks * 64},
         {(void **)&w->scratch, 256},
         {(void **)&w->counts_dev, VG_IT_COUNT * sizeof(unsigned long long)},
+        {(void **)&w->work, 256},
+        {(void **)&w->beam_t, (size_t)kBeamDefaultBlocks * sizeof(float)},
     };
     for (auto &a : allocs) {
         if (hipMalloc(a.p, a.bytes) != hipSuccess) {
@@ -920,6 +1274,7 @@ int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
             return fail(VG_E_NOMEM, "hipMalloc(%zu) failed", a.bytes);
         }
     }
+    w->beam_cap = kBeamDefaultBlocks;
     DevWorld &dw = w->dw;
     dw.l4 = w->l4;
     dw.chunk = w->chunk;
@@ -1212,6 +1567,124 @@ int vg_event_elapsed_ms(vg_ctx *ctx, void *start, void *end, float *ms) {
     HIP_TRY(hipEventSynchronize((hipEvent_t)end));
     HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
     return VG_OK;
+}
+
+} // extern "C"
+
+namespace vg {
+// Time `launch` (already bound to its arguments) over every (waves per CU, refill) pair and
+// keep the fastest in the context. Each candidate runs once to warm up, then `reps` times
+// between events on one stream; its score is the fastest repetition.
+template <class L>
+static int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms) {
+    static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
+    static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
+    const int reps = 3;
+    hipStream_t stream;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int old_schedule = c->schedule;
+    c->schedule = VG_SCHEDULE_PERSISTENT;
+    float best = INFINITY;
+    int bw = *wpc_opt, br = *refill_opt, st = VG_OK;
+    for (int wpc : kWpc) {
+        for (int rf : kRefill) {
+            *wpc_opt = wpc;
+            *refill_opt = rf;
+            st = launch(stream);
+            if (st) break;
+            float ms_min = INFINITY;
+            for (int i = 0; i < reps && st == VG_OK; ++i) {
+                float ms = 0.0f;
+                if (hipEventRecord(e0, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
+                if (!st) st = launch(stream);
+                if (!st && hipEventRecord(e1, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
+                if (!st && hipEventSynchronize(e1) != hipSuccess) st = fail(VG_E_DEVICE, "sync");
+                if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = fail(VG_E_DEVICE, "elapsed");
+                ms_min = ms < ms_min ? ms : ms_min;
+            }
+            if (st) break;
+            if (ms_min < best) {
+                best = ms_min;
+                bw = wpc;
+                br = rf;
+            }
+        }
+        if (st) break;
+    }
+    *wpc_opt = bw;
+    *refill_opt = br;
+    c->schedule = old_schedule;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(stream);
+    if (st) return st;
+    if (best_ms) *best_ms = best;
+    return VG_OK;
+}
+
+} // namespace vg
+
+extern "C" {
+
+int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
+    if (!ctx) return fail(VG_E_INVALID, "null ctx");
+    switch (option) {
+    case VG_OPT_SCHEDULE:
+        if (value != VG_SCHEDULE_PERSISTENT && value != VG_SCHEDULE_STATIC)
+            return fail(VG_E_INVALID, "schedule %d", value);
+        ctx->schedule = value;
+        return VG_OK;
+    case VG_OPT_FRAME_WAVES_PER_CU:
+    case VG_OPT_RAYS_WAVES_PER_CU:
+        if (value < 1 || value > 64) return fail(VG_E_INVALID, "waves per CU %d not in 1..64", value);
+        (option == VG_OPT_FRAME_WAVES_PER_CU ? ctx->frame_wpc : ctx->rays_wpc) = value;
+        return VG_OK;
+    case VG_OPT_FRAME_REFILL:
+    case VG_OPT_RAYS_REFILL:
+        if (value < 1 || value > 64) return fail(VG_E_INVALID, "refill threshold %d not in 1..64", value);
+        (option == VG_OPT_FRAME_REFILL ? ctx->frame_refill : ctx->rays_refill) = value;
+        return VG_OK;
+    default:
+        return fail(VG_E_INVALID, "option %d cannot be set", option);
+    }
+}
+
+int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
+    if (!ctx || !value) return fail(VG_E_INVALID, "null ctx/value");
+    switch (option) {
+    case VG_OPT_SCHEDULE: *value = ctx->schedule; return VG_OK;
+    case VG_OPT_FRAME_WAVES_PER_CU: *value = ctx->frame_wpc; return VG_OK;
+    case VG_OPT_FRAME_REFILL: *value = ctx->frame_refill; return VG_OK;
+    case VG_OPT_RAYS_WAVES_PER_CU: *value = ctx->rays_wpc; return VG_OK;
+    case VG_OPT_RAYS_REFILL: *value = ctx->rays_refill; return VG_OK;
+    case VG_OPT_CU_COUNT: *value = ctx->cus; return VG_OK;
+    default: return fail(VG_E_INVALID, "unknown option %d", option);
+    }
+}
+
+int vg_autotune_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                      float primary_tmax, uint32_t primary_flags, const float sun[3],
+                      float shadow_tmax, vg_hit *hits, uint8_t *occluded, float *best_ms) {
+    int st = check_world(w);
+    if (st) return st;
+    vg_ctx *c = w->ctx;
+    return autotune(c, &c->frame_wpc, &c->frame_refill, [&](hipStream_t s) {
+        return vg_trace_frame(w, camera, width, height, primary_tmax, primary_flags, sun,
+                              shadow_tmax, hits, occluded, (void *)s);
+    }, best_ms);
+}
+
+int vg_autotune_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
+                     float *best_ms) {
+    int st = check_world(w);
+    if (st) return st;
+    vg_ctx *c = w->ctx;
+    return autotune(c, &c->rays_wpc, &c->rays_refill, [&](hipStream_t s) {
+        return vg_trace_rays(w, rays, n, flags, hits, (void *)s);
+    }, best_ms);
 }
 
 } // extern "C"

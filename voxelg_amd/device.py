@@ -174,6 +174,15 @@ class Context:
     def world(self, dims) -> "DeviceWorld":
         return DeviceWorld(self, dims)
 
+    def set_option(self, option: int, value: int) -> "Context":
+        check(lib().vg_ctx_set_option(self._h, option, int(value)), "vg_ctx_set_option")
+        return self
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int()
+        check(lib().vg_ctx_get_option(self._h, option, C.byref(v)), "vg_ctx_get_option")
+        return int(v.value)
+
     def close(self):
         if self._h:
             lib().vg_ctx_destroy(self._h)
@@ -282,6 +291,30 @@ class DeviceWorld:
                                           hits.data_ptr(), occluded.data_ptr(), counts),
               "vg_trace_frame_counts")
         return hits, occluded, dict(zip(IT_NAMES, (int(c) for c in counts)))
+
+    def autotune_frame(self, camera: np.ndarray, width: int, height: int, primary_tmax: float,
+                       primary_flags: int, sun, shadow_tmax: float) -> float:
+        """Time every (waves per CU, refill) pair of the persistent frame kernel on this frame
+        and keep the fastest in the context; returns its time in ms."""
+        cam = np.ascontiguousarray(camera, np.float32)
+        hits = self._buf(width * height * 20)
+        occ = self._buf(width * height)
+        ms = C.c_float()
+        check(lib().vg_autotune_frame(self._h, ptr(cam), width, height, primary_tmax,
+                                      primary_flags, f32x3(sun), shadow_tmax, hits.data_ptr(),
+                                      occ.data_ptr(), C.byref(ms)), "vg_autotune_frame")
+        return float(ms.value)
+
+    def autotune_rays(self, rays, n: int | None = None, flags: int = 0) -> float:
+        if isinstance(rays, np.ndarray):
+            n = len(rays) if n is None else n
+            rays = self.ctx.rays(rays)
+        n = rays.nbytes // 32 if n is None else int(n)
+        hits = self._buf(n * 20)
+        ms = C.c_float()
+        check(lib().vg_autotune_rays(self._h, rays.data_ptr(), n, flags, hits.data_ptr(),
+                                     C.byref(ms)), "vg_autotune_rays")
+        return float(ms.value)
 
     def raycast_hit(self, camera_world, direction) -> np.void:
         out = np.zeros(1, HIT_DTYPE)
