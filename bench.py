@@ -6,6 +6,7 @@ LOD past 520) through the seeded benchmark world, then one any-hit shadow ray to
 is generated on the host, uploaded once, and resident in HBM before the timed region.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py --workload batched   # uneven batched queries (96 batches per step)
 
 Prints ONE JSON line on stdout; details go to stderr.
 """
@@ -59,6 +60,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["frame", "batched"], default="frame")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (a plain copy kernel "
                          "of known bytes, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -68,6 +70,9 @@ def main():
               "MI355X (rays are independent; see DESIGN.md for how frames would shard).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        return bench_batched(args)
 
     import numpy as np
 
@@ -185,6 +190,97 @@ def main():
             "kernel": "k_beam + k_frame_persistent", "kernel_ms": round(avg_kern_s * 1e3, 4),
             "algorithmic_bytes": int(bytes_per_frame),
         },
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """One step = 96 uneven batches (sizes log-uniform in [256, 262144]) of mixed primary,
+    shadow and AO rays on the 256x128x256 world, replayed as one captured HIP graph."""
+    import numpy as np
+
+    import voxelg_amd as vg
+    from voxelg_amd import _lib as L
+    from voxelg_amd import workloads
+    from voxelg_amd._lib import (HIT_RECORD_BYTES, MATERIAL_BYTES, RAY_INPUT_BYTES,
+                                 algorithmic_bytes)
+    from voxelg_amd.device import Context, DevicePtr, Graph, decode_hits, device_count
+
+    assert device_count() > 0, "bench.py needs a HIP device"
+    wl = workloads.BATCHED
+    world = wl.world()
+    rays, sizes = wl.rays(world)
+    ctx = Context(0)
+    dw = ctx.world(world.dims).upload(world)
+    rbuf = ctx.rays(rays)
+    hbuf = ctx.alloc(len(rays) * 20)
+    batches, off = [], 0
+    for n in sizes:
+        batches.append((DevicePtr(rbuf, off * 32), DevicePtr(hbuf, off * 20), n, 0))
+        off += n
+    bs = dw.batch_set(batches)
+    tuned = dw.autotune_batches(bs)
+    sched = {"waves_per_cu": ctx.get_option(L.VG_OPT_RAYS_WAVES_PER_CU),
+             "refill_below": ctx.get_option(L.VG_OPT_RAYS_REFILL),
+             "cus": ctx.get_option(L.VG_OPT_CU_COUNT)}
+    log(f"{bs.n} batches, {bs.total} rays; autotuned {sched} ({tuned:.3f} ms)")
+    counts = dw.trace_batches_counts(bs)
+    hits = decode_hits(hbuf)
+    n_hit = int(hits["hit"].sum())
+    bytes_per_step = (algorithmic_bytes(counts) + bs.total * (RAY_INPUT_BYTES + HIT_RECORD_BYTES)
+                      + n_hit * MATERIAL_BYTES)
+    stream = ctx.stream()
+    graph = Graph(ctx, stream, lambda st: dw.trace_batches(bs, st))
+    for _ in range(args.warmup):
+        graph.launch(stream)
+    stream.synchronize()
+    ev = [(ctx.event(), ctx.event()) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        graph.launch(stream)
+        b.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_ms(b) for a, b in ev]
+    value = bs.total * args.steps / wall / 1e6
+    achieved = bytes_per_step / (float(np.mean(kern_ms)) / 1e3) / 1e9
+    log(f"step {wall / args.steps * 1e3:.3f} ms wall, graph {np.mean(kern_ms):.3f} ms, "
+        f"{value:.0f} Mrays/s, {n_hit} hits")
+    base = None
+    if not args.no_cpu_baseline:
+        import oracle
+        ow = oracle.OracleWorld.view(world)
+        threads = min(os.cpu_count() or 1, 256)
+        # whole steps, repeated until about 10 s of CPU work
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 1 or time.perf_counter() - t0 < 10.0:
+            ow.raycast_rays(rays, threads)
+            reps += 1
+        dt = time.perf_counter() - t0
+        base = {"value": len(rays) * reps / dt / 1e6, "unit": "Mrays/s", "cores": threads,
+                "kind": "port",
+                "sample": f"{reps} full steps ({len(rays) * reps} rays), R0 = per-voxel DDA of "
+                          f"src/raycast.rs, {dt:.1f} s"}
+        log(f"cpu baseline: {base}")
+    out = {
+        "metric": "Mrays/s of uneven batched ray queries",
+        "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded world; seeded primary / shadow / AO rays)",
+        "config": {"workload": "batched: 96 batches/step, sizes log-uniform 256..262144, 40% "
+                               "coherent primary / 30% shadow / 30% cosine AO, tmax in "
+                               "{32,128,512,1024}, 256x128x256 world, one HIP graph per step",
+                   "world": list(world.dims), "rays": bs.total, "batches": bs.n,
+                   "schedule": dict(sched, kernel="persistent ray-compacting (batches)")},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_batches_persistent",
+                     "kernel_ms": round(float(np.mean(kern_ms)), 4),
+                     "algorithmic_bytes": int(bytes_per_step)},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

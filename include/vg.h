@@ -261,6 +261,35 @@ int vg_autotune_frame(vg_world *world, const vg_camera *camera_host, int32_t wid
 int vg_autotune_rays(vg_world *world, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,
                      float *best_ms);
 
+/* ------------------------------------------------------- uneven batches -- */
+/* Many batched queries at once (device descriptors): batch b traces n rays of `rays` into
+ * `hits` with `flags` OR-ed into each ray. `prefix` (device, n_batches + 1 entries) holds
+ * the exclusive prefix sums of the batch sizes, prefix[n_batches] == total. One persistent
+ * ray-compacting launch serves all batches. */
+typedef struct vg_batch {
+    const vg_ray *rays;
+    vg_hit *hits;
+    uint32_t n;
+    uint32_t flags;
+} vg_batch;
+int vg_trace_batches(vg_world *world, const vg_batch *batches, const uint32_t *prefix,
+                     uint32_t n_batches, uint32_t total, void *stream);
+int vg_trace_batches_counts(vg_world *world, const vg_batch *batches, const uint32_t *prefix,
+                            uint32_t n_batches, uint32_t total,
+                            uint64_t counts_host[VG_IT_COUNT]);
+int vg_autotune_batches(vg_world *world, const vg_batch *batches, const uint32_t *prefix,
+                        uint32_t n_batches, uint32_t total, float *best_ms);
+
+/* ----------------------------------------------------------- HIP graphs -- */
+/* Capture the launches that vg_trace_* calls enqueue on `stream` (not the null stream)
+ * between begin and end into an executable graph, then replay it with one call. Nothing in
+ * a captured sequence allocates or memsets: counters are reset by kernels. */
+typedef struct vg_graph vg_graph;
+int vg_graph_begin(vg_ctx *ctx, void *stream);
+int vg_graph_end(vg_ctx *ctx, void *stream, vg_graph **out);
+int vg_graph_launch(vg_graph *graph, void *stream);
+int vg_graph_destroy(vg_graph *graph);
+
 /* raycast (src/raycast.rs:21) over a world on the device: one ray, host in and out.
  * Returns VG_OK with hit_host->hit = 0/1. Synchronous. */
 int vg_raycast(vg_world *world, const float camera_world[3], const float dir[3],

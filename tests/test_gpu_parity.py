@@ -249,6 +249,49 @@ def test_autotune_keeps_results(ctx, small, small_world):
         ctx.set_option(L.VG_OPT_FRAME_REFILL, 0)
 
 
+def batched_setup(ctx, small, small_world):
+    from voxelg_amd.device import DevicePtr
+    rays, sizes = workloads.BATCHED.rays(small_world)
+    rbuf = ctx.rays(rays)
+    hbuf = ctx.alloc(len(rays) * 20)
+    batches, off = [], 0
+    for n in sizes:
+        batches.append((DevicePtr(rbuf, off * 32), DevicePtr(hbuf, off * 20), n, 0))
+        off += n
+    return rays, hbuf, small.batch_set(batches)
+
+
+def test_uneven_batches_bit_exact(ctx, small, small_world):
+    rays, hbuf, bs = batched_setup(ctx, small, small_world)
+    print(f"case: {bs.n} uneven batches, {bs.total} rays", flush=True)
+    assert bs.n == 96 and bs.total == len(rays)
+    small.trace_batches(bs)
+    ctx.synchronize()
+    ref, rc = oracle.OracleWorld.view(small_world).trace_rays(rays, 0, THREADS)
+    assert_same(gpu_hits(hbuf), ref, "uneven batches")
+    assert small.trace_batches_counts(bs) == rc
+    assert_same(gpu_hits(hbuf), ref, "uneven batches (counting pass)")
+
+
+def test_graph_replays_match_eager(ctx, small, small_world):
+    from voxelg_amd.device import Graph
+    rays, hbuf, bs = batched_setup(ctx, small, small_world)
+    stream = ctx.stream()
+    small.trace_batches(bs, stream)
+    stream.synchronize()
+    eager = gpu_hits(hbuf).copy()
+    g = Graph(ctx, stream, lambda s: small.trace_batches(bs, s))
+    other = mixed_rays(np.random.default_rng(4), 50_000, small_world.dims)
+    for rep in range(3):
+        print(f"case: graph replay {rep}", flush=True)
+        hbuf.zero()
+        small.trace_rays(ctx.rays(other))  # eager work between replays
+        ctx.synchronize()
+        g.launch(stream)
+        stream.synchronize()
+        assert gpu_hits(hbuf).tobytes() == eager.tobytes(), f"replay {rep}"
+
+
 def test_edge_cases_and_errors(ctx, small, small_world):
     from voxelg_amd._lib import VGError
     # zero rays is a no-op

@@ -22,7 +22,9 @@ HIT_DTYPE = np.dtype(
     [("voxel", "<i4", (3,)), ("t", "<f4"), ("face", "u1"), ("material", "u1"), ("hit", "u1"),
      ("pad", "u1")]
 )
+BATCH_DTYPE = np.dtype([("rays", "<u8"), ("hits", "<u8"), ("n", "<u4"), ("flags", "<u4")])
 assert BRICK_DTYPE.itemsize == 72 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 20
+assert BATCH_DTYPE.itemsize == 24
 
 VG_OK = 0
 VG_SCHEDULE_PERSISTENT = 0
@@ -118,6 +120,13 @@ SIGNATURES = {
     "vg_autotune_frame": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _F32P,
                                     C.c_float, _P, _P, C.POINTER(C.c_float)]),
     "vg_autotune_rays": (C.c_int, [_P, _P, C.c_size_t, C.c_uint32, _P, C.POINTER(C.c_float)]),
+    "vg_trace_batches": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
+    "vg_trace_batches_counts": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _U64P]),
+    "vg_autotune_batches": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
+    "vg_graph_begin": (C.c_int, [_P, _P]),
+    "vg_graph_end": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "vg_graph_launch": (C.c_int, [_P, _P]),
+    "vg_graph_destroy": (C.c_int, [_P]),
 }
 
 

@@ -941,6 +941,95 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
     if (COUNT) flush_counts(cnt, counts);
 }
 
+// Persistent, ray-compacting query over many uneven batches at once: ray ids run over the
+// concatenation of all batches (prefix[b] = first id of batch b, prefix[nb] = total), each
+// lane finds its batch by binary search, and the refill discipline is that of
+// k_rays_persistent, so short and long batches share the machine without tail waste.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
+                                                               const vg_batch *__restrict__ bat,
+                                                               const uint32_t *__restrict__ prefix,
+                                                               uint32_t nbat, uint32_t total,
+                                                               uint32_t *work, int threshold,
+                                                               unsigned long long *counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cnt[VG_IT_COUNT];
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    bool busy = false;
+    vg_hit *out = nullptr;
+    uint32_t f = 0;
+    float tmax = 0.0f;
+    bool lod = false;
+    int it = 0;
+    Walk r;
+    WalkCache q;
+    HitRec h;
+    hit_miss(h);
+    cache_reset(q);
+    bool exhausted = false;
+    for (;;) {
+        uint64_t act = __ballot(busy);
+        int nact = __popcll(act);
+        if (!exhausted && nact < threshold) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
+            base = __shfl(base, 0, 64);
+            if (base >= total) {
+                exhausted = true;
+            } else if (!busy) {
+                const uint32_t id = base + lanes_below(~act);
+                if (id < total) {
+                    uint32_t lo = 0, hi = nbat; // last batch with prefix <= id
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (prefix[mid] <= id) lo = mid;
+                        else hi = mid;
+                    }
+                    const vg_batch b = bat[lo];
+                    const uint32_t j = id - prefix[lo];
+                    if (j < b.n) { // ids past a batch's n (an inconsistent prefix) are ignored
+                    const float4 *rp = reinterpret_cast<const float4 *>(b.rays + j);
+                    const float4 a = rp[0], c = rp[1];
+                    const float o[3] = {a.x, a.y, a.z}, din[3] = {c.x, c.y, c.z};
+                    float d[3];
+                    out = b.hits + j;
+                    f = b.flags | __float_as_uint(c.w);
+                    tmax = a.w;
+                    lod = (f & VG_RAY_LOD) && !(f & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
+                    hit_miss(h);
+                    if (normalize_or_zero(din, d) && walk_init(r, w, o, d)) {
+                        cache_reset(q);
+                        it = 0;
+                        busy = true;
+                    } else {
+                        store_hit(out, h);
+                    }
+                    }
+                }
+            }
+            nact = __popcll(__ballot(busy));
+        }
+        if (nact == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        do {
+            if (busy) {
+                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
+                done = done || ++it >= max_iter;
+                if (done) {
+                    store_hit(out, h);
+                    busy = false;
+                }
+            }
+        } while (__popcll(__ballot(busy)) >= threshold);
+    }
+    if (COUNT) flush_counts(cnt, counts);
+}
+
 // Upload relayout: reference-order bricks (72 B AoS) -> nested occupancy + materials.
 __global__ void k_relayout_bricks(const vg_brick *__restrict__ staged,
                                   const uint8_t *__restrict__ bu_ref, int64_t first, int64_t count,
@@ -1685,6 +1774,103 @@ int vg_autotune_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, 
     return autotune(c, &c->rays_wpc, &c->rays_refill, [&](hipStream_t s) {
         return vg_trace_rays(w, rays, n, flags, hits, (void *)s);
     }, best_ms);
+}
+
+int vg_trace_batches(vg_world *w, const vg_batch *batches, const uint32_t *prefix,
+                     uint32_t n_batches, uint32_t total, void *stream) {
+    int st = check_world(w);
+    if (st) return st;
+    if (total == 0 || n_batches == 0) return VG_OK;
+    if (!batches || !prefix) return fail(VG_E_INVALID, "null batches/prefix");
+    if (total > 0xF0000000u) return fail(VG_E_INVALID, "too many rays");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, w->work);
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
+    hipLaunchKernelGGL((k_batches_persistent<false>), grid, block, 0, s, w->dw, batches, prefix,
+                       n_batches, total, w->work, w->ctx->rays_refill, nullptr);
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
+}
+
+int vg_trace_batches_counts(vg_world *w, const vg_batch *batches, const uint32_t *prefix,
+                            uint32_t n_batches, uint32_t total, uint64_t counts_host[VG_IT_COUNT]) {
+    if (!counts_host) return fail(VG_E_INVALID, "null counts");
+    int st = check_world(w);
+    if (st) return st;
+    HIP_TRY(hipMemset(w->counts_dev, 0, VG_IT_COUNT * sizeof(unsigned long long)));
+    if (total && n_batches) {
+        if (!batches || !prefix) return fail(VG_E_INVALID, "null batches/prefix");
+        hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, 0, w->work);
+        const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
+        hipLaunchKernelGGL((k_batches_persistent<true>), grid, block, 0, 0, w->dw, batches, prefix,
+                           n_batches, total, w->work, w->ctx->rays_refill, w->counts_dev);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpy(counts_host, w->counts_dev, VG_IT_COUNT * 8, hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_autotune_batches(vg_world *w, const vg_batch *batches, const uint32_t *prefix,
+                        uint32_t n_batches, uint32_t total, float *best_ms) {
+    int st = check_world(w);
+    if (st) return st;
+    vg_ctx *c = w->ctx;
+    return autotune(c, &c->rays_wpc, &c->rays_refill, [&](hipStream_t s) {
+        return vg_trace_batches(w, batches, prefix, n_batches, total, (void *)s);
+    }, best_ms);
+}
+
+struct vg_graph {
+    int device;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int vg_graph_begin(vg_ctx *ctx, void *stream) {
+    CTX_DEVICE(ctx);
+    if (!stream) return fail(VG_E_INVALID, "capture needs a stream (not the null stream)");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return VG_OK;
+}
+
+int vg_graph_end(vg_ctx *ctx, void *stream, vg_graph **out) {
+    CTX_DEVICE(ctx);
+    if (!stream || !out) return fail(VG_E_INVALID, "null stream/out");
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(VG_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    vg_graph *gr = new (std::nothrow) vg_graph;
+    if (!gr) {
+        (void)hipGraphExecDestroy(ex);
+        (void)hipGraphDestroy(g);
+        return fail(VG_E_NOMEM, "graph");
+    }
+    gr->device = ctx->device;
+    gr->graph = g;
+    gr->exec = ex;
+    *out = gr;
+    return VG_OK;
+}
+
+int vg_graph_launch(vg_graph *g, void *stream) {
+    if (!g) return fail(VG_E_INVALID, "null graph");
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return VG_OK;
+}
+
+int vg_graph_destroy(vg_graph *g) {
+    if (!g) return VG_OK;
+    (void)hipSetDevice(g->device);
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    return VG_OK;
 }
 
 } // extern "C"

@@ -17,7 +17,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (HIT_DTYPE, IT_NAMES, RAY_DTYPE, VG_IT_COUNT, check, f32x3, i32x3, lib, ptr)
+from ._lib import (BATCH_DTYPE, HIT_DTYPE, IT_NAMES, RAY_DTYPE, VG_IT_COUNT, check, f32x3, i32x3,
+                   lib, ptr)
 from .world import World
 
 
@@ -80,6 +81,17 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class DevicePtr:
+    """A device address inside another allocation (keeps the owner alive)."""
+
+    def __init__(self, owner, offset_bytes: int):
+        self.owner = owner
+        self._p = owner.data_ptr() + int(offset_bytes)
+
+    def data_ptr(self) -> int:
+        return self._p
 
 
 class Event:
@@ -316,6 +328,28 @@ class DeviceWorld:
                                      C.byref(ms)), "vg_autotune_rays")
         return float(ms.value)
 
+    # ------------------------------------------------------------ batches --
+    def batch_set(self, batches) -> "BatchSet":
+        """Device descriptors for many uneven batches: a list of (rays, hits, n, flags) with
+        rays / hits DeviceBuffers (or anything with data_ptr())."""
+        return BatchSet(self, batches)
+
+    def trace_batches(self, bs: "BatchSet", stream=None):
+        check(lib().vg_trace_batches(self._h, bs.desc.data_ptr(), bs.prefix.data_ptr(), bs.n,
+                                     bs.total, _stream(stream)), "vg_trace_batches")
+
+    def trace_batches_counts(self, bs: "BatchSet") -> dict:
+        counts = (C.c_uint64 * VG_IT_COUNT)()
+        check(lib().vg_trace_batches_counts(self._h, bs.desc.data_ptr(), bs.prefix.data_ptr(),
+                                            bs.n, bs.total, counts), "vg_trace_batches_counts")
+        return dict(zip(IT_NAMES, (int(c) for c in counts)))
+
+    def autotune_batches(self, bs: "BatchSet") -> float:
+        ms = C.c_float()
+        check(lib().vg_autotune_batches(self._h, bs.desc.data_ptr(), bs.prefix.data_ptr(), bs.n,
+                                        bs.total, C.byref(ms)), "vg_autotune_batches")
+        return float(ms.value)
+
     def raycast_hit(self, camera_world, direction) -> np.void:
         out = np.zeros(1, HIT_DTYPE)
         check(lib().vg_raycast(self._h, f32x3(camera_world), f32x3(direction), ptr(out)),
@@ -332,3 +366,47 @@ def raycast(camera_world, direction, world: DeviceWorld, world_origin=None):
     if not h["hit"]:
         return None
     return PickHit(tuple(int(v) for v in h["voxel"]), FACE_NORMALS[int(h["face"])])
+
+
+class BatchSet:
+    """vg_batch descriptors and their prefix sums, resident on the device."""
+
+    def __init__(self, world: DeviceWorld, batches):
+        d = np.zeros(len(batches), BATCH_DTYPE)
+        sizes = []
+        for i, (rays, hits, n, flags) in enumerate(batches):
+            d[i] = (rays.data_ptr(), hits.data_ptr(), int(n), int(flags))
+            sizes.append(int(n))
+        prefix = np.zeros(len(batches) + 1, np.uint32)
+        prefix[1:] = np.cumsum(sizes, dtype=np.uint64).astype(np.uint32)
+        self.n = len(batches)
+        self.total = int(prefix[-1])
+        self.sizes = sizes
+        self._keep = batches
+        self.desc = world.ctx.to_device(d)
+        self.prefix = world.ctx.to_device(prefix)
+
+
+class Graph:
+    """Launches captured from a stream into an executable HIP graph."""
+
+    def __init__(self, ctx: Context, stream: Stream, fn):
+        check(lib().vg_graph_begin(ctx._h, stream.handle), "vg_graph_begin")
+        try:
+            fn(stream)
+        finally:
+            g = C.c_void_p()
+            st = lib().vg_graph_end(ctx._h, stream.handle, C.byref(g))
+        check(st, "vg_graph_end")
+        self._g = g
+        self.ctx = ctx
+
+    def launch(self, stream: Stream):
+        check(lib().vg_graph_launch(self._g, stream.handle), "vg_graph_launch")
+
+    def __del__(self):
+        try:
+            lib().vg_graph_destroy(self._g)
+        except Exception:
+            pass
+

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import VG_RAY_BEAM, VG_RAY_LOD
+from ._lib import RAY_DTYPE, VG_RAY_ANY_HIT, VG_RAY_BEAM, VG_RAY_LOD
 from .world import Camera, World, sun_direction, terrain_height
 
 MAX_RAY_DIST = 700.0
@@ -77,3 +77,89 @@ class FrameWorkload:
 SMALL = FrameWorkload("small", (256, 128, 256), 1280, 720, 512.0, 0, False)
 LARGE = FrameWorkload("large", (2048, 256, 2048), 1920, 1080, MAX_RAY_DIST,
                       VG_RAY_LOD | VG_RAY_BEAM, True)
+
+
+@dataclass(frozen=True)
+class BatchedWorkload:
+    """Uneven batched ray queries on the small world: n_batches batches per step with sizes
+    exp(U[ln lo, ln hi]); 40% coherent primary rays (64x64-pixel tiles of random cameras),
+    30% shadow rays toward the sun and 30% cosine-hemisphere AO / bounce rays, both from
+    random surface points; tmax uniform in {32, 128, 512, 1024}."""
+    name: str = "batched"
+    dims: tuple = (256, 128, 256)
+    n_batches: int = 96
+    size_lo: int = 256
+    size_hi: int = 262144
+    mix: tuple = (0.4, 0.3, 0.3)
+    tmaxes: tuple = (32.0, 128.0, 512.0, 1024.0)
+    seed: int = 0x5EED_0003
+
+    def world(self, threads: int = 0) -> World:
+        return World(self.dims).generate(SMALL.seed, threads)
+
+    def rays(self, world: World):
+        """(rays [total] RAY_DTYPE, batch sizes) for one step."""
+        rng = np.random.default_rng(self.seed)
+        sizes = np.exp(rng.uniform(np.log(self.size_lo), np.log(self.size_hi),
+                                   self.n_batches)).astype(np.int64)
+        total = int(sizes.sum())
+        n_p = int(total * self.mix[0])
+        n_s = int(total * self.mix[1])
+        n_a = total - n_p - n_s
+        prim = self._primary(rng, n_p)
+        top = world.top_surface()
+        shad = self._surface_rays(rng, world, top, n_s, "shadow")
+        ao = self._surface_rays(rng, world, top, n_a, "ao")
+        allr = np.concatenate([prim, shad, ao])
+        # interleave the three kinds in chunks of 1024 rays, each chunk kept coherent
+        chunks = np.arange(0, total, 1024)
+        order = rng.permutation(len(chunks))
+        perm = np.concatenate([np.arange(c, min(c + 1024, total)) for c in chunks[order]])
+        allr = allr[perm]
+        allr["tmax"] = rng.choice(np.asarray(self.tmaxes, np.float32), total)
+        return allr, [int(x) for x in sizes]
+
+    def _primary(self, rng, n):
+        out = np.zeros(n, RAY_DTYPE)
+        W, H, T = 1280, 720, 64
+        done = 0
+        while done < n:
+            x = self.dims[0] * (0.25 + 0.5 * rng.random())
+            z = self.dims[2] * (0.25 + 0.5 * rng.random())
+            cam = Camera(pos=(x, 90.0, z), yaw=2 * math.pi * rng.random(), pitch=-0.35)
+            u = cam.uniform(W, H)
+            tx, ty = rng.integers(0, W // T), rng.integers(0, H // T)
+            px, py = np.meshgrid(np.arange(tx * T, tx * T + T), np.arange(ty * T, ty * T + T))
+            px, py = px.ravel().astype(np.float32), py.ravel().astype(np.float32)
+            ndc_x = (px + 0.5) / W * 2 - 1
+            ndc_y = 1 - (py + 0.5) / H * 2
+            a = ndc_x * u[15] * (W / H)
+            b = ndc_y * u[15]
+            d = u[4:7][None] + u[8:11][None] * a[:, None] + u[12:15][None] * b[:, None]
+            k = min(len(d), n - done)
+            out["o"][done:done + k] = u[0:3]
+            out["d"][done:done + k] = d[:k]
+            done += k
+        return out
+
+    def _surface_rays(self, rng, world, top, n, kind):
+        out = np.zeros(n, RAY_DTYPE)
+        xs = rng.integers(0, self.dims[0], n)
+        zs = rng.integers(0, self.dims[2], n)
+        ys = top[xs, zs]
+        ok = ys >= 0
+        o = np.stack([xs + rng.random(n), ys + 1.0 + 1e-3, zs + rng.random(n)], 1)
+        out["o"] = o.astype(np.float32)
+        if kind == "shadow":
+            out["d"] = LARGE.sun()
+            out["flags"] = VG_RAY_ANY_HIT
+        else:
+            r1, phi = rng.random(n), 2 * math.pi * rng.random(n)
+            s = np.sqrt(r1)
+            out["d"] = np.stack([s * np.cos(phi), np.sqrt(1 - r1), s * np.sin(phi)], 1)
+        out["d"][~ok] = 0.0  # no surface in this column: a zero direction misses
+        return out
+
+
+BATCHED = BatchedWorkload()
+

@@ -113,6 +113,19 @@ class World:
         out[inside] = np.where(occ == 1, m, 0)
         return out
 
+    def top_surface(self) -> np.ndarray:
+        """Window-local y of the highest solid voxel of every (x, z) column (-1 if none),
+        shape (X, Z)."""
+        X, Y, Z = self.dims
+        BX, BY, BZ = self.brick_dims
+        occ = np.ascontiguousarray(self.bricks["occupancy"]).reshape(BZ, BY, BX)
+        bits = np.unpackbits(occ.view(np.uint8).reshape(BZ, BY, BX, 8), axis=-1,
+                             bitorder="little").reshape(BZ, BY, BX, 4, 4, 4)  # ly, lz, lx
+        vox = bits.transpose(2, 5, 1, 3, 0, 4).reshape(X, Y, Z)  # x, y, z
+        has = vox.any(axis=1)
+        top = Y - 1 - np.argmax(vox[:, ::-1, :], axis=1)
+        return np.where(has, top, -1)
+
     def solid_fraction(self) -> float:
         occ = np.ascontiguousarray(self.bricks["occupancy"])
         bits = np.unpackbits(occ.view(np.uint8)).sum(dtype=np.int64)
