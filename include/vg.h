@@ -290,6 +290,91 @@ int vg_graph_end(vg_ctx *ctx, void *stream, vg_graph **out);
 int vg_graph_launch(vg_graph *graph, void *stream);
 int vg_graph_destroy(vg_graph *graph);
 
+/* ------------------------------------------------------------ GI probes -- */
+/* GiProbe (shaders/gi_probes.wgsl:25-48): 12 x vec4 f32 = 192 bytes. SH-L1 coefficient
+ * order in each vec4 is (c0, c1y, c1z, c1x) (sh_basis, :72-74). */
+typedef struct vg_probe {
+    float sh_r[4], sh_g[4], sh_b[4]; /* displayed SH-L1 radiance, per channel */
+    float sh_dist[4], sh_dist2[4];   /* visible distance and distance^2 moments */
+    float info[4];                   /* xyz world grid coordinate held, w sample age (0 new) */
+    float st_r[4], st_g[4], st_b[4]; /* staging sums of this cycle's subset estimates */
+    float st_dist[4], st_dist2[4];
+    float st_info[4];                /* x subset estimates this cycle, yzw relocation offset */
+} vg_probe;
+
+#define VG_GI_SKIP_BURIED 1u /* a probe whose voxel is solid traces nothing and stays empty */
+
+/* One probe update (cs_gi_probe_update, shaders/gi_probes.wgsl:274-396). The first block
+ * mirrors the shader's constants and camera fields; vg_gi_params_default derives the
+ * lighting block from a sun direction with the reference's sun_color, ambient_color and
+ * sky_color (shaders/common.wgsl:63-83, shaders/raymarch.wgsl:2532-2579, 2870-2876). */
+typedef struct vg_gi_params {
+    int32_t grid[3];        /* PROBE_DIM_X, _Y, _Z (src/world_dims.rs:44-48) */
+    int32_t spacing;        /* PROBE_SPACING, voxels; probe g sits at (g + 0.5) * spacing */
+    int32_t rays_per_probe; /* PROBE_RAYS: fresh gather rays per probe per update */
+    int32_t epochs;         /* GI_DIR_EPOCHS: the spherical-Fibonacci set has
+                               rays_per_probe * epochs points, one subset per update */
+    int32_t update_div;     /* GI_UPDATE_DIV: update probes i * div + round % div */
+    int32_t round;          /* camera.gi_round */
+    int32_t bounces;        /* 1: ambient tail at the first hit (the reference's model);
+                               2: one cosine-sampled bounce ray from the first hit */
+    uint32_t seed;          /* bounce-direction sampling seed */
+    uint32_t flags;         /* VG_GI_* */
+    float hysteresis;       /* PROBE_FOLD_HYSTERESIS (0.6) */
+    float gi_dist;          /* GI_DIST (40): reach of gather and bounce rays */
+    float shadow_tmax;      /* SHADOW_MAX_DIST (480) */
+    float sun[3];           /* unit vector toward the sun */
+    float gi_sky;           /* GI_SKY (0.32): escaped rays gather sky * gi_sky */
+    float sun_color[3];     /* sun_color(sun) */
+    float ambient_tail;     /* 0.35: a last bounce adds albedo * ambient * ambient_tail */
+    float ambient[3];       /* ambient_color() */
+    float day_t;            /* sun_intensity(sun) */
+    float sky_horizon[3];   /* mix(night, day, day_t) horizon colour */
+    float dusk_w;           /* sunset warmth weight */
+    float sky_zenith[3];    /* mix(night, day, day_t) zenith colour */
+    float halo;             /* day_t * 0.45 */
+    float sun_xz[2];        /* normalize(sun.xz) (zero when the sun is overhead) */
+    float _pad[2];
+} vg_gi_params;
+
+typedef struct vg_probes vg_probes;
+
+/* The reference's defaults (64 x 32 x 64 probes at spacing 8, 8 rays, 8 epochs, div 8,
+ * one bounce) with lighting derived from `sun` (need not be normalized). */
+int vg_gi_params_default(const float sun[3], vg_gi_params *out);
+/* default_palette (src/renderer.rs:201-245): 256 RGBA albedo entries. */
+int vg_gi_default_palette(float rgba_host[256 * 4]);
+/* The gather directions of the spherical-Fibonacci set, point i = k * epochs + epoch:
+ * z = 1 - 2 (i + 0.5) / n, phi = i * golden angle, d = (r cos phi, z, r sin phi) (computed
+ * in double, rounded to f32): n = rays_per_probe * epochs entries of 4 floats (w = 0). */
+int vg_gi_directions(int32_t n, float *xyzw_host);
+/* The cosine-weighted hemisphere table bounce rays draw from: VG_GI_BOUNCE_TABLE entries
+ * (t, b, n, 0) with n = cos(theta) >= 0, from a Hammersley set (computed in double). */
+#define VG_GI_BOUNCE_TABLE 4096
+int vg_gi_bounce_table(float *xyzw_host);
+
+/* Probe storage for one grid (params->grid, rays_per_probe and epochs fix the sizes) on a
+ * world's device. Probes start empty (age 0). */
+int vg_probes_create(vg_world *world, const vg_gi_params *params, vg_probes **out);
+int vg_probes_destroy(vg_probes *probes);
+/* Replace the albedo palette (256 RGBA entries, host). */
+int vg_probes_set_palette(vg_probes *probes, const float *rgba_host);
+/* Every probe back to empty. */
+int vg_probes_reset(vg_probes *probes, void *stream);
+/* Device address of the PROBE_TOTAL vg_probe records (slot x + y*X + z*X*Y). */
+int vg_probes_data(vg_probes *probes, vg_probe **dev_out);
+/* Copy the records to / from the host. Synchronous. */
+int vg_probes_download(vg_probes *probes, vg_probe *host);
+int vg_probes_upload(vg_probes *probes, const vg_probe *host);
+/* cs_gi_probe_update: gather rays_per_probe rays per updated probe (the round's subset),
+ * shade every hit (sun with a shadow ray, plus the ambient tail or one bounce ray), project
+ * onto SH-L1 with weight 4 pi / rays, fold complete cycles with hysteresis, relocate. */
+int vg_probes_update(vg_probes *probes, const vg_gi_params *params, void *stream);
+/* The same update reporting the traced rays (gather, shadow, bounce) and the walk's
+ * iteration-class totals. Synchronous. */
+int vg_probes_update_counts(vg_probes *probes, const vg_gi_params *params,
+                            uint64_t rays_host[3], uint64_t counts_host[VG_IT_COUNT]);
+
 /* raycast (src/raycast.rs:21) over a world on the device: one ray, host in and out.
  * Returns VG_OK with hit_host->hit = 0/1. Synchronous. */
 int vg_raycast(vg_world *world, const float camera_world[3], const float dir[3],

@@ -37,6 +37,20 @@ IT_NAMES = ("l4_skip", "chunk_skip", "tile_skip", "tile_lod", "tile_uniform", "b
 IT_BYTES = (8, 8, 16, 17, 17, 25, 26, 35, 34, 8, 16, 25, 34, 8)
 
 
+# orc_gi_params (oracle.h): the byte layout of vg_gi_params.
+GI_PARAMS_DTYPE = np.dtype([
+    ("grid", "<i4", (3,)), ("spacing", "<i4"), ("rays_per_probe", "<i4"), ("epochs", "<i4"),
+    ("update_div", "<i4"), ("round", "<i4"), ("bounces", "<i4"), ("seed", "<u4"),
+    ("flags", "<u4"), ("hysteresis", "<f4"), ("gi_dist", "<f4"), ("shadow_tmax", "<f4"),
+    ("sun", "<f4", (3,)), ("gi_sky", "<f4"), ("sun_color", "<f4", (3,)),
+    ("ambient_tail", "<f4"), ("ambient", "<f4", (3,)), ("day_t", "<f4"),
+    ("sky_horizon", "<f4", (3,)), ("dusk_w", "<f4"), ("sky_zenith", "<f4", (3,)),
+    ("halo", "<f4"), ("sun_xz", "<f4", (2,)), ("pad", "<f4", (2,))])
+GI_SKIP_BURIED = 1
+GI_BOUNCE_TABLE = 4096
+PROBE_FIELDS = 48
+
+
 class OrcWorld(C.Structure):
     _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_int32 * 3), ("bricks", C.c_void_p),
                 ("tile_mask", C.c_void_p), ("chunk_mask", C.c_void_p), ("l4_mask", C.c_void_p),
@@ -86,6 +100,13 @@ def lib():
             "orc_frame_r0": (None, [W, P, C.c_int32, C.c_int32, C.c_float, F32P, C.c_float,
                                     C.c_int32, C.c_int32, P, P, C.c_int]),
             "orc_shadow_origin": (None, [F32P, F32P, P, F32P]),
+            "orc_trace_dir": (C.c_int, [W, F32P, F32P, C.c_float, C.c_uint32, P, U64P]),
+            "orc_gi_sky": (None, [P, F32P, C.POINTER(C.c_double)]),
+            "orc_gi_bounce_index": (C.c_uint32, [C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32]),
+            "orc_gi_sample": (None, [W, P, P, P, F32P, F32P, C.c_uint32, C.c_uint32,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_uint8), F32P, U64P]),
+            "orc_gi_update": (None, [W, P, P, P, P, P, C.c_int, U64P]),
+            "orc_gi_reference": (None, [W, P, C.c_int32, P, P, P, P, P, C.c_int64, P, C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -93,6 +114,11 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def _threads(threads: int) -> int:
+    """Worker threads: the request, else the host's cores capped at 16 (one GPU's CPU share)."""
+    return int(threads) if threads else min(16, os.cpu_count() or 1)
 
 
 def _f3(v):
@@ -214,14 +240,14 @@ class OracleWorld:
         out = np.zeros(len(rays), HIT_DTYPE)
         cnt = (C.c_uint64 * IT_COUNT)()
         lib().orc_trace_r1_batch(C.byref(self.cstruct()), _p(rays), len(rays), flags, _p(out),
-                                 cnt, threads or os.cpu_count())
+                                 cnt, _threads(threads))
         return out, dict(zip(IT_NAMES, (int(c) for c in cnt)))
 
     def raycast_rays(self, rays: np.ndarray, threads=0) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(len(rays), HIT_DTYPE)
         lib().orc_raycast_r0_batch(C.byref(self.cstruct()), _p(rays), len(rays), _p(out),
-                                   threads or os.cpu_count())
+                                   _threads(threads))
         return out
 
     def frame(self, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
@@ -237,7 +263,7 @@ class OracleWorld:
         cam = np.ascontiguousarray(camera, np.float32)
         lib().orc_frame(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
                         primary_flags, _f3(sun if sun is not None else (0, 0, 0)),
-                        shadow_tmax, r0, r1, _p(hits), _p(shadow), threads or os.cpu_count(), cnt,
+                        shadow_tmax, r0, r1, _p(hits), _p(shadow), _threads(threads), cnt,
                         _p(it) if iters else None)
         out = (hits, shadow, dict(zip(IT_NAMES, (int(c) for c in cnt))))
         return out + (it,) if iters else out
@@ -251,8 +277,73 @@ class OracleWorld:
         cam = np.ascontiguousarray(camera, np.float32)
         lib().orc_frame_r0(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
                            _f3(sun if sun is not None else (0, 0, 0)), shadow_tmax, r0, r1,
-                           _p(hits), _p(shadow), threads or os.cpu_count())
+                           _p(hits), _p(shadow), _threads(threads))
         return hits, shadow
+
+    # ---------------------------------------------------------- GI probes --
+    def trace_dir(self, o, d, tmax, flags):
+        """R1 along d exactly as given; returns (started, hit)."""
+        out = np.zeros(1, HIT_DTYPE)
+        started = lib().orc_trace_dir(C.byref(self.cstruct()), _f3(o), _f3(d), float(tmax),
+                                      int(flags), _p(out), None)
+        return bool(started), out[0]
+
+    def gi_sample(self, params, btab, palette, o, d, probe, k):
+        """One gather sample: (rgb + visible distance as float64[4], face, t, rays[3])."""
+        prm = _gi_params(params)
+        out = (C.c_double * 4)()
+        face = C.c_uint8()
+        t = C.c_float()
+        rays = (C.c_uint64 * 3)()
+        lib().orc_gi_sample(C.byref(self.cstruct()), _p(prm), _p(_f4(btab)), _p(_f4(palette)),
+                            _f3(o), _f3(d), int(probe), int(k), out, C.byref(face), C.byref(t),
+                            rays)
+        return np.array(out[:]), int(face.value), float(t.value), [int(r) for r in rays]
+
+    def gi_update(self, params, dirs, btab, palette, probes: np.ndarray, threads=0):
+        """One probe update in place on float64 records [total, 48]; returns rays[3]."""
+        prm = _gi_params(params)
+        assert probes.dtype == np.float64 and probes.flags["C_CONTIGUOUS"]
+        rays = (C.c_uint64 * 3)()
+        lib().orc_gi_update(C.byref(self.cstruct()), _p(prm), _p(_f4(dirs)), _p(_f4(btab)),
+                            _p(_f4(palette)), _p(probes), _threads(threads), rays)
+        return [int(r) for r in rays]
+
+    def gi_reference(self, params, dirs, btab, palette, slots, origins, threads=0):
+        """FP64 estimate of SH-L1 (r, g, b, dist, dist2) x 4 per listed probe slot from the
+        given origins over the full direction set `dirs`: float64 [n, 20]."""
+        prm = _gi_params(params)
+        dirs = _f4(dirs)
+        slots = np.ascontiguousarray(slots, np.int64)
+        origins = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        out = np.zeros((len(slots), 20), np.float64)
+        lib().orc_gi_reference(C.byref(self.cstruct()), _p(prm), len(dirs), _p(dirs),
+                               _p(_f4(btab)), _p(_f4(palette)), _p(slots), _p(origins),
+                               len(slots), _p(out), _threads(threads))
+        return out
+
+
+def _gi_params(params) -> np.ndarray:
+    a = np.frombuffer(bytes(params), dtype=GI_PARAMS_DTYPE) if not isinstance(params, np.ndarray) \
+        else params.view(GI_PARAMS_DTYPE).reshape(-1)
+    assert a.size == 1
+    return np.ascontiguousarray(a)
+
+
+def _f4(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32)
+    assert a.ndim == 2 and a.shape[1] == 4
+    return a
+
+
+def gi_sky(params, d):
+    out = (C.c_double * 3)()
+    lib().orc_gi_sky(_p(_gi_params(params)), _f3(d), out)
+    return np.array(out[:])
+
+
+def gi_bounce_index(seed, rnd, probe, k):
+    return int(lib().orc_gi_bounce_index(int(seed) & 0xFFFFFFFF, int(rnd), int(probe), int(k)))
 
 
 def make_rays(o, d, tmax=np.inf, flags=0) -> np.ndarray:

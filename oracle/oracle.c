@@ -884,6 +884,15 @@ static void trace_core(const orc_world *w, const float o[3], const float dir[3],
     trace_core_from(w, o, dir, tmax, flags, 0.0f, out, cnt);
 }
 
+int orc_trace_dir(const orc_world *w, const float o[3], const float dir[3], float tmax,
+                  uint32_t flags, orc_hit *out, uint64_t counts[ORC_IT_COUNT]) {
+    miss(out);
+    r1_t s;
+    if (!r1_init(&s, w, o, dir)) return 0;
+    r1_walk(&s, tmax, flags, out, counts);
+    return 1;
+}
+
 void orc_trace_r1(const orc_world *w, const orc_ray *r, uint32_t flags, orc_hit *out,
                   uint64_t counts[ORC_IT_COUNT]) {
     float dir[3];

@@ -138,6 +138,11 @@ void orc_trace_flat(const orc_world *w, const orc_ray *r, uint32_t flags, orc_hi
 /* Brute force over every solid voxel (tiny worlds only): nearest slab entry. */
 void orc_trace_brute(const orc_world *w, const orc_ray *r, orc_hit *out, int32_t *n_ties);
 
+/* R1 from an origin along dir exactly as given (no normalization); returns 0 when the ray
+ * never enters the window (out is then a miss). */
+int orc_trace_dir(const orc_world *w, const float o[3], const float dir[3], float tmax,
+                  uint32_t flags, orc_hit *out, uint64_t counts[ORC_IT_COUNT]);
+
 void orc_trace_r1_batch(const orc_world *w, const orc_ray *rays, size_t n, uint32_t flags,
                         orc_hit *out, uint64_t counts[ORC_IT_COUNT], int threads);
 void orc_raycast_r0_batch(const orc_world *w, const orc_ray *rays, size_t n, orc_hit *out,
@@ -160,6 +165,64 @@ void orc_shadow_origin(const float o[3], const float d[3], const orc_hit *h, flo
 /* Beam prologue start t of the 8x8 pixel block (bx, by); tests counts tested tiles. */
 float orc_beam_t(const orc_world *w, const float *camera176, int32_t bx, int32_t by, float tmax,
                  uint64_t *tests);
+
+/* ---------------------------------------------------------------- GI probes --
+ * FP64 restatement of the probe update (shaders/gi_probes.wgsl:187-396) over the same
+ * inputs as the device: the ray origins, the traversal (R1) and every branch are computed
+ * exactly as the device computes them (fp32), the radiance, SH projection and fold in
+ * double. Parameters have the byte layout of vg_gi_params (include/vg.h). */
+typedef struct orc_gi_params {
+    int32_t grid[3];
+    int32_t spacing;
+    int32_t rays_per_probe;
+    int32_t epochs;
+    int32_t update_div;
+    int32_t round;
+    int32_t bounces;
+    uint32_t seed;
+    uint32_t flags;
+    float hysteresis;
+    float gi_dist;
+    float shadow_tmax;
+    float sun[3];
+    float gi_sky;
+    float sun_color[3];
+    float ambient_tail;
+    float ambient[3];
+    float day_t;
+    float sky_horizon[3];
+    float dusk_w;
+    float sky_zenith[3];
+    float halo;
+    float sun_xz[2];
+    float pad[2];
+} orc_gi_params;
+
+#define ORC_GI_SKIP_BURIED 1u
+#define ORC_GI_BOUNCE_TABLE 4096
+#define ORC_PROBE_FIELDS 48 /* vg_probe: 12 x vec4 */
+
+/* sky_color for a direction (radiance before GI_SKY), double. */
+void orc_gi_sky(const orc_gi_params *p, const float d[3], double out[3]);
+/* Bounce-table index of sample k of probe slot `probe` in `round`. */
+uint32_t orc_gi_bounce_index(uint32_t seed, int32_t round, uint32_t probe, uint32_t k);
+/* One gather sample from o along d: radiance rgb and visible distance (out[4]); face of
+ * the gather hit (ORC_FACE_NONE if none); rays[3] += gather, shadow, bounce rays traced. */
+void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab,
+                   const float *palette, const float o[3], const float d[3], uint32_t probe,
+                   uint32_t k, double out[4], uint8_t *face, float *t_hit, uint64_t rays[3]);
+/* One probe update over total = grid product records of ORC_PROBE_FIELDS doubles. */
+void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs,
+                   const float *btab, const float *palette, double *probes, int threads,
+                   uint64_t rays[3]);
+/* The judge: for each listed probe slot, an n_dirs-direction estimate (dirs: the
+ * n_dirs-point Fibonacci set) of its SH-L1 radiance and distance moments from origin
+ * origins[3i..3i+2], bounce samples drawn for sample k = 0..n_dirs-1 of `round`. out: 20
+ * doubles per probe (r, g, b, dist, dist2 x 4 coefficients). */
+void orc_gi_reference(const orc_world *w, const orc_gi_params *p, int32_t n_dirs,
+                      const float *dirs, const float *btab, const float *palette,
+                      const int64_t *slots, const float *origins, int64_t n, double *out,
+                      int threads);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,396 @@
+/*
+ * CPU ORACLE — test infrastructure only (see oracle.h). FP64 restatement of the GI probe
+ * update, cs_gi_probe_update / probe_ray_radiance (shaders/gi_probes.wgsl:187-396), with
+ * the build's 2-bounce extension.
+ *
+ * What must equal the device bit for bit is computed in fp32 with the device's operation
+ * order: probe origins and relocation offsets (gi_probes.wgsl:298-306, 381-392), gather /
+ * shadow / bounce ray origins and directions, the traversal (R1) and the sun-disc test.
+ * Everything that is only summed is double: radiance, SH projection (4 pi / N weights),
+ * staging sums and the hysteresis fold.
+ *
+ * Parity pins (tests/test_gi_oracle.py): closed forms (constant sky -> c0 = 4 pi L 0.282095
+ * and zero linear terms on the symmetric set; a sealed box -> albedo * ambient * tail;
+ * 2-bounce sealed box -> albedo^2 * ambient * tail); sky_color at directions where the
+ * reference formula reduces to its constants; the fold/relocation recurrences against a
+ * hand-stepped sequence; Fibonacci and bounce tables against their closed forms.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+static double smooth_d(double e0, double e1, double x) {
+    double t = (x - e0) / (e1 - e0);
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    return t * t * (3.0 - 2.0 * t);
+}
+
+void orc_gi_sky(const orc_gi_params *p, const float d[3], double out[3]) {
+    const double dy = d[1];
+    const double up = dy < -0.2 ? -0.2 : (dy > 1.0 ? 1.0 : dy);
+    const double zf = smooth_d(0.0, 0.55, up);
+    for (int k = 0; k < 3; ++k)
+        out[k] = (double)p->sky_horizon[k] * (1.0 - zf) + (double)p->sky_zenith[k] * zf;
+    if (p->dusk_w > 0.0f) {
+        const double l = sqrt((double)d[0] * d[0] + (double)d[2] * d[2]);
+        double azi = 0.0;
+        if (l > 0.0) {
+            azi = (d[0] / l) * p->sun_xz[0] + (d[2] / l) * p->sun_xz[1];
+            azi = azi < 0.0 ? 0.0 : (azi > 1.0 ? 1.0 : azi);
+        }
+        const double wgt = (double)p->dusk_w * azi * azi * azi * (1.0 - zf);
+        out[0] += 1.50 * wgt;
+        out[1] += 0.55 * wgt;
+        out[2] += 0.18 * wgt;
+    }
+    double sa = (double)d[0] * p->sun[0] + (double)d[1] * p->sun[1] + (double)d[2] * p->sun[2];
+    sa = sa > 0.0 ? sa : 0.0;
+    const double h = pow(sa, 6.0) * (double)p->halo;
+    out[0] += 1.0 * h;
+    out[1] += 0.78 * h;
+    out[2] += 0.52 * h;
+    /* the disc is a threshold: decided in fp32 exactly as the device does */
+    float saf = d[0] * p->sun[0] + d[1] * p->sun[1] + d[2] * p->sun[2];
+    saf = saf > 0.0f ? saf : 0.0f;
+    if (saf > 0.9985f) {
+        out[0] = out[0] * 0.1 + 2.4 * 0.9;
+        out[1] = out[1] * 0.1 + 2.0 * 0.9;
+        out[2] = out[2] * 0.1 + 1.5 * 0.9;
+    }
+}
+
+static uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+uint32_t orc_gi_bounce_index(uint32_t seed, int32_t round, uint32_t probe, uint32_t k) {
+    uint32_t x = mix32(seed ^ ((uint32_t)round * 0x85EBCA6Bu));
+    x = mix32(x ^ probe);
+    x = mix32(x ^ k);
+    return x >> 20;
+}
+
+static float face_n(uint32_t f, int k) {
+    return (f < 6u && (int)(f >> 1) == k) ? ((f & 1u) ? -1.0f : 1.0f) : 0.0f;
+}
+
+static float face_ndl(uint32_t f, const float s[3]) {
+    if (f >= 6u) return 0.0f;
+    const float v = (f & 1u) ? -s[f >> 1] : s[f >> 1];
+    return v > 0.0f ? v : 0.0f;
+}
+
+static void offset_origin(const float o[3], const float d[3], float t, uint32_t f, float out[3]) {
+    for (int k = 0; k < 3; ++k) {
+        float p = o[k] + d[k] * t;
+        out[k] = p + face_n(f, k) * 0.03f;
+    }
+}
+
+static void bounce_dir(const float *e, uint32_t f, float out[3]) {
+    const int ax = (int)(f >> 1);
+    const float sg = (f & 1u) ? -1.0f : 1.0f;
+    out[ax] = sg * e[2];
+    out[ax == 2 ? 0 : ax + 1] = e[0];
+    out[ax == 0 ? 2 : ax - 1] = e[1];
+}
+
+/* 1 when the sun is visible from o (no solid voxel within shadow_tmax, water included). */
+static double sun_visible(const orc_world *w, const orc_gi_params *p, const float o[3],
+                          uint64_t rays[3]) {
+    orc_hit h;
+    if (orc_trace_dir(w, o, p->sun, p->shadow_tmax, ORC_ANY_HIT, &h, NULL)) rays[1]++;
+    return h.hit ? 0.0 : 1.0;
+}
+
+void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab,
+                   const float *palette, const float o[3], const float d[3], uint32_t probe,
+                   uint32_t k, double out[4], uint8_t *face, float *t_hit, uint64_t rays[3]) {
+    orc_hit h;
+    if (orc_trace_dir(w, o, d, p->gi_dist, ORC_SKIP_TRANSPARENT, &h, NULL)) rays[0]++;
+    if (!h.hit) {
+        double sky[3];
+        orc_gi_sky(p, d, sky);
+        for (int c = 0; c < 3; ++c) out[c] = sky[c] * (double)p->gi_sky;
+        out[3] = p->gi_dist;
+        *face = ORC_FACE_NONE;
+        *t_hit = p->gi_dist;
+        return;
+    }
+    const uint32_t fX = h.face;
+    const float tX = h.t;
+    const float *aX = palette + 4 * h.material;
+    const double ndlX = face_ndl(fX, p->sun);
+    float so[3];
+    offset_origin(o, d, tX, fX, so);
+    const double visX = ndlX > 0.0 ? sun_visible(w, p, so, rays) : 0.0;
+    double lb[3];
+    if (p->bounces == 2) {
+        float bd[3];
+        bounce_dir(btab + 4 * orc_gi_bounce_index(p->seed, p->round, probe, k), fX, bd);
+        orc_hit hy;
+        if (orc_trace_dir(w, so, bd, p->gi_dist, ORC_SKIP_TRANSPARENT, &hy, NULL)) rays[2]++;
+        if (!hy.hit) {
+            double sky[3];
+            orc_gi_sky(p, bd, sky);
+            for (int c = 0; c < 3; ++c) lb[c] = sky[c] * (double)p->gi_sky;
+        } else {
+            const float *aY = palette + 4 * hy.material;
+            const double ndlY = face_ndl(hy.face, p->sun);
+            float yo[3];
+            offset_origin(so, bd, hy.t, hy.face, yo);
+            const double visY = ndlY > 0.0 ? sun_visible(w, p, yo, rays) : 0.0;
+            for (int c = 0; c < 3; ++c)
+                lb[c] = (double)aY[c] * ((double)p->sun_color[c] * ndlY * visY +
+                                         (double)p->ambient[c] * (double)p->ambient_tail);
+        }
+    } else {
+        for (int c = 0; c < 3; ++c) lb[c] = (double)p->ambient[c] * (double)p->ambient_tail;
+    }
+    for (int c = 0; c < 3; ++c)
+        out[c] = (double)aX[c] * ((double)p->sun_color[c] * ndlX * visX + lb[c]);
+    out[3] = tX;
+    *face = (uint8_t)fX;
+    *t_hit = tX;
+}
+
+static int pos_mod_i(int32_t a, int32_t b) {
+    int32_t r = a % b;
+    return r < 0 ? r + b : r;
+}
+
+static void probe_cell(const orc_gi_params *p, const orc_world *w, int64_t idx, int32_t g[3]) {
+    const int32_t sx = (int32_t)(idx % p->grid[0]);
+    const int32_t sy = (int32_t)((idx / p->grid[0]) % p->grid[1]);
+    const int32_t sz = (int32_t)(idx / ((int64_t)p->grid[0] * p->grid[1]));
+    const int32_t ox = w->origin[0] / p->spacing, oz = w->origin[2] / p->spacing;
+    g[0] = ox + pos_mod_i(sx - ox, p->grid[0]);
+    g[1] = sy;
+    g[2] = oz + pos_mod_i(sz - oz, p->grid[2]);
+}
+
+static int solid_for_gi(uint32_t m) {
+    if (m == 0u || m == 33u) return 0;
+    if ((m >= 5u && m <= 12u) || m == 18u) return 0;
+    return 1;
+}
+
+/* SH-L1 basis (gi_probes.wgsl:72-74) in double, the reference's truncated constants. */
+static void sh_basis(const float d[3], double b[4]) {
+    b[0] = 0.282095;
+    b[1] = 0.488603 * d[1];
+    b[2] = 0.488603 * d[2];
+    b[3] = 0.488603 * d[0];
+}
+
+typedef struct {
+    const orc_world *w;
+    const orc_gi_params *p;
+    const float *dirs, *btab, *pal;
+    double *probes;
+    uint32_t n_upd, phase, epoch;
+    int64_t next;
+    pthread_mutex_t mu;
+    uint64_t rays[3];
+} upd_job;
+
+static void update_probe(upd_job *J, uint32_t j, uint64_t rays[3]) {
+    const orc_gi_params *p = J->p;
+    const int64_t idx = (int64_t)j * p->update_div + J->phase;
+    int32_t g[3];
+    probe_cell(p, J->w, idx, g);
+    double *pr = J->probes + idx * ORC_PROBE_FIELDS;
+    double *info = pr + 20, *st = pr + 24, *st_info = pr + 44;
+    const int is_new = info[3] < 0.5 || info[0] != (double)g[0] || info[1] != (double)g[1] ||
+                       info[2] != (double)g[2];
+    float off[3];
+    for (int k = 0; k < 3; ++k) off[k] = is_new ? 0.0f : (float)st_info[1 + k];
+    float o[3];
+    for (int k = 0; k < 3; ++k) o[k] = ((float)g[k] + 0.5f) * (float)p->spacing + off[k];
+    if (p->flags & ORC_GI_SKIP_BURIED) {
+        const uint32_t m = orc_material_at_world(J->w, (int32_t)floorf(o[0]), (int32_t)floorf(o[1]),
+                                                 (int32_t)floorf(o[2]));
+        if (solid_for_gi(m)) {
+            memset(pr, 0, ORC_PROBE_FIELDS * sizeof(double));
+            info[0] = g[0];
+            info[1] = g[1];
+            info[2] = g[2];
+            return;
+        }
+    }
+    const uint32_t N = (uint32_t)p->rays_per_probe;
+    const double wpp = 4.0 * M_PI / (double)N;
+    double c[5][4];
+    memset(c, 0, sizeof(c));
+    float near_t = 1.0e30f;
+    uint32_t near_f = ORC_FACE_NONE;
+    for (uint32_t k = 0; k < N; ++k) {
+        const float *dd = J->dirs + 4 * (k * (uint32_t)p->epochs + J->epoch);
+        const float d[3] = {dd[0], dd[1], dd[2]};
+        double rad[4];
+        uint8_t f;
+        float t;
+        orc_gi_sample(J->w, p, J->btab, J->pal, o, d, (uint32_t)idx, k, rad, &f, &t, rays);
+        double b[4];
+        sh_basis(d, b);
+        const double ws[5] = {rad[0], rad[1], rad[2], rad[3], rad[3] * rad[3]};
+        for (int a = 0; a < 5; ++a)
+            for (int q = 0; q < 4; ++q) c[a][q] += wpp * ws[a] * b[q];
+        if (f != ORC_FACE_NONE && t < near_t) {
+            near_t = t;
+            near_f = f;
+        }
+    }
+    if (is_new) {
+        for (int a = 0; a < 5; ++a)
+            for (int q = 0; q < 4; ++q) pr[a * 4 + q] = c[a][q];
+        memset(st, 0, 24 * sizeof(double));
+    } else {
+        for (int a = 0; a < 5; ++a)
+            for (int q = 0; q < 4; ++q) st[a * 4 + q] += c[a][q];
+        st_info[0] += 1.0;
+        if (J->epoch == (uint32_t)p->epochs - 1u) {
+            const double cnt = st_info[0] > 1.0 ? st_info[0] : 1.0;
+            const double h = p->hysteresis;
+            for (int i = 0; i < 20; ++i) pr[i] = (st[i] / cnt) * (1.0 - h) + pr[i] * h;
+            memset(st, 0, 24 * sizeof(double));
+        }
+    }
+    /* relocation in fp32, as the device computes it */
+    float n[3];
+    for (int k = 0; k < 3; ++k) n[k] = face_n(near_f, k);
+    if (near_t < 1.2f) {
+        const float m = (1.2f - near_t) < 0.35f ? (1.2f - near_t) : 0.35f;
+        for (int k = 0; k < 3; ++k) {
+            float v = off[k] + n[k] * m;
+            off[k] = v < -3.5f ? -3.5f : (v > 3.5f ? 3.5f : v);
+        }
+    } else if (near_t > 2.5f) {
+        for (int k = 0; k < 3; ++k) off[k] = off[k] * 0.98f;
+    }
+    for (int k = 0; k < 3; ++k) st_info[1 + k] = off[k];
+    info[0] = g[0];
+    info[1] = g[1];
+    info[2] = g[2];
+    info[3] = 1.0;
+}
+
+static void *upd_worker(void *arg) {
+    upd_job *J = (upd_job *)arg;
+    uint64_t rays[3] = {0, 0, 0};
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        int64_t j = J->next++;
+        pthread_mutex_unlock(&J->mu);
+        if (j >= (int64_t)J->n_upd) break;
+        update_probe(J, (uint32_t)j, rays);
+    }
+    pthread_mutex_lock(&J->mu);
+    for (int k = 0; k < 3; ++k) J->rays[k] += rays[k];
+    pthread_mutex_unlock(&J->mu);
+    return NULL;
+}
+
+static int clamp_threads(int threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    return threads;
+}
+
+void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs,
+                   const float *btab, const float *palette, double *probes, int threads,
+                   uint64_t rays[3]) {
+    const int64_t total = (int64_t)p->grid[0] * p->grid[1] * p->grid[2];
+    const uint32_t div = (uint32_t)p->update_div;
+    upd_job J;
+    memset(&J, 0, sizeof(J));
+    J.w = w;
+    J.p = p;
+    J.dirs = dirs;
+    J.btab = btab;
+    J.pal = palette;
+    J.probes = probes;
+    J.phase = (uint32_t)p->round % div;
+    J.epoch = ((uint32_t)p->round / div) % (uint32_t)p->epochs;
+    J.n_upd = total > (int64_t)J.phase ? (uint32_t)((total - J.phase + div - 1) / div) : 0u;
+    pthread_mutex_init(&J.mu, NULL);
+    threads = clamp_threads(threads);
+    pthread_t th[256];
+    for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, upd_worker, &J);
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&J.mu);
+    if (rays)
+        for (int k = 0; k < 3; ++k) rays[k] += J.rays[k];
+}
+
+typedef struct {
+    const orc_world *w;
+    const orc_gi_params *p;
+    int32_t n_dirs;
+    const float *dirs, *btab, *pal, *origins;
+    const int64_t *slots;
+    int64_t n, next;
+    double *out;
+    pthread_mutex_t mu;
+} ref_job;
+
+static void *ref_worker(void *arg) {
+    ref_job *J = (ref_job *)arg;
+    uint64_t rays[3] = {0, 0, 0};
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        int64_t i = J->next++;
+        pthread_mutex_unlock(&J->mu);
+        if (i >= J->n) break;
+        const float *o = J->origins + 3 * i;
+        const double wpp = 4.0 * M_PI / (double)J->n_dirs;
+        double c[20];
+        memset(c, 0, sizeof(c));
+        for (int32_t k = 0; k < J->n_dirs; ++k) {
+            const float d[3] = {J->dirs[4 * k], J->dirs[4 * k + 1], J->dirs[4 * k + 2]};
+            double rad[4], b[4];
+            uint8_t f;
+            float t;
+            orc_gi_sample(J->w, J->p, J->btab, J->pal, o, d, (uint32_t)J->slots[i], (uint32_t)k,
+                          rad, &f, &t, rays);
+            sh_basis(d, b);
+            const double ws[5] = {rad[0], rad[1], rad[2], rad[3], rad[3] * rad[3]};
+            for (int a = 0; a < 5; ++a)
+                for (int q = 0; q < 4; ++q) c[a * 4 + q] += wpp * ws[a] * b[q];
+        }
+        memcpy(J->out + 20 * i, c, sizeof(c));
+    }
+    return NULL;
+}
+
+void orc_gi_reference(const orc_world *w, const orc_gi_params *p, int32_t n_dirs,
+                      const float *dirs, const float *btab, const float *palette,
+                      const int64_t *slots, const float *origins, int64_t n, double *out,
+                      int threads) {
+    ref_job J;
+    memset(&J, 0, sizeof(J));
+    J.w = w;
+    J.p = p;
+    J.n_dirs = n_dirs;
+    J.dirs = dirs;
+    J.btab = btab;
+    J.pal = palette;
+    J.origins = origins;
+    J.slots = slots;
+    J.n = n;
+    J.out = out;
+    pthread_mutex_init(&J.mu, NULL);
+    threads = clamp_threads(threads);
+    pthread_t th[256];
+    for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, ref_worker, &J);
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&J.mu);
+}

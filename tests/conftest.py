@@ -25,3 +25,15 @@ def pytest_sessionstart(session):
 def small_world():
     from voxelg_amd import workloads
     return workloads.SMALL.world()
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from voxelg_amd.device import Context, device_count
+    assert device_count() > 0, "GPU tests need a HIP device"
+    return Context(0)
+
+
+@pytest.fixture(scope="session")
+def small(ctx, small_world):
+    return ctx.world(small_world.dims).upload(small_world)

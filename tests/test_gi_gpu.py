@@ -1,0 +1,129 @@
+"""GPU parity of the probe update (vg_probes_update) against the FP64 oracle on the same
+samples: every gather / shadow / bounce ray is traced bit-exactly on both sides, so the
+records agree to fp32 summation error in the SH sums and exactly in the integer-valued
+fields (grid cell, age, staging count) and the fp32 relocation offsets."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from voxelg_amd import gi
+from voxelg_amd._lib import VGError
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+SUN = (math.cos(math.radians(35.0)) * math.cos(2.1), math.sin(math.radians(35.0)),
+       math.cos(math.radians(35.0)) * math.sin(2.1))
+
+
+def as_fields(probes: np.ndarray) -> np.ndarray:
+    return probes.view(np.float32).reshape(len(probes), 48).astype(np.float64)
+
+
+def assert_records_match(got: np.ndarray, ref: np.ndarray, what: str):
+    g = as_fields(got)
+    # integer-valued and fp32 fields: exact (info, staging count, relocation offset)
+    for col in (20, 21, 22, 23, 44, 45, 46, 47):
+        bad = np.nonzero(g[:, col] != ref[:, col])[0]
+        assert len(bad) == 0, f"{what}: field {col} differs at {bad[:4]}: {g[bad[:2], col]} " \
+                              f"vs {ref[bad[:2], col]}"
+    # SH sums (displayed 0..19, staging 24..43): fp32 summation of the same samples
+    for base in (0, 24):
+        for grp in range(5):
+            cols = slice(base + 4 * grp, base + 4 * grp + 4)
+            scale = max(np.abs(ref[:, base + 4 * grp]).max(), 1e-6)
+            err = np.abs(g[:, cols] - ref[:, cols])
+            tol = 2e-5 * scale + 2e-5 * np.abs(ref[:, cols])
+            bad = np.argwhere(err > tol)
+            assert len(bad) == 0, f"{what}: SH group {base}+{grp} off at {bad[:3]}: " \
+                                  f"{g[bad[0][0], cols]} vs {ref[bad[0][0], cols]}"
+
+
+def run_rounds(ctx, dw, world, params, rounds, check_every=1):
+    dirs = gi.directions(int(params[0]["rays_per_probe"]) * int(params[0]["epochs"]))
+    bt, pal = gi.bounce_table(), gi.default_palette()
+    grid = gi.ProbeGrid(dw, params)
+    ow = oracle.OracleWorld.view(world)
+    ref = np.zeros((grid.total, 48), np.float64)
+    p = params.copy()
+    for r in rounds:
+        p["round"] = r
+        print(f"case: probes {tuple(p[0]['grid'])} rays {p[0]['rays_per_probe']} "
+              f"bounces {p[0]['bounces']} round {r}", flush=True)
+        grid.update(p)
+        ow.gi_update(p, dirs, bt, pal, ref, threads=THREADS)
+        if (r - rounds[0]) % check_every == 0 or r == rounds[-1]:
+            ctx.synchronize()
+            assert_records_match(grid.download(), ref, f"round {r}")
+    return grid, ref
+
+
+def test_plain_two_bounce_matches_oracle(ctx, small, small_world):
+    p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=128, epochs=2,
+                     update_div=1, bounces=2)
+    grid, ref = run_rounds(ctx, small, small_world, p, range(4))
+    live = ref[:, 23] == 1.0
+    assert 0.3 < live.mean() < 1.0  # some probes buried in the terrain, most live
+    assert np.any(ref[live, 44] != 0) or np.any(ref[live, :20] != 0)
+
+
+def test_reference_defaults_one_bounce_matches_oracle(ctx, small, small_world):
+    """The reference's own schedule: 8 rays, 8 direction epochs, 1/8 of the probes per
+    round, one bounce, spacing 8 (a 32 x 16 x 32 grid over the 256 x 128 x 256 world), over
+    72 rounds: every probe seeds, stages all 8 epochs, folds once and relocates."""
+    p = gi.gi_params(SUN, grid=(32, 16, 32))
+    grid, ref = run_rounds(ctx, small, small_world, p, range(72), check_every=8)
+    live = ref[:, 23] == 1.0
+    assert live.mean() > 0.3
+    assert np.count_nonzero(ref[live][:, 45:48]) > 0  # some probes relocated
+
+
+def test_ray_counts_match_oracle(ctx, small, small_world):
+    p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=64, epochs=1,
+                     update_div=1, bounces=2)
+    grid = gi.ProbeGrid(small, p)
+    rays, cnt = grid.update_counts(p)
+    ref = np.zeros((grid.total, 48))
+    orays = oracle.OracleWorld.view(small_world).gi_update(
+        p, gi.directions(64), gi.bounce_table(), gi.default_palette(), ref, threads=THREADS)
+    assert [rays["gather"], rays["shadow"], rays["bounce"]] == orays
+    assert rays["bounce"] > 0 and rays["shadow"] > 0
+    assert sum(cnt.values()) > rays["gather"]
+    assert_records_match(grid.download(), ref, "counts run")
+
+
+def test_palette_and_reset(ctx, small, small_world):
+    p = gi.gi_params(SUN, grid=(4, 4, 4), spacing=64, rays_per_probe=64, epochs=1,
+                     update_div=1, bounces=1)
+    pal = gi.default_palette()
+    pal[1:40, :3] *= 0.5
+    grid = gi.ProbeGrid(small, p).set_palette(pal)
+    grid.update(p)
+    ref = np.zeros((grid.total, 48))
+    oracle.OracleWorld.view(small_world).gi_update(p, gi.directions(64), gi.bounce_table(), pal,
+                                                   ref, threads=THREADS)
+    ctx.synchronize()
+    assert_records_match(grid.download(), ref, "palette")
+    grid.reset()
+    ctx.synchronize()
+    assert np.all(as_fields(grid.download()) == 0)
+
+
+def test_param_errors(ctx, small):
+    p = gi.gi_params(SUN, grid=(4, 4, 4), spacing=64, rays_per_probe=16, epochs=1)
+    grid = gi.ProbeGrid(small, p)
+    q = p.copy()
+    q["grid"] = (4, 4, 8)
+    with pytest.raises(VGError):
+        grid.update(q)
+    q = p.copy()
+    q["rays_per_probe"] = 32
+    with pytest.raises(VGError):
+        grid.update(q)
+    q = p.copy()
+    q["bounces"] = 3
+    with pytest.raises(VGError):
+        grid.update(q)

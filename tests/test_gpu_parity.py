@@ -15,18 +15,6 @@ pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    from voxelg_amd.device import Context, device_count
-    assert device_count() > 0, "GPU tests need a HIP device"
-    return Context(0)
-
-
-@pytest.fixture(scope="module")
-def small(ctx, small_world):
-    return ctx.world(small_world.dims).upload(small_world)
-
-
 def gpu_hits(h):
     from voxelg_amd.device import decode_hits
     return decode_hits(h)

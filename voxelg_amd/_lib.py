@@ -23,8 +23,26 @@ HIT_DTYPE = np.dtype(
      ("pad", "u1")]
 )
 BATCH_DTYPE = np.dtype([("rays", "<u8"), ("hits", "<u8"), ("n", "<u4"), ("flags", "<u4")])
+# vg_probe: GiProbe (shaders/gi_probes.wgsl:25-48), 12 x vec4 f32.
+PROBE_DTYPE = np.dtype([
+    ("sh_r", "<f4", (4,)), ("sh_g", "<f4", (4,)), ("sh_b", "<f4", (4,)),
+    ("sh_dist", "<f4", (4,)), ("sh_dist2", "<f4", (4,)), ("info", "<f4", (4,)),
+    ("st_r", "<f4", (4,)), ("st_g", "<f4", (4,)), ("st_b", "<f4", (4,)),
+    ("st_dist", "<f4", (4,)), ("st_dist2", "<f4", (4,)), ("st_info", "<f4", (4,))])
+# vg_gi_params
+GI_PARAMS_DTYPE = np.dtype([
+    ("grid", "<i4", (3,)), ("spacing", "<i4"), ("rays_per_probe", "<i4"), ("epochs", "<i4"),
+    ("update_div", "<i4"), ("round", "<i4"), ("bounces", "<i4"), ("seed", "<u4"),
+    ("flags", "<u4"), ("hysteresis", "<f4"), ("gi_dist", "<f4"), ("shadow_tmax", "<f4"),
+    ("sun", "<f4", (3,)), ("gi_sky", "<f4"), ("sun_color", "<f4", (3,)),
+    ("ambient_tail", "<f4"), ("ambient", "<f4", (3,)), ("day_t", "<f4"),
+    ("sky_horizon", "<f4", (3,)), ("dusk_w", "<f4"), ("sky_zenith", "<f4", (3,)),
+    ("halo", "<f4"), ("sun_xz", "<f4", (2,)), ("pad", "<f4", (2,))])
 assert BRICK_DTYPE.itemsize == 72 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 20
-assert BATCH_DTYPE.itemsize == 24
+assert BATCH_DTYPE.itemsize == 24 and PROBE_DTYPE.itemsize == 192
+assert GI_PARAMS_DTYPE.itemsize == 152
+VG_GI_SKIP_BURIED = 1
+VG_GI_BOUNCE_TABLE = 4096
 
 VG_OK = 0
 VG_SCHEDULE_PERSISTENT = 0
@@ -127,6 +145,19 @@ SIGNATURES = {
     "vg_graph_end": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "vg_graph_launch": (C.c_int, [_P, _P]),
     "vg_graph_destroy": (C.c_int, [_P]),
+    "vg_gi_params_default": (C.c_int, [_F32P, _P]),
+    "vg_gi_default_palette": (C.c_int, [_P]),
+    "vg_gi_directions": (C.c_int, [C.c_int32, _P]),
+    "vg_gi_bounce_table": (C.c_int, [_P]),
+    "vg_probes_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "vg_probes_destroy": (C.c_int, [_P]),
+    "vg_probes_set_palette": (C.c_int, [_P, _P]),
+    "vg_probes_reset": (C.c_int, [_P, _P]),
+    "vg_probes_data": (C.c_int, [_P, C.POINTER(_P)]),
+    "vg_probes_download": (C.c_int, [_P, _P]),
+    "vg_probes_upload": (C.c_int, [_P, _P]),
+    "vg_probes_update": (C.c_int, [_P, _P, _P]),
+    "vg_probes_update_counts": (C.c_int, [_P, _P, _U64P, _U64P]),
 }
 
 

@@ -1,0 +1,732 @@
+// GI probe update for gfx950: cs_gi_probe_update (shaders/gi_probes.wgsl:274-396) as
+// three launches on one stream.
+//   k_probe_prep    one thread per updated probe: its world grid cell, whether the slot is
+//                   new, its (relocated) ray origin and whether it is buried in a solid voxel.
+//   k_probe_paths   persistent, ray-compacting: one lane per gather sample runs the whole
+//                   path (gather ray -> sun shadow ray at the hit -> optional cosine bounce
+//                   ray -> sun shadow ray at the bounce hit) through the exact walk, refilling
+//                   idle lanes like the frame kernel, and writes the sample's radiance and
+//                   visible distance.
+//   k_probe_reduce  one wave per updated probe: SH-L1 projection of its samples in a fixed
+//                   lane order (deterministic), the nearest hit for relocation, the staging
+//                   sum and the hysteresis fold of complete cycles.
+// Samples of one probe have consecutive ids, so a wave's gather rays share their origin.
+#include "vg_world.h"
+
+struct vg_probes {
+    vg_world *world;
+    int32_t grid[3];
+    int32_t rays, epochs;
+    int64_t total;          // probes
+    vg_probe *probes;       // total records
+    float4 *dirs;           // rays * epochs gather directions
+    float4 *bounce;         // VG_GI_BOUNCE_TABLE cosine-hemisphere directions
+    float4 *palette;        // 256 albedo entries
+    float4 *pwork;          // per updated probe: ray origin xyz, w = flags bits
+    float4 *rad;            // per sample: radiance rgb, visible distance
+    uint8_t *face;          // per sample: face code of the gather hit (VG_FACE_NONE: none)
+    uint32_t *ctr;          // persistent counter
+    unsigned long long *counts; // VG_IT_COUNT walk classes + 3 ray kinds
+};
+
+namespace vg {
+
+constexpr uint32_t kProbeNew = 1u, kProbeBuried = 2u;
+constexpr int kGiCounters = VG_IT_COUNT + 3;
+
+// ----------------------------------------------------------------- helpers --
+__device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// sky_color (shaders/raymarch.wgsl:2532-2579) for a daytime or dusk sun: horizon-zenith
+// blend, the dusk warm wash, the forward-scatter halo and the sun disc. The night stars
+// term is not evaluated.
+__device__ __forceinline__ void sky_color(const vg_gi_params &p, const float d[3], float out[3]) {
+    const float up = d[1] < -0.2f ? -0.2f : (d[1] > 1.0f ? 1.0f : d[1]);
+    const float zf = gi_smooth(0.0f, 0.55f, up);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = p.sky_horizon[k] * (1.0f - zf) + p.sky_zenith[k] * zf;
+    if (p.dusk_w > 0.0f) {
+        const float l = sqrtf(d[0] * d[0] + d[2] * d[2]);
+        float azi = 0.0f;
+        if (l > 0.0f) {
+            azi = (d[0] / l) * p.sun_xz[0] + (d[2] / l) * p.sun_xz[1];
+            azi = azi < 0.0f ? 0.0f : (azi > 1.0f ? 1.0f : azi);
+        }
+        const float wgt = p.dusk_w * (azi * azi * azi) * (1.0f - zf);
+        out[0] += 1.50f * wgt;
+        out[1] += 0.55f * wgt;
+        out[2] += 0.18f * wgt;
+    }
+    float sa = d[0] * p.sun[0] + d[1] * p.sun[1] + d[2] * p.sun[2];
+    sa = sa > 0.0f ? sa : 0.0f;
+    const float s2 = sa * sa;
+    const float h = s2 * s2 * s2 * p.halo;
+    out[0] += 1.0f * h;
+    out[1] += 0.78f * h;
+    out[2] += 0.52f * h;
+    if (sa > 0.9985f) {
+        out[0] = out[0] * 0.1f + 2.4f * 0.9f;
+        out[1] = out[1] * 0.1f + 2.0f * 0.9f;
+        out[2] = out[2] * 0.1f + 1.5f * 0.9f;
+    }
+}
+
+// Outward normal component of face code f on axis k.
+__device__ __forceinline__ float face_n(uint32_t f, int k) {
+    return (f < 6u && (int)(f >> 1) == k) ? ((f & 1u) ? -1.0f : 1.0f) : 0.0f;
+}
+
+// max(0, n . sun) for an axis-aligned face normal.
+__device__ __forceinline__ float face_ndl(uint32_t f, const float s[3]) {
+    if (f >= 6u) return 0.0f;
+    const float v = (f & 1u) ? -s[f >> 1] : s[f >> 1];
+    return v > 0.0f ? v : 0.0f;
+}
+
+// Secondary ray origin off a hit: (o + d t) + n * 0.03 (shaders/gi_probes.wgsl:239).
+__device__ __forceinline__ void gi_offset_origin(const float o[3], const float d[3], float t,
+                                                 uint32_t f, float out[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = (o[k] + d[k] * t) + face_n(f, k) * 0.03f;
+}
+
+// lowbias32 integer mixer (C. Wellons): the bounce-table index of a sample.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t bounce_index(uint32_t seed, int32_t round, uint32_t probe,
+                                                 uint32_t k) {
+    uint32_t x = mix32(seed ^ ((uint32_t)round * 0x85EBCA6Bu));
+    x = mix32(x ^ probe);
+    x = mix32(x ^ k);
+    return x >> 20; // VG_GI_BOUNCE_TABLE = 4096 entries
+}
+
+// Cosine-hemisphere table entry (t, b, c) turned to the hit face: c along the outward
+// normal, t and b on the next two axes in cyclic order.
+__device__ __forceinline__ void bounce_dir(const float4 e, uint32_t f, float out[3]) {
+    const int ax = (int)(f >> 1);
+    const float sg = (f & 1u) ? -1.0f : 1.0f;
+    out[ax] = sg * e.z;
+    out[ax == 2 ? 0 : ax + 1] = e.x;
+    out[ax == 0 ? 2 : ax - 1] = e.y;
+}
+
+// Material of a world voxel through the pyramid (0 outside the window).
+__device__ __forceinline__ uint32_t material_at(const DevWorld &w, const int32_t v[3]) {
+    int32_t sl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int32_t l = v[k] - w.origin[k];
+        if ((uint32_t)l >= (uint32_t)w.dims[k]) return 0u;
+        int32_t s = l + w.ro_mod[k];
+        sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
+    }
+    const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
+                         (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
+    if (!((w.l4[l4i] >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1))
+        return 0u;
+    const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                        (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+    const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
+    if (!((w.chunk[ci] >> tb) & 1)) return 0u;
+    const uint32_t tn = ci * 64u + (uint32_t)tb;
+    if (w.tile_uni[tn]) return w.tile_uni[tn];
+    const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
+    if (!((w.tile[tn] >> bb) & 1)) return 0u;
+    const uint32_t bn = tn * 64u + (uint32_t)bb;
+    if (w.brick_uni[bn]) return w.brick_uni[bn];
+    const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
+    if (!((w.occ[bn] >> vb) & 1)) return 0u;
+    return w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+}
+
+// World grid cell of probe slot idx for the current window (shaders/gi_probes.wgsl:281-297).
+__device__ __forceinline__ void probe_cell(const vg_gi_params &p, const DevWorld &w, int64_t idx,
+                                           int32_t g[3]) {
+    const int32_t sx = (int32_t)(idx % p.grid[0]);
+    const int32_t sy = (int32_t)((idx / p.grid[0]) % p.grid[1]);
+    const int32_t sz = (int32_t)(idx / ((int64_t)p.grid[0] * p.grid[1]));
+    const int32_t ox = w.origin[0] / p.spacing, oz = w.origin[2] / p.spacing;
+    int32_t mx = (sx - ox) % p.grid[0];
+    mx = mx < 0 ? mx + p.grid[0] : mx;
+    int32_t mz = (sz - oz) % p.grid[2];
+    mz = mz < 0 ? mz + p.grid[2] : mz;
+    g[0] = ox + mx;
+    g[1] = sy;
+    g[2] = oz + mz;
+}
+
+// ----------------------------------------------------------------- kernels --
+__global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params p, uint32_t n_upd,
+                                                      uint32_t phase, const vg_probe *probes,
+                                                      float4 *pwork, uint32_t *ctr) {
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j == 0) *ctr = 0u; // the path kernel enqueued next consumes the counter
+    if (j >= n_upd) return;
+    const int64_t idx = (int64_t)j * p.update_div + phase;
+    int32_t g[3];
+    probe_cell(p, w, idx, g);
+    const float4 *rec = reinterpret_cast<const float4 *>(probes + idx);
+    const float4 info = rec[5], st_info = rec[11];
+    const bool is_new = info.w < 0.5f || info.x != (float)g[0] || info.y != (float)g[1] ||
+                        info.z != (float)g[2];
+    const float off[3] = {is_new ? 0.0f : st_info.y, is_new ? 0.0f : st_info.z,
+                          is_new ? 0.0f : st_info.w};
+    float o[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = ((float)g[k] + 0.5f) * (float)p.spacing + off[k];
+    uint32_t fl = is_new ? kProbeNew : 0u;
+    if (p.flags & VG_GI_SKIP_BURIED) {
+        const int32_t v[3] = {(int32_t)floorf(o[0]), (int32_t)floorf(o[1]), (int32_t)floorf(o[2])};
+        if (solid_mat(material_at(w, v), VG_RAY_SKIP_TRANSPARENT)) fl |= kProbeBuried;
+    }
+    pwork[j] = make_float4(o[0], o[1], o[2], __uint_as_float(fl));
+}
+
+// Path stages of a lane.
+enum : int { kIdle = 0, kGather = 1, kShadowX = 2, kBounce = 3, kShadowY = 4 };
+
+template <bool COUNT, int BOUNCES>
+__global__ __launch_bounds__(kBlock) void k_probe_paths(
+    DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
+    const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
+    const float4 *__restrict__ btab, const float4 *__restrict__ pal, float4 *__restrict__ rad,
+    uint8_t *__restrict__ face_out, uint32_t *ctr, int threshold, unsigned long long *counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cnt[VG_IT_COUNT];
+    uint32_t nray[3] = {0, 0, 0};
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    const uint32_t N = (uint32_t)p.rays_per_probe;
+    int stage = kIdle;
+    uint32_t sid = 0;
+    float tX = 0.0f, ndlX = 0.0f, visX = 0.0f, ndlY = 0.0f;
+    uint32_t fX = VG_FACE_NONE, mX = 0, mY = 0;
+    float so[3] = {0.0f, 0.0f, 0.0f}; // secondary origin off the gather hit
+    float lb[3] = {0.0f, 0.0f, 0.0f}; // radiance arriving at the gather hit from its bounce
+    int it = 0;
+    Walk r;
+    WalkCache q;
+    HitRec h;
+    hit_miss(h);
+    cache_reset(q);
+    bool exhausted = false;
+
+    // Finish the sample: radiance toward the probe and its visible distance.
+    auto finish_hit = [&]() {
+        const float4 a = pal[mX];
+        const float sx = ndlX * visX;
+        float4 out;
+        out.x = a.x * (p.sun_color[0] * sx + lb[0]);
+        out.y = a.y * (p.sun_color[1] * sx + lb[1]);
+        out.z = a.z * (p.sun_color[2] * sx + lb[2]);
+        out.w = tX;
+        rad[sid] = out;
+        face_out[sid] = (uint8_t)fX;
+        stage = kIdle;
+    };
+    auto tail = [&]() {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lb[k] = p.ambient[k] * p.ambient_tail;
+    };
+    auto start = [&](const float o[3], const float d[3], int st) -> bool {
+        hit_miss(h);
+        if (!walk_init(r, w, o, d)) return false;
+        cache_reset(q);
+        it = 0;
+        stage = st;
+        if (COUNT) nray[st == kGather ? 0 : (st == kBounce ? 2 : 1)]++;
+        return true;
+    };
+    // the gather direction and origin of the current sample
+    auto gather_ray = [&](float o[3], float d[3]) {
+        const float4 po = pwork[sid / N];
+        const float4 dd = dirs[(sid % N) * (uint32_t)p.epochs + epoch];
+        o[0] = po.x;
+        o[1] = po.y;
+        o[2] = po.z;
+        d[0] = dd.x;
+        d[1] = dd.y;
+        d[2] = dd.z;
+    };
+    auto bounce_ray = [&](float d[3]) {
+        const uint32_t j = sid / N;
+        const uint32_t probe = j * (uint32_t)p.update_div + phase;
+        bounce_dir(btab[bounce_index(p.seed, p.round, probe, sid % N)], fX, d);
+    };
+    // After the shadow ray at the gather hit (or none): bounce or tail, then finish.
+    auto after_x = [&]() {
+        if (BOUNCES == 2) {
+            float d[3];
+            bounce_ray(d);
+            if (start(so, d, kBounce)) return;
+            float sky[3];
+            sky_color(p, d, sky);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lb[k] = sky[k] * p.gi_sky;
+        } else {
+            tail();
+        }
+        finish_hit();
+    };
+    auto after_y = [&](float visY) {
+        const float4 a = pal[mY];
+        const float sy = ndlY * visY;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            lb[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
+        finish_hit();
+    };
+
+    for (;;) {
+        uint64_t act = __ballot(stage != kIdle);
+        int nact = __popcll(act);
+        if (!exhausted && nact < threshold) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(ctr, (uint32_t)(64 - nact));
+            base = __shfl(base, 0, 64);
+            if (base >= n_samples) {
+                exhausted = true;
+            } else if (stage == kIdle) {
+                const uint32_t id = base + lanes_below(~act);
+                if (id < n_samples) {
+                    sid = id;
+                    const uint32_t fl = __float_as_uint(pwork[sid / N].w);
+                    if (!(fl & kProbeBuried)) {
+                        float o[3], d[3];
+                        gather_ray(o, d);
+                        if (!start(o, d, kGather)) {
+                            float sky[3];
+                            sky_color(p, d, sky);
+                            rad[sid] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky,
+                                                   sky[2] * p.gi_sky, p.gi_dist);
+                            face_out[sid] = (uint8_t)VG_FACE_NONE;
+                        }
+                    }
+                }
+            }
+            nact = __popcll(__ballot(stage != kIdle));
+        }
+        if (nact == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        do {
+            if (stage != kIdle) {
+                const bool shadow = stage == kShadowX || stage == kShadowY;
+                bool done = walk_iter<COUNT>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                                             shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
+                                             false, h, cnt);
+                done = done || ++it >= max_iter;
+                if (done) {
+                    if (stage == kGather) {
+                        float o[3], d[3];
+                        gather_ray(o, d);
+                        if (!h.hit) {
+                            float sky[3];
+                            sky_color(p, d, sky);
+                            rad[sid] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky,
+                                                   sky[2] * p.gi_sky, p.gi_dist);
+                            face_out[sid] = (uint8_t)VG_FACE_NONE;
+                            stage = kIdle;
+                        } else {
+                            tX = h.t;
+                            fX = h.face;
+                            mX = h.mat;
+                            ndlX = face_ndl(fX, p.sun);
+                            visX = 0.0f;
+                            gi_offset_origin(o, d, tX, fX, so);
+                            if (ndlX > 0.0f) {
+                                if (!start(so, p.sun, kShadowX)) {
+                                    visX = 1.0f;
+                                    after_x();
+                                }
+                            } else {
+                                after_x();
+                            }
+                        }
+                    } else if (stage == kShadowX) {
+                        visX = h.hit ? 0.0f : 1.0f;
+                        after_x();
+                    } else if (stage == kBounce) {
+                        if (!h.hit) {
+                            float d[3], sky[3];
+                            bounce_ray(d);
+                            sky_color(p, d, sky);
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) lb[k] = sky[k] * p.gi_sky;
+                            finish_hit();
+                        } else {
+                            float d[3], yo[3];
+                            bounce_ray(d);
+                            const uint32_t fY = h.face;
+                            mY = h.mat;
+                            ndlY = face_ndl(fY, p.sun);
+                            gi_offset_origin(so, d, h.t, fY, yo);
+                            if (ndlY > 0.0f) {
+                                if (!start(yo, p.sun, kShadowY)) after_y(1.0f);
+                            } else {
+                                after_y(0.0f);
+                            }
+                        }
+                    } else { // kShadowY
+                        after_y(h.hit ? 0.0f : 1.0f);
+                    }
+                }
+            }
+        } while (__popcll(__ballot(stage != kIdle)) >= threshold);
+    }
+    if (COUNT) {
+        flush_counts(cnt, counts);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            unsigned long long v = nray[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0 && v) atomicAdd(&counts[VG_IT_COUNT + k], v);
+        }
+    }
+}
+
+// One wave per updated probe: project its samples, fold, relocate, write the record.
+__global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_params p, uint32_t n_upd,
+                                                        uint32_t epoch, uint32_t phase,
+                                                        const float4 *__restrict__ pwork,
+                                                        const float4 *__restrict__ dirs,
+                                                        const float4 *__restrict__ rad,
+                                                        const uint8_t *__restrict__ face,
+                                                        vg_probe *probes) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t j = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (j >= n_upd) return; // wave-uniform
+    const int64_t idx = (int64_t)j * p.update_div + phase;
+    const uint32_t N = (uint32_t)p.rays_per_probe;
+    const float4 pw = pwork[j];
+    const uint32_t fl = __float_as_uint(pw.w);
+    int32_t g[3];
+    probe_cell(p, w, idx, g);
+    float4 *rec = reinterpret_cast<float4 *>(probes + idx);
+    if (fl & kProbeBuried) {
+        if (lane < 12) rec[lane] = lane == 5 ? make_float4((float)g[0], (float)g[1], (float)g[2], 0.0f)
+                                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float wpp = 12.566371f / (float)N; // 4 pi / N (uniform-sphere MC weight)
+    float c[5][4];
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) c[a][b] = 0.0f;
+    float near_t = 1.0e30f;
+    uint32_t near_k = 0xFFFFFFFFu;
+    for (uint32_t k = lane; k < N; k += 64) {
+        const uint32_t s = j * N + k;
+        const float4 v = rad[s];
+        const float4 dd = dirs[k * (uint32_t)p.epochs + epoch];
+        const float bs[4] = {0.282095f, 0.488603f * dd.y, 0.488603f * dd.z, 0.488603f * dd.x};
+        const float ws[5] = {wpp * v.x, wpp * v.y, wpp * v.z, wpp * v.w, wpp * v.w * v.w};
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) c[a][b] += ws[a] * bs[b];
+        if (face[s] != VG_FACE_NONE && v.w < near_t) {
+            near_t = v.w;
+            near_k = k;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            float v = c[a][b];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            c[a][b] = v;
+        }
+    // nearest hit, ties to the lowest sample (the reference's ascending loop with <)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ot = __shfl_xor(near_t, off, 64);
+        const uint32_t ok = __shfl_xor(near_k, off, 64);
+        if (ot < near_t || (ot == near_t && ok < near_k)) {
+            near_t = ot;
+            near_k = ok;
+        }
+    }
+    if (lane != 0) return;
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    if (near_k != 0xFFFFFFFFu) {
+        const uint32_t f = face[j * N + near_k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) n[k] = face_n(f, k);
+    }
+    float4 pr[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) pr[i] = rec[i];
+    const bool is_new = (fl & kProbeNew) != 0;
+    const float off_prev[3] = {is_new ? 0.0f : pr[11].y, is_new ? 0.0f : pr[11].z,
+                               is_new ? 0.0f : pr[11].w};
+    auto v4 = [](const float *a) { return make_float4(a[0], a[1], a[2], a[3]); };
+    if (is_new) {
+        pr[0] = v4(c[0]);
+        pr[1] = v4(c[1]);
+        pr[2] = v4(c[2]);
+        pr[3] = v4(c[3]);
+        pr[4] = v4(c[4]);
+#pragma unroll
+        for (int i = 6; i < 12; ++i) pr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        float *st = &pr[6].x;
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) st[a * 4 + b] = st[a * 4 + b] + c[a][b];
+        pr[11].x = pr[11].x + 1.0f;
+        if (epoch == (uint32_t)p.epochs - 1u) {
+            const float cnt = pr[11].x > 1.0f ? pr[11].x : 1.0f;
+            const float h = p.hysteresis;
+            float *sh = &pr[0].x;
+#pragma unroll
+            for (int i = 0; i < 20; ++i) sh[i] = (st[i] / cnt) * (1.0f - h) + sh[i] * h;
+#pragma unroll
+            for (int i = 6; i < 12; ++i) pr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    // relocation (shaders/gi_probes.wgsl:381-392)
+    float off[3] = {off_prev[0], off_prev[1], off_prev[2]};
+    if (near_t < 1.2f) {
+        const float m = (1.2f - near_t) < 0.35f ? (1.2f - near_t) : 0.35f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float v = off[k] + n[k] * m;
+            off[k] = v < -3.5f ? -3.5f : (v > 3.5f ? 3.5f : v);
+        }
+    } else if (near_t > 2.5f) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) off[k] = off[k] * 0.98f;
+    }
+    pr[11] = make_float4(pr[11].x, off[0], off[1], off[2]);
+    pr[5] = make_float4((float)g[0], (float)g[1], (float)g[2], 1.0f);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) rec[i] = pr[i];
+}
+
+// --------------------------------------------------------------------- host --
+static int check_params(const vg_probes *pr, const vg_gi_params *p) {
+    if (!p) return fail(VG_E_INVALID, "null params");
+    for (int k = 0; k < 3; ++k)
+        if (p->grid[k] != pr->grid[k])
+            return fail(VG_E_INVALID, "params grid %dx%dx%d does not match the probes' %dx%dx%d",
+                        p->grid[0], p->grid[1], p->grid[2], pr->grid[0], pr->grid[1], pr->grid[2]);
+    if (p->rays_per_probe != pr->rays || p->epochs != pr->epochs)
+        return fail(VG_E_INVALID, "rays/epochs %d/%d do not match the probes' %d/%d",
+                    p->rays_per_probe, p->epochs, pr->rays, pr->epochs);
+    if (p->spacing <= 0) return fail(VG_E_INVALID, "spacing %d", p->spacing);
+    if (p->update_div <= 0) return fail(VG_E_INVALID, "update_div %d", p->update_div);
+    if (p->round < 0) return fail(VG_E_INVALID, "round %d", p->round);
+    if (p->bounces != 1 && p->bounces != 2)
+        return fail(VG_E_UNSUPPORTED, "bounces %d (1 or 2)", p->bounces);
+    return VG_OK;
+}
+
+static int probes_update(vg_probes *pr, const vg_gi_params *p, bool count, hipStream_t stream) {
+    if (!pr) return fail(VG_E_INVALID, "null probes");
+    int st = check_world(pr->world);
+    if (st) return st;
+    st = check_params(pr, p);
+    if (st) return st;
+    vg_world *w = pr->world;
+    const uint32_t div = (uint32_t)p->update_div;
+    const uint32_t phase = (uint32_t)p->round % div;
+    const uint32_t epoch = ((uint32_t)p->round / div) % (uint32_t)p->epochs;
+    const int64_t n_upd64 = pr->total > (int64_t)phase ? (pr->total - phase + div - 1) / div : 0;
+    const uint32_t n_upd = (uint32_t)n_upd64;
+    if (n_upd == 0) return VG_OK;
+    const uint64_t n_samples = (uint64_t)n_upd * (uint64_t)pr->rays;
+    unsigned long long *cd = count ? pr->counts : nullptr;
+    hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                       w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr);
+    HIP_TRY(hipGetLastError());
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
+    const int thr = w->ctx->rays_refill;
+#define LAUNCH_G(C, B)                                                                         \
+    hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
+                       (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
+                       pr->palette, pr->rad, pr->face, pr->ctr, thr, cd)
+    if (count) {
+        if (p->bounces == 2) LAUNCH_G(true, 2);
+        else LAUNCH_G(true, 1);
+    } else {
+        if (p->bounces == 2) LAUNCH_G(false, 2);
+        else LAUNCH_G(false, 1);
+    }
+#undef LAUNCH_G
+    HIP_TRY(hipGetLastError());
+    const uint32_t wpb = kBlock / 64;
+    hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + wpb - 1) / wpb), dim3(kBlock), 0, stream,
+                       w->dw, *p, n_upd, epoch, phase, pr->pwork, pr->dirs, pr->rad, pr->face,
+                       pr->probes);
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
+}
+
+static void free_probes(vg_probes *p) {
+    void *ptrs[] = {p->probes, p->dirs, p->bounce, p->palette, p->pwork,
+                    p->rad,    p->face, p->ctr,    p->counts};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+}
+
+} // namespace vg
+
+using namespace vg;
+
+extern "C" {
+
+int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
+    if (!world || !p || !out) return fail(VG_E_INVALID, "null argument");
+    *out = nullptr;
+    int st = set_device(world);
+    if (st) return st;
+    for (int k = 0; k < 3; ++k)
+        if (p->grid[k] <= 0) return fail(VG_E_INVALID, "grid[%d] = %d", k, p->grid[k]);
+    if (p->rays_per_probe <= 0 || p->rays_per_probe > 65536 || p->epochs <= 0 ||
+        (int64_t)p->rays_per_probe * p->epochs > (1 << 24))
+        return fail(VG_E_INVALID, "rays_per_probe %d, epochs %d", p->rays_per_probe, p->epochs);
+    const int64_t total = (int64_t)p->grid[0] * p->grid[1] * p->grid[2];
+    if (total > (1ll << 26) || total * p->rays_per_probe > 0xF0000000ll)
+        return fail(VG_E_INVALID, "probe grid too large");
+    vg_probes *pr = new vg_probes();
+    std::memset(pr, 0, sizeof(*pr));
+    pr->world = world;
+    for (int k = 0; k < 3; ++k) pr->grid[k] = p->grid[k];
+    pr->rays = p->rays_per_probe;
+    pr->epochs = p->epochs;
+    pr->total = total;
+    const size_t nd = (size_t)p->rays_per_probe * p->epochs;
+    const size_t ns = (size_t)total * p->rays_per_probe;
+    bool ok = hipMalloc((void **)&pr->probes, (size_t)total * sizeof(vg_probe)) == hipSuccess &&
+              hipMalloc((void **)&pr->dirs, nd * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->bounce, VG_GI_BOUNCE_TABLE * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->palette, 256 * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->pwork, (size_t)total * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->rad, ns * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->face, ns) == hipSuccess &&
+              hipMalloc((void **)&pr->ctr, 256) == hipSuccess &&
+              hipMalloc((void **)&pr->counts, kGiCounters * sizeof(unsigned long long)) ==
+                  hipSuccess;
+    if (!ok) {
+        free_probes(pr);
+        delete pr;
+        return fail(VG_E_NOMEM, "probe storage for %lld probes x %d rays", (long long)total,
+                    p->rays_per_probe);
+    }
+    std::vector<float> tab(nd * 4);
+    vg_gi_directions((int32_t)nd, tab.data());
+    std::vector<float> bt(VG_GI_BOUNCE_TABLE * 4), pal(256 * 4);
+    vg_gi_bounce_table(bt.data());
+    vg_gi_default_palette(pal.data());
+    hipError_t e = hipMemcpy(pr->dirs, tab.data(), nd * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(pr->bounce, bt.data(), bt.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(pr->palette, pal.data(), pal.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(pr->probes, 0, (size_t)total * sizeof(vg_probe));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_probes(pr);
+        delete pr;
+        return fail(VG_E_DEVICE, "probe setup: %s", hipGetErrorString(e));
+    }
+    *out = pr;
+    return VG_OK;
+}
+
+int vg_probes_destroy(vg_probes *pr) {
+    if (!pr) return VG_OK;
+    (void)set_device(pr->world);
+    (void)hipDeviceSynchronize();
+    free_probes(pr);
+    delete pr;
+    return VG_OK;
+}
+
+int vg_probes_set_palette(vg_probes *pr, const float *rgba) {
+    if (!pr || !rgba) return fail(VG_E_INVALID, "null argument");
+    int st = set_device(pr->world);
+    if (st) return st;
+    HIP_TRY(hipMemcpy(pr->palette, rgba, 256 * sizeof(float4), hipMemcpyHostToDevice));
+    return VG_OK;
+}
+
+int vg_probes_reset(vg_probes *pr, void *stream) {
+    if (!pr) return fail(VG_E_INVALID, "null probes");
+    int st = set_device(pr->world);
+    if (st) return st;
+    HIP_TRY(hipMemsetAsync(pr->probes, 0, (size_t)pr->total * sizeof(vg_probe),
+                           (hipStream_t)stream));
+    return VG_OK;
+}
+
+int vg_probes_data(vg_probes *pr, vg_probe **out) {
+    if (!pr || !out) return fail(VG_E_INVALID, "null argument");
+    *out = pr->probes;
+    return VG_OK;
+}
+
+int vg_probes_download(vg_probes *pr, vg_probe *host) {
+    if (!pr || !host) return fail(VG_E_INVALID, "null argument");
+    int st = set_device(pr->world);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(host, pr->probes, (size_t)pr->total * sizeof(vg_probe),
+                      hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_probes_upload(vg_probes *pr, const vg_probe *host) {
+    if (!pr || !host) return fail(VG_E_INVALID, "null argument");
+    int st = set_device(pr->world);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(pr->probes, host, (size_t)pr->total * sizeof(vg_probe),
+                      hipMemcpyHostToDevice));
+    return VG_OK;
+}
+
+int vg_probes_update(vg_probes *pr, const vg_gi_params *p, void *stream) {
+    return probes_update(pr, p, false, (hipStream_t)stream);
+}
+
+int vg_probes_update_counts(vg_probes *pr, const vg_gi_params *p, uint64_t rays_host[3],
+                            uint64_t counts_host[VG_IT_COUNT]) {
+    if (!pr) return fail(VG_E_INVALID, "null probes");
+    int st = set_device(pr->world);
+    if (st) return st;
+    HIP_TRY(hipMemset(pr->counts, 0, kGiCounters * sizeof(unsigned long long)));
+    st = probes_update(pr, p, true, nullptr);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned long long c[kGiCounters];
+    HIP_TRY(hipMemcpy(c, pr->counts, sizeof(c), hipMemcpyDeviceToHost));
+    if (counts_host)
+        for (int k = 0; k < VG_IT_COUNT; ++k) counts_host[k] = c[k];
+    if (rays_host)
+        for (int k = 0; k < 3; ++k) rays_host[k] = c[VG_IT_COUNT + k];
+    return VG_OK;
+}
+
+} // extern "C"

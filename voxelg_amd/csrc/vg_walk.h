@@ -1,0 +1,572 @@
+// Device-side world view and the exact hierarchical voxel walk shared by every kernel
+// file of the engine (trace, probes, edits).
+//
+// HBM layout (all arrays owned by vg_world):
+//   l4[n_l4]          u64  reference order (x fastest)             one bit per child chunk
+//   chunk[n_chunks]   u64  reference order                         one bit per child tile
+//   tile[c*64 + tb]   u64  nested: chunk index, then tile bit      one bit per child brick
+//   tile_uni[same]    u8
+//   occ[t*64 + bb]    u64  nested: nested tile, then brick bit     one bit per voxel
+//   brick_uni[same]   u8
+//   mat[b*64 + vb]    u8   nested brick, then voxel bit
+// so the 64 child words of any tile (or chunk) are one contiguous 512-byte run that a
+// wave reads in one coalesced pass, and a brick's 64 materials are one 64-byte line.
+//
+// Exactness: the walk uses only IEEE +, -, *, / and floor in fp32 (contraction off,
+// correctly rounded divide and sqrt), recomputes every plane-crossing t from integer
+// coordinates, and skips empty cells with an integer-exact exit computation, so its
+// result equals a one-voxel-at-a-time walk bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "vg_internal.h"
+
+namespace vg {
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(VG_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+struct DevWorld {
+    const uint64_t *l4;
+    const uint64_t *chunk;
+    const uint64_t *tile;
+    const uint64_t *occ;
+    const uint8_t *tile_uni;
+    const uint8_t *brick_uni;
+    const uint8_t *mat;
+    int32_t dims[3];
+    int32_t ro_mod[3];
+    int32_t origin[3];
+    int32_t cdim[3];
+    int32_t l4dim[3];
+    int32_t fold; // 1 when the window origin folds any axis (toroidal slots)
+};
+
+struct CamParams {
+    float origin[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float tan_half_fov;
+    float res[2];
+    float jitter[2];
+};
+
+// ------------------------------------------------------------ material rules --
+// shaders/raymarch.wgsl:891-920 predicates; src/raycast.rs:88-89 fringe exclusion.
+__device__ __forceinline__ bool is_transparent(uint32_t m) { return (m >= 5u && m <= 12u) || m == 18u; }
+__device__ __forceinline__ bool is_foliage(uint32_t m) {
+    return m == 14u || m == 25u || m == 26u || m == 27u || m == 30u || m == 31u || m == 33u ||
+           m == 34u || m == 35u || m == 36u || m == 37u;
+}
+__device__ __forceinline__ bool uniform_optimisable(uint32_t m) {
+    return !is_foliage(m) && !is_transparent(m);
+}
+__device__ __forceinline__ bool solid_mat(uint32_t m, uint32_t flags) {
+    if (m == 0u || m == 33u) return false;
+    if ((flags & VG_RAY_SKIP_TRANSPARENT) && is_transparent(m)) return false;
+    if ((flags & VG_RAY_ANY_HIT) && m == 35u) return false;
+    return true;
+}
+
+__device__ __forceinline__ float fsel_min(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fsel_max(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float safe_inv(float x) { return fabsf(x) < 1e-8f ? 1e30f : 1.0f / x; }
+__device__ __forceinline__ int cbit(int32_t x, int32_t y, int32_t z) { return x + z * 4 + y * 16; }
+
+// --------------------------------------------------------------- walk state --
+struct Walk {
+    float org[3], dir[3], inv[3];
+    float tn[3];     // t of the next plane crossing on each axis (+inf for an axis that never steps)
+    int32_t step[3];
+    int32_t l[3];    // current voxel, window-local
+    float t_cur;     // entry t of the current voxel
+    int32_t last_axis;
+    int32_t entry_face;
+    int32_t face_pack; // 4 bits per axis: face code of an entry through that axis
+};
+
+struct HitRec {
+    int32_t v[3];
+    float t;
+    uint32_t face, mat, hit;
+};
+
+__device__ __forceinline__ void hit_miss(HitRec &h) {
+    h.v[0] = h.v[1] = h.v[2] = 0;
+    h.t = -1.0f;
+    h.face = VG_FACE_NONE;
+    h.mat = 0;
+    h.hit = 0;
+}
+
+// t of the plane crossed to enter voxel e along axis k (plane e when stepping up, e + 1
+// when stepping down), always from the integer coordinate.
+__device__ __forceinline__ float t_plane(const Walk &r, int k, int32_t e) {
+    int32_t p = r.step[k] > 0 ? e : e + 1;
+    return ((float)p - r.org[k]) * r.inv[k];
+}
+
+__device__ __forceinline__ float t_next(const Walk &r, int k) {
+    return r.step[k] != 0 ? t_plane(r, k, r.l[k] + r.step[k]) : INFINITY;
+}
+
+// Ray setup: window-local origin, slab entry, 1e-3 bias clamped into the window
+// (shaders/raymarch.wgsl:3369-3411, src/raycast.rs:34-50). dir is already normalized.
+__device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const float o[3],
+                                          const float d[3]) {
+    float tmn[3], tmx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.org[k] = o[k] - (float)w.origin[k];
+        r.dir[k] = d[k];
+        r.inv[k] = safe_inv(d[k]);
+        float t0 = (0.0f - r.org[k]) * r.inv[k];
+        float t1 = ((float)w.dims[k] - r.org[k]) * r.inv[k];
+        tmn[k] = fsel_min(t0, t1);
+        tmx[k] = fsel_max(t0, t1);
+        r.step[k] = fabsf(d[k]) < 1e-8f ? 0 : (d[k] > 0.0f ? 1 : -1);
+    }
+    if (r.step[0] == 0 && r.step[1] == 0 && r.step[2] == 0) return false; // cannot advance
+    float t_enter = fsel_max(fsel_max(tmn[0], fsel_max(tmn[1], tmn[2])), 0.0f);
+    float t_exit = fsel_min(tmx[0], fsel_min(tmx[1], tmx[2]));
+    if (t_enter >= t_exit || t_exit < 0.0f) return false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float p = r.org[k] + d[k] * (t_enter + 1e-3f);
+        p = fsel_min(fsel_max(p, 1e-3f), (float)w.dims[k] - 1e-3f);
+        r.l[k] = (int32_t)floorf(p);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.tn[k] = t_next(r, k);
+    r.t_cur = t_enter;
+    r.last_axis = -1;
+    r.face_pack = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int f = r.step[k] > 0 ? 2 * k + 1 : (r.step[k] < 0 ? 2 * k : VG_FACE_NONE);
+        r.face_pack |= f << (4 * k);
+    }
+    int la = (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) ? 0 : (tmn[1] >= tmn[2] ? 1 : 2);
+    r.entry_face = (r.face_pack >> (4 * la)) & 15;
+    return true;
+}
+
+// Advance out of the aligned cell of size S (1 = just the current voxel) that holds the
+// current voxel, to exactly the state a one-voxel-at-a-time walk reaches at its first
+// crossing of the cell's faces (T*, ax*): crossings are processed in (t, axis) order, ties
+// to the lowest axis (src/raycast.rs:92-103), every t from its integer plane. The crossed
+// axis lands just outside the face; another axis moves only if its cached next crossing
+// orders before (T*, ax*) - then it lands on the last in-cell plane that does, found from a
+// floor estimate corrected by direct t comparisons (t along an axis is monotone in the
+// plane coordinate), and the first rejected plane's t becomes its next crossing.
+// For S = 1 this is the plain voxel step; for S > 1 it replaces skip_to_cell
+// (shaders/raymarch.wgsl:4807-4864) with an integer-exact exit.
+__device__ __forceinline__ void walk_advance(Walk &r, const int32_t sl[3], int32_t S) {
+    int32_t lo[3];
+    float tf[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = r.l[k] - (sl[k] & (S - 1));
+        int32_t plane = r.step[k] > 0 ? lo[k] + S : lo[k];
+        tf[k] = r.step[k] != 0 ? ((float)plane - r.org[k]) * r.inv[k] : INFINITY;
+    }
+    int ax = 0;
+    float ts = tf[0];
+    if (tf[1] < ts) {
+        ts = tf[1];
+        ax = 1;
+    }
+    if (tf[2] < ts) {
+        ts = tf[2];
+        ax = 2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k == ax) {
+            r.l[k] = r.step[k] > 0 ? lo[k] + S : lo[k] - 1;
+            r.tn[k] = t_plane(r, k, r.l[k] + r.step[k]);
+        } else if (r.tn[k] < ts || (r.tn[k] == ts && k < ax)) {
+            // this axis crosses at least one in-cell plane before the exit (so S > 1)
+            const int32_t st = r.step[k];
+            const int32_t first = r.l[k];
+            const int32_t last = st > 0 ? lo[k] + S - 1 : lo[k];
+            int32_t e = (int32_t)floorf(r.org[k] + r.dir[k] * ts);
+            if (st > 0) {
+                e = e < first ? first : e;
+                e = e > last ? last : e;
+            } else {
+                e = e > first ? first : e;
+                e = e < last ? last : e;
+            }
+            // bounded by the cell width; normally one test each way
+            for (int g = 0; g < 258 && e != first; ++g) {
+                float t = t_plane(r, k, e);
+                if (t < ts || (t == ts && k < ax)) break;
+                e -= st;
+            }
+            float nxt = tf[k]; // the cell face, if e ends on the last in-cell voxel
+            for (int g = 0; g < 258 && e != last; ++g) {
+                float t = t_plane(r, k, e + st);
+                if (!(t < ts || (t == ts && k < ax))) {
+                    nxt = t;
+                    break;
+                }
+                e += st;
+            }
+            r.l[k] = e;
+            r.tn[k] = nxt;
+        }
+    }
+    r.t_cur = ts;
+    r.last_axis = ax;
+}
+
+// Jump to the state reached once every plane crossing with t < tb has been processed (a
+// prefix of the (t, axis) order): each axis lands on its last plane before tb (floor
+// estimate corrected by direct t comparisons), the entry t is the latest crossing, ties to
+// the higher axis (processed last). Used after the beam prologue proved every voxel the
+// ray visits before tb empty, so the walk continues exactly as if it had stepped there.
+__device__ __forceinline__ void walk_advance_to(Walk &r, float tb) {
+    float tl = -INFINITY;
+    int la = -1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (r.step[k] != 0 && r.tn[k] < tb) {
+            const int32_t st = r.step[k];
+            const int32_t nearest = r.l[k] + st;
+            int32_t e = (int32_t)floorf(r.org[k] + r.dir[k] * tb);
+            if (st > 0 ? e < nearest : e > nearest) e = nearest;
+            for (int g = 0; g < 4096 && e != nearest && !(t_plane(r, k, e) < tb); ++g) e -= st;
+            for (int g = 0; g < 4096 && t_plane(r, k, e + st) < tb; ++g) e += st;
+            const float tk = t_plane(r, k, e);
+            r.l[k] = e;
+            r.tn[k] = t_plane(r, k, e + st);
+            if (tk > tl || (tk == tl && k > la)) {
+                tl = tk;
+                la = k;
+            }
+        }
+    }
+    if (la >= 0) {
+        r.t_cur = tl;
+        r.last_axis = la;
+    }
+}
+
+__device__ __forceinline__ void emit(const Walk &r, const DevWorld &w, uint32_t m, HitRec &h) {
+    h.hit = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h.v[k] = r.l[k] + w.origin[k];
+    h.t = r.t_cur;
+    h.mat = m;
+    h.face = r.last_axis < 0 ? (uint32_t)r.entry_face
+                             : (uint32_t)((r.face_pack >> (4 * r.last_axis)) & 15);
+}
+
+// shaders/raymarch.wgsl:253-295: topmost non-turf voxel's material, layer y = 3 first.
+__device__ __forceinline__ uint32_t topmost_material(const DevWorld &w, uint32_t bn, uint64_t occ) {
+#pragma unroll
+    for (int y = 3; y >= 0; --y) {
+        uint32_t layer = (uint32_t)(occ >> (16 * y)) & 0xFFFFu;
+        if (layer) {
+            uint32_t m = w.mat[(uint64_t)bn * 64u + 16u * y + __builtin_ctz(layer)];
+            if (m != 35u) return m;
+        }
+    }
+    return occ ? 35u : 0u;
+}
+
+// The 8 child bits of the 2x2x2 group of children holding slot coordinate sl inside a
+// parent u64 whose children are 2^(sh-1) voxels wide (bits lx + 4 lz + 16 ly over the
+// group's octant). An empty group is an aligned empty cube twice the child size, left
+// exactly like any other empty cell; it is not a level of the reference's pyramid.
+__device__ __forceinline__ uint64_t group_mask(const int32_t sl[3], int sh) {
+    const int ox = (sl[0] >> sh) & 1, oy = (sl[1] >> sh) & 1, oz = (sl[2] >> sh) & 1;
+    return (uint64_t)0x330033u << (2 * ox + 8 * oz + 32 * oy);
+}
+
+#define LOD_BRICK_T 400.0f
+#define TILE_LOD_T 520.0f
+
+// The walk: phase 1 leaves solid voxels the ray starts in (src/raycast.rs:108-115),
+// phase 2 returns the first solid voxel. Each iteration tests the coarsest level and
+// skips the largest empty aligned cell (shaders/raymarch.wgsl:4626-4775). The words of
+// the current L4 cell, chunk, tile and brick stay in registers until the walk leaves
+// them. COUNT tallies iteration classes for algorithmic-byte accounting.
+// Register-cached pyramid words of the cells holding the current voxel, and the phase.
+struct WalkCache {
+    uint32_t cur_l4, cur_c, cur_t, cur_b;
+    uint64_t l4w, cw, tw, ow;
+    uint32_t tu, bu;
+    bool phase1;
+};
+
+__device__ __forceinline__ void cache_reset(WalkCache &q) {
+    q.cur_l4 = q.cur_c = q.cur_t = q.cur_b = ~0u;
+    q.l4w = q.cw = q.tw = q.ow = 0;
+    q.tu = q.bu = 0;
+    q.phase1 = true;
+}
+
+// One iteration of the walk: phase 1 leaves solid voxels the ray starts in
+// (src/raycast.rs:108-115), phase 2 returns the first solid voxel. The iteration tests the
+// coarsest level first and leaves the largest empty aligned cell (the reference's levels,
+// shaders/raymarch.wgsl:4626-4775, plus exact 2x2x2 groups of empty children). The words of
+// the current L4 cell, chunk, tile and brick stay in registers until the walk leaves them.
+// Returns true when the walk has ended (h then holds the hit, or stays a miss). COUNT
+// tallies iteration classes for algorithmic-byte accounting.
+template <bool COUNT>
+__device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld &w, float tmax,
+                                          uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
+    if (r.t_cur > tmax) return true;
+    if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
+        (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
+        return true;
+    int32_t sl[3] = {r.l[0], r.l[1], r.l[2]};
+    if (w.fold) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t s = r.l[k] + w.ro_mod[k];
+            sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
+        }
+    }
+    int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
+    do {
+        const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
+                             (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
+        if (l4i != q.cur_l4) {
+            q.cur_l4 = l4i;
+            q.l4w = w.l4[l4i];
+        }
+        if (q.l4w == 0) {
+            if (COUNT) cnt[VG_IT_L4_SKIP]++;
+            q.phase1 = false;
+            S = 256;
+            break;
+        }
+        if (!((q.l4w >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
+            q.phase1 = false;
+            const bool grp = !(q.l4w & group_mask(sl, 7));
+            if (COUNT) cnt[grp ? VG_IT_CHUNK_GROUP_SKIP : VG_IT_CHUNK_SKIP]++;
+            S = grp ? 128 : 64;
+            break;
+        }
+        const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                            (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+        if (ci != q.cur_c) {
+            q.cur_c = ci;
+            q.cw = w.chunk[ci];
+        }
+        const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
+        if (!((q.cw >> tb) & 1)) {
+            q.phase1 = false;
+            const bool grp = !(q.cw & group_mask(sl, 5));
+            if (COUNT) cnt[grp ? VG_IT_TILE_GROUP_SKIP : VG_IT_TILE_SKIP]++;
+            S = grp ? 32 : 16;
+            break;
+        }
+        const uint32_t tn = ci * 64u + (uint32_t)tb;
+        if (tn != q.cur_t) {
+            q.cur_t = tn;
+            q.tw = w.tile[tn];
+            q.tu = w.tile_uni[tn];
+        }
+        if (lod && !q.phase1 && r.t_cur > TILE_LOD_T) {
+            if (COUNT) cnt[VG_IT_TILE_LOD]++;
+            if (q.tw) {
+                const uint32_t fb = tn * 64u + (uint32_t)__builtin_ctzll(q.tw);
+                uint32_t lm = topmost_material(w, fb, w.occ[fb]);
+                if (lm != 0u && uniform_optimisable(lm)) {
+                    emit(r, w, lm, h);
+                    return true;
+                }
+            }
+        }
+        if (q.tu != 0u) {
+            if (COUNT) cnt[VG_IT_TILE_UNIFORM]++;
+            if (solid_mat(q.tu, flags)) {
+                if (!q.phase1) {
+                    emit(r, w, q.tu, h);
+                    return true;
+                }
+            } else {
+                q.phase1 = false;
+            }
+            S = 16;
+            break;
+        }
+        const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
+        if (!((q.tw >> bb) & 1)) {
+            q.phase1 = false;
+            // inside an occupied tile a tile-LOD test may fire at any visited voxel past
+            // t 520, so a group skip there must end before t 520: its exit lies within
+            // the 8^3 cell's diagonal (< 14) of the current voxel's entry
+            const bool grp = !(q.tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T);
+            if (COUNT) cnt[grp ? VG_IT_BRICK_GROUP_SKIP : VG_IT_BRICK_SKIP]++;
+            S = grp ? 8 : 4;
+            break;
+        }
+        const uint32_t bn = tn * 64u + (uint32_t)bb;
+        if (bn != q.cur_b) {
+            q.cur_b = bn;
+            q.ow = w.occ[bn];
+            q.bu = w.brick_uni[bn];
+        }
+        if (q.bu != 0u) {
+            if (COUNT) cnt[VG_IT_BRICK_UNIFORM]++;
+            if (solid_mat(q.bu, flags)) {
+                if (!q.phase1) {
+                    emit(r, w, q.bu, h);
+                    return true;
+                }
+            } else {
+                q.phase1 = false;
+            }
+            S = 4;
+            break;
+        }
+        if (lod && !q.phase1 && r.t_cur > LOD_BRICK_T) {
+            if (COUNT) cnt[VG_IT_BRICK_LOD]++;
+            uint32_t m = topmost_material(w, bn, q.ow);
+            if (!is_foliage(m)) {
+                emit(r, w, m, h);
+                return true;
+            }
+        }
+        const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
+        if ((q.ow >> vb) & 1) {
+            if (COUNT) cnt[VG_IT_VOXEL]++;
+            uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+            if (solid_mat(m, flags)) {
+                if (!q.phase1) {
+                    emit(r, w, m, h);
+                    return true;
+                }
+                break; // phase 1: step out of the solid voxel
+            }
+            q.phase1 = false;
+            break;
+        }
+        q.phase1 = false;
+        // an empty 2x2x2 voxel octant is left whole; inside an occupied brick a brick-LOD
+        // test may fire at any visited voxel past t 400, so the octant (diagonal < 4)
+        // must be left before it
+        const bool grp = !(q.ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T);
+        if (COUNT) cnt[grp ? VG_IT_VOXEL_GROUP_SKIP : VG_IT_VOXEL]++;
+        S = grp ? 2 : 1;
+    } while (false);
+    walk_advance(r, sl, S);
+    return false;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax, uint32_t flags,
+                                         HitRec &h, uint32_t *cnt) {
+    const bool lod = (flags & VG_RAY_LOD) && !(flags & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
+    WalkCache q;
+    cache_reset(q);
+    // every iteration crosses at least one voxel plane inside the window
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    for (int it = 0; it < max_iter; ++it)
+        if (walk_iter<COUNT>(r, q, w, tmax, flags, lod, h, cnt)) return;
+}
+
+// glam normalize_or_zero: d * (1 / sqrt(dot)); false when that is not a finite positive
+// reciprocal or the result is zero.
+__device__ __forceinline__ bool normalize_or_zero(const float in[3], float out[3]) {
+    float len = sqrtf(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+    float rcp = 1.0f / len;
+    if (!(isfinite(rcp) && rcp > 0.0f)) return false;
+    out[0] = in[0] * rcp;
+    out[1] = in[1] * rcp;
+    out[2] = in[2] * rcp;
+    return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
+}
+
+// Pinhole direction (shaders/raymarch.wgsl:430-438, 600).
+__device__ __forceinline__ void primary_dir(const CamParams &c, int32_t px, int32_t py,
+                                            float out[3]) {
+    float u = ((float)px + 0.5f + c.jitter[0]) / c.res[0];
+    float v = ((float)py + 0.5f + c.jitter[1]) / c.res[1];
+    float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
+    float aspect = c.res[0] / c.res[1];
+    float a = nx * c.tan_half_fov * aspect, b = ny * c.tan_half_fov;
+    float r[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = c.forward[k] + c.right[k] * a + c.up[k] * b;
+    float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    float rcp = 1.0f / len;
+    out[0] = r[0] * rcp;
+    out[1] = r[1] * rcp;
+    out[2] = r[2] * rcp;
+}
+
+// p_hit + n * 0.001 (shaders/raymarch.wgsl:3955).
+__device__ __forceinline__ void shadow_origin(const float o[3], const float d[3], const HitRec &h,
+                                              float out[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float n = 0.0f;
+        if (h.face < 6u && (int)(h.face >> 1) == k) n = (h.face & 1u) ? -1.0f : 1.0f;
+        float p = o[k] + d[k] * h.t;
+        out[k] = p + n * 0.001f;
+    }
+}
+
+__device__ __forceinline__ void store_hit(vg_hit *dst, const HitRec &h) {
+    uint32_t *p = reinterpret_cast<uint32_t *>(dst);
+    p[0] = (uint32_t)h.v[0];
+    p[1] = (uint32_t)h.v[1];
+    p[2] = (uint32_t)h.v[2];
+    p[3] = __float_as_uint(h.t);
+    p[4] = (h.face & 0xFFu) | ((h.mat & 0xFFu) << 8) | ((h.hit & 0xFFu) << 16);
+}
+
+__device__ __forceinline__ void load_hit(const vg_hit *src, HitRec &h) {
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(src);
+    h.v[0] = (int32_t)p[0];
+    h.v[1] = (int32_t)p[1];
+    h.v[2] = (int32_t)p[2];
+    h.t = __uint_as_float(p[3]);
+    h.face = p[4] & 0xFFu;
+    h.mat = (p[4] >> 8) & 0xFFu;
+    h.hit = (p[4] >> 16) & 0xFFu;
+}
+
+// Wave-reduce the per-lane counters and add them to the global totals.
+__device__ __forceinline__ void flush_counts(const uint32_t *cnt, unsigned long long *counts) {
+#pragma unroll
+    for (int k = 0; k < VG_IT_COUNT; ++k) {
+        unsigned long long v = cnt[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counts[k], v);
+    }
+}
+
+
+constexpr int kBlock = 256;
+
+// Spread consecutive launch-order blocks round-robin over the 8 XCDs back into
+// contiguous ranges per XCD, so neighbouring screen blocks share one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nb) {
+    uint32_t full = nb & ~7u;
+    if (bid >= full) return bid;
+    return (bid & 7u) * (full >> 3) + (bid >> 3);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+} // namespace vg
