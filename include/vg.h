@@ -375,6 +375,96 @@ int vg_probes_update(vg_probes *probes, const vg_gi_params *params, void *stream
 int vg_probes_update_counts(vg_probes *probes, const vg_gi_params *params,
                             uint64_t rays_host[3], uint64_t counts_host[VG_IT_COUNT]);
 
+/* ------------------------------------------------------ radiance cache -- */
+/* A learned radiance cache for the probe update. It has no counterpart in the reference:
+ * it stands in for the bounce tail of gather rays (the reference ends them with an ambient
+ * term, shaders/gi_probes.wgsl:226-241). Position is encoded by a multiresolution hash
+ * grid (per level: trilinear interpolation of 8 hashed corners, hash
+ * x*1 ^ y*2654435761 ^ z*805459861 mod 2^log2_table, 2 features), concatenated with the
+ * face one-hot (inputs 32..37), the direction (38..40) and a constant 1 (41) into 64
+ * inputs, then three 64x64 ReLU layers and a 16x64 output layer whose rows 0..2 are the
+ * rgb prediction. Inference and training run as bf16 MFMA kernels with fp32
+ * accumulation; the optimizer (Adam, fp32 master parameters) skips grid entries whose
+ * gradient is exactly zero. Loss: sum over queries of w * |y - t|^2 / (2 n).
+ *
+ * Parameter vectors (vg_nrc_get_params / set_params, fp32): mlp = W0, W1, W2 (64x64,
+ * row = output, column = input) then W3 (16x64; rows 3..15 stay zero); grid =
+ * [level][entry][feature], levels * 2^log2_table * 2 floats. */
+#define VG_NRC_MAX_LEVELS 16
+#define VG_NRC_MLP_PARAMS (3 * 64 * 64 + 16 * 64)
+#define VG_NRC_MAX_BATCH (1 << 20) /* queries per vg_nrc_train_step / vg_nrc_gradients */
+
+typedef struct vg_nrc_config {
+    int32_t levels;                   /* 1..16 */
+    int32_t log2_table;               /* entries per level = 2^log2_table, 4..22 */
+    float scale[VG_NRC_MAX_LEVELS];   /* grid cells per voxel at each level (vg_nrc_scales) */
+    float lr_grid, lr_mlp;            /* Adam learning rates */
+    float beta1, beta2, eps;          /* Adam */
+    float init_grid;                  /* grid features start uniform in [-init_grid, init_grid] */
+    uint32_t seed;                    /* initialisation and training-sample seed */
+    int32_t batch;                    /* training samples per step of the learned probe update */
+    int32_t steps;                    /* training steps per learned probe update */
+    int32_t _pad;
+} vg_nrc_config;
+
+/* One cache query: a surface point (world voxel coordinates), the face it lies on (low byte;
+ * VG_FACE_NONE for none) and the direction that reached it. aux is carried, not read. */
+typedef struct vg_nrc_query {
+    float pos[3];
+    uint32_t face;
+    float dir[3];
+    uint32_t aux;
+} vg_nrc_query;
+
+typedef struct vg_nrc vg_nrc;
+
+/* Defaults: 16 levels of 2^19 entries, cells from 128 voxels down to 1, Adam lr 1e-2 (grid
+ * and MLP), beta 0.9 / 0.99, eps 1e-10, init 1e-4, 65536 samples x 1 step per update. */
+int vg_nrc_config_default(vg_nrc_config *out);
+/* scale[l] = b^l / coarse_cell with b = (coarse_cell / fine_cell)^(1 / (levels - 1)). */
+int vg_nrc_scales(int32_t levels, float coarse_cell, float fine_cell, float *scale_out);
+/* A cache on ctx's device, parameters initialised from cfg->seed (MLP: Xavier uniform). */
+int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out);
+int vg_nrc_destroy(vg_nrc *nrc);
+/* out[0] = MLP parameters (VG_NRC_MLP_PARAMS), out[1] = grid parameters. */
+int vg_nrc_param_counts(const vg_nrc *nrc, int64_t out[2]);
+/* fp32 master parameters to / from the host (either pointer may be NULL). Setting
+ * parameters also resets the optimizer (moments and step count). Synchronous. */
+int vg_nrc_get_params(vg_nrc *nrc, float *mlp_host, float *grid_host);
+int vg_nrc_set_params(vg_nrc *nrc, const float *mlp_host, const float *grid_host);
+/* Raw predictions for n device queries: out4[i] = (r, g, b, 0), device. */
+int vg_nrc_infer(vg_nrc *nrc, const vg_nrc_query *queries, uint32_t n, float *out4,
+                 void *stream);
+/* One Adam step on n device queries with device targets (r, g, b, weight). */
+int vg_nrc_train_step(vg_nrc *nrc, const vg_nrc_query *queries, const float *target4,
+                      uint32_t n, void *stream);
+/* The loss gradient of n device queries without an update (for inspection): mlp and grid
+ * gradients and the loss to the host (any pointer may be NULL). Synchronous. */
+int vg_nrc_gradients(vg_nrc *nrc, const vg_nrc_query *queries, const float *target4,
+                     uint32_t n, float *mlp_grad_host, float *grid_grad_host, float *loss_host);
+/* Optimizer steps taken and the loss of the last step. Synchronous. */
+int vg_nrc_stats(vg_nrc *nrc, int64_t *steps_out, float *loss_out);
+/* The last training batch the learned probe update drew (cfg batch entries): queries with
+ * aux = the bounce-table index used, and targets (r, g, b, weight). Synchronous. */
+int vg_nrc_last_batch(vg_nrc *nrc, vg_nrc_query *queries_host, float *target4_host);
+
+/* The learned probe update (the reference's cs_gi_probe_update with the cache as the
+ * bounce tail): gather rays_per_probe rays per updated probe and the sun shadow ray at
+ * each hit, then the tail albedo * max(0, cache(hit point, face, direction)) for every hit
+ * (one inference launch over the compacted hits), the SH-L1 projection, fold and
+ * relocation of vg_probes_update. With VG_NRC_TRAIN the cache then trains on this
+ * update's hits: cfg->steps steps of cfg->batch hits drawn with replacement, each
+ * target the traced bounce tail (a cosine bounce ray and the sun shadow ray at its hit).
+ * params->bounces is not read. */
+#define VG_NRC_TRAIN 1u
+int vg_probes_update_learned(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
+                             uint32_t flags, void *stream);
+/* The same, reporting rays traced (gather, shadow, training bounce) and the cache queries
+ * (compacted gather hits). Synchronous. */
+int vg_probes_update_learned_counts(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
+                                    uint32_t flags, uint64_t rays_host[3],
+                                    uint64_t *queries_host);
+
 /* raycast (src/raycast.rs:21) over a world on the device: one ray, host in and out.
  * Returns VG_OK with hit_host->hit = 0/1. Synchronous. */
 int vg_raycast(vg_world *world, const float camera_world[3], const float dir[3],

@@ -46,6 +46,16 @@ GI_PARAMS_DTYPE = np.dtype([
     ("ambient_tail", "<f4"), ("ambient", "<f4", (3,)), ("day_t", "<f4"),
     ("sky_horizon", "<f4", (3,)), ("dusk_w", "<f4"), ("sky_zenith", "<f4", (3,)),
     ("halo", "<f4"), ("sun_xz", "<f4", (2,)), ("pad", "<f4", (2,))])
+# orc_nrc_config / orc_nrc_query (oracle.h): the byte layouts of vg_nrc_config / vg_nrc_query.
+NRC_CONFIG_DTYPE = np.dtype([
+    ("levels", "<i4"), ("log2_table", "<i4"), ("scale", "<f4", (16,)), ("lr_grid", "<f4"),
+    ("lr_mlp", "<f4"), ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("init_grid", "<f4"),
+    ("seed", "<u4"), ("batch", "<i4"), ("steps", "<i4"), ("pad", "<i4")])
+NRC_QUERY_DTYPE = np.dtype([("pos", "<f4", (3,)), ("face", "<u4"), ("dir", "<f4", (3,)),
+                            ("aux", "<u4")])
+NRC_WIDTH = 64
+NRC_MLP_PARAMS = 3 * 64 * 64 + 16 * 64
+NRC_IN_FACE, NRC_IN_DIR, NRC_IN_ONE = 32, 38, 41
 GI_SKIP_BURIED = 1
 GI_BOUNCE_TABLE = 4096
 PROBE_FIELDS = 48
@@ -107,6 +117,18 @@ def lib():
                                      C.POINTER(C.c_double), C.POINTER(C.c_uint8), F32P, U64P]),
             "orc_gi_update": (None, [W, P, P, P, P, P, C.c_int, U64P]),
             "orc_gi_reference": (None, [W, P, C.c_int32, P, P, P, P, P, C.c_int64, P, C.c_int]),
+            "orc_gi_update_learned": (None, [W, P, P, P, P, P, P, C.c_int, U64P]),
+            "orc_gi_bounce_tail": (None, [W, P, P, P, P, C.c_uint32, P, U64P]),
+            "orc_gi_bounce_tails": (None, [W, P, P, P, C.c_int64, P, P, P, C.c_int]),
+            "orc_bf16_round": (C.c_double, [C.c_double]),
+            "orc_nrc_hash": (C.c_uint32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+            "orc_nrc_scales": (None, [C.c_int32, C.c_float, C.c_float, P]),
+            "orc_nrc_encode": (None, [P, P, P, C.c_int, P]),
+            "orc_nrc_forward": (None, [P, P, P, C.c_int64, P, C.c_int, P]),
+            "orc_nrc_loss": (C.c_double, [P, P, P, C.c_int64, P, P, C.c_int]),
+            "orc_nrc_loss_grad": (C.c_double, [P, P, P, C.c_int64, P, P, C.c_int, P, P]),
+            "orc_nrc_adam": (None, [C.c_int64, P, P, P, P, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_int64, C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -309,6 +331,30 @@ class OracleWorld:
                             _p(_f4(palette)), _p(probes), _threads(threads), rays)
         return [int(r) for r in rays]
 
+    def gi_update_learned(self, params, dirs, btab, palette, cache: "NrcOracle",
+                          probes: np.ndarray, threads=0):
+        """One learned probe update in place (bounce tails from `cache`); returns rays[3]."""
+        prm = _gi_params(params)
+        assert probes.dtype == np.float64 and probes.flags["C_CONTIGUOUS"]
+        rays = (C.c_uint64 * 3)()
+        ev = cache.eval_struct()
+        lib().orc_gi_update_learned(C.byref(self.cstruct()), _p(prm), _p(_f4(dirs)),
+                                    _p(_f4(btab)), _p(_f4(palette)), C.addressof(ev),
+                                    _p(probes), _threads(threads), rays)
+        return [int(r) for r in rays]
+
+    def gi_bounce_tails(self, params, btab, palette, queries, bidx, threads=0) -> np.ndarray:
+        """Traced bounce tail (rgb, float64 [n, 3]) at each cache query point."""
+        prm = _gi_params(params)
+        q = np.ascontiguousarray(queries, NRC_QUERY_DTYPE).reshape(-1)
+        b = np.ascontiguousarray(bidx, np.uint32).reshape(-1)
+        assert len(b) == len(q)
+        out = np.zeros((len(q), 3), np.float64)
+        lib().orc_gi_bounce_tails(C.byref(self.cstruct()), _p(prm), _p(_f4(btab)),
+                                  _p(_f4(palette)), len(q), _p(q), _p(b), _p(out),
+                                  _threads(threads))
+        return out
+
     def gi_reference(self, params, dirs, btab, palette, slots, origins, threads=0):
         """FP64 estimate of SH-L1 (r, g, b, dist, dist2) x 4 per listed probe slot from the
         given origins over the full direction set `dirs`: float64 [n, 20]."""
@@ -402,3 +448,80 @@ def terrain_height(dims, seed, x, z):
 def brick_uniform_of(brick: np.ndarray) -> int:
     b = np.ascontiguousarray(np.asarray(brick, BRICK_DTYPE).reshape(1))
     return int(lib().orc_brick_uniform_of(_p(b)))
+
+
+class NrcOracle:
+    """FP64 radiance cache (oracle_nrc.c) over a config and logical fp32 parameters.
+    emulate=True rounds what the device holds in bf16 (parameters, layer inputs, layer
+    gradients) at the same points."""
+
+    def __init__(self, config: np.ndarray, mlp: np.ndarray, grid: np.ndarray, emulate=False):
+        self.cfg = np.ascontiguousarray(config.view(NRC_CONFIG_DTYPE).reshape(1))
+        c = self.cfg[0]
+        self.grid_params = int(c["levels"]) * (1 << int(c["log2_table"])) * 2
+        self.mlp = np.ascontiguousarray(mlp, np.float32).reshape(NRC_MLP_PARAMS)
+        self.grid = np.ascontiguousarray(grid, np.float32).reshape(self.grid_params)
+        self.emulate = int(bool(emulate))
+
+    @staticmethod
+    def _q(queries):
+        q = np.ascontiguousarray(queries, NRC_QUERY_DTYPE).reshape(-1)
+        return q
+
+    def encode(self, query) -> np.ndarray:
+        q = self._q(query)
+        out = np.zeros(NRC_WIDTH, np.float64)
+        lib().orc_nrc_encode(_p(self.cfg), _p(self.grid), _p(q[:1]), self.emulate, _p(out))
+        return out
+
+    def forward(self, queries) -> np.ndarray:
+        q = self._q(queries)
+        out = np.zeros((len(q), 3), np.float64)
+        lib().orc_nrc_forward(_p(self.cfg), _p(self.mlp), _p(self.grid), len(q), _p(q),
+                              self.emulate, _p(out))
+        return out
+
+    def loss(self, queries, targets) -> float:
+        q = self._q(queries)
+        t = np.ascontiguousarray(targets, np.float32).reshape(len(q), 4)
+        return lib().orc_nrc_loss(_p(self.cfg), _p(self.mlp), _p(self.grid), len(q), _p(q),
+                                  _p(t), self.emulate)
+
+    def loss_grad(self, queries, targets):
+        """(loss, d loss / d mlp [13312], d loss / d grid [levels * 2^log2_table * 2])."""
+        q = self._q(queries)
+        t = np.ascontiguousarray(targets, np.float32).reshape(len(q), 4)
+        gm = np.zeros(NRC_MLP_PARAMS, np.float64)
+        gg = np.zeros(self.grid_params, np.float64)
+        loss = lib().orc_nrc_loss_grad(_p(self.cfg), _p(self.mlp), _p(self.grid), len(q), _p(q),
+                                       _p(t), self.emulate, _p(gm), _p(gg))
+        return loss, gm, gg
+
+    def eval_struct(self):
+        """orc_nrc_eval for orc_gi_update_learned (keeps this object alive while used)."""
+        class _Eval(C.Structure):
+            _fields_ = [("cfg", C.c_void_p), ("mlp", C.c_void_p), ("grid", C.c_void_p),
+                        ("emulate", C.c_int)]
+        return _Eval(_p(self.cfg), _p(self.mlp), _p(self.grid), self.emulate)
+
+
+def nrc_scales(levels, coarse_cell, fine_cell) -> np.ndarray:
+    out = np.zeros(int(levels), np.float32)
+    lib().orc_nrc_scales(int(levels), float(coarse_cell), float(fine_cell), _p(out))
+    return out
+
+
+def nrc_hash(x, y, z, log2_table) -> int:
+    return int(lib().orc_nrc_hash(int(x), int(y), int(z), int(log2_table)))
+
+
+def bf16_round(v) -> float:
+    return float(lib().orc_bf16_round(float(v)))
+
+
+def nrc_adam(p, m, v, g, lr, beta1, beta2, eps, step, sparse):
+    """One Adam step in place on float64 arrays."""
+    for a in (p, m, v, g):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    lib().orc_nrc_adam(len(p), _p(p), _p(m), _p(v), _p(g), float(lr), float(beta1), float(beta2),
+                       float(eps), int(step), int(bool(sparse)))

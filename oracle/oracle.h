@@ -202,6 +202,60 @@ typedef struct orc_gi_params {
 #define ORC_GI_BOUNCE_TABLE 4096
 #define ORC_PROBE_FIELDS 48 /* vg_probe: 12 x vec4 */
 
+/* ------------------------------------------------------------ radiance cache --
+ * The learned probe update's cache (vg_nrc_config / vg_nrc_query in include/vg.h, same
+ * bytes): hash-grid encoding of the position + face one-hot + direction + a constant 1,
+ * a 64-wide input, three 64x64 ReLU layers and a 16x64 output layer of which rows 0..2 are
+ * the rgb prediction. Parameters are the logical fp32 vectors: mlp = W0, W1, W2 (64x64,
+ * row = output) then W3 (16x64); grid[(level * 2^log2_table + entry) * 2 + feature]. */
+#define ORC_NRC_WIDTH 64
+#define ORC_NRC_MAX_LEVELS 16
+#define ORC_NRC_IN_FACE 32 /* inputs 32..37: face one-hot */
+#define ORC_NRC_IN_DIR 38  /* inputs 38..40: direction */
+#define ORC_NRC_IN_ONE 41  /* input 41: constant 1 (the first layer's bias) */
+#define ORC_NRC_MLP_PARAMS (3 * 64 * 64 + 16 * 64)
+
+typedef struct orc_nrc_config {
+    int32_t levels;
+    int32_t log2_table;
+    float scale[ORC_NRC_MAX_LEVELS];
+    float lr_grid, lr_mlp;
+    float beta1, beta2, eps;
+    float init_grid;
+    uint32_t seed;
+    int32_t batch;
+    int32_t steps;
+    int32_t pad;
+} orc_nrc_config;
+
+typedef struct orc_nrc_query {
+    float pos[3];
+    uint32_t face; /* low byte: face code (6 = none) */
+    float dir[3];
+    uint32_t aux;
+} orc_nrc_query;
+
+typedef struct orc_nrc_eval {
+    const orc_nrc_config *cfg;
+    const float *mlp, *grid;
+    int emulate;
+} orc_nrc_eval;
+
+double orc_bf16_round(double v);
+uint32_t orc_nrc_hash(int32_t x, int32_t y, int32_t z, int32_t log2_table);
+void orc_nrc_scales(int32_t levels, float coarse_cell, float fine_cell, float *out);
+void orc_nrc_encode(const orc_nrc_config *c, const float *grid, const orc_nrc_query *q,
+                    int emulate, double x[ORC_NRC_WIDTH]);
+void orc_nrc_forward(const orc_nrc_config *c, const float *mlp, const float *grid, int64_t n,
+                     const orc_nrc_query *q, int emulate, double *out3);
+double orc_nrc_loss(const orc_nrc_config *c, const float *mlp, const float *grid, int64_t n,
+                    const orc_nrc_query *q, const float *target4, int emulate);
+double orc_nrc_loss_grad(const orc_nrc_config *c, const float *mlp, const float *grid, int64_t n,
+                         const orc_nrc_query *q, const float *target4, int emulate,
+                         double *g_mlp, double *g_grid);
+void orc_nrc_adam(int64_t n, double *p, double *m, double *v, const double *g, double lr,
+                  double beta1, double beta2, double eps, int64_t step, int sparse);
+
 /* sky_color for a direction (radiance before GI_SKY), double. */
 void orc_gi_sky(const orc_gi_params *p, const float d[3], double out[3]);
 /* Bounce-table index of sample k of probe slot `probe` in `round`. */
@@ -215,6 +269,21 @@ void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab
 void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs,
                    const float *btab, const float *palette, double *probes, int threads,
                    uint64_t rays[3]);
+/* The learned update: gather rays and the sun shadow at their hits as orc_gi_update, the
+ * bounce tail of every hit taken from the cache (max(0, prediction) at the hit point
+ * o + d t, its face and the gather direction) instead of a traced bounce. */
+void orc_gi_update_learned(const orc_world *w, const orc_gi_params *p, const float *dirs,
+                           const float *btab, const float *palette, const orc_nrc_eval *cache,
+                           double *probes, int threads, uint64_t rays[3]);
+/* The traced bounce tail at a cache query point (pos = the gather hit point, face): the
+ * secondary origin pos + n * 0.03, the bounce direction from bounce-table entry bidx. */
+void orc_gi_bounce_tail(const orc_world *w, const orc_gi_params *p, const float *btab,
+                        const float *palette, const orc_nrc_query *q, uint32_t bidx,
+                        double out[3], uint64_t rays[3]);
+/* orc_gi_bounce_tail over n queries with their bounce-table indices; out: 3 doubles each. */
+void orc_gi_bounce_tails(const orc_world *w, const orc_gi_params *p, const float *btab,
+                         const float *palette, int64_t n, const orc_nrc_query *q,
+                         const uint32_t *bidx, double *out, int threads);
 /* The judge: for each listed probe slot, an n_dirs-direction estimate (dirs: the
  * n_dirs-point Fibonacci set) of its SH-L1 radiance and distance moments from origin
  * origins[3i..3i+2], bounce samples drawn for sample k = 0..n_dirs-1 of `round`. out: 20

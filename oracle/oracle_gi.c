@@ -111,9 +111,45 @@ static double sun_visible(const orc_world *w, const orc_gi_params *p, const floa
     return h.hit ? 0.0 : 1.0;
 }
 
-void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab,
-                   const float *palette, const float o[3], const float d[3], uint32_t probe,
-                   uint32_t k, double out[4], uint8_t *face, float *t_hit, uint64_t rays[3]) {
+/* Radiance arriving at a surface point from its cosine-sampled bounce ray (the tail of a
+ * 2-bounce gather sample, and the learned update's training target): so is the offset
+ * secondary origin, fX the face the bounce leaves through, e the bounce-table entry. */
+static void bounce_tail(const orc_world *w, const orc_gi_params *p, const float *palette,
+                        const float so[3], uint32_t fX, const float *e, double lb[3],
+                        uint64_t rays[3]) {
+    float bd[3];
+    bounce_dir(e, fX, bd);
+    orc_hit hy;
+    if (orc_trace_dir(w, so, bd, p->gi_dist, ORC_SKIP_TRANSPARENT, &hy, NULL)) rays[2]++;
+    if (!hy.hit) {
+        double sky[3];
+        orc_gi_sky(p, bd, sky);
+        for (int c = 0; c < 3; ++c) lb[c] = sky[c] * (double)p->gi_sky;
+        return;
+    }
+    const float *aY = palette + 4 * hy.material;
+    const double ndlY = face_ndl(hy.face, p->sun);
+    float yo[3];
+    offset_origin(so, bd, hy.t, hy.face, yo);
+    const double visY = ndlY > 0.0 ? sun_visible(w, p, yo, rays) : 0.0;
+    for (int c = 0; c < 3; ++c)
+        lb[c] = (double)aY[c] * ((double)p->sun_color[c] * ndlY * visY +
+                                 (double)p->ambient[c] * (double)p->ambient_tail);
+}
+
+void orc_gi_bounce_tail(const orc_world *w, const orc_gi_params *p, const float *btab,
+                        const float *palette, const orc_nrc_query *q, uint32_t bidx,
+                        double out[3], uint64_t rays[3]) {
+    const uint32_t f = q->face & 0xFFu;
+    float so[3];
+    for (int k = 0; k < 3; ++k) so[k] = q->pos[k] + face_n(f, k) * 0.03f;
+    bounce_tail(w, p, palette, so, f, btab + 4 * (bidx & (ORC_GI_BOUNCE_TABLE - 1)), out, rays);
+}
+
+static void gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab,
+                      const float *palette, const float o[3], const float d[3], uint32_t probe,
+                      uint32_t k, const orc_nrc_eval *cache, double out[4], uint8_t *face,
+                      float *t_hit, uint64_t rays[3]) {
     orc_hit h;
     if (orc_trace_dir(w, o, d, p->gi_dist, ORC_SKIP_TRANSPARENT, &h, NULL)) rays[0]++;
     if (!h.hit) {
@@ -133,25 +169,19 @@ void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab
     offset_origin(o, d, tX, fX, so);
     const double visX = ndlX > 0.0 ? sun_visible(w, p, so, rays) : 0.0;
     double lb[3];
-    if (p->bounces == 2) {
-        float bd[3];
-        bounce_dir(btab + 4 * orc_gi_bounce_index(p->seed, p->round, probe, k), fX, bd);
-        orc_hit hy;
-        if (orc_trace_dir(w, so, bd, p->gi_dist, ORC_SKIP_TRANSPARENT, &hy, NULL)) rays[2]++;
-        if (!hy.hit) {
-            double sky[3];
-            orc_gi_sky(p, bd, sky);
-            for (int c = 0; c < 3; ++c) lb[c] = sky[c] * (double)p->gi_sky;
-        } else {
-            const float *aY = palette + 4 * hy.material;
-            const double ndlY = face_ndl(hy.face, p->sun);
-            float yo[3];
-            offset_origin(so, bd, hy.t, hy.face, yo);
-            const double visY = ndlY > 0.0 ? sun_visible(w, p, yo, rays) : 0.0;
-            for (int c = 0; c < 3; ++c)
-                lb[c] = (double)aY[c] * ((double)p->sun_color[c] * ndlY * visY +
-                                         (double)p->ambient[c] * (double)p->ambient_tail);
-        }
+    if (cache) {
+        /* learned: the path ends at the gather hit; the cache gives its bounce tail */
+        orc_nrc_query q;
+        for (int c = 0; c < 3; ++c) q.pos[c] = o[c] + d[c] * tX;
+        for (int c = 0; c < 3; ++c) q.dir[c] = d[c];
+        q.face = fX;
+        q.aux = 0;
+        double y[3];
+        orc_nrc_forward(cache->cfg, cache->mlp, cache->grid, 1, &q, cache->emulate, y);
+        for (int c = 0; c < 3; ++c) lb[c] = y[c] > 0.0 ? y[c] : 0.0;
+    } else if (p->bounces == 2) {
+        bounce_tail(w, p, palette, so, fX,
+                    btab + 4 * orc_gi_bounce_index(p->seed, p->round, probe, k), lb, rays);
     } else {
         for (int c = 0; c < 3; ++c) lb[c] = (double)p->ambient[c] * (double)p->ambient_tail;
     }
@@ -160,6 +190,12 @@ void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab
     out[3] = tX;
     *face = (uint8_t)fX;
     *t_hit = tX;
+}
+
+void orc_gi_sample(const orc_world *w, const orc_gi_params *p, const float *btab,
+                   const float *palette, const float o[3], const float d[3], uint32_t probe,
+                   uint32_t k, double out[4], uint8_t *face, float *t_hit, uint64_t rays[3]) {
+    gi_sample(w, p, btab, palette, o, d, probe, k, NULL, out, face, t_hit, rays);
 }
 
 static int pos_mod_i(int32_t a, int32_t b) {
@@ -197,6 +233,7 @@ typedef struct {
     const float *dirs, *btab, *pal;
     double *probes;
     uint32_t n_upd, phase, epoch;
+    const orc_nrc_eval *cache;
     int64_t next;
     pthread_mutex_t mu;
     uint64_t rays[3];
@@ -238,7 +275,7 @@ static void update_probe(upd_job *J, uint32_t j, uint64_t rays[3]) {
         double rad[4];
         uint8_t f;
         float t;
-        orc_gi_sample(J->w, p, J->btab, J->pal, o, d, (uint32_t)idx, k, rad, &f, &t, rays);
+        gi_sample(J->w, p, J->btab, J->pal, o, d, (uint32_t)idx, k, J->cache, rad, &f, &t, rays);
         double b[4];
         sh_basis(d, b);
         const double ws[5] = {rad[0], rad[1], rad[2], rad[3], rad[3] * rad[3]};
@@ -305,9 +342,9 @@ static int clamp_threads(int threads) {
     return threads;
 }
 
-void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs,
-                   const float *btab, const float *palette, double *probes, int threads,
-                   uint64_t rays[3]) {
+static void gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs,
+                      const float *btab, const float *palette, const orc_nrc_eval *cache,
+                      double *probes, int threads, uint64_t rays[3]) {
     const int64_t total = (int64_t)p->grid[0] * p->grid[1] * p->grid[2];
     const uint32_t div = (uint32_t)p->update_div;
     upd_job J;
@@ -318,6 +355,7 @@ void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs
     J.btab = btab;
     J.pal = palette;
     J.probes = probes;
+    J.cache = cache;
     J.phase = (uint32_t)p->round % div;
     J.epoch = ((uint32_t)p->round / div) % (uint32_t)p->epochs;
     J.n_upd = total > (int64_t)J.phase ? (uint32_t)((total - J.phase + div - 1) / div) : 0u;
@@ -329,6 +367,18 @@ void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs
     pthread_mutex_destroy(&J.mu);
     if (rays)
         for (int k = 0; k < 3; ++k) rays[k] += J.rays[k];
+}
+
+void orc_gi_update(const orc_world *w, const orc_gi_params *p, const float *dirs,
+                   const float *btab, const float *palette, double *probes, int threads,
+                   uint64_t rays[3]) {
+    gi_update(w, p, dirs, btab, palette, NULL, probes, threads, rays);
+}
+
+void orc_gi_update_learned(const orc_world *w, const orc_gi_params *p, const float *dirs,
+                           const float *btab, const float *palette, const orc_nrc_eval *cache,
+                           double *probes, int threads, uint64_t rays[3]) {
+    gi_update(w, p, dirs, btab, palette, cache, probes, threads, rays);
 }
 
 typedef struct {
@@ -391,6 +441,55 @@ void orc_gi_reference(const orc_world *w, const orc_gi_params *p, int32_t n_dirs
     threads = clamp_threads(threads);
     pthread_t th[256];
     for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, ref_worker, &J);
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&J.mu);
+}
+
+typedef struct {
+    const orc_world *w;
+    const orc_gi_params *p;
+    const float *btab, *pal;
+    const orc_nrc_query *q;
+    const uint32_t *bidx;
+    double *out;
+    int64_t n, next;
+    pthread_mutex_t mu;
+} tail_job;
+
+static void *tail_worker(void *arg) {
+    tail_job *J = (tail_job *)arg;
+    uint64_t rays[3] = {0, 0, 0};
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        const int64_t i0 = J->next;
+        J->next += 256;
+        pthread_mutex_unlock(&J->mu);
+        if (i0 >= J->n) break;
+        const int64_t i1 = i0 + 256 < J->n ? i0 + 256 : J->n;
+        for (int64_t i = i0; i < i1; ++i)
+            orc_gi_bounce_tail(J->w, J->p, J->btab, J->pal, J->q + i, J->bidx[i], J->out + 3 * i,
+                               rays);
+    }
+    return NULL;
+}
+
+void orc_gi_bounce_tails(const orc_world *w, const orc_gi_params *p, const float *btab,
+                         const float *palette, int64_t n, const orc_nrc_query *q,
+                         const uint32_t *bidx, double *out, int threads) {
+    tail_job J;
+    memset(&J, 0, sizeof(J));
+    J.w = w;
+    J.p = p;
+    J.btab = btab;
+    J.pal = palette;
+    J.q = q;
+    J.bidx = bidx;
+    J.out = out;
+    J.n = n;
+    pthread_mutex_init(&J.mu, NULL);
+    threads = clamp_threads(threads);
+    pthread_t th[256];
+    for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, tail_worker, &J);
     for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
     pthread_mutex_destroy(&J.mu);
 }

@@ -1,0 +1,212 @@
+"""GPU parity of the radiance cache (bf16 MFMA inference / training, Adam) and of the learned
+probe update against the FP64 oracle, and the online-training gain.
+
+The oracle runs with bf16 emulation: it rounds parameters, layer inputs and layer gradients
+to bf16 where the device holds them in bf16 and sums in double. Device and oracle then differ
+by fp32 accumulation order (and the rare bf16 rounding flip it causes), so the tolerances
+below are norm-wise: 2e-3 for predictions, 1e-2 for gradients.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from voxelg_amd import gi, nrc
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+SUN = (math.cos(math.radians(35.0)) * math.cos(2.1), math.sin(math.radians(35.0)),
+       math.cos(math.radians(35.0)) * math.sin(2.1))
+
+
+def case(msg):
+    print(f"case: {msg}", flush=True)
+
+
+def rand_queries(rng, n, hi=300.0):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return nrc.queries(rng.uniform(0.0, hi, (n, 3)), rng.integers(0, 7, n), d,
+                       aux=np.arange(n))
+
+
+def rand_params(rng, cache, scale_mlp=0.2, scale_grid=0.5):
+    mlp = (rng.normal(size=cache.n_mlp) * scale_mlp).astype(np.float32)
+    mlp[12288 + 3 * 64:] = 0.0
+    grid = (rng.normal(size=cache.n_grid) * scale_grid).astype(np.float32)
+    return mlp, grid
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def cache12(ctx):
+    """16 levels of 2^12 entries: a grid small enough to compare whole gradients."""
+    return nrc.RadianceCache(ctx, nrc.default_config(log2_table=12, batch=4096))
+
+
+def test_infer_matches_oracle(cache12):
+    rng = np.random.default_rng(11)
+    mlp, grid = rand_params(rng, cache12)
+    cache12.set_params(mlp, grid)
+    for n in (1, 63, 3001):
+        case(f"nrc infer n={n}")
+        q = rand_queries(rng, n)
+        y = cache12.infer(q)
+        ref = oracle.NrcOracle(cache12.config, mlp, grid, emulate=True).forward(q)
+        assert rel(y, ref) < 2e-3, rel(y, ref)
+        assert np.abs(y - ref).max() < 2e-2 * np.abs(ref).max() + 1e-6
+    exact = oracle.NrcOracle(cache12.config, mlp, grid).forward(q)
+    assert rel(y, exact) < 5e-2  # bf16 versus fp64 throughout
+
+
+def test_gradients_match_oracle(cache12):
+    rng = np.random.default_rng(12)
+    mlp, grid = rand_params(rng, cache12)
+    cache12.set_params(mlp, grid)
+    n = 1500  # not a multiple of the 256-sample tile
+    q = rand_queries(rng, n, hi=120.0)
+    t = rng.uniform(-0.5, 1.5, (n, 4)).astype(np.float32)
+    t[:, 3] = 1.0
+    t[::7, 3] = 0.0  # zero-weight samples contribute nothing
+    case(f"nrc gradients n={n}")
+    loss, gm, gg = cache12.gradients(q, t)
+    lo, om, og = oracle.NrcOracle(cache12.config, mlp, grid, emulate=True).loss_grad(q, t)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    for lyr in range(4):
+        sl = slice(lyr * 4096, lyr * 4096 + (4096 if lyr < 3 else 1024))
+        assert rel(gm[sl], om[sl]) < 1e-2, (lyr, rel(gm[sl], om[sl]))
+    assert np.all(gm[12288 + 3 * 64:] == 0)
+    assert rel(gg, og) < 1e-2, rel(gg, og)
+    # entries no query touches stay exactly zero
+    assert np.all(gg[og == 0] == 0)
+    # a second call gives the same MLP gradient (deterministic reduction), grid up to atomics
+    _, gm2, gg2 = cache12.gradients(q, t)
+    np.testing.assert_array_equal(gm2, gm)
+    assert rel(gg2, gg) < 1e-5
+
+
+def test_train_step_is_adam_on_the_gradient(cache12):
+    rng = np.random.default_rng(13)
+    mlp, grid = rand_params(rng, cache12)
+    cache12.set_params(mlp, grid)
+    q = rand_queries(rng, 2048, hi=120.0)
+    t = rng.uniform(0.0, 1.0, (2048, 4)).astype(np.float32)
+    t[:, 3] = 1.0
+    case("nrc train step")
+    _, gm, gg = cache12.gradients(q, t)
+    cache12.train_step(q, t)
+    steps, loss = cache12.stats()
+    assert steps == 1
+    m1, g1 = cache12.params()
+    c = cache12.config[0]
+    for p0, g, got, lr in ((mlp, gm, m1, c["lr_mlp"]), (grid, gg, g1, c["lr_grid"])):
+        p = p0.astype(np.float64)
+        oracle.nrc_adam(p, np.zeros_like(p), np.zeros_like(p), g.astype(np.float64), float(lr),
+                        float(c["beta1"]), float(c["beta2"]), float(c["eps"]), 1, True)
+        live = g != 0
+        np.testing.assert_allclose(got[live], p[live], rtol=1e-5, atol=1e-7)
+        np.testing.assert_array_equal(got[~live], p0[~live])
+    # the second step uses the gradient at the new parameters
+    cache12.train_step(q, t)
+    assert cache12.stats()[0] == 2
+
+
+def probe_setup(small, grid=(8, 4, 8), rays=32):
+    p = gi.gi_params(SUN, grid=grid, spacing=32, rays_per_probe=rays, epochs=1, update_div=1,
+                     bounces=2, flags=1)
+    return p, gi.directions(rays), gi.bounce_table(), gi.default_palette()
+
+
+def test_learned_update_matches_oracle(ctx, small, small_world):
+    cache = nrc.RadianceCache(ctx, nrc.default_config(log2_table=14, batch=2048))
+    rng = np.random.default_rng(14)
+    mlp, grid = rand_params(rng, cache, scale_mlp=0.15, scale_grid=0.3)
+    cache.set_params(mlp, grid)
+    p, dirs, bt, pal = probe_setup(small)
+    pg = gi.ProbeGrid(small, p)
+    ow = oracle.OracleWorld.view(small_world)
+    ref = np.zeros((pg.total, 48))
+    ev = oracle.NrcOracle(cache.config, mlp, grid, emulate=True)
+    for r in (0, 1):
+        p["round"] = r
+        case(f"learned probe update round {r}")
+        rays, nq = pg.update_learned_counts(cache, p, train=False)
+        orays = ow.gi_update_learned(p, dirs, bt, pal, ev, ref, threads=THREADS)
+        assert rays["gather"] == orays[0] and rays["shadow"] == orays[1] and rays["bounce"] == 0
+        assert 100 < nq <= rays["gather"]
+        got = pg.download().view(np.float32).reshape(pg.total, 48).astype(np.float64)
+        for col in (20, 21, 22, 23, 44, 45, 46, 47):
+            np.testing.assert_array_equal(got[:, col], ref[:, col])
+        live = ref[:, 23] == 1
+        assert live.sum() > 50
+        for base in (0, 24):
+            sh, rsh = got[live, base:base + 20], ref[live, base:base + 20]
+            if np.abs(rsh).max() > 0:
+                assert rel(sh, rsh) < 2e-3, (base, rel(sh, rsh))
+    assert cache.stats()[0] == 0  # no training without the flag
+
+
+def test_training_targets_match_oracle(ctx, small, small_world):
+    cache = nrc.RadianceCache(ctx, nrc.default_config(log2_table=14, batch=3000))
+    p, dirs, bt, pal = probe_setup(small)
+    p["round"] = 5
+    pg = gi.ProbeGrid(small, p)
+    case("learned update with training: targets")
+    rays, nq = pg.update_learned_counts(cache, p, train=True)
+    assert cache.stats()[0] == 1
+    q, t = cache.last_batch()
+    assert len(q) == 3000 and np.all(t[:, 3] == 1.0)
+    assert rays["bounce"] == 3000  # every training sample traces its bounce ray
+    ow = oracle.OracleWorld.view(small_world)
+    ref = ow.gi_bounce_tails(p, bt, pal, q, q["aux"], threads=THREADS)
+    np.testing.assert_allclose(t[:, :3], ref, rtol=2e-6, atol=1e-7)
+    assert np.all(q["face"] < 6) and np.all(q["aux"] < 4096)
+
+
+def test_online_training_gains_over_no_cache(ctx, small, small_world):
+    """32 gather rays per probe: the trained cache's bounce tail versus the reference's
+    ambient tail (the method without the network) and versus an untrained cache, all
+    judged against the FP64 4096-ray 2-bounce estimate."""
+    p, dirs, bt, pal = probe_setup(small)
+    ow = oracle.OracleWorld.view(small_world)
+    pg = gi.ProbeGrid(small, p)
+    total = pg.total
+    slots = np.arange(total)
+    origins = gi.probe_origins(np.zeros(total, gi.PROBE_DTYPE), p, slots)
+    pref = p.copy()
+    pref["rays_per_probe"] = 4096
+    ref = ow.gi_reference(pref, gi.directions(4096), bt, pal, slots, origins, threads=THREADS)
+
+    def err():
+        rec = pg.download()
+        live = rec["info"][:, 3] == 1
+        return float(gi.irradiance_error(gi.sh_coefficients(rec)[live], ref[live]).mean())
+
+    cache = nrc.RadianceCache(ctx, nrc.default_config(batch=16384))
+    case("untrained cache")
+    pg.reset().update_learned(cache, p, train=False)
+    e_untrained = err()
+    p1 = p.copy()
+    p1["bounces"] = 1
+    pg.reset().update(p1)
+    e_ambient = err()
+    for r in range(80):
+        p["round"] = r
+        pg.update_learned(cache, p, train=True)
+    ctx.synchronize()
+    steps, loss = cache.stats()
+    assert steps == 80
+    p["round"] = 1000
+    case("trained cache")
+    pg.reset().update_learned(cache, p, train=False)
+    e_trained = err()
+    print(f"32-ray irradiance error: trained cache {e_trained:.4f}, ambient tail {e_ambient:.4f}, "
+          f"untrained cache {e_untrained:.4f} (last loss {loss:.5f})", flush=True)
+    assert e_trained < 0.75 * e_ambient
+    assert e_trained < 0.75 * e_untrained

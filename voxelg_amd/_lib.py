@@ -38,6 +38,17 @@ GI_PARAMS_DTYPE = np.dtype([
     ("ambient_tail", "<f4"), ("ambient", "<f4", (3,)), ("day_t", "<f4"),
     ("sky_horizon", "<f4", (3,)), ("dusk_w", "<f4"), ("sky_zenith", "<f4", (3,)),
     ("halo", "<f4"), ("sun_xz", "<f4", (2,)), ("pad", "<f4", (2,))])
+# vg_nrc_config / vg_nrc_query (radiance cache)
+NRC_CONFIG_DTYPE = np.dtype([
+    ("levels", "<i4"), ("log2_table", "<i4"), ("scale", "<f4", (16,)), ("lr_grid", "<f4"),
+    ("lr_mlp", "<f4"), ("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("init_grid", "<f4"),
+    ("seed", "<u4"), ("batch", "<i4"), ("steps", "<i4"), ("pad", "<i4")])
+NRC_QUERY_DTYPE = np.dtype([("pos", "<f4", (3,)), ("face", "<u4"), ("dir", "<f4", (3,)),
+                            ("aux", "<u4")])
+assert NRC_CONFIG_DTYPE.itemsize == 112 and NRC_QUERY_DTYPE.itemsize == 32
+VG_NRC_MLP_PARAMS = 3 * 64 * 64 + 16 * 64
+VG_NRC_MAX_BATCH = 1 << 20
+VG_NRC_TRAIN = 1
 assert BRICK_DTYPE.itemsize == 72 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 20
 assert BATCH_DTYPE.itemsize == 24 and PROBE_DTYPE.itemsize == 192
 assert GI_PARAMS_DTYPE.itemsize == 152
@@ -158,6 +169,20 @@ SIGNATURES = {
     "vg_probes_upload": (C.c_int, [_P, _P]),
     "vg_probes_update": (C.c_int, [_P, _P, _P]),
     "vg_probes_update_counts": (C.c_int, [_P, _P, _U64P, _U64P]),
+    "vg_nrc_config_default": (C.c_int, [_P]),
+    "vg_nrc_scales": (C.c_int, [C.c_int32, C.c_float, C.c_float, _P]),
+    "vg_nrc_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "vg_nrc_destroy": (C.c_int, [_P]),
+    "vg_nrc_param_counts": (C.c_int, [_P, _I64P]),
+    "vg_nrc_get_params": (C.c_int, [_P, _P, _P]),
+    "vg_nrc_set_params": (C.c_int, [_P, _P, _P]),
+    "vg_nrc_infer": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "vg_nrc_train_step": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    "vg_nrc_gradients": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, _P]),
+    "vg_nrc_stats": (C.c_int, [_P, _I64P, _F32P]),
+    "vg_nrc_last_batch": (C.c_int, [_P, _P, _P]),
+    "vg_probes_update_learned": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    "vg_probes_update_learned_counts": (C.c_int, [_P, _P, _P, C.c_uint32, _U64P, _U64P]),
 }
 
 

@@ -11,7 +11,11 @@
 //                   lane order (deterministic), the nearest hit for relocation, the staging
 //                   sum and the hysteresis fold of complete cycles.
 // Samples of one probe have consecutive ids, so a wave's gather rays share their origin.
-#include "vg_world.h"
+// The learned update (vg_probes_update_learned) runs the path kernel without the bounce:
+// every gather hit is appended to a compacted list of cache queries, one inference launch
+// of the radiance cache adds its bounce tail, and k_nrc_targets traces the training
+// targets (cosine bounce + sun shadow) for a batch of those hits.
+#include "vg_nrc.h"
 
 struct vg_probes {
     vg_world *world;
@@ -25,7 +29,8 @@ struct vg_probes {
     float4 *pwork;          // per updated probe: ray origin xyz, w = flags bits
     float4 *rad;            // per sample: radiance rgb, visible distance
     uint8_t *face;          // per sample: face code of the gather hit (VG_FACE_NONE: none)
-    uint32_t *ctr;          // persistent counter
+    vg_nrc_query *queries;  // learned update: compacted gather hits (one per sample at most)
+    uint32_t *ctr;          // counters: [0] path work, [1] queries, [2 + s] training step s work
     unsigned long long *counts; // VG_IT_COUNT walk classes + 3 ray kinds
 };
 
@@ -33,6 +38,7 @@ namespace vg {
 
 constexpr uint32_t kProbeNew = 1u, kProbeBuried = 2u;
 constexpr int kGiCounters = VG_IT_COUNT + 3;
+constexpr int kGiCtrs = 2 + 64; // work, queries, one per training step (vg_nrc_config.steps <= 64)
 
 // ----------------------------------------------------------------- helpers --
 __device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params 
                                                       uint32_t phase, const vg_probe *probes,
                                                       float4 *pwork, uint32_t *ctr) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j == 0) *ctr = 0u; // the path kernel enqueued next consumes the counter
+    if (j < (uint32_t)kGiCtrs) ctr[j] = 0u; // the kernels enqueued next consume the counters
     if (j >= n_upd) return;
     const int64_t idx = (int64_t)j * p.update_div + phase;
     int32_t g[3];
@@ -202,7 +208,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
     const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
     const float4 *__restrict__ btab, const float4 *__restrict__ pal, float4 *__restrict__ rad,
-    uint8_t *__restrict__ face_out, uint32_t *ctr, int threshold, unsigned long long *counts) {
+    uint8_t *__restrict__ face_out, vg_nrc_query *__restrict__ queries, uint32_t *ctr,
+    int threshold, unsigned long long *counts) {
+    // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail)
     const uint32_t lane = threadIdx.x & 63;
     uint32_t cnt[VG_IT_COUNT];
     uint32_t nray[3] = {0, 0, 0};
@@ -224,6 +232,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     hit_miss(h);
     cache_reset(q);
     bool exhausted = false;
+    bool emit = false; // learned: this lane holds a cache query to append
+    float eX[3] = {0.0f, 0.0f, 0.0f}, eD[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t eF = 0, eS = 0;
 
     // Finish the sample: radiance toward the probe and its visible distance.
     auto finish_hit = [&]() {
@@ -269,6 +280,21 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     };
     // After the shadow ray at the gather hit (or none): bounce or tail, then finish.
     auto after_x = [&]() {
+        if (BOUNCES == 3) {
+            float o[3], d[3];
+            gather_ray(o, d);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                eX[k] = o[k] + d[k] * tX; // the hit point (the secondary origin before its offset)
+                eD[k] = d[k];
+                lb[k] = 0.0f;
+            }
+            eF = fX | (mX << 8);
+            eS = sid;
+            emit = true;
+            finish_hit();
+            return;
+        }
         if (BOUNCES == 2) {
             float d[3];
             bounce_ray(d);
@@ -384,6 +410,20 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         }
                     } else { // kShadowY
                         after_y(h.hit ? 0.0f : 1.0f);
+                    }
+                }
+            }
+            if (BOUNCES == 3) { // append this iteration's finished hits, one atomic per wave
+                const uint64_t em = __ballot(emit);
+                if (em) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(ctr + 1, (uint32_t)__popcll(em));
+                    base = __shfl(base, 0, 64);
+                    if (emit) {
+                        float4 *dst = reinterpret_cast<float4 *>(queries + base + lanes_below(em));
+                        dst[0] = make_float4(eX[0], eX[1], eX[2], __uint_as_float(eF));
+                        dst[1] = make_float4(eD[0], eD[1], eD[2], __uint_as_float(eS));
+                        emit = false;
                     }
                 }
             }
@@ -524,8 +564,159 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     for (int i = 0; i < 12; ++i) rec[i] = pr[i];
 }
 
+// Training targets of the learned update: `batch` samples drawn with replacement from the
+// update's compacted gather hits (`queries`, count qn[0]); each traces the hit's cosine
+// bounce ray and the sun shadow ray at the bounce hit, like the plain 2-bounce path, and
+// writes the query (aux = bounce-table index) and its target (bounce tail rgb, weight 1).
+// With no hits at all every target gets weight 0. Persistent and ray-compacting.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_nrc_targets(
+    DevWorld w, vg_gi_params p, uint32_t batch, uint32_t tseed,
+    const vg_nrc_query *__restrict__ queries, const uint32_t *qn,
+    const float4 *__restrict__ btab, const float4 *__restrict__ pal,
+    vg_nrc_query *__restrict__ tq, float4 *__restrict__ tt, uint32_t *ctr, int threshold,
+    unsigned long long *counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cnt[VG_IT_COUNT];
+    uint32_t nray[3] = {0, 0, 0};
+    if (COUNT)
+#pragma unroll
+        for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    const uint32_t nq = *qn;
+    int stage = kIdle;
+    uint32_t j = 0, bidx = 0, fq = 0, mY = 0;
+    float qp[3] = {0.0f, 0.0f, 0.0f}, qd[3] = {0.0f, 0.0f, 0.0f};
+    float so[3] = {0.0f, 0.0f, 0.0f}, bd[3] = {0.0f, 0.0f, 0.0f};
+    float ndlY = 0.0f;
+    int it = 0;
+    Walk r;
+    WalkCache q;
+    HitRec h;
+    hit_miss(h);
+    cache_reset(q);
+    bool exhausted = false;
+    auto write = [&](const float lb[3]) {
+        float4 *dst = reinterpret_cast<float4 *>(tq + j);
+        dst[0] = make_float4(qp[0], qp[1], qp[2], __uint_as_float(fq));
+        dst[1] = make_float4(qd[0], qd[1], qd[2], __uint_as_float(bidx));
+        tt[j] = make_float4(lb[0], lb[1], lb[2], 1.0f);
+        stage = kIdle;
+    };
+    auto sky_tail = [&]() {
+        float sky[3], lb[3];
+        sky_color(p, bd, sky);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lb[k] = sky[k] * p.gi_sky;
+        write(lb);
+    };
+    auto hit_tail = [&](float visY) {
+        const float4 a = pal[mY];
+        const float sy = ndlY * visY;
+        float lb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            lb[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
+        write(lb);
+    };
+    auto start = [&](const float o[3], const float d[3], int st) -> bool {
+        hit_miss(h);
+        if (!walk_init(r, w, o, d)) return false;
+        cache_reset(q);
+        it = 0;
+        stage = st;
+        if (COUNT) nray[st == kBounce ? 2 : 1]++;
+        return true;
+    };
+    for (;;) {
+        uint64_t act = __ballot(stage != kIdle);
+        int nact = __popcll(act);
+        if (!exhausted && nact < threshold) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(ctr, (uint32_t)(64 - nact));
+            base = __shfl(base, 0, 64);
+            if (base >= batch) {
+                exhausted = true;
+            } else if (stage == kIdle) {
+                const uint32_t id = base + lanes_below(~act);
+                if (id < batch) {
+                    j = id;
+                    if (nq == 0) {
+                        float4 *dst = reinterpret_cast<float4 *>(tq + j);
+                        dst[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)VG_FACE_NONE));
+                        dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        tt[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    } else {
+                        uint32_t x = mix32(tseed ^ ((uint32_t)p.round * 0x85EBCA6Bu));
+                        x = mix32(x ^ j);
+                        const uint32_t qi = x % nq;
+                        bidx = mix32(x ^ 0x68E31DA4u) >> 20;
+                        const float4 *src = reinterpret_cast<const float4 *>(queries + qi);
+                        const float4 a = src[0], b = src[1];
+                        qp[0] = a.x;
+                        qp[1] = a.y;
+                        qp[2] = a.z;
+                        fq = __float_as_uint(a.w) & 0xFFu;
+                        qd[0] = b.x;
+                        qd[1] = b.y;
+                        qd[2] = b.z;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) so[k] = qp[k] + face_n(fq, k) * 0.03f;
+                        bounce_dir(btab[bidx], fq, bd);
+                        if (!start(so, bd, kBounce)) sky_tail();
+                    }
+                }
+            }
+            nact = __popcll(__ballot(stage != kIdle));
+        }
+        if (nact == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        do {
+            if (stage != kIdle) {
+                const bool shadow = stage == kShadowY;
+                bool done = walk_iter<COUNT>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                                             shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
+                                             false, h, cnt);
+                done = done || ++it >= max_iter;
+                if (done) {
+                    if (stage == kBounce) {
+                        if (!h.hit) {
+                            sky_tail();
+                        } else {
+                            float yo[3];
+                            const uint32_t fY = h.face;
+                            mY = h.mat;
+                            ndlY = face_ndl(fY, p.sun);
+                            gi_offset_origin(so, bd, h.t, fY, yo);
+                            if (ndlY > 0.0f) {
+                                if (!start(yo, p.sun, kShadowY)) hit_tail(1.0f);
+                            } else {
+                                hit_tail(0.0f);
+                            }
+                        }
+                    } else { // kShadowY
+                        hit_tail(h.hit ? 0.0f : 1.0f);
+                    }
+                }
+            }
+        } while (__popcll(__ballot(stage != kIdle)) >= threshold);
+    }
+    if (COUNT) {
+        flush_counts(cnt, counts);
+#pragma unroll
+        for (int k = 1; k < 3; ++k) {
+            unsigned long long v = nray[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0 && v) atomicAdd(&counts[VG_IT_COUNT + k], v);
+        }
+    }
+}
+
 // --------------------------------------------------------------------- host --
-static int check_params(const vg_probes *pr, const vg_gi_params *p) {
+static int check_params(const vg_probes *pr, const vg_gi_params *p, bool learned) {
     if (!p) return fail(VG_E_INVALID, "null params");
     for (int k = 0; k < 3; ++k)
         if (p->grid[k] != pr->grid[k])
@@ -537,18 +728,23 @@ static int check_params(const vg_probes *pr, const vg_gi_params *p) {
     if (p->spacing <= 0) return fail(VG_E_INVALID, "spacing %d", p->spacing);
     if (p->update_div <= 0) return fail(VG_E_INVALID, "update_div %d", p->update_div);
     if (p->round < 0) return fail(VG_E_INVALID, "round %d", p->round);
-    if (p->bounces != 1 && p->bounces != 2)
+    if (!learned && p->bounces != 1 && p->bounces != 2)
         return fail(VG_E_UNSUPPORTED, "bounces %d (1 or 2)", p->bounces);
     return VG_OK;
 }
 
-static int probes_update(vg_probes *pr, const vg_gi_params *p, bool count, hipStream_t stream) {
+// One probe update: prep, paths, (learned: cache inference), reduce, (learned + train:
+// targets and optimizer steps). nrc == nullptr is the plain update.
+static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32_t nflags,
+                         bool count, hipStream_t stream) {
     if (!pr) return fail(VG_E_INVALID, "null probes");
     int st = check_world(pr->world);
     if (st) return st;
-    st = check_params(pr, p);
+    st = check_params(pr, p, nrc != nullptr);
     if (st) return st;
     vg_world *w = pr->world;
+    if (nrc && nrc->ctx->device != w->ctx->device)
+        return fail(VG_E_INVALID, "the cache and the probes live on different devices");
     const uint32_t div = (uint32_t)p->update_div;
     const uint32_t phase = (uint32_t)p->round % div;
     const uint32_t epoch = ((uint32_t)p->round / div) % (uint32_t)p->epochs;
@@ -565,27 +761,77 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, bool count, hipSt
 #define LAUNCH_G(C, B)                                                                         \
     hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
-                       pr->palette, pr->rad, pr->face, pr->ctr, thr, cd)
+                       pr->palette, pr->rad, pr->face, pr->queries, pr->ctr, thr, cd)
+    const int mode = nrc ? 3 : p->bounces;
     if (count) {
-        if (p->bounces == 2) LAUNCH_G(true, 2);
+        if (mode == 3) LAUNCH_G(true, 3);
+        else if (mode == 2) LAUNCH_G(true, 2);
         else LAUNCH_G(true, 1);
     } else {
-        if (p->bounces == 2) LAUNCH_G(false, 2);
+        if (mode == 3) LAUNCH_G(false, 3);
+        else if (mode == 2) LAUNCH_G(false, 2);
         else LAUNCH_G(false, 1);
     }
 #undef LAUNCH_G
     HIP_TRY(hipGetLastError());
+    if (nrc) {
+        st = nrc_infer_launch(nrc, pr->queries, pr->ctr + 1, (uint32_t)n_samples, nullptr, pr->rad,
+                              pr->palette, stream);
+        if (st) return st;
+    }
     const uint32_t wpb = kBlock / 64;
     hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + wpb - 1) / wpb), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, epoch, phase, pr->pwork, pr->dirs, pr->rad, pr->face,
                        pr->probes);
     HIP_TRY(hipGetLastError());
+    if (nrc && (nflags & VG_NRC_TRAIN)) {
+        for (int s = 0; s < nrc->cfg.steps; ++s) {
+            const uint32_t tseed = nrc->cfg.seed ^ (0x9E3779B9u * (uint32_t)(s + 1));
+            if (count)
+                hipLaunchKernelGGL(k_nrc_targets<true>, grid, block, 0, stream, w->dw, *p,
+                                   (uint32_t)nrc->tcap, tseed, pr->queries, pr->ctr + 1,
+                                   pr->bounce, pr->palette, nrc->tq, nrc->tt, pr->ctr + 2 + s, thr,
+                                   cd);
+            else
+                hipLaunchKernelGGL(k_nrc_targets<false>, grid, block, 0, stream, w->dw, *p,
+                                   (uint32_t)nrc->tcap, tseed, pr->queries, pr->ctr + 1,
+                                   pr->bounce, pr->palette, nrc->tq, nrc->tt, pr->ctr + 2 + s, thr,
+                                   cd);
+            HIP_TRY(hipGetLastError());
+            st = nrc_train_launch(nrc, nrc->tq, nrc->tt, (uint32_t)nrc->tcap, true, stream);
+            if (st) return st;
+        }
+    }
+    return VG_OK;
+}
+
+static int probes_update_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc,
+                                uint32_t nflags, uint64_t rays_host[3], uint64_t counts_host[],
+                                uint64_t *queries_host) {
+    if (!pr) return fail(VG_E_INVALID, "null probes");
+    int st = set_device(pr->world);
+    if (st) return st;
+    HIP_TRY(hipMemset(pr->counts, 0, kGiCounters * sizeof(unsigned long long)));
+    st = probes_update(pr, p, nrc, nflags, true, nullptr);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned long long c[kGiCounters];
+    HIP_TRY(hipMemcpy(c, pr->counts, sizeof(c), hipMemcpyDeviceToHost));
+    if (counts_host)
+        for (int k = 0; k < VG_IT_COUNT; ++k) counts_host[k] = c[k];
+    if (rays_host)
+        for (int k = 0; k < 3; ++k) rays_host[k] = c[VG_IT_COUNT + k];
+    if (queries_host) {
+        uint32_t nq = 0;
+        HIP_TRY(hipMemcpy(&nq, pr->ctr + 1, sizeof(nq), hipMemcpyDeviceToHost));
+        *queries_host = nq;
+    }
     return VG_OK;
 }
 
 static void free_probes(vg_probes *p) {
     void *ptrs[] = {p->probes, p->dirs, p->bounce, p->palette, p->pwork,
-                    p->rad,    p->face, p->ctr,    p->counts};
+                    p->rad,    p->face, p->queries, p->ctr, p->counts};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -625,7 +871,8 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
               hipMalloc((void **)&pr->pwork, (size_t)total * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->rad, ns * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->face, ns) == hipSuccess &&
-              hipMalloc((void **)&pr->ctr, 256) == hipSuccess &&
+              hipMalloc((void **)&pr->queries, ns * sizeof(vg_nrc_query)) == hipSuccess &&
+              hipMalloc((void **)&pr->ctr, kGiCtrs * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void **)&pr->counts, kGiCounters * sizeof(unsigned long long)) ==
                   hipSuccess;
     if (!ok) {
@@ -708,25 +955,25 @@ int vg_probes_upload(vg_probes *pr, const vg_probe *host) {
 }
 
 int vg_probes_update(vg_probes *pr, const vg_gi_params *p, void *stream) {
-    return probes_update(pr, p, false, (hipStream_t)stream);
+    return probes_update(pr, p, nullptr, 0u, false, (hipStream_t)stream);
+}
+
+int vg_probes_update_learned(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32_t flags,
+                             void *stream) {
+    if (!nrc) return fail(VG_E_INVALID, "null cache");
+    return probes_update(pr, p, nrc, flags, false, (hipStream_t)stream);
+}
+
+int vg_probes_update_learned_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc,
+                                    uint32_t flags, uint64_t rays_host[3],
+                                    uint64_t *queries_host) {
+    if (!nrc) return fail(VG_E_INVALID, "null cache");
+    return probes_update_counts(pr, p, nrc, flags, rays_host, nullptr, queries_host);
 }
 
 int vg_probes_update_counts(vg_probes *pr, const vg_gi_params *p, uint64_t rays_host[3],
                             uint64_t counts_host[VG_IT_COUNT]) {
-    if (!pr) return fail(VG_E_INVALID, "null probes");
-    int st = set_device(pr->world);
-    if (st) return st;
-    HIP_TRY(hipMemset(pr->counts, 0, kGiCounters * sizeof(unsigned long long)));
-    st = probes_update(pr, p, true, nullptr);
-    if (st) return st;
-    HIP_TRY(hipDeviceSynchronize());
-    unsigned long long c[kGiCounters];
-    HIP_TRY(hipMemcpy(c, pr->counts, sizeof(c), hipMemcpyDeviceToHost));
-    if (counts_host)
-        for (int k = 0; k < VG_IT_COUNT; ++k) counts_host[k] = c[k];
-    if (rays_host)
-        for (int k = 0; k < 3; ++k) rays_host[k] = c[VG_IT_COUNT + k];
-    return VG_OK;
+    return probes_update_counts(pr, p, nullptr, 0u, rays_host, counts_host, nullptr);
 }
 
 } // extern "C"

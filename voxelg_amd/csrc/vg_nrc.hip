@@ -1,0 +1,847 @@
+// Radiance cache for gfx950: multiresolution hash-grid encoding + 3x64 ReLU MLP, inference
+// and training as bf16 MFMA kernels with fp32 accumulation, Adam with fp32 master weights.
+//
+// MFMA formulation. Every layer computes Y^T = W X^T with v_mfma_f32_16x16x32_bf16: the A
+// operand is a 16x32 block of W (rows = outputs), the B operand a 32x16 block of the layer
+// input with the 16 columns = 16 samples (one per lane of a 16-lane group), so a wave holds
+// 4 column blocks = 64 samples. The C layout (lane c + 16 g holds rows 4g..4g+3 of sample
+// c) is fed back as the next layer's B operand without leaving the registers: the k-th
+// input of k-step ks held by lane group g, element j, is feature
+//     phi(ks, g, j) = 32 ks + 16 (j / 4) + 4 g + j % 4,
+// i.e. elements 0..3 are C-block 2ks and 4..7 are C-block 2ks+1 of the previous layer. The
+// weight fragments are pre-permuted by phi (k_nrc_adam_mlp packs them after every step)
+// and staged in LDS. The input layer uses the same phi over the logical input order, so a
+// lane encodes hash levels {2g, 2g+1, 8+2g, 9+2g} of its sample.
+// Training stages a layer's input and output gradient (bf16) through LDS to form the
+// weight gradient dW = dY X^T as MFMAs over samples; per-block partial sums are reduced in
+// a second kernel (deterministic), grid gradients are scattered with fp32 atomics.
+#include "vg_nrc.h"
+
+#include <vector>
+
+namespace vg {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+struct NrcDev {
+    const uint32_t *grid_h;
+    const u32x4_t *frag_fwd;
+    const u32x4_t *frag_bwd;
+    float scale[VG_NRC_MAX_LEVELS];
+    int32_t levels, log2_table;
+};
+
+constexpr int kSRow = kNrcTile + 8; // bf16 per LDS staging row (padded)
+
+// ----------------------------------------------------------------- helpers --
+__device__ __forceinline__ uint32_t bf16_bits(float f) { // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    return bf16_bits(a) | (bf16_bits(b) << 16);
+}
+__device__ __forceinline__ float bf_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
+__device__ __forceinline__ float relu(float v) { return v > 0.0f ? v : 0.0f; }
+
+__device__ __forceinline__ f32x4_t mfma(u32x4_t a, u32x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                   __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+
+// Grid entries (level base included) and trilinear weights of the 8 corners of level l.
+__device__ __forceinline__ void level_corners(const NrcDev &m, int l, const float p[3],
+                                              uint32_t idx[8], float w[8]) {
+    const float s = m.scale[l];
+    float fr[3];
+    int32_t i0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float v = p[k] * s;
+        const float fl = floorf(v);
+        i0[k] = (int32_t)fl;
+        fr[k] = v - fl;
+    }
+    const uint32_t mask = (1u << m.log2_table) - 1u, base = (uint32_t)l << m.log2_table;
+    const uint32_t hx[2] = {(uint32_t)i0[0], (uint32_t)i0[0] + 1u};
+    const uint32_t hy[2] = {(uint32_t)i0[1] * 2654435761u, ((uint32_t)i0[1] + 1u) * 2654435761u};
+    const uint32_t hz[2] = {(uint32_t)i0[2] * 805459861u, ((uint32_t)i0[2] + 1u) * 805459861u};
+    const float wx[2] = {1.0f - fr[0], fr[0]}, wy[2] = {1.0f - fr[1], fr[1]},
+                wz[2] = {1.0f - fr[2], fr[2]};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int cx = q & 1, cy = (q >> 1) & 1, cz = q >> 2;
+        idx[q] = base + ((hx[cx] ^ hy[cy] ^ hz[cz]) & mask);
+        w[q] = (wx[cx] * wy[cy]) * wz[cz];
+    }
+}
+
+__device__ __forceinline__ void level_feat(const NrcDev &m, int l, const float p[3], float &f0,
+                                           float &f1) {
+    f0 = 0.0f;
+    f1 = 0.0f;
+    if (l >= m.levels) return;
+    uint32_t idx[8];
+    float w[8];
+    level_corners(m, l, p, idx, w);
+    uint32_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = m.grid_h[idx[q]];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        f0 = __builtin_fmaf(w[q], bf_lo(v[q]), f0);
+        f1 = __builtin_fmaf(w[q], bf_hi(v[q]), f1);
+    }
+}
+
+// Input k-steps of lane group g for one sample (phi order over the logical inputs).
+__device__ __forceinline__ void build_input(const NrcDev &m, int g, const float p[3],
+                                            const float d[3], uint32_t face, u32x4_t &x0,
+                                            u32x4_t &x1) {
+    float a0, a1, b0, b1, c0, c1, e0, e1;
+    level_feat(m, 2 * g, p, a0, a1);
+    level_feat(m, 2 * g + 1, p, b0, b1);
+    level_feat(m, 8 + 2 * g, p, c0, c1);
+    level_feat(m, 9 + 2 * g, p, e0, e1);
+    x0 = u32x4_t{pack2(a0, a1), pack2(b0, b1), pack2(c0, c1), pack2(e0, e1)};
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * g + j; // input 32 + k
+        float x = 0.0f;
+        if (k < 6) x = (face == (uint32_t)k) ? 1.0f : 0.0f;
+        else if (k < 9) x = d[k - 6];
+        else if (k == 9) x = 1.0f;
+        v[j] = x;
+    }
+    x1 = u32x4_t{pack2(v[0], v[1]), pack2(v[2], v[3]), 0u, 0u};
+}
+
+__device__ __forceinline__ void load_query(const vg_nrc_query *q, uint32_t s, bool valid,
+                                           float p[3], float d[3], uint32_t &face, uint32_t &aux) {
+    if (valid) {
+        const float4 *r = reinterpret_cast<const float4 *>(q + s);
+        const float4 a = r[0], b = r[1];
+        p[0] = a.x;
+        p[1] = a.y;
+        p[2] = a.z;
+        face = __float_as_uint(a.w);
+        d[0] = b.x;
+        d[1] = b.y;
+        d[2] = b.z;
+        aux = __float_as_uint(b.w);
+    } else {
+        p[0] = p[1] = p[2] = 0.0f;
+        d[0] = d[1] = d[2] = 0.0f;
+        face = VG_FACE_NONE;
+        aux = 0u;
+    }
+}
+
+__device__ __forceinline__ void load_frags(u32x4_t *dst, const u32x4_t *src) {
+    for (int i = threadIdx.x; i < kNrcFrags * 64; i += blockDim.x) dst[i] = src[i];
+}
+
+// acc[ob][sb] = sum over k-steps of fragment (lyr, ob, ks) x input[ks][sb].
+template <int NOB>
+__device__ __forceinline__ void mlp_layer(const u32x4_t *sf, int lyr, const u32x4_t x[2][4],
+                                          f32x4_t acc[4][4], int lane) {
+#pragma unroll
+    for (int ob = 0; ob < NOB; ++ob) {
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) acc[ob][sb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const u32x4_t a = sf[((lyr * 4 + ob) * 2 + ks) * 64 + lane];
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) acc[ob][sb] = mfma(a, x[ks][sb], acc[ob][sb]);
+        }
+    }
+}
+
+__device__ __forceinline__ void relu_pack(const f32x4_t acc[4][4], u32x4_t x[2][4]) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const f32x4_t a = acc[2 * ks][sb], b = acc[2 * ks + 1][sb];
+            x[ks][sb] = u32x4_t{pack2(relu(a[0]), relu(a[1])), pack2(relu(a[2]), relu(a[3])),
+                                pack2(relu(b[0]), relu(b[1])), pack2(relu(b[2]), relu(b[3]))};
+        }
+}
+
+// ---------------------------------------------------------------- inference --
+template <bool PROBE>
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                   const uint32_t *n_dev, uint32_t n_max,
+                                                   float4 *__restrict__ out,
+                                                   float4 *__restrict__ rad,
+                                                   const float4 *__restrict__ pal) {
+    __shared__ u32x4_t sf[kNrcFrags * 64];
+    load_frags(sf, m.frag_fwd);
+    __syncthreads();
+    uint32_t n = n_dev ? *n_dev : n_max;
+    n = n < n_max ? n : n_max;
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t tile = blockIdx.x * 4u + wave; (uint64_t)tile * 64u < n; tile += gridDim.x * 4u) {
+        u32x4_t x[2][4];
+        uint32_t face[4], aux[4];
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const uint32_t s = tile * 64u + (uint32_t)(sb * 16 + c);
+            float p[3], d[3];
+            load_query(q, s, s < n, p, d, face[sb], aux[sb]);
+            build_input(m, g, p, d, face[sb] & 0xFFu, x[0][sb], x[1][sb]);
+        }
+        f32x4_t acc[4][4];
+        mlp_layer<4>(sf, 0, x, acc, lane);
+        relu_pack(acc, x);
+        mlp_layer<4>(sf, 1, x, acc, lane);
+        relu_pack(acc, x);
+        mlp_layer<4>(sf, 2, x, acc, lane);
+        relu_pack(acc, x);
+        mlp_layer<1>(sf, 3, x, acc, lane);
+        if (g == 0) {
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) {
+                const uint32_t s = tile * 64u + (uint32_t)(sb * 16 + c);
+                if (s >= n) continue;
+                const f32x4_t y = acc[0][sb];
+                if (PROBE) {
+                    const float4 a = pal[(face[sb] >> 8) & 0xFFu];
+                    float4 r = rad[aux[sb]];
+                    r.x += a.x * relu(y[0]);
+                    r.y += a.y * relu(y[1]);
+                    r.z += a.z * relu(y[2]);
+                    rad[aux[sb]] = r;
+                } else {
+                    out[s] = make_float4(y[0], y[1], y[2], 0.0f);
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------- training --
+// x[ks][sb] element j -> LDS row phi(ks, g, j) (rows >= nrows dropped), column = sample.
+__device__ __forceinline__ void stage(uint16_t *s, const u32x4_t x[2][4], int nrows, int g, int c,
+                                      int wave) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const int col = wave * 64 + sb * 16 + c;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t u = x[ks][sb][e];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int j = 2 * e + h;
+                    const int row = 32 * ks + 16 * (j >> 2) + 4 * g + (j & 3);
+                    if (row < nrows) s[row * kSRow + col] = (uint16_t)(h ? (u >> 16) : (u & 0xFFFFu));
+                }
+            }
+        }
+}
+
+// dW block (ob, ib) over the tile's 256 samples: A = D rows 16 ob + r, B = X rows 16 ib + r,
+// k = samples 32 kk + 8 g + j. Lane c + 16 g gets dW[16 ob + 4 g + i][16 ib + c].
+__device__ __forceinline__ f32x4_t dw_block(const uint16_t *sD, const uint16_t *sX, int ob, int ib,
+                                            int g, int r) {
+    f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < kNrcTile / 32; ++kk) {
+        const u32x4_t a =
+            *reinterpret_cast<const u32x4_t *>(sD + (16 * ob + r) * kSRow + 32 * kk + 8 * g);
+        const u32x4_t b =
+            *reinterpret_cast<const u32x4_t *>(sX + (16 * ib + r) * kSRow + 32 * kk + 8 * g);
+        acc = mfma(a, b, acc);
+    }
+    return acc;
+}
+
+// Weight gradient of layer lyr for this tile, added to the block partial (stored on the
+// block's first tile). Every (block, weight) entry belongs to one wave.
+__device__ __forceinline__ void weight_grad(uint16_t *sX, uint16_t *sD, const u32x4_t X[2][4],
+                                            const u32x4_t D[2][4], int lyr, float *part,
+                                            bool first, int g, int c, int wave) {
+    const int nrows = lyr == 3 ? 16 : 64;
+    stage(sX, X, 64, g, c, wave);
+    stage(sD, D, nrows, g, c, wave);
+    __syncthreads();
+    const int nblk = (nrows / 16) * 4;
+    for (int b = wave; b < nblk; b += 4) {
+        const int ob = b >> 2, ib = b & 3;
+        const f32x4_t a = dw_block(sD, sX, ob, ib, g, c);
+        float *dst = part + lyr * 4096 + (16 * ob + 4 * g) * 64 + 16 * ib + c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[i * 64] = first ? a[i] : dst[i * 64] + a[i];
+    }
+    __syncthreads();
+}
+
+// D_in = (W^T D_out) through the ReLU of the layer input Xin, packed as the next B operand.
+__device__ __forceinline__ void back_layer(const u32x4_t *sbf, int lyr, int nks,
+                                           const u32x4_t Dout[2][4], const u32x4_t Xin[2][4],
+                                           u32x4_t Din[2][4], int lane) {
+    f32x4_t acc[4][4];
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) acc[ib][sb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < nks; ++ks) {
+            const u32x4_t a = sbf[((lyr * 4 + ib) * 2 + ks) * 64 + lane];
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) acc[ib][sb] = mfma(a, Dout[ks][sb], acc[ib][sb]);
+        }
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            uint32_t o[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int j0 = 2 * e, j1 = 2 * e + 1;
+                const float v0 = acc[2 * ks + (j0 >> 2)][sb][j0 & 3];
+                const float v1 = acc[2 * ks + (j1 >> 2)][sb][j1 & 3];
+                const uint32_t u = Xin[ks][sb][e];
+                o[e] = pack2((u & 0xFFFFu) ? v0 : 0.0f, (u >> 16) ? v1 : 0.0f);
+            }
+            Din[ks][sb] = u32x4_t{o[0], o[1], o[2], o[3]};
+        }
+}
+
+__global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                   const float4 *__restrict__ tgt, uint32_t n,
+                                                   float inv_n, float *__restrict__ partial,
+                                                   float *__restrict__ grid_g) {
+    __shared__ u32x4_t sf[kNrcFrags * 64];
+    __shared__ u32x4_t sbw[kNrcFrags * 64];
+    __shared__ __align__(16) uint16_t sX[kNrcWidth * kSRow];
+    __shared__ __align__(16) uint16_t sD[kNrcWidth * kSRow];
+    __shared__ float sl[4];
+    load_frags(sf, m.frag_fwd);
+    load_frags(sbw, m.frag_bwd);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int wave = threadIdx.x >> 6;
+    float *part = partial + (size_t)blockIdx.x * kNrcPartial;
+    float loss = 0.0f;
+    bool first = true;
+    for (uint32_t tile = blockIdx.x; (uint64_t)tile * kNrcTile < n; tile += gridDim.x) {
+        u32x4_t X[4][2][4];
+        float pos[4][3], tw[4], tr[4][3];
+        bool live[4];
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const uint32_t s = tile * kNrcTile + (uint32_t)(wave * 64 + sb * 16 + c);
+            const bool valid = s < n;
+            float d[3];
+            uint32_t face, aux;
+            load_query(q, s, valid, pos[sb], d, face, aux);
+            const float4 t = valid ? tgt[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            tr[sb][0] = t.x;
+            tr[sb][1] = t.y;
+            tr[sb][2] = t.z;
+            tw[sb] = t.w;
+            live[sb] = valid && t.w != 0.0f;
+            build_input(m, g, pos[sb], d, face & 0xFFu, X[0][0][sb], X[0][1][sb]);
+        }
+        f32x4_t acc[4][4];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            mlp_layer<4>(sf, l, X[l], acc, lane);
+            relu_pack(acc, X[l + 1]);
+        }
+        mlp_layer<1>(sf, 3, X[3], acc, lane);
+        // output gradient: rows 0..2 of lane group 0
+        u32x4_t D[2][4], Dn[2][4];
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            float dy[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float e = 0.0f;
+                if (g == 0 && i < 3) {
+                    const float r = acc[0][sb][i] - tr[sb][i];
+                    e = tw[sb] * r;
+                    loss += e * r;
+                }
+                dy[i] = e * inv_n;
+            }
+            D[0][sb] = u32x4_t{pack2(dy[0], dy[1]), pack2(dy[2], dy[3]), 0u, 0u};
+            D[1][sb] = u32x4_t{0u, 0u, 0u, 0u};
+        }
+        weight_grad(sX, sD, X[3], D, 3, part, first, g, c, wave);
+        back_layer(sbw, 3, 1, D, X[3], Dn, lane);
+        weight_grad(sX, sD, X[2], Dn, 2, part, first, g, c, wave);
+        back_layer(sbw, 2, 2, Dn, X[2], D, lane);
+        weight_grad(sX, sD, X[1], D, 1, part, first, g, c, wave);
+        back_layer(sbw, 1, 2, D, X[1], Dn, lane);
+        weight_grad(sX, sD, X[0], Dn, 0, part, first, g, c, wave);
+        // encoding gradient (inputs 0..31 = C-blocks 0 and 1), scattered to the grid corners
+        f32x4_t ga[2][4];
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib) {
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) ga[ib][sb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const u32x4_t a = sbw[((0 * 4 + ib) * 2 + ks) * 64 + lane];
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb) ga[ib][sb] = mfma(a, Dn[ks][sb], ga[ib][sb]);
+            }
+        }
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            if (!live[sb]) continue;
+#pragma unroll
+            for (int lv = 0; lv < 4; ++lv) {
+                const int l = lv < 2 ? 2 * g + lv : 8 + 2 * g + (lv - 2);
+                if (l >= m.levels) continue;
+                const float d0 = ga[lv >> 1][sb][(lv & 1) * 2];
+                const float d1 = ga[lv >> 1][sb][(lv & 1) * 2 + 1];
+                if (d0 == 0.0f && d1 == 0.0f) continue;
+                uint32_t idx[8];
+                float w[8];
+                level_corners(m, l, pos[sb], idx, w);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    unsafeAtomicAdd(grid_g + 2u * idx[k], w[k] * d0);
+                    unsafeAtomicAdd(grid_g + 2u * idx[k] + 1u, w[k] * d1);
+                }
+            }
+        }
+        first = false;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) sl[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[VG_NRC_MLP_PARAMS] = ((sl[0] + sl[1]) + (sl[2] + sl[3])) * 0.5f * inv_n;
+}
+
+// Sum of the block partials: the MLP gradient and the loss; with `update`, one more step.
+__global__ void k_nrc_reduce(const float *__restrict__ partial, int nblocks, float *mlp_g,
+                             float *loss, uint32_t *step, int update) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > VG_NRC_MLP_PARAMS) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kNrcPartial + p];
+    if (p < VG_NRC_MLP_PARAMS) {
+        mlp_g[p] = s;
+    } else {
+        *loss = s;
+        if (update) *step += 1u;
+    }
+}
+
+__device__ __forceinline__ void adam1(float &p, float &m, float &v, float g, float lr, float b1,
+                                      float b2, float eps, float c1, float c2) {
+    m = b1 * m + (1.0f - b1) * g;
+    v = b2 * v + (1.0f - b2) * (g * g);
+    p -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+// Sparse Adam over the grid: entries with a zero gradient are left alone; the gradient is
+// cleared for the next step and the bf16 copy the kernels read is refreshed.
+__global__ void k_nrc_adam_grid(float2 *__restrict__ p, float2 *__restrict__ mm,
+                                float2 *__restrict__ vv, float2 *__restrict__ gg,
+                                uint32_t *__restrict__ ph, int64_t n_entries, float lr, float b1,
+                                float b2, float eps, const uint32_t *step) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_entries) return;
+    const float2 g = gg[e];
+    if (g.x == 0.0f && g.y == 0.0f) return;
+    const float t = (float)*step;
+    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+    float2 pp = p[e], m = mm[e], v = vv[e];
+    if (g.x != 0.0f) adam1(pp.x, m.x, v.x, g.x, lr, b1, b2, eps, c1, c2);
+    if (g.y != 0.0f) adam1(pp.y, m.y, v.y, g.y, lr, b1, b2, eps, c1, c2);
+    p[e] = pp;
+    mm[e] = m;
+    vv[e] = v;
+    gg[e] = make_float2(0.0f, 0.0f);
+    ph[e] = pack2(pp.x, pp.y);
+}
+
+// bf16 copy of the whole grid (after an upload or initialisation).
+__global__ void k_nrc_grid_pack(const float2 *__restrict__ p, uint32_t *__restrict__ ph,
+                                int64_t n_entries) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_entries) return;
+    const float2 v = p[e];
+    ph[e] = pack2(v.x, v.y);
+}
+
+// Dense Adam over the MLP (with `update`), then the forward and backward fragments of the
+// (new) weights. One block.
+__global__ __launch_bounds__(1024) void k_nrc_adam_mlp(float *__restrict__ p, float *__restrict__ mm,
+                                                       float *__restrict__ vv,
+                                                       const float *__restrict__ g,
+                                                       uint16_t *__restrict__ ffwd,
+                                                       uint16_t *__restrict__ fbwd, float lr,
+                                                       float b1, float b2, float eps,
+                                                       const uint32_t *step, int update) {
+    if (update) {
+        const float t = (float)*step;
+        const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+        for (int i = threadIdx.x; i < VG_NRC_MLP_PARAMS; i += blockDim.x) {
+            float pp = p[i], m = mm[i], v = vv[i];
+            adam1(pp, m, v, g[i], lr, b1, b2, eps, c1, c2);
+            p[i] = pp;
+            mm[i] = m;
+            vv[i] = v;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kNrcFrags * 64 * 8; e += blockDim.x) {
+        const int f = e >> 9, lane = (e >> 3) & 63, j = e & 7;
+        const int lyr = f >> 3, blk = (f >> 1) & 3, ks = f & 1;
+        const int r = lane & 15, gg = lane >> 4;
+        const int phi = 32 * ks + 16 * (j >> 2) + 4 * gg + (j & 3);
+        float wf, wb;
+        if (lyr < 3) {
+            wf = p[lyr * 4096 + (16 * blk + r) * 64 + phi];
+            wb = p[lyr * 4096 + phi * 64 + 16 * blk + r];
+        } else {
+            wf = blk == 0 ? p[12288 + r * 64 + phi] : 0.0f;
+            wb = phi < 16 ? p[12288 + phi * 64 + 16 * blk + r] : 0.0f;
+        }
+        ffwd[e] = (uint16_t)bf16_bits(wf);
+        fbwd[e] = (uint16_t)bf16_bits(wb);
+    }
+}
+
+// ---------------------------------------------------------------------- host --
+static NrcDev dev_view(const vg_nrc *m) {
+    NrcDev d;
+    d.grid_h = m->grid_h;
+    d.frag_fwd = reinterpret_cast<const u32x4_t *>(m->frag_fwd);
+    d.frag_bwd = reinterpret_cast<const u32x4_t *>(m->frag_bwd);
+    for (int l = 0; l < VG_NRC_MAX_LEVELS; ++l) d.scale[l] = m->cfg.scale[l];
+    d.levels = m->cfg.levels;
+    d.log2_table = m->cfg.log2_table;
+    return d;
+}
+
+static int set_dev(const vg_nrc *m) {
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    return VG_OK;
+}
+
+int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, const uint32_t *n_dev, uint32_t n_max,
+                     float4 *out4, float4 *rad, const float4 *pal, hipStream_t stream) {
+    if (n_max == 0) return VG_OK;
+    const uint64_t tiles = ((uint64_t)n_max + 63) / 64;
+    uint64_t blocks = (tiles + 3) / 4;
+    const uint64_t cap = (uint64_t)m->ctx->cus * 8;
+    if (blocks > cap) blocks = cap;
+    if (rad)
+        hipLaunchKernelGGL(k_nrc_infer<true>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                           dev_view(m), q, n_dev, n_max, out4, rad, pal);
+    else
+        hipLaunchKernelGGL(k_nrc_infer<false>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                           dev_view(m), q, n_dev, n_max, out4, rad, pal);
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
+}
+
+int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
+                     hipStream_t stream) {
+    if (n == 0) return VG_OK;
+    const uint64_t tiles = ((uint64_t)n + kNrcTile - 1) / kNrcTile;
+    const int blocks = (int)(tiles < (uint64_t)m->partial_blocks ? tiles : m->partial_blocks);
+    hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, stream, dev_view(m), q, t, n,
+                       1.0f / (float)n, m->partial, m->grid_g);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_nrc_reduce, dim3((VG_NRC_MLP_PARAMS + 1 + 255) / 256), dim3(256), 0,
+                       stream, m->partial, blocks, m->mlp_g, m->loss, m->step, update ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    if (update) {
+        const vg_nrc_config &c = m->cfg;
+        hipLaunchKernelGGL(k_nrc_adam_grid, dim3((uint32_t)((m->n_entries + 255) / 256)), dim3(256),
+                           0, stream, reinterpret_cast<float2 *>(m->grid),
+                           reinterpret_cast<float2 *>(m->grid_m),
+                           reinterpret_cast<float2 *>(m->grid_v),
+                           reinterpret_cast<float2 *>(m->grid_g), m->grid_h, m->n_entries,
+                           c.lr_grid, c.beta1, c.beta2, c.eps, m->step);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(1), dim3(1024), 0, stream, m->mlp, m->mlp_m,
+                           m->mlp_v, m->mlp_g, reinterpret_cast<uint16_t *>(m->frag_fwd),
+                           reinterpret_cast<uint16_t *>(m->frag_bwd), c.lr_mlp, c.beta1, c.beta2,
+                           c.eps, m->step, 1);
+        HIP_TRY(hipGetLastError());
+    }
+    return VG_OK;
+}
+
+static uint32_t hmix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Uniform in [-1, 1) from (seed, stream, index).
+static float unit_sym(uint32_t seed, uint32_t stream, uint64_t i) {
+    uint32_t x = hmix32(seed ^ hmix32(stream * 0x9E3779B9u));
+    x = hmix32(x ^ (uint32_t)i);
+    x = hmix32(x ^ (uint32_t)(i >> 32));
+    return (float)((double)(x >> 8) * (2.0 / 16777216.0) - 1.0);
+}
+
+static int check_config(const vg_nrc_config *c) {
+    if (!c) return fail(VG_E_INVALID, "null config");
+    if (c->levels < 1 || c->levels > VG_NRC_MAX_LEVELS)
+        return fail(VG_E_INVALID, "levels %d (1..%d)", c->levels, VG_NRC_MAX_LEVELS);
+    if (c->log2_table < 4 || c->log2_table > 22)
+        return fail(VG_E_INVALID, "log2_table %d (4..22)", c->log2_table);
+    for (int l = 0; l < c->levels; ++l)
+        if (!(c->scale[l] > 0.0f) || !std::isfinite(c->scale[l]))
+            return fail(VG_E_INVALID, "scale[%d] = %g", l, (double)c->scale[l]);
+    if (c->batch < 1 || c->batch > VG_NRC_MAX_BATCH)
+        return fail(VG_E_INVALID, "batch %d (1..%d)", c->batch, VG_NRC_MAX_BATCH);
+    if (c->steps < 0 || c->steps > 64) return fail(VG_E_INVALID, "steps %d (0..64)", c->steps);
+    if (!(c->beta1 >= 0.0f && c->beta1 < 1.0f && c->beta2 >= 0.0f && c->beta2 < 1.0f &&
+          c->eps >= 0.0f && c->lr_grid >= 0.0f && c->lr_mlp >= 0.0f && c->init_grid >= 0.0f))
+        return fail(VG_E_INVALID, "optimizer settings out of range");
+    return VG_OK;
+}
+
+static void free_nrc(vg_nrc *m) {
+    void *ptrs[] = {m->mlp,    m->mlp_m,  m->mlp_v,  m->mlp_g,  m->partial, m->frag_fwd,
+                    m->frag_bwd, m->grid, m->grid_m, m->grid_v, m->grid_g,  m->grid_h,
+                    m->step,   m->loss,   m->tq,     m->tt};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+// Upload parameters, clear the optimizer, refresh the bf16 copies. Synchronous.
+static int load_params(vg_nrc *m, const float *mlp, const float *grid) {
+    const size_t mb = VG_NRC_MLP_PARAMS * sizeof(float), gb = (size_t)m->n_grid * sizeof(float);
+    if (mlp) HIP_TRY(hipMemcpy(m->mlp, mlp, mb, hipMemcpyHostToDevice));
+    if (grid) HIP_TRY(hipMemcpy(m->grid, grid, gb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(m->mlp_m, 0, mb));
+    HIP_TRY(hipMemset(m->mlp_v, 0, mb));
+    HIP_TRY(hipMemset(m->mlp_g, 0, mb));
+    HIP_TRY(hipMemset(m->grid_m, 0, gb));
+    HIP_TRY(hipMemset(m->grid_v, 0, gb));
+    HIP_TRY(hipMemset(m->grid_g, 0, gb));
+    HIP_TRY(hipMemset(m->step, 0, sizeof(uint32_t)));
+    HIP_TRY(hipMemset(m->loss, 0, sizeof(float)));
+    hipLaunchKernelGGL(k_nrc_grid_pack, dim3((uint32_t)((m->n_entries + 255) / 256)), dim3(256), 0,
+                       nullptr, reinterpret_cast<const float2 *>(m->grid), m->grid_h, m->n_entries);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(1), dim3(1024), 0, nullptr, m->mlp, m->mlp_m, m->mlp_v,
+                       m->mlp_g, reinterpret_cast<uint16_t *>(m->frag_fwd),
+                       reinterpret_cast<uint16_t *>(m->frag_bwd), 0.0f, 0.0f, 0.0f, 0.0f, m->step, 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return VG_OK;
+}
+
+static int check_batch(const vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    if (n > VG_NRC_MAX_BATCH) return fail(VG_E_INVALID, "n %u above %d", n, VG_NRC_MAX_BATCH);
+    if (n && (!q || !t)) return fail(VG_E_INVALID, "null queries or targets");
+    return set_dev(m);
+}
+
+} // namespace vg
+
+using namespace vg;
+
+extern "C" {
+
+int vg_nrc_scales(int32_t levels, float coarse_cell, float fine_cell, float *out) {
+    if (!out || levels < 1 || levels > VG_NRC_MAX_LEVELS || !(coarse_cell > 0.0f) ||
+        !(fine_cell > 0.0f))
+        return fail(VG_E_INVALID, "bad scale arguments");
+    const double b = levels > 1 ? std::pow((double)coarse_cell / fine_cell, 1.0 / (levels - 1)) : 1.0;
+    for (int l = 0; l < levels; ++l) out[l] = (float)(std::pow(b, l) / (double)coarse_cell);
+    return VG_OK;
+}
+
+int vg_nrc_config_default(vg_nrc_config *c) {
+    if (!c) return fail(VG_E_INVALID, "null config");
+    std::memset(c, 0, sizeof(*c));
+    c->levels = 16;
+    c->log2_table = 19;
+    vg_nrc_scales(16, 128.0f, 1.0f, c->scale);
+    c->lr_grid = 1e-2f;
+    c->lr_mlp = 1e-2f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.99f;
+    c->eps = 1e-10f;
+    c->init_grid = 1e-4f;
+    c->seed = 0x5EEDCAC4u;
+    c->batch = 65536;
+    c->steps = 1;
+    return VG_OK;
+}
+
+int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
+    if (!ctx || !out) return fail(VG_E_INVALID, "null argument");
+    *out = nullptr;
+    int st = check_config(cfg);
+    if (st) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    vg_nrc *m = new vg_nrc();
+    std::memset(m, 0, sizeof(*m));
+    m->ctx = ctx;
+    m->cfg = *cfg;
+    m->n_entries = (int64_t)cfg->levels << cfg->log2_table;
+    m->n_grid = 2 * m->n_entries;
+    m->partial_blocks = 1024;
+    m->tcap = cfg->batch;
+    const size_t mb = VG_NRC_MLP_PARAMS * sizeof(float), gb = (size_t)m->n_grid * sizeof(float);
+    const size_t fb = (size_t)kNrcFrags * 64 * 16;
+    bool ok = hipMalloc((void **)&m->mlp, mb) == hipSuccess &&
+              hipMalloc((void **)&m->mlp_m, mb) == hipSuccess &&
+              hipMalloc((void **)&m->mlp_v, mb) == hipSuccess &&
+              hipMalloc((void **)&m->mlp_g, mb) == hipSuccess &&
+              hipMalloc((void **)&m->partial,
+                        (size_t)m->partial_blocks * kNrcPartial * sizeof(float)) == hipSuccess &&
+              hipMalloc((void **)&m->frag_fwd, fb) == hipSuccess &&
+              hipMalloc((void **)&m->frag_bwd, fb) == hipSuccess &&
+              hipMalloc((void **)&m->grid, gb) == hipSuccess &&
+              hipMalloc((void **)&m->grid_m, gb) == hipSuccess &&
+              hipMalloc((void **)&m->grid_v, gb) == hipSuccess &&
+              hipMalloc((void **)&m->grid_g, gb) == hipSuccess &&
+              hipMalloc((void **)&m->grid_h, (size_t)m->n_entries * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->step, 256) == hipSuccess &&
+              hipMalloc((void **)&m->loss, 256) == hipSuccess &&
+              hipMalloc((void **)&m->tq, (size_t)m->tcap * sizeof(vg_nrc_query)) == hipSuccess &&
+              hipMalloc((void **)&m->tt, (size_t)m->tcap * sizeof(float4)) == hipSuccess;
+    if (!ok) {
+        free_nrc(m);
+        delete m;
+        return fail(VG_E_NOMEM, "radiance cache storage (%lld grid parameters)",
+                    (long long)(2 * ((int64_t)cfg->levels << cfg->log2_table)));
+    }
+    // Xavier-uniform MLP (W3 rows >= 3 zero), uniform grid features.
+    std::vector<float> mlp(VG_NRC_MLP_PARAMS, 0.0f), grid((size_t)m->n_grid);
+    const float s_hidden = std::sqrt(6.0f / 128.0f), s_out = std::sqrt(6.0f / 67.0f);
+    for (int i = 0; i < 3 * 4096; ++i) mlp[i] = unit_sym(cfg->seed, 1, (uint64_t)i) * s_hidden;
+    for (int i = 0; i < 3 * 64; ++i) mlp[12288 + i] = unit_sym(cfg->seed, 2, (uint64_t)i) * s_out;
+    for (int64_t i = 0; i < m->n_grid; ++i) grid[(size_t)i] = unit_sym(cfg->seed, 3, (uint64_t)i) * cfg->init_grid;
+    HIP_TRY(hipMemset(m->tq, 0, (size_t)m->tcap * sizeof(vg_nrc_query)));
+    HIP_TRY(hipMemset(m->tt, 0, (size_t)m->tcap * sizeof(float4)));
+    st = load_params(m, mlp.data(), grid.data());
+    if (st) {
+        free_nrc(m);
+        delete m;
+        return st;
+    }
+    *out = m;
+    return VG_OK;
+}
+
+int vg_nrc_destroy(vg_nrc *m) {
+    if (!m) return VG_OK;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipDeviceSynchronize();
+    free_nrc(m);
+    delete m;
+    return VG_OK;
+}
+
+int vg_nrc_param_counts(const vg_nrc *m, int64_t out[2]) {
+    if (!m || !out) return fail(VG_E_INVALID, "null argument");
+    out[0] = VG_NRC_MLP_PARAMS;
+    out[1] = m->n_grid;
+    return VG_OK;
+}
+
+int vg_nrc_get_params(vg_nrc *m, float *mlp, float *grid) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    int st = set_dev(m);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    if (mlp) HIP_TRY(hipMemcpy(mlp, m->mlp, VG_NRC_MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+    if (grid)
+        HIP_TRY(hipMemcpy(grid, m->grid, (size_t)m->n_grid * sizeof(float), hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+int vg_nrc_set_params(vg_nrc *m, const float *mlp, const float *grid) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    int st = set_dev(m);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    return load_params(m, mlp, grid);
+}
+
+int vg_nrc_infer(vg_nrc *m, const vg_nrc_query *q, uint32_t n, float *out4, void *stream) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    if (n && (!q || !out4)) return fail(VG_E_INVALID, "null queries or output");
+    int st = set_dev(m);
+    if (st) return st;
+    return nrc_infer_launch(m, q, nullptr, n, reinterpret_cast<float4 *>(out4), nullptr, nullptr,
+                            (hipStream_t)stream);
+}
+
+int vg_nrc_train_step(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n, void *stream) {
+    int st = check_batch(m, q, t, n);
+    if (st) return st;
+    if (n == 0) return fail(VG_E_INVALID, "empty batch");
+    return nrc_train_launch(m, q, reinterpret_cast<const float4 *>(t), n, true, (hipStream_t)stream);
+}
+
+int vg_nrc_gradients(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n,
+                     float *mlp_grad, float *grid_grad, float *loss) {
+    int st = check_batch(m, q, t, n);
+    if (st) return st;
+    if (n == 0) return fail(VG_E_INVALID, "empty batch");
+    HIP_TRY(hipDeviceSynchronize());
+    st = nrc_train_launch(m, q, reinterpret_cast<const float4 *>(t), n, false, nullptr);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    if (mlp_grad)
+        HIP_TRY(hipMemcpy(mlp_grad, m->mlp_g, VG_NRC_MLP_PARAMS * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    if (grid_grad)
+        HIP_TRY(hipMemcpy(grid_grad, m->grid_g, (size_t)m->n_grid * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    if (loss) HIP_TRY(hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
+    // the optimizer expects a clear grid gradient
+    HIP_TRY(hipMemset(m->grid_g, 0, (size_t)m->n_grid * sizeof(float)));
+    HIP_TRY(hipDeviceSynchronize());
+    return VG_OK;
+}
+
+int vg_nrc_stats(vg_nrc *m, int64_t *steps, float *loss) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    int st = set_dev(m);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    uint32_t s = 0;
+    float l = 0.0f;
+    HIP_TRY(hipMemcpy(&s, m->step, sizeof(s), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&l, m->loss, sizeof(l), hipMemcpyDeviceToHost));
+    if (steps) *steps = s;
+    if (loss) *loss = l;
+    return VG_OK;
+}
+
+int vg_nrc_last_batch(vg_nrc *m, vg_nrc_query *q, float *t) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    int st = set_dev(m);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    if (q) HIP_TRY(hipMemcpy(q, m->tq, (size_t)m->tcap * sizeof(vg_nrc_query), hipMemcpyDeviceToHost));
+    if (t) HIP_TRY(hipMemcpy(t, m->tt, (size_t)m->tcap * sizeof(float4), hipMemcpyDeviceToHost));
+    return VG_OK;
+}
+
+} // extern "C"

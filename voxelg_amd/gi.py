@@ -148,6 +148,26 @@ class ProbeGrid:
         return (dict(zip(("gather", "shadow", "bounce"), (int(r) for r in rays))),
                 dict(zip(IT_NAMES, (int(c) for c in cnt))))
 
+    def update_learned(self, cache, params: np.ndarray, train: bool = True,
+                       stream=None) -> "ProbeGrid":
+        """The learned update: gather + sun shadow, the cache's bounce tail, projection and
+        fold; with `train`, the cache's online training steps on this update's hits."""
+        p = np.ascontiguousarray(params, GI_PARAMS_DTYPE)
+        check(lib().vg_probes_update_learned(self._h, ptr(p), cache._h, 1 if train else 0,
+                                             _stream(stream)), "vg_probes_update_learned")
+        return self
+
+    def update_learned_counts(self, cache, params: np.ndarray, train: bool = True):
+        """One learned update reporting (rays dict: gather/shadow/bounce, cache queries)."""
+        p = np.ascontiguousarray(params, GI_PARAMS_DTYPE)
+        rays = (C.c_uint64 * 3)()
+        nq = C.c_uint64()
+        check(lib().vg_probes_update_learned_counts(self._h, ptr(p), cache._h, 1 if train else 0,
+                                                    rays, C.byref(nq)),
+              "vg_probes_update_learned_counts")
+        return (dict(zip(("gather", "shadow", "bounce"), (int(r) for r in rays))),
+                int(nq.value))
+
     def download(self) -> np.ndarray:
         out = np.zeros(self.total, PROBE_DTYPE)
         check(lib().vg_probes_download(self._h, ptr(out)), "vg_probes_download")
