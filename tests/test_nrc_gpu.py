@@ -85,10 +85,11 @@ def test_gradients_match_oracle(cache12):
     assert rel(gg, og) < 1e-2, rel(gg, og)
     # entries no query touches stay exactly zero
     assert np.all(gg[og == 0] == 0)
-    # a second call gives the same MLP gradient (deterministic reduction), grid up to atomics
+    # a second call gives the same MLP gradient (deterministic reduction); the grid gradient
+    # agrees up to the order of its atomics (fp32 coarse levels, bf16 fine levels)
     _, gm2, gg2 = cache12.gradients(q, t)
     np.testing.assert_array_equal(gm2, gm)
-    assert rel(gg2, gg) < 1e-5
+    assert rel(gg2, gg) < 5e-3
 
 
 def test_train_step_is_adam_on_the_gradient(cache12):
@@ -105,13 +106,22 @@ def test_train_step_is_adam_on_the_gradient(cache12):
     assert steps == 1
     m1, g1 = cache12.params()
     c = cache12.config[0]
-    for p0, g, got, lr in ((mlp, gm, m1, c["lr_mlp"]), (grid, gg, g1, c["lr_grid"])):
+    for name, p0, g, got, lr in (("mlp", mlp, gm, m1, c["lr_mlp"]),
+                                 ("grid", grid, gg, g1, c["lr_grid"])):
         p = p0.astype(np.float64)
         oracle.nrc_adam(p, np.zeros_like(p), np.zeros_like(p), g.astype(np.float64), float(lr),
                         float(c["beta1"]), float(c["beta2"]), float(c["eps"]), 1, True)
         live = g != 0
-        np.testing.assert_allclose(got[live], p[live], rtol=1e-5, atol=1e-7)
-        np.testing.assert_array_equal(got[~live], p0[~live])
+        bad = np.abs(got - p) > 1e-5 * np.abs(p) + 1e-7
+        if name == "mlp":  # deterministic gradient: every weight follows Adam
+            assert not bad.any()
+        else:
+            # the train step recomputes the grid gradient, whose atomics may sum in another
+            # order: a gradient that nearly cancels can change sign or reach exactly zero in
+            # one of the two evaluations (a move of up to 2 lr); only such entries differ
+            assert bad.sum() <= 1e-3 * live.sum(), bad.sum()
+            assert np.all(np.abs(g[bad]) < 1e-2 * np.abs(g).max())
+            assert np.all(np.abs(got[bad] - p0[bad]) <= 2.0 * float(lr) * (1 + 1e-5))
     # the second step uses the gradient at the new parameters
     cache12.train_step(q, t)
     assert cache12.stats()[0] == 2
@@ -153,13 +163,21 @@ def test_learned_update_matches_oracle(ctx, small, small_world):
 
 
 def test_training_targets_match_oracle(ctx, small, small_world):
+    """A learned update trains on the previous learned update's hits: the first has none
+    (every target weight 0), the second traces a bounce for each of its batch."""
     cache = nrc.RadianceCache(ctx, nrc.default_config(log2_table=14, batch=3000))
     p, dirs, bt, pal = probe_setup(small)
-    p["round"] = 5
     pg = gi.ProbeGrid(small, p)
+    p["round"] = 4
+    case("learned update with training, no previous hits")
+    rays0, nq0 = pg.update_learned_counts(cache, p, train=True)
+    assert cache.stats()[0] == 1 and rays0["bounce"] == 0
+    q, t = cache.last_batch()
+    assert np.all(t == 0) and np.all(q["face"] == 6)
+    p["round"] = 5
     case("learned update with training: targets")
     rays, nq = pg.update_learned_counts(cache, p, train=True)
-    assert cache.stats()[0] == 1
+    assert cache.stats()[0] == 2
     q, t = cache.last_batch()
     assert len(q) == 3000 and np.all(t[:, 3] == 1.0)
     assert rays["bounce"] == 3000  # every training sample traces its bounce ray
@@ -167,6 +185,9 @@ def test_training_targets_match_oracle(ctx, small, small_world):
     ref = ow.gi_bounce_tails(p, bt, pal, q, q["aux"], threads=THREADS)
     np.testing.assert_allclose(t[:, :3], ref, rtol=2e-6, atol=1e-7)
     assert np.all(q["face"] < 6) and np.all(q["aux"] < 4096)
+    # the batch is stratified over the previous update's nq0 hits: every hit is drawn
+    keys = {(tuple(r["pos"]), int(r["face"])) for r in q}
+    assert len(keys) == nq0
 
 
 def test_online_training_gains_over_no_cache(ctx, small, small_world):

@@ -13,12 +13,15 @@
 // Samples of one probe have consecutive ids, so a wave's gather rays share their origin.
 // The learned update (vg_probes_update_learned) runs the path kernel without the bounce:
 // every gather hit is appended to a compacted list of cache queries, one inference launch
-// of the radiance cache adds its bounce tail, and k_nrc_targets traces the training
-// targets (cosine bounce + sun shadow) for a batch of those hits.
+// of the radiance cache adds its bounce tail. The same path launch also traces the training
+// targets (cosine bounce + sun shadow) of a batch drawn from the previous learned update's
+// hits (the list is double-buffered; the half in use is a device-side parity), so the long
+// training paths overlap the gather work instead of forming a launch of their own.
 #include "vg_nrc.h"
 
 struct vg_probes {
     vg_world *world;
+    int device;             // the world's device (destroy does not touch the world)
     int32_t grid[3];
     int32_t rays, epochs;
     int64_t total;          // probes
@@ -29,8 +32,9 @@ struct vg_probes {
     float4 *pwork;          // per updated probe: ray origin xyz, w = flags bits
     float4 *rad;            // per sample: radiance rgb, visible distance
     uint8_t *face;          // per sample: face code of the gather hit (VG_FACE_NONE: none)
-    vg_nrc_query *queries;  // learned update: compacted gather hits (one per sample at most)
-    uint32_t *ctr;          // counters: [0] path work, [1] queries, [2 + s] training step s work
+    vg_nrc_query *queries;  // learned update: compacted gather hits, two halves of `ns` entries
+    size_t ns;              // samples per update at most (total probes x rays)
+    uint32_t *ctr;          // device counters, see kGiCtrs
     unsigned long long *counts; // VG_IT_COUNT walk classes + 3 ray kinds
 };
 
@@ -38,7 +42,10 @@ namespace vg {
 
 constexpr uint32_t kProbeNew = 1u, kProbeBuried = 2u;
 constexpr int kGiCounters = VG_IT_COUNT + 3;
-constexpr int kGiCtrs = 2 + 64; // work, queries, one per training step (vg_nrc_config.steps <= 64)
+// Device counters of a probe grid: [0] path work and [1] this update's cache queries (one
+// 64-bit word), [2] the previous learned update's query count, [3] the query-list half this
+// update writes, [4 + s - 1] work of training step s >= 1 (vg_nrc_config.steps <= 64).
+constexpr int kGiCtrs = 4 + 64;
 
 // ----------------------------------------------------------------- helpers --
 __device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
@@ -118,6 +125,16 @@ __device__ __forceinline__ uint32_t bounce_index(uint32_t seed, int32_t round, u
     return x >> 20; // VG_GI_BOUNCE_TABLE = 4096 entries
 }
 
+// Training sample j of a batch of n drawn from a list of nq hits: stratified over the list
+// (whose order follows probe order) from a per-step random offset, and its bounce-table
+// index.
+__device__ __forceinline__ void train_pick(uint32_t tseed, int32_t round, uint32_t j, uint32_t n,
+                                           uint32_t nq, uint32_t &qi, uint32_t &bidx) {
+    const uint32_t x = mix32(tseed ^ ((uint32_t)round * 0x85EBCA6Bu));
+    qi = (uint32_t)(((uint64_t)(x % nq) + (uint64_t)j * nq / n) % nq);
+    bidx = mix32(x ^ mix32(j ^ 0x68E31DA4u)) >> 20;
+}
+
 // Cosine-hemisphere table entry (t, b, c) turned to the hit face: c along the outward
 // normal, t and b on the next two axes in cyclic order.
 __device__ __forceinline__ void bounce_dir(const float4 e, uint32_t f, float out[3]) {
@@ -176,9 +193,14 @@ __device__ __forceinline__ void probe_cell(const vg_gi_params &p, const DevWorld
 // ----------------------------------------------------------------- kernels --
 __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params p, uint32_t n_upd,
                                                       uint32_t phase, const vg_probe *probes,
-                                                      float4 *pwork, uint32_t *ctr) {
+                                                      float4 *pwork, uint32_t *ctr, int learned) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j < (uint32_t)kGiCtrs) ctr[j] = 0u; // the kernels enqueued next consume the counters
+    if (j == 0) { // the kernels enqueued next consume the counters
+        const uint32_t prev = ctr[1], par = ctr[3];
+        for (int k = 0; k < kGiCtrs; ++k) ctr[k] = 0u;
+        ctr[2] = learned ? prev : 0u;
+        ctr[3] = learned ? (par ^ 1u) : par;
+    }
     if (j >= n_upd) return;
     const int64_t idx = (int64_t)j * p.update_div + phase;
     int32_t g[3];
@@ -203,15 +225,33 @@ __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params 
 // Path stages of a lane.
 enum : int { kIdle = 0, kGather = 1, kShadowX = 2, kBounce = 3, kShadowY = 4 };
 
+// Learned-mode arguments of the path kernel.
+struct LearnedArgs {
+    vg_nrc_query *queries; // two halves of `half` entries (this update's / the previous hits)
+    uint32_t half;
+    uint32_t train_n;      // training targets traced by this launch (work ids [0, train_n))
+    uint32_t tseed;
+    vg_nrc_query *tq;      // training batch: queries (aux = bounce-table index)
+    float4 *tt;            // ... and targets (rgb, weight)
+};
+
 template <bool COUNT, int BOUNCES>
 __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
     const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
     const float4 *__restrict__ btab, const float4 *__restrict__ pal, float4 *__restrict__ rad,
-    uint8_t *__restrict__ face_out, vg_nrc_query *__restrict__ queries, uint32_t *ctr,
-    int threshold, unsigned long long *counts) {
-    // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail)
+    uint8_t *__restrict__ face_out, LearnedArgs la, uint32_t *ctr, int threshold,
+    unsigned long long *counts) {
+    // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail;
+    // the lanes' kBounce / kShadowY stages then trace training targets)
     const uint32_t lane = threadIdx.x & 63;
+    const uint32_t par = BOUNCES == 3 ? ctr[3] : 0u;
+    vg_nrc_query *qcur = la.queries + (size_t)par * la.half;
+    const vg_nrc_query *qprev = la.queries + (size_t)(par ^ 1u) * la.half;
+    const uint32_t nprev = BOUNCES == 3 ? ctr[2] : 0u;
+    const uint32_t train_n = BOUNCES == 3 ? la.train_n : 0u;
+    const uint32_t total_ids = n_samples + train_n;
+    uint32_t tqi = 0, tbidx = 0; // training lane: query index and bounce-table index
     uint32_t cnt[VG_IT_COUNT];
     uint32_t nray[3] = {0, 0, 0};
     if (COUNT)
@@ -317,19 +357,90 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         finish_hit();
     };
 
+    // Learned: a finished hit stays pending in its (idle) lane until the wave's next refill,
+    // whose one 64-bit atomic on ctr[0..1] takes both the new work (low word) and the query
+    // slots (high word); the last pending hits go with the wave's exit.
+    auto put_query = [&](uint32_t slot) {
+        float4 *dst = reinterpret_cast<float4 *>(qcur + slot);
+        dst[0] = make_float4(eX[0], eX[1], eX[2], __uint_as_float(eF));
+        dst[1] = make_float4(eD[0], eD[1], eD[2], __uint_as_float(eS));
+        emit = false;
+    };
+    unsigned long long *ctr64 = reinterpret_cast<unsigned long long *>(ctr);
+    // Training lanes (learned): sid holds the batch index, fX the query's face, so the
+    // secondary origin; the query is re-read from the previous list when written out.
+    auto train_dir = [&](float d[3]) { bounce_dir(btab[tbidx], fX, d); };
+    auto train_write = [&](const float lb3[3]) {
+        const float4 *src = reinterpret_cast<const float4 *>(qprev + tqi);
+        const float4 a = src[0], b = src[1];
+        float4 *dst = reinterpret_cast<float4 *>(la.tq + sid);
+        dst[0] = make_float4(a.x, a.y, a.z, __uint_as_float(fX));
+        dst[1] = make_float4(b.x, b.y, b.z, __uint_as_float(tbidx));
+        la.tt[sid] = make_float4(lb3[0], lb3[1], lb3[2], 1.0f);
+        stage = kIdle;
+    };
+    auto train_sky = [&]() {
+        float d[3], sky[3], t[3];
+        train_dir(d);
+        sky_color(p, d, sky);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t[k] = sky[k] * p.gi_sky;
+        train_write(t);
+    };
+    auto train_hit = [&](float visY) {
+        const float4 a = pal[mY];
+        const float sy = ndlY * visY;
+        float t[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            t[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
+        train_write(t);
+    };
+    auto train_start = [&](uint32_t j) {
+        sid = j;
+        if (nprev == 0u) {
+            float4 *dst = reinterpret_cast<float4 *>(la.tq + j);
+            dst[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)VG_FACE_NONE));
+            dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            la.tt[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return;
+        }
+        train_pick(la.tseed, p.round, j, train_n, nprev, tqi, tbidx);
+        const float4 a = reinterpret_cast<const float4 *>(qprev + tqi)[0];
+        fX = __float_as_uint(a.w) & 0xFFu;
+        const float qp[3] = {a.x, a.y, a.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) so[k] = qp[k] + face_n(fX, k) * 0.03f;
+        float d[3];
+        train_dir(d);
+        if (!start(so, d, kBounce)) train_sky();
+    };
     for (;;) {
         uint64_t act = __ballot(stage != kIdle);
         int nact = __popcll(act);
         if (!exhausted && nact < threshold) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(ctr, (uint32_t)(64 - nact));
-            base = __shfl(base, 0, 64);
-            if (base >= n_samples) {
+            if (BOUNCES == 3) {
+                const uint64_t em = __ballot(emit);
+                unsigned long long old = 0;
+                if (lane == 0)
+                    old = atomicAdd(ctr64, ((unsigned long long)__popcll(em) << 32) |
+                                               (unsigned long long)(uint32_t)(64 - nact));
+                old = __shfl(old, 0, 64);
+                base = (uint32_t)old;
+                if (emit) put_query((uint32_t)(old >> 32) + lanes_below(em));
+            } else {
+                if (lane == 0) base = atomicAdd(ctr, (uint32_t)(64 - nact));
+                base = __shfl(base, 0, 64);
+            }
+            if (base >= total_ids) {
                 exhausted = true;
             } else if (stage == kIdle) {
                 const uint32_t id = base + lanes_below(~act);
-                if (id < n_samples) {
-                    sid = id;
+                if (id < train_n) {
+                    train_start(id);
+                } else if (id < total_ids) {
+                    sid = id - train_n;
                     const uint32_t fl = __float_as_uint(pwork[sid / N].w);
                     if (!(fl & kProbeBuried)) {
                         float o[3], d[3];
@@ -347,7 +458,19 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             nact = __popcll(__ballot(stage != kIdle));
         }
         if (nact == 0) {
-            if (exhausted) break;
+            if (exhausted) {
+                if (BOUNCES == 3) {
+                    const uint64_t em = __ballot(emit);
+                    if (em) {
+                        unsigned long long old = 0;
+                        if (lane == 0)
+                            old = atomicAdd(ctr64, (unsigned long long)__popcll(em) << 32);
+                        old = __shfl(old, 0, 64);
+                        if (emit) put_query((uint32_t)(old >> 32) + lanes_below(em));
+                    }
+                }
+                break;
+            }
             continue;
         }
         do {
@@ -387,6 +510,24 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     } else if (stage == kShadowX) {
                         visX = h.hit ? 0.0f : 1.0f;
                         after_x();
+                    } else if (BOUNCES == 3 && stage == kBounce) {
+                        if (!h.hit) {
+                            train_sky();
+                        } else {
+                            float d[3], yo[3];
+                            train_dir(d);
+                            const uint32_t fY = h.face;
+                            mY = h.mat;
+                            ndlY = face_ndl(fY, p.sun);
+                            gi_offset_origin(so, d, h.t, fY, yo);
+                            if (ndlY > 0.0f) {
+                                if (!start(yo, p.sun, kShadowY)) train_hit(1.0f);
+                            } else {
+                                train_hit(0.0f);
+                            }
+                        }
+                    } else if (BOUNCES == 3) { // kShadowY of a training lane
+                        train_hit(h.hit ? 0.0f : 1.0f);
                     } else if (stage == kBounce) {
                         if (!h.hit) {
                             float d[3], sky[3];
@@ -410,20 +551,6 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         }
                     } else { // kShadowY
                         after_y(h.hit ? 0.0f : 1.0f);
-                    }
-                }
-            }
-            if (BOUNCES == 3) { // append this iteration's finished hits, one atomic per wave
-                const uint64_t em = __ballot(emit);
-                if (em) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(ctr + 1, (uint32_t)__popcll(em));
-                    base = __shfl(base, 0, 64);
-                    if (emit) {
-                        float4 *dst = reinterpret_cast<float4 *>(queries + base + lanes_below(em));
-                        dst[0] = make_float4(eX[0], eX[1], eX[2], __uint_as_float(eF));
-                        dst[1] = make_float4(eD[0], eD[1], eD[2], __uint_as_float(eS));
-                        emit = false;
                     }
                 }
             }
@@ -564,18 +691,20 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     for (int i = 0; i < 12; ++i) rec[i] = pr[i];
 }
 
-// Training targets of the learned update: `batch` samples drawn with replacement from the
-// update's compacted gather hits (`queries`, count qn[0]); each traces the hit's cosine
-// bounce ray and the sun shadow ray at the bounce hit, like the plain 2-bounce path, and
-// writes the query (aux = bounce-table index) and its target (bounce tail rgb, weight 1).
-// With no hits at all every target gets weight 0. Persistent and ray-compacting.
+// Training targets of the learned update's steps after the first (the first step's are
+// traced by the path kernel): `batch` samples drawn from the previous learned update's
+// hits (train_pick); each traces the hit's cosine bounce ray and the sun shadow ray at the
+// bounce hit, like the plain 2-bounce path, and writes the query (aux = bounce-table index)
+// and its target (bounce tail rgb, weight 1). With no hits every target gets weight 0.
+// Persistent and ray-compacting.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_nrc_targets(
     DevWorld w, vg_gi_params p, uint32_t batch, uint32_t tseed,
-    const vg_nrc_query *__restrict__ queries, const uint32_t *qn,
+    const vg_nrc_query *__restrict__ qbase, uint32_t half, const uint32_t *gctr,
     const float4 *__restrict__ btab, const float4 *__restrict__ pal,
     vg_nrc_query *__restrict__ tq, float4 *__restrict__ tt, uint32_t *ctr, int threshold,
     unsigned long long *counts) {
+    const vg_nrc_query *queries = qbase + (size_t)(gctr[3] ^ 1u) * half;
     const uint32_t lane = threadIdx.x & 63;
     uint32_t cnt[VG_IT_COUNT];
     uint32_t nray[3] = {0, 0, 0};
@@ -583,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_nrc_targets(
 #pragma unroll
         for (int k = 0; k < VG_IT_COUNT; ++k) cnt[k] = 0;
     const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
-    const uint32_t nq = *qn;
+    const uint32_t nq = gctr[2];
     int stage = kIdle;
     uint32_t j = 0, bidx = 0, fq = 0, mY = 0;
     float qp[3] = {0.0f, 0.0f, 0.0f}, qd[3] = {0.0f, 0.0f, 0.0f};
@@ -647,10 +776,8 @@ __global__ __launch_bounds__(kBlock) void k_nrc_targets(
                         dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         tt[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     } else {
-                        uint32_t x = mix32(tseed ^ ((uint32_t)p.round * 0x85EBCA6Bu));
-                        x = mix32(x ^ j);
-                        const uint32_t qi = x % nq;
-                        bidx = mix32(x ^ 0x68E31DA4u) >> 20;
+                        uint32_t qi;
+                        train_pick(tseed, p.round, j, batch, nq, qi, bidx);
                         const float4 *src = reinterpret_cast<const float4 *>(queries + qi);
                         const float4 a = src[0], b = src[1];
                         qp[0] = a.x;
@@ -754,14 +881,22 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     const uint64_t n_samples = (uint64_t)n_upd * (uint64_t)pr->rays;
     unsigned long long *cd = count ? pr->counts : nullptr;
     hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
-                       w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr);
+                       w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0);
     HIP_TRY(hipGetLastError());
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
     const int thr = w->ctx->rays_refill;
+    const bool train = nrc && (nflags & VG_NRC_TRAIN) && nrc->cfg.steps > 0;
+    LearnedArgs la;
+    la.queries = pr->queries;
+    la.half = (uint32_t)pr->ns;
+    la.train_n = train ? (uint32_t)nrc->tcap : 0u;
+    la.tseed = nrc ? nrc->cfg.seed ^ 0x9E3779B9u : 0u;
+    la.tq = nrc ? nrc->tq : nullptr;
+    la.tt = nrc ? nrc->tt : nullptr;
 #define LAUNCH_G(C, B)                                                                         \
     hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
-                       pr->palette, pr->rad, pr->face, pr->queries, pr->ctr, thr, cd)
+                       pr->palette, pr->rad, pr->face, la, pr->ctr, thr, cd)
     const int mode = nrc ? 3 : p->bounces;
     if (count) {
         if (mode == 3) LAUNCH_G(true, 3);
@@ -775,8 +910,8 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
 #undef LAUNCH_G
     HIP_TRY(hipGetLastError());
     if (nrc) {
-        st = nrc_infer_launch(nrc, pr->queries, pr->ctr + 1, (uint32_t)n_samples, nullptr, pr->rad,
-                              pr->palette, stream);
+        st = nrc_infer_launch(nrc, pr->queries, pr->ns, pr->ctr + 3, pr->ctr + 1,
+                              (uint32_t)n_samples, nullptr, pr->rad, pr->palette, stream);
         if (st) return st;
     }
     const uint32_t wpb = kBlock / 64;
@@ -784,20 +919,22 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
                        w->dw, *p, n_upd, epoch, phase, pr->pwork, pr->dirs, pr->rad, pr->face,
                        pr->probes);
     HIP_TRY(hipGetLastError());
-    if (nrc && (nflags & VG_NRC_TRAIN)) {
+    if (train) {
         for (int s = 0; s < nrc->cfg.steps; ++s) {
-            const uint32_t tseed = nrc->cfg.seed ^ (0x9E3779B9u * (uint32_t)(s + 1));
-            if (count)
-                hipLaunchKernelGGL(k_nrc_targets<true>, grid, block, 0, stream, w->dw, *p,
-                                   (uint32_t)nrc->tcap, tseed, pr->queries, pr->ctr + 1,
-                                   pr->bounce, pr->palette, nrc->tq, nrc->tt, pr->ctr + 2 + s, thr,
-                                   cd);
-            else
-                hipLaunchKernelGGL(k_nrc_targets<false>, grid, block, 0, stream, w->dw, *p,
-                                   (uint32_t)nrc->tcap, tseed, pr->queries, pr->ctr + 1,
-                                   pr->bounce, pr->palette, nrc->tq, nrc->tt, pr->ctr + 2 + s, thr,
-                                   cd);
-            HIP_TRY(hipGetLastError());
+            if (s > 0) { // step 0's targets came with the path launch
+                const uint32_t tseed = nrc->cfg.seed ^ (0x9E3779B9u * (uint32_t)(s + 1));
+                if (count)
+                    hipLaunchKernelGGL(k_nrc_targets<true>, grid, block, 0, stream, w->dw, *p,
+                                       (uint32_t)nrc->tcap, tseed, pr->queries, (uint32_t)pr->ns,
+                                       pr->ctr, pr->bounce, pr->palette, nrc->tq, nrc->tt,
+                                       pr->ctr + 4 + (s - 1), thr, cd);
+                else
+                    hipLaunchKernelGGL(k_nrc_targets<false>, grid, block, 0, stream, w->dw, *p,
+                                       (uint32_t)nrc->tcap, tseed, pr->queries, (uint32_t)pr->ns,
+                                       pr->ctr, pr->bounce, pr->palette, nrc->tq, nrc->tt,
+                                       pr->ctr + 4 + (s - 1), thr, cd);
+                HIP_TRY(hipGetLastError());
+            }
             st = nrc_train_launch(nrc, nrc->tq, nrc->tt, (uint32_t)nrc->tcap, true, stream);
             if (st) return st;
         }
@@ -858,6 +995,8 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
     vg_probes *pr = new vg_probes();
     std::memset(pr, 0, sizeof(*pr));
     pr->world = world;
+    pr->device = world->ctx->device;
+    pr->ns = (size_t)total * p->rays_per_probe;
     for (int k = 0; k < 3; ++k) pr->grid[k] = p->grid[k];
     pr->rays = p->rays_per_probe;
     pr->epochs = p->epochs;
@@ -871,7 +1010,7 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
               hipMalloc((void **)&pr->pwork, (size_t)total * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->rad, ns * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->face, ns) == hipSuccess &&
-              hipMalloc((void **)&pr->queries, ns * sizeof(vg_nrc_query)) == hipSuccess &&
+              hipMalloc((void **)&pr->queries, 2 * ns * sizeof(vg_nrc_query)) == hipSuccess &&
               hipMalloc((void **)&pr->ctr, kGiCtrs * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void **)&pr->counts, kGiCounters * sizeof(unsigned long long)) ==
                   hipSuccess;
@@ -892,6 +1031,7 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
     if (e == hipSuccess)
         e = hipMemcpy(pr->palette, pal.data(), pal.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(pr->probes, 0, (size_t)total * sizeof(vg_probe));
+    if (e == hipSuccess) e = hipMemset(pr->ctr, 0, kGiCtrs * sizeof(uint32_t));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_probes(pr);
@@ -904,7 +1044,7 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
 
 int vg_probes_destroy(vg_probes *pr) {
     if (!pr) return VG_OK;
-    (void)set_device(pr->world);
+    (void)hipSetDevice(pr->device);
     (void)hipDeviceSynchronize();
     free_probes(pr);
     delete pr;

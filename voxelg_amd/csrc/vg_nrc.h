@@ -6,6 +6,7 @@
 
 struct vg_nrc {
     vg_ctx *ctx;
+    int device;             // ctx's device (destroy does not touch ctx)
     vg_nrc_config cfg;
     int64_t n_entries;      // levels * 2^log2_table
     int64_t n_grid;         // grid parameters = 2 * n_entries
@@ -14,13 +15,17 @@ struct vg_nrc {
     int partial_blocks;     // blocks the partial buffer holds
     uint32_t *frag_fwd;     // bf16 MFMA A-operand fragments of W (4 layers x 4 x 2 x 64 lanes x 16 B)
     uint32_t *frag_bwd;     // ... of W^T for the backward pass
-    float *grid, *grid_m, *grid_v, *grid_g;  // fp32 master, moments, gradient (kept zero between steps)
+    float *grid, *grid_m, *grid_v, *grid_g;  // fp32 master, moments, coarse-level gradient (zero between steps)
+    uint32_t *grid_gh;      // fine-level gradient, bf16 pair per entry (zero between steps)
     uint32_t *grid_h;       // bf16 feature pairs the kernels read, one u32 per entry
     uint32_t *step;         // optimizer steps taken (device)
     float *loss;            // loss of the last step (device)
     vg_nrc_query *tq;       // the learned probe update's last training batch
     float4 *tt;             // ... and its targets
     int32_t tcap;           // entries of tq / tt
+    uint32_t *feat;         // training batch encoding: bf16 pair per (sample, level)
+    float *dfeat;           // ... its gradient: 32 fp32 per sample
+    int32_t fcap;           // samples feat / dfeat hold
 };
 
 namespace vg {
@@ -29,12 +34,15 @@ constexpr int kNrcWidth = 64;
 constexpr int kNrcFrags = 32;       // [layer 4][block 4][k-step 2]
 constexpr int kNrcTile = 256;       // training samples per block per tile
 constexpr int kNrcPartial = VG_NRC_MLP_PARAMS + 16; // floats per block partial (loss at +0)
+constexpr int kNrcCoarse = 6;       // levels below this accumulate fp32 gradients
 
-// Inference over queries (n_dev: device count read by the kernel, else n_max). out4
-// receives raw predictions; with rad/pal set (probe epilogue) the kernel instead adds
-// albedo * max(0, prediction) to rad[aux] (aux = sample id, face >> 8 = material).
-int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, const uint32_t *n_dev, uint32_t n_max,
-                     float4 *out4, float4 *rad, const float4 *pal, hipStream_t stream);
+// Inference over the queries at q + sel[0] * q_stride (sel may be null: q itself), n_dev[0]
+// of them (else n_max; never more than n_max). out4 receives raw predictions; with rad/pal
+// set (probe epilogue) the kernel instead adds albedo * max(0, prediction) to rad[aux]
+// (aux = sample id, face >> 8 = material).
+int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const uint32_t *sel,
+                     const uint32_t *n_dev, uint32_t n_max, float4 *out4, float4 *rad,
+                     const float4 *pal, hipStream_t stream);
 // Loss gradient of n queries into the partial buffer and the grid gradient, reduced to
 // mlp_g; with update, the optimizer step follows (grid gradient cleared by it).
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,

@@ -24,6 +24,7 @@ namespace vg {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
 struct NrcDev {
     const uint32_t *grid_h;
@@ -98,16 +99,9 @@ __device__ __forceinline__ void level_feat(const NrcDev &m, int l, const float p
     }
 }
 
-// Input k-steps of lane group g for one sample (phi order over the logical inputs).
-__device__ __forceinline__ void build_input(const NrcDev &m, int g, const float p[3],
-                                            const float d[3], uint32_t face, u32x4_t &x0,
-                                            u32x4_t &x1) {
-    float a0, a1, b0, b1, c0, c1, e0, e1;
-    level_feat(m, 2 * g, p, a0, a1);
-    level_feat(m, 2 * g + 1, p, b0, b1);
-    level_feat(m, 8 + 2 * g, p, c0, c1);
-    level_feat(m, 9 + 2 * g, p, e0, e1);
-    x0 = u32x4_t{pack2(a0, a1), pack2(b0, b1), pack2(c0, c1), pack2(e0, e1)};
+// Second input k-step of lane group g: inputs 32 + 4g + j (face one-hot, direction, the
+// constant 1; elements 4..7 are inputs >= 48, all zero).
+__device__ __forceinline__ u32x4_t input_extras(int g, const float d[3], uint32_t face) {
     float v[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -118,7 +112,21 @@ __device__ __forceinline__ void build_input(const NrcDev &m, int g, const float 
         else if (k == 9) x = 1.0f;
         v[j] = x;
     }
-    x1 = u32x4_t{pack2(v[0], v[1]), pack2(v[2], v[3]), 0u, 0u};
+    return u32x4_t{pack2(v[0], v[1]), pack2(v[2], v[3]), 0u, 0u};
+}
+
+// Input k-steps of lane group g for one sample (phi order over the logical inputs): the
+// first holds hash levels 2g, 2g+1, 8+2g, 9+2g as bf16 feature pairs.
+__device__ __forceinline__ void build_input(const NrcDev &m, int g, const float p[3],
+                                            const float d[3], uint32_t face, u32x4_t &x0,
+                                            u32x4_t &x1) {
+    float a0, a1, b0, b1, c0, c1, e0, e1;
+    level_feat(m, 2 * g, p, a0, a1);
+    level_feat(m, 2 * g + 1, p, b0, b1);
+    level_feat(m, 8 + 2 * g, p, c0, c1);
+    level_feat(m, 9 + 2 * g, p, e0, e1);
+    x0 = u32x4_t{pack2(a0, a1), pack2(b0, b1), pack2(c0, c1), pack2(e0, e1)};
+    x1 = input_extras(g, d, face);
 }
 
 __device__ __forceinline__ void load_query(const vg_nrc_query *q, uint32_t s, bool valid,
@@ -176,12 +184,14 @@ __device__ __forceinline__ void relu_pack(const f32x4_t acc[4][4], u32x4_t x[2][
 
 // ---------------------------------------------------------------- inference --
 template <bool PROBE>
-__global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query *__restrict__ q,
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query *__restrict__ qb,
+                                                   size_t q_stride, const uint32_t *sel,
                                                    const uint32_t *n_dev, uint32_t n_max,
                                                    float4 *__restrict__ out,
                                                    float4 *__restrict__ rad,
                                                    const float4 *__restrict__ pal) {
     __shared__ u32x4_t sf[kNrcFrags * 64];
+    const vg_nrc_query *q = qb + (sel ? (size_t)sel[0] * q_stride : (size_t)0);
     load_frags(sf, m.frag_fwd);
     __syncthreads();
     uint32_t n = n_dev ? *n_dev : n_max;
@@ -317,10 +327,99 @@ __device__ __forceinline__ void back_layer(const u32x4_t *sbf, int lyr, int nks,
         }
 }
 
+// Encoding of a training batch: one thread per (sample, level) writes the level's bf16
+// feature pair, feat[s * 16 + l] (zero for levels beyond the config).
+__global__ __launch_bounds__(256) void k_nrc_encode(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                    uint32_t n, uint32_t *__restrict__ feat) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint32_t s = (uint32_t)(t >> 4), l = (uint32_t)(t & 15u);
+    if (s >= n) return;
+    float f0 = 0.0f, f1 = 0.0f;
+    if ((int)l < m.levels) {
+        const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
+        const float p[3] = {a.x, a.y, a.z};
+        level_feat(m, (int)l, p, f0, f1);
+    }
+    feat[t] = pack2(f0, f1);
+}
+
+// Grid gradient of one level (blockIdx.y): one thread per sample spreads d loss / d feature
+// (dfeat, fp32) over the level's 8 corners with their trilinear weights.
+//   coarse levels (< kNrcCoarse): many samples share a corner; a wave first sums the
+//     contributions of its lanes per distinct corner entry (training batches come in
+//     query-list order, which follows probe order, so neighbouring lanes share corners),
+//     then one fp32 atomic per entry and feature;
+//   fine levels: corners are rarely shared; one packed bf16x2 atomic per corner
+//     (global_atomic_pk_add_bf16) into grid_gh.
+__global__ __launch_bounds__(256) void k_nrc_scatter(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                     const float2 *__restrict__ dfeat, uint32_t n,
+                                                     float *__restrict__ grid_g,
+                                                     uint32_t *__restrict__ grid_gh) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const int l = (int)blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    float2 d = make_float2(0.0f, 0.0f);
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    if (valid) {
+        d = dfeat[(size_t)s * 16u + (uint32_t)l];
+        const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
+        p[0] = a.x;
+        p[1] = a.y;
+        p[2] = a.z;
+    }
+    const bool live = valid && (d.x != 0.0f || d.y != 0.0f);
+    if (l < kNrcCoarse) {
+        if (__ballot(live) == 0) return; // wave-uniform
+        uint32_t idx[8];
+        float w[8];
+        level_corners(m, l, p, idx, w);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            bool pend = live;
+            const float v0 = w[k] * d.x, v1 = w[k] * d.y;
+            uint64_t todo = __ballot(pend);
+            while (todo) {
+                const int leader = __ffsll((long long)todo) - 1;
+                const uint32_t key = __shfl(idx[k], leader, 64);
+                const bool mine = pend && idx[k] == key;
+                const uint64_t same = __ballot(mine);
+                float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    s0 += __shfl_xor(s0, off, 64);
+                    s1 += __shfl_xor(s1, off, 64);
+                }
+                if (lane == leader) {
+                    unsafeAtomicAdd(grid_g + 2u * key, s0);
+                    unsafeAtomicAdd(grid_g + 2u * key + 1u, s1);
+                }
+                pend = pend && !mine;
+                todo &= ~same;
+            }
+        }
+    } else {
+        if (!live) return;
+        uint32_t idx[8];
+        float w[8];
+        level_corners(m, l, p, idx, w);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t bits = pack2(w[k] * d.x, w[k] * d.y);
+            __builtin_amdgcn_global_atomic_fadd_v2bf16(
+                (__attribute__((address_space(1))) bf16x2_t *)(grid_gh + idx[k]),
+                __builtin_bit_cast(bf16x2_t, bits));
+        }
+    }
+}
+
+// Forward and backward of a training batch from its encoding (feat): weight-gradient sums
+// per block into `partial`, the encoding gradient per sample into dfeat[s * 32 + input].
 __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                   const uint32_t *__restrict__ feat,
                                                    const float4 *__restrict__ tgt, uint32_t n,
                                                    float inv_n, float *__restrict__ partial,
-                                                   float *__restrict__ grid_g) {
+                                                   float *__restrict__ dfeat) {
     __shared__ u32x4_t sf[kNrcFrags * 64];
     __shared__ u32x4_t sbw[kNrcFrags * 64];
     __shared__ __align__(16) uint16_t sX[kNrcWidth * kSRow];
@@ -336,22 +435,27 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query 
     bool first = true;
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * kNrcTile < n; tile += gridDim.x) {
         u32x4_t X[4][2][4];
-        float pos[4][3], tw[4], tr[4][3];
-        bool live[4];
+        float tw[4], tr[4][3];
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
             const uint32_t s = tile * kNrcTile + (uint32_t)(wave * 64 + sb * 16 + c);
             const bool valid = s < n;
-            float d[3];
+            float pos[3], d[3];
             uint32_t face, aux;
-            load_query(q, s, valid, pos[sb], d, face, aux);
+            load_query(q, s, valid, pos, d, face, aux);
             const float4 t = valid ? tgt[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             tr[sb][0] = t.x;
             tr[sb][1] = t.y;
             tr[sb][2] = t.z;
             tw[sb] = t.w;
-            live[sb] = valid && t.w != 0.0f;
-            build_input(m, g, pos[sb], d, face & 0xFFu, X[0][0][sb], X[0][1][sb]);
+            if (valid) {
+                const uint2 lo = *reinterpret_cast<const uint2 *>(feat + (size_t)s * 16u + 2u * g);
+                const uint2 hi = *reinterpret_cast<const uint2 *>(feat + (size_t)s * 16u + 8u + 2u * g);
+                X[0][0][sb] = u32x4_t{lo.x, lo.y, hi.x, hi.y};
+            } else {
+                X[0][0][sb] = u32x4_t{0u, 0u, 0u, 0u};
+            }
+            X[0][1][sb] = input_extras(g, d, face & 0xFFu);
         }
         f32x4_t acc[4][4];
 #pragma unroll
@@ -400,23 +504,11 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query 
         }
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
-            if (!live[sb]) continue;
-#pragma unroll
-            for (int lv = 0; lv < 4; ++lv) {
-                const int l = lv < 2 ? 2 * g + lv : 8 + 2 * g + (lv - 2);
-                if (l >= m.levels) continue;
-                const float d0 = ga[lv >> 1][sb][(lv & 1) * 2];
-                const float d1 = ga[lv >> 1][sb][(lv & 1) * 2 + 1];
-                if (d0 == 0.0f && d1 == 0.0f) continue;
-                uint32_t idx[8];
-                float w[8];
-                level_corners(m, l, pos[sb], idx, w);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    unsafeAtomicAdd(grid_g + 2u * idx[k], w[k] * d0);
-                    unsafeAtomicAdd(grid_g + 2u * idx[k] + 1u, w[k] * d1);
-                }
-            }
+            const uint32_t s = tile * kNrcTile + (uint32_t)(wave * 64 + sb * 16 + c);
+            if (s >= n) continue;
+            float4 *dst = reinterpret_cast<float4 *>(dfeat + (size_t)s * 32u);
+            dst[g] = make_float4(ga[0][sb][0], ga[0][sb][1], ga[0][sb][2], ga[0][sb][3]);
+            dst[4 + g] = make_float4(ga[1][sb][0], ga[1][sb][1], ga[1][sb][2], ga[1][sb][3]);
         }
         first = false;
     }
@@ -429,16 +521,27 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query 
 }
 
 // Sum of the block partials: the MLP gradient and the loss; with `update`, one more step.
-__global__ void k_nrc_reduce(const float *__restrict__ partial, int nblocks, float *mlp_g,
-                             float *loss, uint32_t *step, int update) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > VG_NRC_MLP_PARAMS) return;
-    float s = 0.0f;
-    for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kNrcPartial + p];
+// A block of 1024 threads owns 64 consecutive entries; its 16 thread rows sum every 16th
+// block partial, then the rows are added in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void k_nrc_reduce(const float *__restrict__ partial, int nblocks,
+                                                     float *mlp_g, float *loss, uint32_t *step,
+                                                     int update) {
+    __shared__ float sr[16][64];
+    const int pl = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + pl;
+    float acc = 0.0f;
+    if (p <= VG_NRC_MLP_PARAMS)
+        for (int b = row; b < nblocks; b += 16) acc += partial[(size_t)b * kNrcPartial + p];
+    sr[row][pl] = acc;
+    __syncthreads();
+    if (row != 0 || p > VG_NRC_MLP_PARAMS) return;
+    float sum = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) sum += sr[r][pl];
     if (p < VG_NRC_MLP_PARAMS) {
-        mlp_g[p] = s;
+        mlp_g[p] = sum;
     } else {
-        *loss = s;
+        *loss = sum;
         if (update) *step += 1u;
     }
 }
@@ -450,16 +553,28 @@ __device__ __forceinline__ void adam1(float &p, float &m, float &v, float g, flo
     p -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
-// Sparse Adam over the grid: entries with a zero gradient are left alone; the gradient is
-// cleared for the next step and the bf16 copy the kernels read is refreshed.
+// Sparse Adam over the grid: entries with a zero gradient are left alone; the gradient
+// (fp32 pair for coarse levels, bf16 pair for fine levels) is cleared for the next step and
+// the bf16 copy the kernels read is refreshed.
 __global__ void k_nrc_adam_grid(float2 *__restrict__ p, float2 *__restrict__ mm,
                                 float2 *__restrict__ vv, float2 *__restrict__ gg,
-                                uint32_t *__restrict__ ph, int64_t n_entries, float lr, float b1,
+                                uint32_t *__restrict__ gh, uint32_t *__restrict__ ph,
+                                int64_t n_entries, int64_t coarse_entries, float lr, float b1,
                                 float b2, float eps, const uint32_t *step) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n_entries) return;
-    const float2 g = gg[e];
-    if (g.x == 0.0f && g.y == 0.0f) return;
+    float2 g;
+    if (e < coarse_entries) {
+        g = gg[e];
+        if (g.x == 0.0f && g.y == 0.0f) return;
+        gg[e] = make_float2(0.0f, 0.0f);
+    } else {
+        const uint32_t h = gh[e];
+        if (h == 0u) return;
+        g = make_float2(bf_lo(h), bf_hi(h));
+        gh[e] = 0u;
+        if (g.x == 0.0f && g.y == 0.0f) return;
+    }
     const float t = (float)*step;
     const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
     float2 pp = p[e], m = mm[e], v = vv[e];
@@ -468,7 +583,6 @@ __global__ void k_nrc_adam_grid(float2 *__restrict__ p, float2 *__restrict__ mm,
     p[e] = pp;
     mm[e] = m;
     vv[e] = v;
-    gg[e] = make_float2(0.0f, 0.0f);
     ph[e] = pack2(pp.x, pp.y);
 }
 
@@ -532,13 +646,19 @@ static NrcDev dev_view(const vg_nrc *m) {
     return d;
 }
 
+static int64_t coarse_entries(const vg_nrc *m) {
+    const int lc = m->cfg.levels < kNrcCoarse ? m->cfg.levels : kNrcCoarse;
+    return (int64_t)lc << m->cfg.log2_table;
+}
+
 static int set_dev(const vg_nrc *m) {
-    HIP_TRY(hipSetDevice(m->ctx->device));
+    HIP_TRY(hipSetDevice(m->device));
     return VG_OK;
 }
 
-int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, const uint32_t *n_dev, uint32_t n_max,
-                     float4 *out4, float4 *rad, const float4 *pal, hipStream_t stream) {
+int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const uint32_t *sel,
+                     const uint32_t *n_dev, uint32_t n_max, float4 *out4, float4 *rad,
+                     const float4 *pal, hipStream_t stream) {
     if (n_max == 0) return VG_OK;
     const uint64_t tiles = ((uint64_t)n_max + 63) / 64;
     uint64_t blocks = (tiles + 3) / 4;
@@ -546,10 +666,10 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, const uint32_t *n_dev, ui
     if (blocks > cap) blocks = cap;
     if (rad)
         hipLaunchKernelGGL(k_nrc_infer<true>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                           dev_view(m), q, n_dev, n_max, out4, rad, pal);
+                           dev_view(m), q, q_stride, sel, n_dev, n_max, out4, rad, pal);
     else
         hipLaunchKernelGGL(k_nrc_infer<false>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                           dev_view(m), q, n_dev, n_max, out4, rad, pal);
+                           dev_view(m), q, q_stride, sel, n_dev, n_max, out4, rad, pal);
     HIP_TRY(hipGetLastError());
     return VG_OK;
 }
@@ -557,12 +677,22 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, const uint32_t *n_dev, ui
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
                      hipStream_t stream) {
     if (n == 0) return VG_OK;
+    if (n > (uint32_t)m->fcap)
+        return fail(VG_E_INVALID, "batch of %u above the training buffers' %d", n, m->fcap);
+    const NrcDev dv = dev_view(m);
+    const uint32_t eblocks = (uint32_t)(((uint64_t)n * 16u + 255u) / 256u);
+    hipLaunchKernelGGL(k_nrc_encode, dim3(eblocks), dim3(256), 0, stream, dv, q, n, m->feat);
+    HIP_TRY(hipGetLastError());
     const uint64_t tiles = ((uint64_t)n + kNrcTile - 1) / kNrcTile;
     const int blocks = (int)(tiles < (uint64_t)m->partial_blocks ? tiles : m->partial_blocks);
-    hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, stream, dev_view(m), q, t, n,
-                       1.0f / (float)n, m->partial, m->grid_g);
+    hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, stream, dv, q, m->feat, t, n,
+                       1.0f / (float)n, m->partial, m->dfeat);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_nrc_reduce, dim3((VG_NRC_MLP_PARAMS + 1 + 255) / 256), dim3(256), 0,
+    hipLaunchKernelGGL(k_nrc_scatter, dim3((n + 255) / 256, (uint32_t)m->cfg.levels), dim3(256),
+                       0, stream, dv, q, reinterpret_cast<const float2 *>(m->dfeat), n, m->grid_g,
+                       m->grid_gh);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_nrc_reduce, dim3((VG_NRC_MLP_PARAMS + 1 + 63) / 64), dim3(1024), 0,
                        stream, m->partial, blocks, m->mlp_g, m->loss, m->step, update ? 1 : 0);
     HIP_TRY(hipGetLastError());
     if (update) {
@@ -571,8 +701,9 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
                            0, stream, reinterpret_cast<float2 *>(m->grid),
                            reinterpret_cast<float2 *>(m->grid_m),
                            reinterpret_cast<float2 *>(m->grid_v),
-                           reinterpret_cast<float2 *>(m->grid_g), m->grid_h, m->n_entries,
-                           c.lr_grid, c.beta1, c.beta2, c.eps, m->step);
+                           reinterpret_cast<float2 *>(m->grid_g), m->grid_gh, m->grid_h,
+                           m->n_entries, coarse_entries(m), c.lr_grid, c.beta1, c.beta2, c.eps,
+                           m->step);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(1), dim3(1024), 0, stream, m->mlp, m->mlp_m,
                            m->mlp_v, m->mlp_g, reinterpret_cast<uint16_t *>(m->frag_fwd),
@@ -621,7 +752,8 @@ static int check_config(const vg_nrc_config *c) {
 static void free_nrc(vg_nrc *m) {
     void *ptrs[] = {m->mlp,    m->mlp_m,  m->mlp_v,  m->mlp_g,  m->partial, m->frag_fwd,
                     m->frag_bwd, m->grid, m->grid_m, m->grid_v, m->grid_g,  m->grid_h,
-                    m->step,   m->loss,   m->tq,     m->tt};
+                    m->step,   m->loss,   m->tq,     m->tt,     m->feat,    m->dfeat,
+                    m->grid_gh};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -637,6 +769,7 @@ static int load_params(vg_nrc *m, const float *mlp, const float *grid) {
     HIP_TRY(hipMemset(m->grid_m, 0, gb));
     HIP_TRY(hipMemset(m->grid_v, 0, gb));
     HIP_TRY(hipMemset(m->grid_g, 0, gb));
+    HIP_TRY(hipMemset(m->grid_gh, 0, (size_t)m->n_entries * sizeof(uint32_t)));
     HIP_TRY(hipMemset(m->step, 0, sizeof(uint32_t)));
     HIP_TRY(hipMemset(m->loss, 0, sizeof(float)));
     hipLaunchKernelGGL(k_nrc_grid_pack, dim3((uint32_t)((m->n_entries + 255) / 256)), dim3(256), 0,
@@ -650,11 +783,27 @@ static int load_params(vg_nrc *m, const float *mlp, const float *grid) {
     return VG_OK;
 }
 
-static int check_batch(const vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n) {
+// Validates a host-called batch and grows the training buffers to hold it (synchronous
+// reallocation: a batch above the configured size must not be enqueued during capture).
+static int check_batch(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n) {
     if (!m) return fail(VG_E_INVALID, "null cache");
     if (n > VG_NRC_MAX_BATCH) return fail(VG_E_INVALID, "n %u above %d", n, VG_NRC_MAX_BATCH);
     if (n && (!q || !t)) return fail(VG_E_INVALID, "null queries or targets");
-    return set_dev(m);
+    int st = set_dev(m);
+    if (st) return st;
+    if (n > (uint32_t)m->fcap) {
+        HIP_TRY(hipDeviceSynchronize());
+        (void)hipFree(m->feat);
+        (void)hipFree(m->dfeat);
+        m->feat = nullptr;
+        m->dfeat = nullptr;
+        m->fcap = 0;
+        if (hipMalloc((void **)&m->feat, (size_t)n * 16 * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc((void **)&m->dfeat, (size_t)n * 32 * sizeof(float)) != hipSuccess)
+            return fail(VG_E_NOMEM, "training buffers for %u queries", n);
+        m->fcap = (int32_t)n;
+    }
+    return VG_OK;
 }
 
 } // namespace vg
@@ -699,11 +848,13 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
     vg_nrc *m = new vg_nrc();
     std::memset(m, 0, sizeof(*m));
     m->ctx = ctx;
+    m->device = ctx->device;
     m->cfg = *cfg;
     m->n_entries = (int64_t)cfg->levels << cfg->log2_table;
     m->n_grid = 2 * m->n_entries;
     m->partial_blocks = 1024;
     m->tcap = cfg->batch;
+    m->fcap = cfg->batch > 65536 ? cfg->batch : 65536;
     const size_t mb = VG_NRC_MLP_PARAMS * sizeof(float), gb = (size_t)m->n_grid * sizeof(float);
     const size_t fb = (size_t)kNrcFrags * 64 * 16;
     bool ok = hipMalloc((void **)&m->mlp, mb) == hipSuccess &&
@@ -719,10 +870,13 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
               hipMalloc((void **)&m->grid_v, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_g, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_h, (size_t)m->n_entries * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->grid_gh, (size_t)m->n_entries * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void **)&m->step, 256) == hipSuccess &&
               hipMalloc((void **)&m->loss, 256) == hipSuccess &&
               hipMalloc((void **)&m->tq, (size_t)m->tcap * sizeof(vg_nrc_query)) == hipSuccess &&
-              hipMalloc((void **)&m->tt, (size_t)m->tcap * sizeof(float4)) == hipSuccess;
+              hipMalloc((void **)&m->tt, (size_t)m->tcap * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&m->feat, (size_t)m->fcap * 16 * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->dfeat, (size_t)m->fcap * 32 * sizeof(float)) == hipSuccess;
     if (!ok) {
         free_nrc(m);
         delete m;
@@ -749,7 +903,7 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
 
 int vg_nrc_destroy(vg_nrc *m) {
     if (!m) return VG_OK;
-    (void)hipSetDevice(m->ctx->device);
+    (void)hipSetDevice(m->device);
     (void)hipDeviceSynchronize();
     free_nrc(m);
     delete m;
@@ -787,8 +941,8 @@ int vg_nrc_infer(vg_nrc *m, const vg_nrc_query *q, uint32_t n, float *out4, void
     if (n && (!q || !out4)) return fail(VG_E_INVALID, "null queries or output");
     int st = set_dev(m);
     if (st) return st;
-    return nrc_infer_launch(m, q, nullptr, n, reinterpret_cast<float4 *>(out4), nullptr, nullptr,
-                            (hipStream_t)stream);
+    return nrc_infer_launch(m, q, 0, nullptr, nullptr, n, reinterpret_cast<float4 *>(out4),
+                            nullptr, nullptr, (hipStream_t)stream);
 }
 
 int vg_nrc_train_step(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n, void *stream) {
@@ -810,12 +964,24 @@ int vg_nrc_gradients(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t 
     if (mlp_grad)
         HIP_TRY(hipMemcpy(mlp_grad, m->mlp_g, VG_NRC_MLP_PARAMS * sizeof(float),
                           hipMemcpyDeviceToHost));
-    if (grid_grad)
-        HIP_TRY(hipMemcpy(grid_grad, m->grid_g, (size_t)m->n_grid * sizeof(float),
+    if (grid_grad) {
+        const int64_t ce = coarse_entries(m);
+        HIP_TRY(hipMemcpy(grid_grad, m->grid_g, (size_t)ce * 2 * sizeof(float),
                           hipMemcpyDeviceToHost));
+        std::vector<uint32_t> gh((size_t)(m->n_entries - ce));
+        if (!gh.empty())
+            HIP_TRY(hipMemcpy(gh.data(), m->grid_gh + ce, gh.size() * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < gh.size(); ++i) {
+            const uint32_t lo = gh[i] << 16, hi = gh[i] & 0xFFFF0000u;
+            std::memcpy(grid_grad + 2 * (ce + (int64_t)i), &lo, 4);
+            std::memcpy(grid_grad + 2 * (ce + (int64_t)i) + 1, &hi, 4);
+        }
+    }
     if (loss) HIP_TRY(hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
     // the optimizer expects a clear grid gradient
     HIP_TRY(hipMemset(m->grid_g, 0, (size_t)m->n_grid * sizeof(float)));
+    HIP_TRY(hipMemset(m->grid_gh, 0, (size_t)m->n_entries * sizeof(uint32_t)));
     HIP_TRY(hipDeviceSynchronize());
     return VG_OK;
 }
