@@ -321,11 +321,18 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         stream.synchronize()
         return float(np.mean([a.elapsed_ms(b) for a, b in ev]))
 
+    from voxelg_amd import _lib as L
+
+    def sched():
+        return {"waves_per_cu": ctx.get_option(L.VG_OPT_PROBE_WAVES_PER_CU),
+                "refill_below": ctx.get_option(L.VG_OPT_PROBE_REFILL)}
+
     g256 = gi.ProbeGrid(dw, p256)
     rays256, it256 = g256.update_counts(p256)
-    g256.update(p256, stream)
+    g256.autotune(p256)
+    sched_plain = sched()
     plain_ms = timed(lambda p: g256.update(p, stream), p256, steps)
-    log(f"gi plain: {plain_ms:.3f} ms/update, rays {rays256}")
+    log(f"gi plain: {plain_ms:.3f} ms/update ({sched_plain}), rays {rays256}")
 
     cache = nrc.RadianceCache(ctx)
     g32 = gi.ProbeGrid(dw, p32)
@@ -336,8 +343,14 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     stream.synchronize()
     log(f"gi learned: {train_updates} training updates in {time.perf_counter() - t0:.2f} s, "
         f"loss {cache.stats()[1]:.5f}")
+    g32.autotune(p32, cache, train=True)  # (its repetitions train the cache further)
+    sched_learned = sched()
     learned_ms = timed(lambda p: g32.update_learned(cache, p, train=True, stream=stream), p32,
                        steps)
+    frozen_ms = timed(lambda p: g32.update_learned(cache, p, train=False, stream=stream), p32,
+                      steps)
+    log(f"gi learned: {learned_ms:.3f} ms/update with training, {frozen_ms:.3f} without "
+        f"({sched_learned})")
     p32["round"] = 5000
     rays32, nq = g32.update_learned_counts(cache, p32, train=True)
     log(f"gi learned: {learned_ms:.3f} ms/update, rays {rays32}, cache queries {nq}")
@@ -378,11 +391,14 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         "probes": int(total), "live_probes": int(live.sum()), "grid": list(grid), "spacing": 32,
         "plain_ms": round(plain_ms, 4), "learned_ms": round(learned_ms, 4),
         "speedup": round(plain_ms / learned_ms, 3),
+        "learned_inference_only_ms": round(frozen_ms, 4),
+        "speedup_inference_only": round(plain_ms / frozen_ms, 3),
+        "schedule_plain": sched_plain, "schedule_learned": sched_learned,
         "err_plain256": round(err(rec_plain), 5), "err_learned32": round(err(rec_learned), 5),
         "err_plain32_2bounce": round(err(rec_plain32), 5),
         "err_ambient_tail32": round(err(rec_ambient), 5),
         "rays_plain": rays256, "rays_learned": rays32, "cache_queries": nq,
-        "train_updates": train_updates + steps + 1, "train_batch": cache.batch,
+        "train_steps": cache.stats()[0], "train_batch": cache.batch,
         "reference": f"FP64 4096 rays/probe 2-bounce, oracle on {threads} host threads, "
                      f"{ref_s:.1f} s",
     }

@@ -250,7 +250,9 @@ enum {
     VG_OPT_FRAME_REFILL = 3,
     VG_OPT_RAYS_WAVES_PER_CU = 4,
     VG_OPT_RAYS_REFILL = 5,
-    VG_OPT_CU_COUNT = 6 /* read only */
+    VG_OPT_CU_COUNT = 6, /* read only */
+    VG_OPT_PROBE_WAVES_PER_CU = 7, /* probe path kernels (vg_probes_update*) */
+    VG_OPT_PROBE_REFILL = 8
 };
 int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
 int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);
@@ -459,6 +461,11 @@ int vg_nrc_last_batch(vg_nrc *nrc, vg_nrc_query *queries_host, float *target4_ho
 #define VG_NRC_TRAIN 1u
 int vg_probes_update_learned(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                              uint32_t flags, void *stream);
+/* Times every (waves per CU, refill) pair of the probe path kernels on this very update
+ * (repeated; nrc may be NULL for the plain update, with VG_NRC_TRAIN the cache trains on
+ * every repetition) and keeps the fastest in the context (VG_OPT_PROBE_*). */
+int vg_autotune_probes(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
+                       uint32_t flags, float *best_ms);
 /* The same, reporting rays traced (gather, shadow, training bounce) and the cache queries
  * (compacted gather hits). Synchronous. */
 int vg_probes_update_learned_counts(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,

@@ -64,6 +64,8 @@ VG_OPT_FRAME_REFILL = 3
 VG_OPT_RAYS_WAVES_PER_CU = 4
 VG_OPT_RAYS_REFILL = 5
 VG_OPT_CU_COUNT = 6
+VG_OPT_PROBE_WAVES_PER_CU = 7
+VG_OPT_PROBE_REFILL = 8
 VG_RAY_LOD = 1
 VG_RAY_SKIP_TRANSPARENT = 2
 VG_RAY_ANY_HIT = 4
@@ -183,6 +185,7 @@ SIGNATURES = {
     "vg_nrc_last_batch": (C.c_int, [_P, _P, _P]),
     "vg_probes_update_learned": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
     "vg_probes_update_learned_counts": (C.c_int, [_P, _P, _P, C.c_uint32, _U64P, _U64P]),
+    "vg_autotune_probes": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
 }
 
 

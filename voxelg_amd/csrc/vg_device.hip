@@ -728,6 +728,8 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->frame_refill = 32;
     c->rays_wpc = 16;
     c->rays_refill = 32;
+    c->probe_wpc = 8;
+    c->probe_refill = 8;
     *out = c;
     return VG_OK;
 }
@@ -1072,56 +1074,6 @@ namespace vg {
 // Time `launch` (already bound to its arguments) over every (waves per CU, refill) pair and
 // keep the fastest in the context. Each candidate runs once to warm up, then `reps` times
 // between events on one stream; its score is the fastest repetition.
-template <class L>
-static int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms) {
-    static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
-    static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
-    const int reps = 3;
-    hipStream_t stream;
-    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    const int old_schedule = c->schedule;
-    c->schedule = VG_SCHEDULE_PERSISTENT;
-    float best = INFINITY;
-    int bw = *wpc_opt, br = *refill_opt, st = VG_OK;
-    for (int wpc : kWpc) {
-        for (int rf : kRefill) {
-            *wpc_opt = wpc;
-            *refill_opt = rf;
-            st = launch(stream);
-            if (st) break;
-            float ms_min = INFINITY;
-            for (int i = 0; i < reps && st == VG_OK; ++i) {
-                float ms = 0.0f;
-                if (hipEventRecord(e0, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
-                if (!st) st = launch(stream);
-                if (!st && hipEventRecord(e1, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
-                if (!st && hipEventSynchronize(e1) != hipSuccess) st = fail(VG_E_DEVICE, "sync");
-                if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = fail(VG_E_DEVICE, "elapsed");
-                ms_min = ms < ms_min ? ms : ms_min;
-            }
-            if (st) break;
-            if (ms_min < best) {
-                best = ms_min;
-                bw = wpc;
-                br = rf;
-            }
-        }
-        if (st) break;
-    }
-    *wpc_opt = bw;
-    *refill_opt = br;
-    c->schedule = old_schedule;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(stream);
-    if (st) return st;
-    if (best_ms) *best_ms = best;
-    return VG_OK;
-}
-
 } // namespace vg
 
 extern "C" {
@@ -1136,13 +1088,19 @@ int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
         return VG_OK;
     case VG_OPT_FRAME_WAVES_PER_CU:
     case VG_OPT_RAYS_WAVES_PER_CU:
+    case VG_OPT_PROBE_WAVES_PER_CU:
         if (value < 1 || value > 64) return fail(VG_E_INVALID, "waves per CU %d not in 1..64", value);
-        (option == VG_OPT_FRAME_WAVES_PER_CU ? ctx->frame_wpc : ctx->rays_wpc) = value;
+        (option == VG_OPT_FRAME_WAVES_PER_CU ? ctx->frame_wpc
+         : option == VG_OPT_RAYS_WAVES_PER_CU ? ctx->rays_wpc
+                                              : ctx->probe_wpc) = value;
         return VG_OK;
     case VG_OPT_FRAME_REFILL:
     case VG_OPT_RAYS_REFILL:
+    case VG_OPT_PROBE_REFILL:
         if (value < 1 || value > 64) return fail(VG_E_INVALID, "refill threshold %d not in 1..64", value);
-        (option == VG_OPT_FRAME_REFILL ? ctx->frame_refill : ctx->rays_refill) = value;
+        (option == VG_OPT_FRAME_REFILL ? ctx->frame_refill
+         : option == VG_OPT_RAYS_REFILL ? ctx->rays_refill
+                                        : ctx->probe_refill) = value;
         return VG_OK;
     default:
         return fail(VG_E_INVALID, "option %d cannot be set", option);
@@ -1157,6 +1115,8 @@ int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
     case VG_OPT_FRAME_REFILL: *value = ctx->frame_refill; return VG_OK;
     case VG_OPT_RAYS_WAVES_PER_CU: *value = ctx->rays_wpc; return VG_OK;
     case VG_OPT_RAYS_REFILL: *value = ctx->rays_refill; return VG_OK;
+    case VG_OPT_PROBE_WAVES_PER_CU: *value = ctx->probe_wpc; return VG_OK;
+    case VG_OPT_PROBE_REFILL: *value = ctx->probe_refill; return VG_OK;
     case VG_OPT_CU_COUNT: *value = ctx->cus; return VG_OK;
     default: return fail(VG_E_INVALID, "unknown option %d", option);
     }

@@ -883,8 +883,8 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0);
     HIP_TRY(hipGetLastError());
-    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
-    const int thr = w->ctx->rays_refill;
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
+    const int thr = w->ctx->probe_refill;
     const bool train = nrc && (nflags & VG_NRC_TRAIN) && nrc->cfg.steps > 0;
     LearnedArgs la;
     la.queries = pr->queries;
@@ -1102,6 +1102,17 @@ int vg_probes_update_learned(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, 
                              void *stream) {
     if (!nrc) return fail(VG_E_INVALID, "null cache");
     return probes_update(pr, p, nrc, flags, false, (hipStream_t)stream);
+}
+
+int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32_t flags,
+                       float *best_ms) {
+    if (!pr) return fail(VG_E_INVALID, "null probes");
+    int st = check_world(pr->world);
+    if (st) return st;
+    vg_ctx *c = pr->world->ctx;
+    return autotune(c, &c->probe_wpc, &c->probe_refill, [&](hipStream_t s) {
+        return probes_update(pr, p, nrc, flags, false, s);
+    }, best_ms);
 }
 
 int vg_probes_update_learned_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc,

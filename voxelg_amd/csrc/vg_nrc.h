@@ -15,8 +15,7 @@ struct vg_nrc {
     int partial_blocks;     // blocks the partial buffer holds
     uint32_t *frag_fwd;     // bf16 MFMA A-operand fragments of W (4 layers x 4 x 2 x 64 lanes x 16 B)
     uint32_t *frag_bwd;     // ... of W^T for the backward pass
-    float *grid, *grid_m, *grid_v, *grid_g;  // fp32 master, moments, coarse-level gradient (zero between steps)
-    uint32_t *grid_gh;      // fine-level gradient, bf16 pair per entry (zero between steps)
+    float *grid, *grid_m, *grid_v;  // fp32 master and Adam moments
     uint32_t *grid_h;       // bf16 feature pairs the kernels read, one u32 per entry
     uint32_t *step;         // optimizer steps taken (device)
     float *loss;            // loss of the last step (device)
@@ -25,7 +24,14 @@ struct vg_nrc {
     int32_t tcap;           // entries of tq / tt
     uint32_t *feat;         // training batch encoding: bf16 pair per (sample, level)
     float *dfeat;           // ... its gradient: 32 fp32 per sample
-    int32_t fcap;           // samples feat / dfeat hold
+    int32_t fcap;           // samples the training buffers hold
+    // grid-gradient records (samples x levels x 8 corners): entry keys and fp32 pairs, their
+    // sorted copies, the reduce-by-key runs (unique entries, sums, count) and the scratch
+    uint32_t *ck_in, *ck_out, *ck_uniq, *c_runs;
+    float2 *cv_in, *cv_out, *cv_agg;
+    void *ctemp;
+    size_t ctemp_bytes;
+    int key_bits;           // bits of an entry key (the sentinel n_entries included)
 };
 
 namespace vg {
@@ -34,7 +40,6 @@ constexpr int kNrcWidth = 64;
 constexpr int kNrcFrags = 32;       // [layer 4][block 4][k-step 2]
 constexpr int kNrcTile = 256;       // training samples per block per tile
 constexpr int kNrcPartial = VG_NRC_MLP_PARAMS + 16; // floats per block partial (loss at +0)
-constexpr int kNrcCoarse = 6;       // levels below this accumulate fp32 gradients
 
 // Inference over the queries at q + sel[0] * q_stride (sel may be null: q itself), n_dev[0]
 // of them (else n_max; never more than n_max). out4 receives raw predictions; with rad/pal

@@ -14,8 +14,13 @@
 // lane encodes hash levels {2g, 2g+1, 8+2g, 9+2g} of its sample.
 // Training stages a layer's input and output gradient (bf16) through LDS to form the
 // weight gradient dW = dY X^T as MFMAs over samples; per-block partial sums are reduced in
-// a second kernel (deterministic), grid gradients are scattered with fp32 atomics.
+// a second kernel (deterministic). The grid gradient is built without atomics: every
+// (sample, level, corner) contribution becomes an (entry, fp32 pair) record, a stable radix
+// sort (rocPRIM through hipCUB) groups them by entry, a reduce-by-key sums each group, and
+// Adam then updates exactly the entries that received a gradient.
 #include "vg_nrc.h"
+
+#include <hipcub/hipcub.hpp>
 
 #include <vector>
 
@@ -343,75 +348,43 @@ __global__ __launch_bounds__(256) void k_nrc_encode(NrcDev m, const vg_nrc_query
     feat[t] = pack2(f0, f1);
 }
 
-// Grid gradient of one level (blockIdx.y): one thread per sample spreads d loss / d feature
-// (dfeat, fp32) over the level's 8 corners with their trilinear weights.
-//   coarse levels (< kNrcCoarse): many samples share a corner; a wave first sums the
-//     contributions of its lanes per distinct corner entry (training batches come in
-//     query-list order, which follows probe order, so neighbouring lanes share corners),
-//     then one fp32 atomic per entry and feature;
-//   fine levels: corners are rarely shared; one packed bf16x2 atomic per corner
-//     (global_atomic_pk_add_bf16) into grid_gh.
-__global__ __launch_bounds__(256) void k_nrc_scatter(NrcDev m, const vg_nrc_query *__restrict__ q,
+// Grid-gradient records of one level (blockIdx.y): one thread per sample writes its 8
+// corner contributions (entry, w * d loss / d feature pair) at ((l * n + s) * 8 + corner);
+// samples with no gradient write the sentinel entry n_entries, which sorts last.
+__global__ __launch_bounds__(256) void k_nrc_contrib(NrcDev m, const vg_nrc_query *__restrict__ q,
                                                      const float2 *__restrict__ dfeat, uint32_t n,
-                                                     float *__restrict__ grid_g,
-                                                     uint32_t *__restrict__ grid_gh) {
+                                                     uint32_t sentinel, uint32_t *__restrict__ keys,
+                                                     float2 *__restrict__ vals) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     const int l = (int)blockIdx.y;
-    const int lane = threadIdx.x & 63;
-    const bool valid = s < n;
-    float2 d = make_float2(0.0f, 0.0f);
-    float p[3] = {0.0f, 0.0f, 0.0f};
-    if (valid) {
-        d = dfeat[(size_t)s * 16u + (uint32_t)l];
-        const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
-        p[0] = a.x;
-        p[1] = a.y;
-        p[2] = a.z;
+    if (s >= n) return;
+    const size_t base = ((size_t)l * n + s) * 8u;
+    const float2 d = dfeat[(size_t)s * 16u + (uint32_t)l];
+    if (d.x == 0.0f && d.y == 0.0f) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            keys[base + k] = sentinel;
+            vals[base + k] = make_float2(0.0f, 0.0f);
+        }
+        return;
     }
-    const bool live = valid && (d.x != 0.0f || d.y != 0.0f);
-    if (l < kNrcCoarse) {
-        if (__ballot(live) == 0) return; // wave-uniform
-        uint32_t idx[8];
-        float w[8];
-        level_corners(m, l, p, idx, w);
+    const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
+    const float p[3] = {a.x, a.y, a.z};
+    uint32_t idx[8];
+    float w[8];
+    level_corners(m, l, p, idx, w);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            bool pend = live;
-            const float v0 = w[k] * d.x, v1 = w[k] * d.y;
-            uint64_t todo = __ballot(pend);
-            while (todo) {
-                const int leader = __ffsll((long long)todo) - 1;
-                const uint32_t key = __shfl(idx[k], leader, 64);
-                const bool mine = pend && idx[k] == key;
-                const uint64_t same = __ballot(mine);
-                float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    s0 += __shfl_xor(s0, off, 64);
-                    s1 += __shfl_xor(s1, off, 64);
-                }
-                if (lane == leader) {
-                    unsafeAtomicAdd(grid_g + 2u * key, s0);
-                    unsafeAtomicAdd(grid_g + 2u * key + 1u, s1);
-                }
-                pend = pend && !mine;
-                todo &= ~same;
-            }
-        }
-    } else {
-        if (!live) return;
-        uint32_t idx[8];
-        float w[8];
-        level_corners(m, l, p, idx, w);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t bits = pack2(w[k] * d.x, w[k] * d.y);
-            __builtin_amdgcn_global_atomic_fadd_v2bf16(
-                (__attribute__((address_space(1))) bf16x2_t *)(grid_gh + idx[k]),
-                __builtin_bit_cast(bf16x2_t, bits));
-        }
+    for (int k = 0; k < 8; ++k) {
+        keys[base + k] = idx[k];
+        vals[base + k] = make_float2(w[k] * d.x, w[k] * d.y);
     }
 }
+
+struct Sum2 {
+    __device__ __host__ float2 operator()(const float2 &a, const float2 &b) const {
+        return make_float2(a.x + b.x, a.y + b.y);
+    }
+};
 
 // Forward and backward of a training batch from its encoding (feat): weight-gradient sums
 // per block into `partial`, the encoding gradient per sample into dfeat[s * 32 + input].
@@ -553,28 +526,20 @@ __device__ __forceinline__ void adam1(float &p, float &m, float &v, float g, flo
     p -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
-// Sparse Adam over the grid: entries with a zero gradient are left alone; the gradient
-// (fp32 pair for coarse levels, bf16 pair for fine levels) is cleared for the next step and
-// the bf16 copy the kernels read is refreshed.
-__global__ void k_nrc_adam_grid(float2 *__restrict__ p, float2 *__restrict__ mm,
-                                float2 *__restrict__ vv, float2 *__restrict__ gg,
-                                uint32_t *__restrict__ gh, uint32_t *__restrict__ ph,
-                                int64_t n_entries, int64_t coarse_entries, float lr, float b1,
-                                float b2, float eps, const uint32_t *step) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_entries) return;
-    float2 g;
-    if (e < coarse_entries) {
-        g = gg[e];
-        if (g.x == 0.0f && g.y == 0.0f) return;
-        gg[e] = make_float2(0.0f, 0.0f);
-    } else {
-        const uint32_t h = gh[e];
-        if (h == 0u) return;
-        g = make_float2(bf_lo(h), bf_hi(h));
-        gh[e] = 0u;
-        if (g.x == 0.0f && g.y == 0.0f) return;
-    }
+// Adam over the grid entries that received a gradient (the reduce-by-key runs: unique
+// entries and their summed gradient pairs); a feature whose sum is exactly zero is left
+// alone, like an entry no sample touched. The bf16 copy the kernels read is refreshed.
+__global__ void k_nrc_adam_runs(const uint32_t *__restrict__ ukeys, const float2 *__restrict__ ug,
+                                const uint32_t *n_runs, int64_t n_entries, float2 *__restrict__ p,
+                                float2 *__restrict__ mm, float2 *__restrict__ vv,
+                                uint32_t *__restrict__ ph, float lr, float b1, float b2, float eps,
+                                const uint32_t *step) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_runs) return;
+    const uint32_t e = ukeys[i];
+    if ((int64_t)e >= n_entries) return; // the sentinel run
+    const float2 g = ug[i];
+    if (g.x == 0.0f && g.y == 0.0f) return;
     const float t = (float)*step;
     const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
     float2 pp = p[e], m = mm[e], v = vv[e];
@@ -646,11 +611,6 @@ static NrcDev dev_view(const vg_nrc *m) {
     return d;
 }
 
-static int64_t coarse_entries(const vg_nrc *m) {
-    const int lc = m->cfg.levels < kNrcCoarse ? m->cfg.levels : kNrcCoarse;
-    return (int64_t)lc << m->cfg.log2_table;
-}
-
 static int set_dev(const vg_nrc *m) {
     HIP_TRY(hipSetDevice(m->device));
     return VG_OK;
@@ -679,6 +639,8 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
     if (n == 0) return VG_OK;
     if (n > (uint32_t)m->fcap)
         return fail(VG_E_INVALID, "batch of %u above the training buffers' %d", n, m->fcap);
+    if (n > (uint32_t)(1u << 31) / (8u * (uint32_t)m->cfg.levels))
+        return fail(VG_E_INVALID, "batch of %u: too many grid-gradient records", n);
     const NrcDev dv = dev_view(m);
     const uint32_t eblocks = (uint32_t)(((uint64_t)n * 16u + 255u) / 256u);
     hipLaunchKernelGGL(k_nrc_encode, dim3(eblocks), dim3(256), 0, stream, dv, q, n, m->feat);
@@ -688,22 +650,28 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
     hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, stream, dv, q, m->feat, t, n,
                        1.0f / (float)n, m->partial, m->dfeat);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_nrc_scatter, dim3((n + 255) / 256, (uint32_t)m->cfg.levels), dim3(256),
-                       0, stream, dv, q, reinterpret_cast<const float2 *>(m->dfeat), n, m->grid_g,
-                       m->grid_gh);
+    const size_t nc = (size_t)n * (size_t)m->cfg.levels * 8u;
+    hipLaunchKernelGGL(k_nrc_contrib, dim3((n + 255) / 256, (uint32_t)m->cfg.levels), dim3(256),
+                       0, stream, dv, q, reinterpret_cast<const float2 *>(m->dfeat), n,
+                       (uint32_t)m->n_entries, m->ck_in, m->cv_in);
     HIP_TRY(hipGetLastError());
+    size_t tb = m->ctemp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(m->ctemp, tb, m->ck_in, m->ck_out, m->cv_in,
+                                               m->cv_out, (int)nc, 0, m->key_bits, stream));
+    tb = m->ctemp_bytes;
+    HIP_TRY(hipcub::DeviceReduce::ReduceByKey(m->ctemp, tb, m->ck_out, m->ck_uniq, m->cv_out,
+                                              m->cv_agg, m->c_runs, Sum2(), (int)nc, stream));
     hipLaunchKernelGGL(k_nrc_reduce, dim3((VG_NRC_MLP_PARAMS + 1 + 63) / 64), dim3(1024), 0,
                        stream, m->partial, blocks, m->mlp_g, m->loss, m->step, update ? 1 : 0);
     HIP_TRY(hipGetLastError());
     if (update) {
         const vg_nrc_config &c = m->cfg;
-        hipLaunchKernelGGL(k_nrc_adam_grid, dim3((uint32_t)((m->n_entries + 255) / 256)), dim3(256),
-                           0, stream, reinterpret_cast<float2 *>(m->grid),
+        hipLaunchKernelGGL(k_nrc_adam_runs, dim3((uint32_t)((nc + 255) / 256)), dim3(256), 0, stream,
+                           m->ck_uniq, m->cv_agg, m->c_runs, m->n_entries,
+                           reinterpret_cast<float2 *>(m->grid),
                            reinterpret_cast<float2 *>(m->grid_m),
-                           reinterpret_cast<float2 *>(m->grid_v),
-                           reinterpret_cast<float2 *>(m->grid_g), m->grid_gh, m->grid_h,
-                           m->n_entries, coarse_entries(m), c.lr_grid, c.beta1, c.beta2, c.eps,
-                           m->step);
+                           reinterpret_cast<float2 *>(m->grid_v), m->grid_h, c.lr_grid, c.beta1,
+                           c.beta2, c.eps, m->step);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(1), dim3(1024), 0, stream, m->mlp, m->mlp_m,
                            m->mlp_v, m->mlp_g, reinterpret_cast<uint16_t *>(m->frag_fwd),
@@ -749,11 +717,52 @@ static int check_config(const vg_nrc_config *c) {
     return VG_OK;
 }
 
+static void free_sort(vg_nrc *m) {
+    void *ptrs[] = {m->ck_in, m->ck_out, m->ck_uniq, m->cv_in, m->cv_out, m->cv_agg, m->ctemp};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    m->ck_in = m->ck_out = m->ck_uniq = nullptr;
+    m->cv_in = m->cv_out = m->cv_agg = nullptr;
+    m->ctemp = nullptr;
+    m->ctemp_bytes = 0;
+}
+
+// Training buffers for batches of up to `cap` samples: encoding, its gradient, and the
+// grid-gradient records with their sort / reduce scratch.
+static int alloc_train(vg_nrc *m, int32_t cap) {
+    (void)hipFree(m->feat);
+    (void)hipFree(m->dfeat);
+    m->feat = nullptr;
+    m->dfeat = nullptr;
+    free_sort(m);
+    m->fcap = 0;
+    const size_t nc = (size_t)cap * (size_t)m->cfg.levels * 8u;
+    bool ok = hipMalloc((void **)&m->feat, (size_t)cap * 16 * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->dfeat, (size_t)cap * 32 * sizeof(float)) == hipSuccess &&
+              hipMalloc((void **)&m->ck_in, nc * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->ck_out, nc * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->ck_uniq, nc * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->cv_in, nc * sizeof(float2)) == hipSuccess &&
+              hipMalloc((void **)&m->cv_out, nc * sizeof(float2)) == hipSuccess &&
+              hipMalloc((void **)&m->cv_agg, nc * sizeof(float2)) == hipSuccess;
+    if (!ok) return fail(VG_E_NOMEM, "training buffers for %d samples", cap);
+    size_t a = 0, b = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, a, m->ck_in, m->ck_out, m->cv_in,
+                                               m->cv_out, (int)nc, 0, m->key_bits));
+    HIP_TRY(hipcub::DeviceReduce::ReduceByKey(nullptr, b, m->ck_out, m->ck_uniq, m->cv_out,
+                                              m->cv_agg, m->c_runs, Sum2(), (int)nc));
+    m->ctemp_bytes = a > b ? a : b;
+    if (hipMalloc(&m->ctemp, m->ctemp_bytes) != hipSuccess)
+        return fail(VG_E_NOMEM, "sort scratch of %zu bytes", m->ctemp_bytes);
+    m->fcap = cap;
+    return VG_OK;
+}
+
 static void free_nrc(vg_nrc *m) {
     void *ptrs[] = {m->mlp,    m->mlp_m,  m->mlp_v,  m->mlp_g,  m->partial, m->frag_fwd,
-                    m->frag_bwd, m->grid, m->grid_m, m->grid_v, m->grid_g,  m->grid_h,
-                    m->step,   m->loss,   m->tq,     m->tt,     m->feat,    m->dfeat,
-                    m->grid_gh};
+                    m->frag_bwd, m->grid, m->grid_m, m->grid_v, m->grid_h,  m->step,
+                    m->loss,   m->tq,     m->tt,     m->feat,   m->dfeat,   m->c_runs};
+    free_sort(m);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -768,8 +777,6 @@ static int load_params(vg_nrc *m, const float *mlp, const float *grid) {
     HIP_TRY(hipMemset(m->mlp_g, 0, mb));
     HIP_TRY(hipMemset(m->grid_m, 0, gb));
     HIP_TRY(hipMemset(m->grid_v, 0, gb));
-    HIP_TRY(hipMemset(m->grid_g, 0, gb));
-    HIP_TRY(hipMemset(m->grid_gh, 0, (size_t)m->n_entries * sizeof(uint32_t)));
     HIP_TRY(hipMemset(m->step, 0, sizeof(uint32_t)));
     HIP_TRY(hipMemset(m->loss, 0, sizeof(float)));
     hipLaunchKernelGGL(k_nrc_grid_pack, dim3((uint32_t)((m->n_entries + 255) / 256)), dim3(256), 0,
@@ -793,15 +800,8 @@ static int check_batch(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_
     if (st) return st;
     if (n > (uint32_t)m->fcap) {
         HIP_TRY(hipDeviceSynchronize());
-        (void)hipFree(m->feat);
-        (void)hipFree(m->dfeat);
-        m->feat = nullptr;
-        m->dfeat = nullptr;
-        m->fcap = 0;
-        if (hipMalloc((void **)&m->feat, (size_t)n * 16 * sizeof(uint32_t)) != hipSuccess ||
-            hipMalloc((void **)&m->dfeat, (size_t)n * 32 * sizeof(float)) != hipSuccess)
-            return fail(VG_E_NOMEM, "training buffers for %u queries", n);
-        m->fcap = (int32_t)n;
+        st = alloc_train(m, (int32_t)n);
+        if (st) return st;
     }
     return VG_OK;
 }
@@ -854,7 +854,8 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
     m->n_grid = 2 * m->n_entries;
     m->partial_blocks = 1024;
     m->tcap = cfg->batch;
-    m->fcap = cfg->batch > 65536 ? cfg->batch : 65536;
+    m->key_bits = 0;
+    while ((1ll << m->key_bits) <= m->n_entries) ++m->key_bits; // entries and the sentinel
     const size_t mb = VG_NRC_MLP_PARAMS * sizeof(float), gb = (size_t)m->n_grid * sizeof(float);
     const size_t fb = (size_t)kNrcFrags * 64 * 16;
     bool ok = hipMalloc((void **)&m->mlp, mb) == hipSuccess &&
@@ -868,15 +869,13 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
               hipMalloc((void **)&m->grid, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_m, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_v, gb) == hipSuccess &&
-              hipMalloc((void **)&m->grid_g, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_h, (size_t)m->n_entries * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc((void **)&m->grid_gh, (size_t)m->n_entries * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->c_runs, 256) == hipSuccess &&
               hipMalloc((void **)&m->step, 256) == hipSuccess &&
               hipMalloc((void **)&m->loss, 256) == hipSuccess &&
               hipMalloc((void **)&m->tq, (size_t)m->tcap * sizeof(vg_nrc_query)) == hipSuccess &&
-              hipMalloc((void **)&m->tt, (size_t)m->tcap * sizeof(float4)) == hipSuccess &&
-              hipMalloc((void **)&m->feat, (size_t)m->fcap * 16 * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc((void **)&m->dfeat, (size_t)m->fcap * 32 * sizeof(float)) == hipSuccess;
+              hipMalloc((void **)&m->tt, (size_t)m->tcap * sizeof(float4)) == hipSuccess;
+    if (ok) ok = alloc_train(m, cfg->batch > 65536 ? cfg->batch : 65536) == VG_OK;
     if (!ok) {
         free_nrc(m);
         delete m;
@@ -965,23 +964,22 @@ int vg_nrc_gradients(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t 
         HIP_TRY(hipMemcpy(mlp_grad, m->mlp_g, VG_NRC_MLP_PARAMS * sizeof(float),
                           hipMemcpyDeviceToHost));
     if (grid_grad) {
-        const int64_t ce = coarse_entries(m);
-        HIP_TRY(hipMemcpy(grid_grad, m->grid_g, (size_t)ce * 2 * sizeof(float),
-                          hipMemcpyDeviceToHost));
-        std::vector<uint32_t> gh((size_t)(m->n_entries - ce));
-        if (!gh.empty())
-            HIP_TRY(hipMemcpy(gh.data(), m->grid_gh + ce, gh.size() * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < gh.size(); ++i) {
-            const uint32_t lo = gh[i] << 16, hi = gh[i] & 0xFFFF0000u;
-            std::memcpy(grid_grad + 2 * (ce + (int64_t)i), &lo, 4);
-            std::memcpy(grid_grad + 2 * (ce + (int64_t)i) + 1, &hi, 4);
+        uint32_t runs = 0;
+        HIP_TRY(hipMemcpy(&runs, m->c_runs, sizeof(runs), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> keys(runs);
+        std::vector<float2> g(runs);
+        if (runs) {
+            HIP_TRY(hipMemcpy(keys.data(), m->ck_uniq, runs * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(g.data(), m->cv_agg, runs * sizeof(float2), hipMemcpyDeviceToHost));
+        }
+        std::memset(grid_grad, 0, (size_t)m->n_grid * sizeof(float));
+        for (uint32_t i = 0; i < runs; ++i) {
+            if ((int64_t)keys[i] >= m->n_entries) continue;
+            grid_grad[2 * (size_t)keys[i]] = g[i].x;
+            grid_grad[2 * (size_t)keys[i] + 1] = g[i].y;
         }
     }
     if (loss) HIP_TRY(hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
-    // the optimizer expects a clear grid gradient
-    HIP_TRY(hipMemset(m->grid_g, 0, (size_t)m->n_grid * sizeof(float)));
-    HIP_TRY(hipMemset(m->grid_gh, 0, (size_t)m->n_entries * sizeof(uint32_t)));
     HIP_TRY(hipDeviceSynchronize());
     return VG_OK;
 }

@@ -13,6 +13,8 @@ struct vg_ctx {
     int frame_refill;    // ... refill when fewer lanes of a wave are busy
     int rays_wpc;        // persistent batched kernel: waves per CU
     int rays_refill;
+    int probe_wpc;       // persistent probe path kernel: waves per CU
+    int probe_refill;
 };
 
 struct vg_world {
@@ -50,6 +52,57 @@ inline bool host_normalize(const float in[3], float out[3]) {
     for (int k = 0; k < 3; ++k) out[k] = in[k] * rcp;
     return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
 }
+
+template <class L>
+inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms) {
+    static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
+    static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
+    const int reps = 3;
+    hipStream_t stream;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int old_schedule = c->schedule;
+    c->schedule = VG_SCHEDULE_PERSISTENT;
+    float best = INFINITY;
+    int bw = *wpc_opt, br = *refill_opt, st = VG_OK;
+    for (int wpc : kWpc) {
+        for (int rf : kRefill) {
+            *wpc_opt = wpc;
+            *refill_opt = rf;
+            st = launch(stream);
+            if (st) break;
+            float ms_min = INFINITY;
+            for (int i = 0; i < reps && st == VG_OK; ++i) {
+                float ms = 0.0f;
+                if (hipEventRecord(e0, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
+                if (!st) st = launch(stream);
+                if (!st && hipEventRecord(e1, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
+                if (!st && hipEventSynchronize(e1) != hipSuccess) st = fail(VG_E_DEVICE, "sync");
+                if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = fail(VG_E_DEVICE, "elapsed");
+                ms_min = ms < ms_min ? ms : ms_min;
+            }
+            if (st) break;
+            if (ms_min < best) {
+                best = ms_min;
+                bw = wpc;
+                br = rf;
+            }
+        }
+        if (st) break;
+    }
+    *wpc_opt = bw;
+    *refill_opt = br;
+    c->schedule = old_schedule;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(stream);
+    if (st) return st;
+    if (best_ms) *best_ms = best;
+    return VG_OK;
+}
+
 
 inline int persistent_grid(const vg_ctx *c, int wpc) {
     int waves = c->cus * (wpc > 0 ? wpc : 1);

@@ -168,6 +168,15 @@ class ProbeGrid:
         return (dict(zip(("gather", "shadow", "bounce"), (int(r) for r in rays))),
                 int(nq.value))
 
+    def autotune(self, params: np.ndarray, cache=None, train: bool = False) -> float:
+        """Pick the probe path kernels' schedule (waves per CU, refill) on this update; the
+        context keeps it. Returns the best time in ms."""
+        p = np.ascontiguousarray(params, GI_PARAMS_DTYPE)
+        ms = C.c_float()
+        check(lib().vg_autotune_probes(self._h, ptr(p), None if cache is None else cache._h,
+                                       1 if train else 0, C.byref(ms)), "vg_autotune_probes")
+        return float(ms.value)
+
     def download(self) -> np.ndarray:
         out = np.zeros(self.total, PROBE_DTYPE)
         check(lib().vg_probes_download(self._h, ptr(out)), "vg_probes_download")
