@@ -234,6 +234,48 @@ int vg_trace_frame_counts(vg_world *world, const vg_camera *camera_host, int32_t
 int vg_trace_rays_counts(vg_world *world, const vg_ray *rays, size_t n, uint32_t flags,
                          vg_hit *hits, uint64_t counts_host[VG_IT_COUNT]);
 
+/* --------------------------------------------------------- dynamic world -- */
+/* A sphere edit (apply_sphere, src/app.rs:199-210): every voxel at offset d from the centre
+ * with |d|^2 <= radius^2 is set to `material` (0 carves) through apply_edit
+ * (src/voxel.rs:450-474): skipped outside the window, stored at its toroidal slot. */
+typedef struct vg_edit {
+    int32_t center[3]; /* world voxel coordinates */
+    uint8_t radius;    /* at most 64 */
+    uint8_t material;
+    uint8_t pad[2];
+} vg_edit;
+
+/* Apply n sphere edits (host array) to an uploaded world, in order: a voxel covered by
+ * several spheres ends with the last one's material, as sequential set_voxel calls
+ * (src/voxel.rs:871-946) leave it. The brick occupancy / materials, the brick and tile
+ * uniform tables and the tile, chunk and L4 masks are updated on the device, bottom-up for
+ * the changed nodes only. The window origin must be chunk-aligned (a multiple of 64), as
+ * the reference's always is. Enqueued on `stream`; the edits are copied before return. */
+int vg_world_apply_edits(vg_world *world, const vg_edit *edits_host, uint32_t n, void *stream);
+
+/* Words of a frame block mask: one bit per 8x8 pixel block (row-major over
+ * ceil(width/8) x ceil(height/8) blocks) plus one whole-frame flag word. */
+int vg_frame_mask_words(int32_t width, int32_t height, uint32_t *words_out);
+/* The reference's project_brick_to_tiles (src/temporal.rs:16-111) for the tiles the last
+ * vg_world_apply_edits changed, made conservative: marks (device mask) every 8x8 block with
+ * a pixel whose primary ray can enter a changed 16^3 tile, and (sun not NULL) every block
+ * whose primary hit can cast a sun shadow ray (within shadow_tmax) through one. A block not
+ * marked keeps its hits and shadow bytes exactly. */
+int vg_frame_dirty_blocks(vg_world *world, const vg_camera *camera_host, int32_t width,
+                          int32_t height, const float sun[3], float shadow_tmax, uint32_t *mask,
+                          void *stream);
+/* vg_trace_frame over the blocks set in a device block mask only (sun NULL: primary only);
+ * pixels of other blocks are left untouched. */
+int vg_trace_frame_blocks(vg_world *world, const vg_camera *camera_host, int32_t width,
+                          int32_t height, float primary_tmax, uint32_t primary_flags,
+                          const float sun[3], float shadow_tmax, const uint32_t *mask,
+                          vg_hit *hits, uint8_t *occluded, void *stream);
+/* The world's device state back in the reference layout (any pointer may be NULL; counts
+ * from vg_world_counts). Synchronous. */
+int vg_world_download(vg_world *world, vg_brick *bricks, uint64_t *tile_mask,
+                      uint64_t *chunk_mask, uint64_t *l4_mask, uint8_t *brick_uniform,
+                      uint8_t *tile_uniform);
+
 /* ------------------------------------------------------------- scheduling -- */
 /* Traces run as persistent, ray-compacting kernels by default: a machine-sized grid whose
  * waves refill idle lanes from a global ray counter whenever fewer than `refill` of their

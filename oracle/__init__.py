@@ -56,6 +56,8 @@ NRC_QUERY_DTYPE = np.dtype([("pos", "<f4", (3,)), ("face", "<u4"), ("dir", "<f4"
 NRC_WIDTH = 64
 NRC_MLP_PARAMS = 3 * 64 * 64 + 16 * 64
 NRC_IN_FACE, NRC_IN_DIR, NRC_IN_ONE = 32, 38, 41
+EDIT_DTYPE = np.dtype([("center", "<i4", (3,)), ("radius", "u1"), ("material", "u1"),
+                       ("pad", "u1", (2,))])
 GI_SKIP_BURIED = 1
 GI_BOUNCE_TABLE = 4096
 PROBE_FIELDS = 48
@@ -88,6 +90,7 @@ def lib():
             "orc_rebuild_all_masks": (None, [W]),
             "orc_rebuild_all_uniform": (None, [W]),
             "orc_set_voxel": (None, [W, C.c_int32, C.c_int32, C.c_int32, C.c_uint8]),
+            "orc_apply_spheres": (None, [W, P, C.c_int64]),
             "orc_material_at_world": (C.c_uint8, [W, C.c_int32, C.c_int32, C.c_int32]),
             "orc_hash2": (C.c_float, [C.c_int32, C.c_int32]),
             "orc_hash3": (C.c_float, [C.c_int32, C.c_int32, C.c_int32]),
@@ -187,6 +190,16 @@ class OracleWorld:
             setattr(o, a, getattr(w, a))
         return o
 
+    @classmethod
+    def copy_of(cls, w) -> "OracleWorld":
+        """An independent copy of a world's arrays."""
+        o = cls.__new__(cls)
+        o.dims, o.origin = tuple(w.dims), tuple(w.origin)
+        for a in ("bricks", "tile_mask", "chunk_mask", "l4_mask", "brick_uniform",
+                  "tile_uniform"):
+            setattr(o, a, np.array(getattr(w, a), copy=True))
+        return o
+
     def cstruct(self) -> OrcWorld:
         s = OrcWorld()
         s.dims = _i3(self.dims)
@@ -222,6 +235,12 @@ class OracleWorld:
             return
         s = [int(np.mod(v, d)) for v, d in zip((wx, wy, wz), self.dims)]
         self.set_voxel(s[0], s[1], s[2], mat)
+
+    def apply_spheres(self, edits: np.ndarray) -> "OracleWorld":
+        """apply_sphere for each edit in order, then a full pyramid / LUT rebuild."""
+        e = np.ascontiguousarray(edits, EDIT_DTYPE)
+        lib().orc_apply_spheres(C.byref(self.cstruct()), _p(e), len(e))
+        return self
 
     def material_at(self, wx, wy, wz) -> int:
         return int(lib().orc_material_at_world(C.byref(self.cstruct()), int(wx), int(wy), int(wz)))

@@ -211,6 +211,44 @@ void orc_set_voxel(orc_world *w, int32_t x, int32_t y, int32_t z, uint8_t mat) {
 }
 
 /* src/voxel.rs:1136-1156: world coordinate -> toroidal slot (x, z folded). */
+/* apply_sphere (src/app.rs:199-210) for each edit in order: every voxel at offset
+ * (dx, dy, dz) with dx^2 + dy^2 + dz^2 <= r^2 goes through apply_edit (src/voxel.rs:450-474):
+ * skipped outside the window, else stored at its toroidal slot (x, z rem_euclid the window
+ * size, y relative to the window) with Brick::set semantics (src/voxel.rs:315-325). The
+ * pyramid and uniform tables are then rebuilt in full, which equals the reference's
+ * per-voxel refresh (test: incremental == full rebuild). */
+void orc_apply_spheres(orc_world *w, const orc_edit *e, int64_t n) {
+    dims_t d = mkdims(w->dims);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t r = e[i].radius, r2 = r * r;
+        for (int32_t dy = -r; dy <= r; ++dy)
+            for (int32_t dx = -r; dx <= r; ++dx)
+                for (int32_t dz = -r; dz <= r; ++dz) {
+                    if (dx * dx + dy * dy + dz * dz > r2) continue;
+                    const int32_t wx = e[i].center[0] + dx, wy = e[i].center[1] + dy,
+                                  wz = e[i].center[2] + dz;
+                    const int32_t rx = wx - w->origin[0], ry = wy - w->origin[1],
+                                  rz = wz - w->origin[2];
+                    if (rx < 0 || ry < 0 || rz < 0 || rx >= w->dims[0] || ry >= w->dims[1] ||
+                        rz >= w->dims[2])
+                        continue;
+                    const int32_t sx = pos_mod(wx, w->dims[0]), sy = ry, sz = pos_mod(wz, w->dims[2]);
+                    orc_brick *b = &w->bricks[lin3(d.b, sx / 4, sy / 4, sz / 4)];
+                    const int vi = child_bit(sx & 3, sy & 3, sz & 3);
+                    const uint64_t bit = (uint64_t)1 << vi;
+                    if (e[i].material == 0) {
+                        b->occupancy &= ~bit;
+                        b->materials[vi] = 0;
+                    } else {
+                        b->occupancy |= bit;
+                        b->materials[vi] = e[i].material;
+                    }
+                }
+    }
+    orc_rebuild_all_masks(w);
+    orc_rebuild_all_uniform(w);
+}
+
 uint8_t orc_material_at_world(const orc_world *w, int32_t wx, int32_t wy, int32_t wz) {
     int32_t rx = wx - w->origin[0], ry = wy - w->origin[1], rz = wz - w->origin[2];
     if (rx < 0 || ry < 0 || rz < 0 || rx >= w->dims[0] || ry >= w->dims[1] || rz >= w->dims[2])

@@ -117,6 +117,17 @@ void orc_rebuild_all_uniform(orc_world *w);
 void orc_set_voxel(orc_world *w, int32_t x, int32_t y, int32_t z, uint8_t mat);
 uint8_t orc_material_at_world(const orc_world *w, int32_t x, int32_t y, int32_t z);
 
+/* A sphere edit (vg_edit in include/vg.h): world voxel centre, radius, material (0 carves). */
+typedef struct orc_edit {
+    int32_t center[3];
+    uint8_t radius;
+    uint8_t material;
+    uint8_t pad[2];
+} orc_edit;
+/* apply_sphere + apply_edit (src/app.rs:199-210, src/voxel.rs:450-474) for each edit in
+ * order, then a full rebuild of masks and uniform tables. */
+void orc_apply_spheres(orc_world *w, const orc_edit *e, int64_t n);
+
 /* noise + generator */
 float orc_hash2(int32_t x, int32_t z);
 float orc_hash3(int32_t x, int32_t y, int32_t z);

@@ -46,6 +46,10 @@ NRC_CONFIG_DTYPE = np.dtype([
 NRC_QUERY_DTYPE = np.dtype([("pos", "<f4", (3,)), ("face", "<u4"), ("dir", "<f4", (3,)),
                             ("aux", "<u4")])
 assert NRC_CONFIG_DTYPE.itemsize == 112 and NRC_QUERY_DTYPE.itemsize == 32
+# vg_edit: a sphere edit (apply_sphere, src/app.rs:199-210)
+EDIT_DTYPE = np.dtype([("center", "<i4", (3,)), ("radius", "u1"), ("material", "u1"),
+                       ("pad", "u1", (2,))])
+assert EDIT_DTYPE.itemsize == 16
 VG_NRC_MLP_PARAMS = 3 * 64 * 64 + 16 * 64
 VG_NRC_MAX_BATCH = 1 << 20
 VG_NRC_TRAIN = 1
@@ -186,6 +190,12 @@ SIGNATURES = {
     "vg_probes_update_learned": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
     "vg_probes_update_learned_counts": (C.c_int, [_P, _P, _P, C.c_uint32, _U64P, _U64P]),
     "vg_autotune_probes": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
+    "vg_world_apply_edits": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "vg_frame_mask_words": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
+    "vg_frame_dirty_blocks": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _F32P, C.c_float, _P, _P]),
+    "vg_trace_frame_blocks": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
+                                        _F32P, C.c_float, _P, _P, _P, _P]),
+    "vg_world_download": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
 }
 
 

@@ -193,14 +193,18 @@ __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams ca
 // Beam pre-pass of the persistent frame: one wave per 8x8 pixel block writes the block's
 // exact start t (beam_start). Its first thread also zeroes the frame's work counter, which
 // the persistent kernel enqueued after it on the same stream consumes.
+// With a block list (list / list_n on the device), the waves take the listed blocks only.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint32_t nbx,
                                                  uint32_t nblocks, float ptmax,
                                                  float *__restrict__ beam_t, uint32_t *work,
+                                                 const uint32_t *__restrict__ list,
+                                                 const uint32_t *list_n,
                                                  unsigned long long *counts) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0u;
-    const uint32_t blk = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (blk >= nblocks) return; // whole waves leave together
+    const uint32_t wi = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (wi >= (list ? *list_n : nblocks)) return; // whole waves leave together
+    const uint32_t blk = list ? list[wi] : wi;
     const uint32_t lane = threadIdx.x & 63;
     const int32_t px = (int32_t)((blk % nbx) * 8 + (lane & 7));
     const int32_t py = (int32_t)((blk / nbx) * 8 + (lane >> 3));
@@ -228,8 +232,11 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     DevWorld w, CamParams cam, int32_t width, int32_t height, uint32_t nbx, uint32_t total_ids,
     float ptmax, uint32_t pflags, float sx, float sy, float sz, float stmax,
     const float *__restrict__ beam_t, vg_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
-    uint32_t *work, int threshold, unsigned long long *counts) {
+    uint32_t *work, int threshold, const uint32_t *__restrict__ list, const uint32_t *list_n,
+    unsigned long long *counts) {
+    // with a block list, ray ids run over the listed 8x8 blocks only
     const uint32_t lane = threadIdx.x & 63;
+    if (list) total_ids = *list_n * 64u;
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
 #pragma unroll
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
             } else if (kind == 0) {
                 const uint32_t id = base + lanes_below(~act);
                 if (id < total_ids) {
-                    const uint32_t blk = id >> 6, qd = id & 63u;
+                    const uint32_t blk = list ? list[id >> 6] : id >> 6, qd = id & 63u;
                     px = (int32_t)((blk % nbx) * 8 + (qd & 7));
                     py = (int32_t)((blk / nbx) * 8 + (qd >> 3));
                     if (px < width && py < height) {
@@ -562,7 +569,9 @@ static int ensure_beam_capacity(vg_world *w, size_t blocks) {
 static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t width, int32_t height,
                                    float ptmax, uint32_t pflags, const float s[3], float stmax,
                                    vg_hit *hits, uint8_t *occluded, bool shadow, bool beam,
-                                   bool count, hipStream_t stream) {
+                                   bool count, hipStream_t stream,
+                                   const uint32_t *list = nullptr,
+                                   const uint32_t *list_n = nullptr) {
     const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
     const uint64_t nblocks = (uint64_t)nbx * nby;
     // the counter may overshoot the ray count by 64 per wave after it runs out
@@ -574,10 +583,10 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
         dim3 g((uint32_t)((nblocks + 3) / 4));
         if (count)
             hipLaunchKernelGGL((k_beam<true>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
-                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, cd);
+                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, list, list_n, cd);
         else
             hipLaunchKernelGGL((k_beam<false>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
-                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, cd);
+                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, list, list_n, cd);
     } else {
         hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, w->work);
     }
@@ -588,7 +597,7 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
 #define LAUNCH_P(C, SH, B)                                                                     \
     hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, cp, width, \
                        height, nbx, total, ptmax, pflags, s[0], s[1], s[2], stmax, w->beam_t,  \
-                       hits, occluded, w->work, thr, cd)
+                       hits, occluded, w->work, thr, list, list_n, cd)
     if (count) {
         if (shadow) {
             if (beam) LAUNCH_P(true, true, true);
@@ -703,6 +712,7 @@ static void free_world(vg_world *w) {
                     w->mat,     w->scratch, w->counts_dev, w->work, w->beam_t};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    vg_edit_state_free(w);
 }
 
 } // namespace vg
@@ -912,6 +922,33 @@ int vg_trace_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t 
                    float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream) {
     return launch_frame(w, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
                         hits, occluded, 1, false, (hipStream_t)stream);
+}
+
+int vg_trace_frame_blocks(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                          float primary_tmax, uint32_t primary_flags, const float sun[3],
+                          float shadow_tmax, const uint32_t *mask, vg_hit *hits,
+                          uint8_t *occluded, void *stream) {
+    int st = check_world(w);
+    if (st) return st;
+    if (!camera || width <= 0 || height <= 0 || !hits || !mask)
+        return fail(VG_E_INVALID, "bad frame args");
+    if (camera->resolution[0] != (float)width || camera->resolution[1] != (float)height)
+        return fail(VG_E_INVALID, "camera resolution %gx%g does not match %dx%d",
+                    camera->resolution[0], camera->resolution[1], width, height);
+    const bool shadow = sun != nullptr;
+    float s[3] = {0.0f, 0.0f, 0.0f};
+    if (shadow) {
+        if (!occluded) return fail(VG_E_INVALID, "null occluded");
+        if (!host_normalize(sun, s)) s[0] = s[1] = s[2] = 0.0f;
+    }
+    const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
+    const uint32_t *list = nullptr, *list_n = nullptr;
+    st = edit_block_list(w, mask, nbx * nby, (hipStream_t)stream, &list, &list_n);
+    if (st) return st;
+    return launch_frame_persistent(w, cam_params(camera), width, height, primary_tmax,
+                                   primary_flags, s, shadow_tmax, hits, occluded, shadow,
+                                   (primary_flags & VG_RAY_BEAM) != 0, false, (hipStream_t)stream,
+                                   list, list_n);
 }
 
 int vg_trace_frame_counts(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,

@@ -17,6 +17,8 @@ struct vg_ctx {
     int probe_refill;
 };
 
+struct vg_edit_state; // sphere-edit buffers (vg_edit.hip), created on first use
+
 struct vg_world {
     vg_ctx *ctx;
     vg::Dims d;
@@ -29,7 +31,10 @@ struct vg_world {
     uint32_t *work;                  // persistent kernels' ray counter
     float *beam_t;                   // beam start t per 8x8 pixel block
     size_t beam_cap;                 // entries of beam_t
+    vg_edit_state *edit;
 };
+
+void vg_edit_state_free(vg_world *w);
 
 namespace vg {
 
@@ -103,6 +108,10 @@ inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *b
     return VG_OK;
 }
 
+
+// Dirty-block list (ids of the set bits of a frame block mask, count on the device).
+int edit_block_list(vg_world *w, const uint32_t *mask, uint32_t nblocks, hipStream_t stream,
+                    const uint32_t **list, const uint32_t **list_n);
 
 inline int persistent_grid(const vg_ctx *c, int wpc) {
     int waves = c->cus * (wpc > 0 ? wpc : 1);

@@ -17,7 +17,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (BATCH_DTYPE, HIT_DTYPE, IT_NAMES, RAY_DTYPE, VG_IT_COUNT, check, f32x3, i32x3,
+from ._lib import (BATCH_DTYPE, EDIT_DTYPE, HIT_DTYPE, IT_NAMES, RAY_DTYPE, VG_IT_COUNT, check,
+                   f32x3, i32x3,
                    lib, ptr)
 from .world import World
 
@@ -291,6 +292,51 @@ class DeviceWorld:
                                    f32x3(sun), shadow_tmax, hits.data_ptr(), occluded.data_ptr(),
                                    _stream(stream)), "vg_trace_frame")
         return hits, occluded
+
+    # ------------------------------------------------------------ dynamic --
+    def apply_edits(self, edits: np.ndarray, stream=None) -> "DeviceWorld":
+        """Sphere edits (EDIT_DTYPE records, applied in order) with the incremental pyramid
+        and uniform-table rebuild on the device."""
+        e = np.ascontiguousarray(edits, EDIT_DTYPE)
+        check(lib().vg_world_apply_edits(self._h, ptr(e) if len(e) else None, len(e),
+                                         _stream(stream)), "vg_world_apply_edits")
+        return self
+
+    def dirty_blocks(self, camera: np.ndarray, width: int, height: int, sun=None,
+                     shadow_tmax: float = 480.0, mask=None, stream=None):
+        """Device block mask of the 8x8 blocks the last edits can change (sun None: primary
+        rays only)."""
+        cam = np.ascontiguousarray(camera, np.float32)
+        n = C.c_uint32()
+        check(lib().vg_frame_mask_words(width, height, C.byref(n)), "vg_frame_mask_words")
+        if mask is None:
+            mask = self._buf(int(n.value) * 4)
+        check(lib().vg_frame_dirty_blocks(self._h, ptr(cam), width, height,
+                                          None if sun is None else f32x3(sun), shadow_tmax,
+                                          mask.data_ptr(), _stream(stream)),
+              "vg_frame_dirty_blocks")
+        return mask
+
+    def trace_frame_blocks(self, camera: np.ndarray, width: int, height: int,
+                           primary_tmax: float, primary_flags: int, sun, shadow_tmax: float,
+                           mask, hits, occluded=None, stream=None):
+        """Retrace the masked 8x8 blocks into existing hit / shadow buffers."""
+        cam = np.ascontiguousarray(camera, np.float32)
+        check(lib().vg_trace_frame_blocks(self._h, ptr(cam), width, height, primary_tmax,
+                                          primary_flags, None if sun is None else f32x3(sun),
+                                          shadow_tmax, mask.data_ptr(), hits.data_ptr(),
+                                          None if occluded is None else occluded.data_ptr(),
+                                          _stream(stream)), "vg_trace_frame_blocks")
+        return hits, occluded
+
+    def download(self) -> World:
+        """The device state back in the reference layout, as a World."""
+        out = World(self.dims, self.origin)
+        check(lib().vg_world_download(self._h, ptr(out.bricks), ptr(out.tile_mask),
+                                      ptr(out.chunk_mask), ptr(out.l4_mask),
+                                      ptr(out.brick_uniform), ptr(out.tile_uniform)),
+              "vg_world_download")
+        return out
 
     def trace_frame_counts(self, camera: np.ndarray, width: int, height: int,
                            primary_tmax: float, primary_flags: int, sun, shadow_tmax: float):

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import RAY_DTYPE, VG_RAY_ANY_HIT, VG_RAY_BEAM, VG_RAY_LOD
+from ._lib import EDIT_DTYPE, RAY_DTYPE, VG_RAY_ANY_HIT, VG_RAY_BEAM, VG_RAY_LOD
 from .world import Camera, World, sun_direction, terrain_height
 
 MAX_RAY_DIST = 700.0
@@ -163,3 +163,43 @@ class BatchedWorkload:
 
 BATCHED = BatchedWorkload()
 
+
+
+@dataclass(frozen=True)
+class DynamicWorkload:
+    """Dynamic world: the small world plus n_edits seeded sphere edits per frame (radius
+    uniform in [r_lo, r_hi], half carve / half fill with a material uniform in 1..12), centred
+    within 8 voxels of the terrain surface; 1920x1080 primary + sun shadow retrace of the
+    dirtied 8x8 blocks; a learned probe update per frame."""
+    name: str = "dynamic"
+    dims: tuple = (256, 128, 256)
+    n_edits: int = 4096
+    r_lo: int = 2
+    r_hi: int = 12
+    width: int = 1920
+    height: int = 1080
+    seed: int = 0x5EED_0005
+
+    def world(self, threads: int = 0) -> World:
+        return World(self.dims).generate(SMALL.seed, threads)
+
+    def camera(self):
+        return SMALL.camera()
+
+    def edits(self, frame: int, world: World, n: int | None = None) -> np.ndarray:
+        """The edits of one frame (EDIT_DTYPE), applied in array order."""
+        n = self.n_edits if n is None else int(n)
+        rng = np.random.default_rng([self.seed, int(frame)])
+        top = world.top_surface()
+        e = np.zeros(n, EDIT_DTYPE)
+        xs = rng.integers(0, self.dims[0], n)
+        zs = rng.integers(0, self.dims[2], n)
+        ys = np.maximum(top[xs, zs], 0) + rng.integers(-8, 9, n)
+        e["center"] = np.stack([xs, ys, zs], 1).astype(np.int32)
+        e["radius"] = rng.integers(self.r_lo, self.r_hi + 1, n).astype(np.uint8)
+        carve = rng.random(n) < 0.5
+        e["material"] = np.where(carve, 0, rng.integers(1, 13, n)).astype(np.uint8)
+        return e
+
+
+DYNAMIC = DynamicWorkload()
