@@ -8,6 +8,7 @@ is generated on the host, uploaded once, and resident in HBM before the timed re
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
   python bench.py --workload batched   # uneven batched queries (96 batches per step)
   python bench.py --workload gi        # GI probe update: plain 256-ray vs learned 32-ray
+  python bench.py --workload dynamic   # 4096 sphere edits per frame + retrace + probe update
 
 Prints ONE JSON line on stdout; details go to stderr.
 """
@@ -61,7 +62,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "batched", "gi"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "batched", "gi", "dynamic"], default="frame")
+    ap.add_argument("--check-every", type=int, default=16,
+                    help="dynamic: bit-exact oracle check of the frame every N frames")
     ap.add_argument("--gi-train-updates", type=int, default=150,
                     help="learned GI: online-training updates before the timed ones")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -78,6 +81,8 @@ def main():
         return bench_batched(args)
     if args.workload == "gi":
         return bench_gi_main(args)
+    if args.workload == "dynamic":
+        return bench_dynamic(args)
 
     import numpy as np
 
@@ -432,6 +437,103 @@ def bench_gi_main(args):
                                "per update) vs plain 256 rays/probe",
                    "world": list(wl.dims)},
         "gi": g,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_dynamic(args):
+    """Config 5: the 256x128x256 world plus 4096 seeded sphere edits per frame (radius 2..12,
+    half carve / half fill). One step = one frame: apply the edits on the device (incremental
+    pyramid / uniform rebuild), mark the dirtied 8x8 blocks (primary and sun-shadow rays),
+    retrace them at 1920x1080 (primary tmax 512 + shadow 480), then a learned probe update
+    (8x4x8 probes at spacing 32, 32 rays per probe, the cache training online). Every
+    --check-every frames the incrementally maintained frame is compared bit for bit with the
+    oracle's full frame of the same edited world (outside the timed region)."""
+    import math
+
+    import numpy as np
+
+    import oracle
+    import voxelg_amd as vg
+    from voxelg_amd import gi, nrc, workloads
+    from voxelg_amd._lib import VG_GI_SKIP_BURIED
+    from voxelg_amd.device import Context, decode_hits, device_count
+
+    assert device_count() > 0, "bench.py needs a HIP device"
+    wl = workloads.DYNAMIC
+    world = wl.world()
+    ref = oracle.OracleWorld.copy_of(world)
+    ctx = Context(0)
+    dw = ctx.world(wl.dims).upload(world)
+    W, H = wl.width, wl.height
+    _, cam = workloads.FrameWorkload("dynamic", wl.dims, W, H, 512.0, 0, True).camera()
+    sun = vg.sun_direction(math.radians(35.0), 2.1)
+    stream = ctx.stream()
+    hits = ctx.alloc(W * H * 20)
+    occ = ctx.alloc(W * H)
+    dw.trace_frame(cam, W, H, 512.0, 0, sun, 480.0, hits, occ, stream)
+    pgrid = (wl.dims[0] // 32, wl.dims[1] // 32, wl.dims[2] // 32)
+    p = gi.gi_params(sun, grid=pgrid, spacing=32, rays_per_probe=32, epochs=1, update_div=1,
+                     bounces=2, flags=VG_GI_SKIP_BURIED)
+    probes = gi.ProbeGrid(dw, p)
+    cache = nrc.RadianceCache(ctx)
+    n_frames = args.warmup + args.steps
+    edits = [wl.edits(f, world) for f in range(n_frames)]
+    phases = ("edits", "dirty", "retrace", "probes")
+    times = {k: [] for k in phases}
+    dirty_frac = []
+    checks = []
+    wall = 0.0
+    nbx, nby = (W + 7) // 8, (H + 7) // 8
+    for f in range(n_frames):
+        ev = [ctx.event() for _ in range(5)]
+        t0 = time.perf_counter()
+        ev[0].record(stream)
+        dw.apply_edits(edits[f], stream)
+        ev[1].record(stream)
+        mask = dw.dirty_blocks(cam, W, H, sun, 480.0, stream=stream)
+        ev[2].record(stream)
+        dw.trace_frame_blocks(cam, W, H, 512.0, 0, sun, 480.0, mask, hits, occ, stream)
+        ev[3].record(stream)
+        p["round"] = f
+        probes.update_learned(cache, p, train=True, stream=stream)
+        ev[4].record(stream)
+        stream.synchronize()
+        dt = time.perf_counter() - t0
+        if f >= args.warmup:
+            wall += dt
+            for k, name in enumerate(phases):
+                times[name].append(ev[k].elapsed_ms(ev[k + 1]))
+            m = mask.numpy(np.uint32)
+            bits = np.unpackbits(m[:-1].view(np.uint8), bitorder="little")[:nbx * nby]
+            dirty_frac.append(1.0 if m[-1] else float(bits.mean()))
+        ref.apply_spheres(edits[f])
+        if args.check_every > 0 and (f + 1) % args.check_every == 0:
+            exp, exp_occ, _ = ref.frame(cam, W, H, 512.0, 0, sun, 480.0,
+                                        threads=min(os.cpu_count() or 1, 16))
+            ok = decode_hits(hits).tobytes() == exp.tobytes() and \
+                np.array_equal(occ.numpy(np.uint8), exp_occ)
+            checks.append({"frame": f, "bit_exact": bool(ok)})
+            log(f"frame {f}: bit-exact check {'ok' if ok else 'FAILED'}")
+            assert ok, f"frame {f}: incrementally maintained frame differs from the oracle"
+    ms = {k: round(float(np.mean(v)), 4) for k, v in times.items()}
+    ms_frame = wall / args.steps * 1e3
+    log(f"dynamic: {ms_frame:.3f} ms/frame wall; phases {ms}; dirty fraction "
+        f"{np.mean(dirty_frac):.3f}; cache steps {cache.stats()[0]}")
+    out = {
+        "metric": "dynamic-world frame ms (4096 sphere edits + incremental rebuild + dirty "
+                  "retrace + learned probe update)",
+        "value": round(ms_frame, 4), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_frame, 4), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
+        "data": "synthetic (seeded world, seeded sphere edits)",
+        "config": {"workload": "dynamic: 256x128x256 world, 4096 sphere edits per frame (r 2..12, "
+                               "50% carve / 50% fill), 1920x1080 primary + sun shadow retrace of "
+                               "the dirty 8x8 blocks, learned probe update (8x4x8 probes, 32 "
+                               "rays, online training)",
+                   "world": list(wl.dims), "resolution": [W, H], "edits_per_frame": wl.n_edits},
+        "phases_ms": ms, "dirty_block_fraction": round(float(np.mean(dirty_frac)), 4),
+        "checks": checks,
     }
     print(json.dumps(out), flush=True)
 
