@@ -4,16 +4,22 @@
 // tables, and the 8x8 screen blocks a frame must retrace afterwards (the reference's
 // project_brick_to_tiles, src/temporal.rs:16-111, made conservative).
 //
-//   pairs   the host sizes, per edit, the bricks its sphere touches inside the window; a
-//           kernel emits (nested brick index, edit index) pairs edit by edit; a stable
-//           radix sort (rocPRIM through hipCUB) groups them by brick keeping edit order.
-//   bricks  one wave per touched brick, one lane per voxel: the lane applies the brick's
-//           spheres in edit order (the last covering sphere wins, exactly as sequential
-//           set_voxel calls), the occupancy word is the wave's ballot, the brick-uniform byte
-//           a wave-wide equality; changed bricks mark their tile dirty.
-//   tiles / chunks / L4   one wave per dirty node: the node word is the ballot of its 64
+//   pairs   one wave per edit enumerates the bricks its sphere touches inside the window
+//           (lane per brick of the clipped box): a count pass (atomic per-brick counters),
+//           an inclusive scan of the counters, and a fill pass that places each (brick, edit)
+//           pair in its brick's run, counting the counter back down to zero. The order inside
+//           a run is arbitrary: the brick pass takes, per voxel, the covering edit of the
+//           HIGHEST index, which is what sequential calls leave.
+//           The count pass also lists the touched tiles (first pair of a brick -> first
+//           brick of a tile -> one append), so later passes visit only those.
+//   bricks  one 1024-thread block per touched tile, one wave per touched brick, one lane per
+//           voxel: the lanes load up to 64 of the brick's edits at once and every lane tests
+//           them all (operands broadcast with readlane); the occupancy word is the wave's
+//           ballot, the brick-uniform byte a wave-wide equality; changed bricks mark their
+//           tile dirty.
+//   tiles / chunks / L4   one wave per dirty node (tiles: from the touched-tile list): the node word is the ballot of its 64
 //           children being non-empty, the tile-uniform byte a wave-wide equality.
-//   blocks  one thread per changed tile projects the tile's box (primary rays) and the box
+//   blocks  one wave per changed tile projects the tile's box (primary rays) and the box
 //           swept against the sun over the shadow range (shadow rays) onto the screen and
 //           marks the covered 8x8 blocks (conservative: 2-pixel margins, the whole screen
 //           when a box straddles the camera plane).
@@ -27,21 +33,24 @@
 struct vg_edit_state {
     size_t cap_edits, cap_pairs;
     vg_edit *edits;          // device copy of the last batch
-    uint32_t *off;           // first pair of each edit (n + 1)
-    uint32_t *pk_in, *pk_out; // pair keys: nested brick index
-    uint32_t *pv_in, *pv_out; // pair values: edit index
+    uint32_t *cnt;           // per nested brick: pairs of the batch (zero between batches)
+    uint32_t *end;           // inclusive scan of cnt: a brick's run is [end[bn-1], end[bn])
+    uint32_t *list;          // edit index of every (brick, edit) pair, grouped by brick
     void *temp;
     size_t temp_bytes;
+    uint32_t *tile_flag;     // per nested tile: listed in this batch (zero between batches)
+    uint32_t *tile_list;     // nested tiles holding a brick some edit of the batch touches
+    uint32_t *tile_n;        // [0] entries of tile_list
     uint32_t *tile_dirty;    // bit per nested tile changed by the last batch
     uint32_t *chunk_dirty;   // bit per chunk
     uint32_t *l4_dirty;      // bit per L4 cell
     uint32_t *blist;         // dirty screen blocks of a masked frame
     uint32_t *blist_n;
     size_t blist_cap;
-    vg_edit *staging;        // pinned host staging of the edit batch
-    uint32_t *staging_off;
-    size_t staging_cap;
-    int key_bits;
+    vg_edit *staging[2];     // pinned host staging of the edit batch, double-buffered
+    hipEvent_t staged[2];    // the copy out of staging[i] has finished
+    bool staged_rec[2];
+    int slot;
 };
 
 namespace vg {
@@ -103,122 +112,175 @@ __host__ __device__ __forceinline__ bool sphere_touches_brick(const vg_edit &e, 
 
 // ------------------------------------------------------------------ kernels --
 __global__ void k_edit_clear(uint32_t *a, uint32_t na, uint32_t *b, uint32_t nb, uint32_t *c,
-                             uint32_t nc) {
+                             uint32_t nc, uint32_t *tile_n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < na) a[i] = 0u;
     if (i < nb) b[i] = 0u;
     if (i < nc) c[i] = 0u;
+    if (i == 0) *tile_n = 0u;
 }
 
-__global__ void k_edit_pairs(DevWorld w, const vg_edit *__restrict__ edits, uint32_t n,
-                             const uint32_t *__restrict__ off, uint32_t *__restrict__ keys,
-                             uint32_t *__restrict__ vals) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// One wave per edit, one lane per brick of its clipped brick box. COUNT: per-brick pair
+// counters; the first pair of a brick flags its tile and the first flag appends the tile to
+// the touched list. FILL: each pair takes a slot of its brick's run
+// [end[bn] - cnt[bn], end[bn]) and counts the counter back down (zero again afterwards).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_edit_enum(DevWorld w, const vg_edit *__restrict__ edits,
+                                                   uint32_t n, uint32_t *__restrict__ cnt,
+                                                   const uint32_t *__restrict__ end,
+                                                   uint32_t *__restrict__ list,
+                                                   uint32_t *__restrict__ tile_flag,
+                                                   uint32_t *__restrict__ tile_list,
+                                                   uint32_t *__restrict__ tile_n) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (i >= n) return; // wave-uniform
     const vg_edit e = edits[i];
     EditRange r;
     if (!edit_range(e, w.origin, w.dims, r)) return;
-    uint32_t o = off[i];
-    const uint32_t end = off[i + 1];
-    for (int32_t y = r.lo[1]; y <= r.hi[1]; ++y)
-        for (int32_t z = r.lo[2]; z <= r.hi[2]; ++z)
-            for (int32_t x = r.lo[0]; x <= r.hi[0]; ++x) {
-                const int32_t wb[3] = {x, y, z};
-                if (!sphere_touches_brick(e, wb)) continue;
-                int32_t sb[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    int32_t s = (wb[k] * 4) % w.dims[k];
-                    s = s < 0 ? s + w.dims[k] : s;
-                    sb[k] = s >> 2;
-                }
-                if (o < end) { // the host sized exactly this many
-                    keys[o] = brick_nested(w, sb);
-                    vals[o] = i;
-                }
-                ++o;
-            }
-}
-
-// One wave per chunk of 64 sorted pairs: every brick whose first pair lies in the chunk is
-// processed by this wave, its spheres in edit order.
-__global__ __launch_bounds__(256) void k_edit_bricks(DevWorld w, const vg_edit *__restrict__ edits,
-                                                     const uint32_t *__restrict__ keys,
-                                                     const uint32_t *__restrict__ vals,
-                                                     uint32_t n_pairs, uint64_t *__restrict__ occ,
-                                                     uint8_t *__restrict__ mat,
-                                                     uint8_t *__restrict__ bu,
-                                                     uint32_t *__restrict__ tile_dirty) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t chunk = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t p = chunk * 64u + lane;
-    if (chunk * 64u >= n_pairs) return; // wave-uniform
-    const bool head = p < n_pairs && (p == 0 || keys[p] != keys[p - 1]);
-    uint64_t heads = __ballot(head);
-    // voxel of this lane inside a brick: bit x + 4z + 16y
-    const int32_t vx = (int32_t)(lane & 3u), vy = (int32_t)(lane >> 4), vz = (int32_t)((lane >> 2) & 3u);
-    while (heads) {
-        const int src = __ffsll((long long)heads) - 1;
-        heads &= heads - 1;
-        const uint32_t hp = chunk * 64u + (uint32_t)src;
-        const uint32_t bn = keys[hp];
+    const uint32_t nx = (uint32_t)(r.hi[0] - r.lo[0] + 1), nz = (uint32_t)(r.hi[2] - r.lo[2] + 1);
+    const uint32_t vol = nx * nz * (uint32_t)(r.hi[1] - r.lo[1] + 1);
+    for (uint32_t j = lane; j < vol; j += 64u) {
+        const uint32_t t = j / nx;
+        const int32_t wb[3] = {r.lo[0] + (int32_t)(j - t * nx), r.lo[1] + (int32_t)(t / nz),
+                               r.lo[2] + (int32_t)(t % nz)};
+        if (!sphere_touches_brick(e, wb)) continue;
         int32_t sb[3];
-        brick_slot(w, bn, sb);
-        const int32_t wv[3] = {slot_to_world(sb[0] * 4 + vx, w.origin[0], w.dims[0]),
-                               slot_to_world(sb[1] * 4 + vy, w.origin[1], w.dims[1]),
-                               slot_to_world(sb[2] * 4 + vz, w.origin[2], w.dims[2])};
-        const uint64_t old_occ = occ[bn];
-        const bool old_bit = (old_occ >> lane) & 1u;
-        const uint32_t raw = mat[(size_t)bn * 64u + lane];
-        // a voxel some sphere covers takes the last covering sphere's material with
-        // Brick::set semantics (src/voxel.rs:315-325); the others keep their bit and byte
-        bool touched = false;
-        uint32_t m = raw;
-        for (uint32_t q = hp; q < n_pairs && keys[q] == bn; ++q) {
-            const vg_edit e = edits[vals[q]];
-            const int32_t dx = wv[0] - e.center[0], dy = wv[1] - e.center[1], dz = wv[2] - e.center[2];
-            if (dx * dx + dy * dy + dz * dz <= (int32_t)e.radius * (int32_t)e.radius) {
-                m = e.material;
-                touched = true;
-            }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t sv = (wb[k] * 4) % w.dims[k];
+            sv = sv < 0 ? sv + w.dims[k] : sv;
+            sb[k] = sv >> 2;
         }
-        const bool bit = touched ? m != 0u : old_bit;
-        const uint64_t new_occ = __ballot(bit);
-        const bool changed = __ballot(touched && (m != raw || bit != old_bit)) != 0;
-        const uint32_t m0 = __shfl(m, 0, 64);
-        const bool all_same = __ballot(m == m0) == ~0ull;
-        if (touched) mat[(size_t)bn * 64u + lane] = (uint8_t)m;
-        if (lane == 0) {
-            occ[bn] = new_occ;
-            bu[bn] = (new_occ == ~0ull && all_same && m0 != 0u) ? (uint8_t)m0 : (uint8_t)0;
-            if (changed) atomicOr(tile_dirty + ((bn >> 6) >> 5), 1u << ((bn >> 6) & 31u));
+        const uint32_t bn = brick_nested(w, sb);
+        if (FILL) {
+            const uint32_t c = atomicSub(cnt + bn, 1u);
+            list[end[bn] - c] = i;
+        } else if (atomicAdd(cnt + bn, 1u) == 0u) {
+            const uint32_t tn = bn >> 6;
+            if (atomicExch(tile_flag + tn, 1u) == 0u) tile_list[atomicAdd(tile_n, 1u)] = tn;
         }
     }
 }
 
-// One wave per nested tile: a dirty tile recomputes its word and uniform byte from its 64
-// bricks and marks its chunk.
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t k) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
+}
+
+// Blocks of 16 waves walk the touched-tile list; in a tile, wave k takes the touched bricks
+// k, k + 16, ... (in brick order), one lane per voxel. Each touched brick applies its edits: a
+// voxel some sphere covers takes the material of the covering edit of the highest index
+// with Brick::set semantics (src/voxel.rs:315-325), the others keep their bit and byte. The
+// tile's flag is cleared for the next batch.
+constexpr uint32_t kBrickWaves = 16;
+__global__ __launch_bounds__(1024) void k_edit_bricks(DevWorld w, const vg_edit *__restrict__ edits,
+                                                      const uint32_t *__restrict__ end,
+                                                      const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ tile_list,
+                                                      const uint32_t *__restrict__ tile_n,
+                                                      uint32_t *__restrict__ tile_flag,
+                                                      uint64_t *__restrict__ occ,
+                                                      uint8_t *__restrict__ mat,
+                                                      uint8_t *__restrict__ bu,
+                                                      uint32_t *__restrict__ tile_dirty) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t nt = *tile_n;
+    // voxel of this lane inside a brick: bit x + 4z + 16y
+    const int32_t vx = (int32_t)(lane & 3u), vy = (int32_t)(lane >> 4), vz = (int32_t)((lane >> 2) & 3u);
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const uint32_t tn = tile_list[t];
+        const uint32_t bl = tn * 64u + lane;
+        const uint32_t hi_l = end[bl];
+        uint32_t lo_l = __shfl_up(hi_l, 1, 64);
+        if (lane == 0) lo_l = bl ? end[bl - 1u] : 0u;
+        uint64_t todo = __ballot(hi_l > lo_l);
+        for (uint32_t k = 0; k < wave && todo; ++k) todo &= todo - 1; // skip to this wave's first
+        bool tile_changed = false;
+        while (todo) {
+            const uint32_t src = (uint32_t)(__ffsll((long long)todo) - 1);
+            for (uint32_t k = 0; k < kBrickWaves && todo; ++k) todo &= todo - 1;
+            const uint32_t bn = tn * 64u + src;
+            const uint32_t lo = lane_read(lo_l, src), hi = lane_read(hi_l, src);
+            int32_t sb[3];
+            brick_slot(w, bn, sb);
+            const int32_t wv[3] = {slot_to_world(sb[0] * 4 + vx, w.origin[0], w.dims[0]),
+                                   slot_to_world(sb[1] * 4 + vy, w.origin[1], w.dims[1]),
+                                   slot_to_world(sb[2] * 4 + vz, w.origin[2], w.dims[2])};
+            const uint64_t old_occ = occ[bn];
+            const bool old_bit = (old_occ >> lane) & 1u;
+            const uint32_t raw = mat[(size_t)bn * 64u + lane];
+            int32_t best = -1;
+            uint32_t m = raw;
+            for (uint32_t base = lo; base < hi; base += 64u) {
+                const uint32_t kn = hi - base < 64u ? hi - base : 64u;
+                uint32_t ei = 0, ex = 0, ey = 0, ez = 0, er = 0;
+                if (lane < kn) {
+                    ei = list[base + lane];
+                    const vg_edit e = edits[ei];
+                    ex = (uint32_t)e.center[0];
+                    ey = (uint32_t)e.center[1];
+                    ez = (uint32_t)e.center[2];
+                    er = ((uint32_t)e.radius * (uint32_t)e.radius) << 8 | (uint32_t)e.material;
+                }
+                for (uint32_t k = 0; k < kn; ++k) {
+                    const int32_t idx = (int32_t)lane_read(ei, k);
+                    const uint32_t rm = lane_read(er, k);
+                    const int32_t dx = wv[0] - (int32_t)lane_read(ex, k);
+                    const int32_t dy = wv[1] - (int32_t)lane_read(ey, k);
+                    const int32_t dz = wv[2] - (int32_t)lane_read(ez, k);
+                    if (dx * dx + dy * dy + dz * dz <= (int32_t)(rm >> 8) && idx > best) {
+                        best = idx;
+                        m = rm & 0xFFu;
+                    }
+                }
+            }
+            const bool touched = best >= 0;
+            const bool bit = touched ? m != 0u : old_bit;
+            const uint64_t new_occ = __ballot(bit);
+            tile_changed |= __ballot(touched && (m != raw || bit != old_bit)) != 0;
+            const uint32_t m0 = __shfl(m, 0, 64);
+            const bool all_same = __ballot(m == m0) == ~0ull;
+            if (touched) mat[(size_t)bn * 64u + lane] = (uint8_t)m;
+            if (lane == 0) {
+                occ[bn] = new_occ;
+                bu[bn] = (new_occ == ~0ull && all_same && m0 != 0u) ? (uint8_t)m0 : (uint8_t)0;
+            }
+        }
+        if (lane == 0) {
+            if (tile_changed) atomicOr(tile_dirty + (tn >> 5), 1u << (tn & 31u));
+            if (wave == 0) tile_flag[tn] = 0u;
+        }
+    }
+}
+
+// Waves walk the touched-tile list: a dirty tile recomputes its word and uniform byte from
+// its 64 bricks and marks its chunk.
 __global__ __launch_bounds__(256) void k_edit_tiles(const uint32_t *__restrict__ tile_dirty,
-                                                    uint32_t n_tiles, const uint64_t *__restrict__ occ,
+                                                    const uint32_t *__restrict__ tile_list,
+                                                    const uint32_t *__restrict__ tile_n,
+                                                    const uint64_t *__restrict__ occ,
                                                     const uint8_t *__restrict__ bu,
                                                     uint64_t *__restrict__ tile,
                                                     uint8_t *__restrict__ tu,
                                                     uint32_t *__restrict__ chunk_dirty) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t tn = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (tn >= n_tiles || !((tile_dirty[tn >> 5] >> (tn & 31u)) & 1u)) return; // wave-uniform
-    const uint32_t bn = tn * 64u + lane;
-    const uint64_t word = __ballot(occ[bn] != 0ull);
-    const uint32_t b = bu[bn];
-    const uint32_t b0 = __shfl(b, 0, 64);
-    const bool uni = b0 != 0u && __ballot(b == b0) == ~0ull;
-    if (lane == 0) {
-        const uint64_t prev = tile[tn];
-        tile[tn] = word;
-        tu[tn] = uni ? (uint8_t)b0 : (uint8_t)0;
-        if ((prev == 0ull) != (word == 0ull)) {
-            const uint32_t ci = tn >> 6;
-            atomicOr(chunk_dirty + (ci >> 5), 1u << (ci & 31u));
+    const uint32_t nt = *tile_n;
+    for (uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6); t < nt; t += gridDim.x * 4u) {
+        const uint32_t tn = tile_list[t];
+        if (!((tile_dirty[tn >> 5] >> (tn & 31u)) & 1u)) continue; // wave-uniform
+        const uint32_t bn = tn * 64u + lane;
+        const uint64_t word = __ballot(occ[bn] != 0ull);
+        const uint32_t b = bu[bn];
+        const uint32_t b0 = __shfl(b, 0, 64);
+        const bool uni = b0 != 0u && __ballot(b == b0) == ~0ull;
+        if (lane == 0) {
+            const uint64_t prev = tile[tn];
+            tile[tn] = word;
+            tu[tn] = uni ? (uint8_t)b0 : (uint8_t)0;
+            if ((prev == 0ull) != (word == 0ull)) {
+                const uint32_t ci = tn >> 6;
+                atomicOr(chunk_dirty + (ci >> 5), 1u << (ci & 31u));
+            }
         }
     }
 }
@@ -306,35 +368,36 @@ __device__ __forceinline__ int project_points(const ProjCam &c, const float (*p)
     return 1;
 }
 
+// Mark the 8x8 blocks a pixel rect touches; the wave's lanes share the (block row, mask
+// word) pairs of the rect, one atomic per pair.
 __device__ __forceinline__ void mark_rect(uint32_t *mask, uint32_t nbx, uint32_t nby, int32_t width,
-                                          int32_t height, const float rect[4]) {
+                                          int32_t height, const float rect[4], uint32_t lane) {
     const float xa = fmaxf(rect[0], 0.0f), ya = fmaxf(rect[1], 0.0f);
     const float xb = fminf(rect[2], (float)(width - 1)), yb = fminf(rect[3], (float)(height - 1));
     if (xa > xb || ya > yb) return;
-    const uint32_t bx0 = (uint32_t)xa / 8u, bx1 = (uint32_t)xb / 8u;
-    const uint32_t by0 = (uint32_t)ya / 8u, by1 = (uint32_t)yb / 8u;
-    for (uint32_t by = by0; by <= by1 && by < nby; ++by) {
-        uint32_t b = by * nbx + bx0;
-        const uint32_t end = by * nbx + (bx1 < nbx ? bx1 : nbx - 1u);
-        while (b <= end) {
-            const uint32_t word = b >> 5, first = b & 31u;
-            const uint32_t last = (end >> 5) == word ? (end & 31u) : 31u;
-            const uint32_t bits = (last == 31u ? ~0u : ((1u << (last + 1u)) - 1u)) & ~((1u << first) - 1u);
-            atomicOr(mask + word, bits);
-            b = word * 32u + last + 1u;
-        }
+    const uint32_t bx0 = (uint32_t)xa / 8u, bx1 = min((uint32_t)xb / 8u, nbx - 1u);
+    const uint32_t by0 = (uint32_t)ya / 8u, by1 = min((uint32_t)yb / 8u, nby - 1u);
+    const uint32_t span = (bx1 - bx0) / 32u + 2u; // words a row of the rect can touch
+    const uint32_t total = (by1 - by0 + 1u) * span;
+    for (uint32_t i = lane; i < total; i += 64u) {
+        const uint32_t by = by0 + i / span, j = i % span;
+        const uint32_t first_b = by * nbx + bx0, last_b = by * nbx + bx1;
+        const uint32_t word = (first_b >> 5) + j;
+        if (word > (last_b >> 5)) continue;
+        const uint32_t lo = word == (first_b >> 5) ? (first_b & 31u) : 0u;
+        const uint32_t hi = word == (last_b >> 5) ? (last_b & 31u) : 31u;
+        const uint32_t bits = (hi == 31u ? ~0u : ((1u << (hi + 1u)) - 1u)) & ~((1u << lo) - 1u);
+        atomicOr(mask + word, bits);
     }
 }
 
-// One thread per nested tile changed by the last edit batch: mark the 8x8 blocks whose
-// primary rays can enter the tile box, and (sun set) whose primary hits can cast a shadow ray
-// through it. mask[nwords] is the whole-screen flag.
-__global__ void k_edit_blocks(DevWorld w, const uint32_t *__restrict__ tile_dirty, uint32_t n_tiles,
-                              ProjCam c, int32_t width, int32_t height, uint32_t nbx, uint32_t nby,
-                              int shadow, float sx, float sy, float sz, float stmax,
-                              uint32_t *__restrict__ mask, uint32_t nwords) {
-    const uint32_t tn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tn >= n_tiles || !((tile_dirty[tn >> 5] >> (tn & 31u)) & 1u)) return;
+// One wave's share of the dirty-block mask for nested tile tn: mark the 8x8 blocks whose
+// primary rays can enter the tile box, and (shadow set) whose primary hits can cast a shadow
+// ray through it. mask[nwords] is the whole-screen flag.
+__device__ void tile_blocks(const DevWorld &w, uint32_t tn, const ProjCam &c, int32_t width,
+                            int32_t height, uint32_t nbx, uint32_t nby, int shadow, float sx,
+                            float sy, float sz, float stmax, uint32_t *mask, uint32_t nwords,
+                            uint32_t lane) {
     int32_t sb[3];
     brick_slot(w, tn * 64u, sb); // the tile's first brick
     float lo[3], hi[3];
@@ -353,10 +416,10 @@ __global__ void k_edit_blocks(DevWorld w, const uint32_t *__restrict__ tile_dirt
     float rect[4];
     int r = project_points(c, pts, 8, rect);
     if (r == 2) {
-        mask[nwords] = 1u;
+        if (lane == 0) mask[nwords] = 1u;
         return;
     }
-    if (r == 1) mark_rect(mask, nbx, nby, width, height, rect);
+    if (r == 1) mark_rect(mask, nbx, nby, width, height, rect, lane);
     if (!shadow) return;
     // hit points p with p + s t inside the box for some t in [0, stmax]: the box swept by
     // -s t, t clipped to where the swept box still meets the window
@@ -374,10 +437,29 @@ __global__ void k_edit_blocks(DevWorld w, const uint32_t *__restrict__ tile_dirt
         for (int k = 0; k < 3; ++k) pts[8 + i][k] = pts[i][k] - s[k] * tmax;
     r = project_points(c, pts, 16, rect);
     if (r == 2) {
-        mask[nwords] = 1u;
+        if (lane == 0) mask[nwords] = 1u;
         return;
     }
-    if (r == 1) mark_rect(mask, nbx, nby, width, height, rect);
+    if (r == 1) mark_rect(mask, nbx, nby, width, height, rect, lane);
+}
+
+// Waves walk the touched-tile list of the last edit batch and mark the blocks of its changed
+// tiles; once the whole-screen flag is set the remaining tiles have nothing to add.
+__global__ __launch_bounds__(256) void k_edit_blocks(DevWorld w, const uint32_t *__restrict__ tile_dirty,
+                                                     const uint32_t *__restrict__ tile_list,
+                                                     const uint32_t *__restrict__ tile_n, ProjCam c,
+                                                     int32_t width, int32_t height, uint32_t nbx,
+                                                     uint32_t nby, int shadow, float sx, float sy,
+                                                     float sz, float stmax, uint32_t *mask,
+                                                     uint32_t nwords) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nt = *tile_n;
+    for (uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6); t < nt; t += gridDim.x * 4u) {
+        const uint32_t tn = tile_list[t];
+        if (!((tile_dirty[tn >> 5] >> (tn & 31u)) & 1u)) continue; // wave-uniform
+        if (__shfl(__atomic_load_n(mask + nwords, __ATOMIC_RELAXED), 0, 64) != 0u) return;
+        tile_blocks(w, tn, c, width, height, nbx, nby, shadow, sx, sy, sz, stmax, mask, nwords, lane);
+    }
 }
 
 __global__ void k_mask_clear(uint32_t *mask, uint32_t n) {
@@ -441,12 +523,14 @@ __global__ void k_download_tiles(const uint64_t *__restrict__ tile, const uint8_
 
 // --------------------------------------------------------------------- host --
 static void free_edit_state(vg_edit_state *s) {
-    void *dev[] = {s->edits, s->off, s->pk_in, s->pk_out, s->pv_in, s->pv_out, s->temp,
-                   s->tile_dirty, s->chunk_dirty, s->l4_dirty, s->blist, s->blist_n};
+    void *dev[] = {s->edits, s->cnt, s->end, s->list, s->temp, s->tile_flag, s->tile_list,
+                   s->tile_n, s->tile_dirty, s->chunk_dirty, s->l4_dirty, s->blist, s->blist_n};
     for (void *p : dev)
         if (p) (void)hipFree(p);
-    if (s->staging) (void)hipHostFree(s->staging);
-    if (s->staging_off) (void)hipHostFree(s->staging_off);
+    for (int i = 0; i < 2; ++i) {
+        if (s->staging[i]) (void)hipHostFree(s->staging[i]);
+        if (s->staged[i]) (void)hipEventDestroy(s->staged[i]);
+    }
     delete s;
 }
 
@@ -455,21 +539,39 @@ static int edit_state(vg_world *w, vg_edit_state **out) {
         vg_edit_state *s = new vg_edit_state();
         std::memset(s, 0, sizeof(*s));
         const Dims &d = w->d;
-        s->key_bits = 1;
-        while ((1ll << s->key_bits) < d.n_bricks) ++s->key_bits;
         const size_t tw = (size_t)(d.n_tiles + 31) / 32, cw = (size_t)(d.n_chunks + 31) / 32,
                      lw = (size_t)(d.n_l4 + 31) / 32;
-        bool ok = hipMalloc((void **)&s->tile_dirty, tw * 4) == hipSuccess &&
+        bool ok = hipMalloc((void **)&s->tile_flag, (size_t)d.n_tiles * 4) == hipSuccess &&
+                  hipMalloc((void **)&s->tile_list, (size_t)d.n_tiles * 4) == hipSuccess &&
+                  hipMalloc((void **)&s->tile_n, 256) == hipSuccess &&
+                  hipMalloc((void **)&s->tile_dirty, tw * 4) == hipSuccess &&
                   hipMalloc((void **)&s->chunk_dirty, cw * 4) == hipSuccess &&
                   hipMalloc((void **)&s->l4_dirty, lw * 4) == hipSuccess &&
-                  hipMalloc((void **)&s->blist_n, 256) == hipSuccess;
-        if (ok) ok = hipMemset(s->tile_dirty, 0, tw * 4) == hipSuccess &&
+                  hipMalloc((void **)&s->blist_n, 256) == hipSuccess &&
+                  hipMalloc((void **)&s->cnt, (size_t)d.n_bricks * 4) == hipSuccess &&
+                  hipMalloc((void **)&s->end, (size_t)d.n_bricks * 4) == hipSuccess;
+        if (ok) ok = hipMemset(s->tile_flag, 0, (size_t)d.n_tiles * 4) == hipSuccess &&
+                     hipMemset(s->tile_n, 0, 256) == hipSuccess &&
+                     hipMemset(s->tile_dirty, 0, tw * 4) == hipSuccess &&
                      hipMemset(s->chunk_dirty, 0, cw * 4) == hipSuccess &&
-                     hipMemset(s->l4_dirty, 0, lw * 4) == hipSuccess;
+                     hipMemset(s->l4_dirty, 0, lw * 4) == hipSuccess &&
+                     hipMemset(s->cnt, 0, (size_t)d.n_bricks * 4) == hipSuccess;
+        for (int i = 0; i < 2 && ok; ++i)
+            ok = hipEventCreateWithFlags(&s->staged[i], hipEventDisableTiming) == hipSuccess;
+        size_t tb = 0;
+        if (ok) ok = hipcub::DeviceScan::InclusiveSum(nullptr, tb, s->cnt, s->end, (int)d.n_bricks) ==
+                     hipSuccess;
+        if (ok) ok = hipMalloc(&s->temp, tb) == hipSuccess;
         if (!ok) {
             free_edit_state(s);
             return fail(VG_E_NOMEM, "edit state");
         }
+        // the counters must be zero before the first batch's kernels run on any stream
+        if (hipDeviceSynchronize() != hipSuccess) {
+            free_edit_state(s);
+            return fail(VG_E_DEVICE, "edit state");
+        }
+        s->temp_bytes = tb;
         w->edit = s;
     }
     *out = w->edit;
@@ -477,45 +579,31 @@ static int edit_state(vg_world *w, vg_edit_state **out) {
 }
 
 static int ensure_edit_capacity(vg_edit_state *s, size_t n_edits, size_t n_pairs) {
-    if (n_edits > s->cap_edits || n_edits + 1 > s->staging_cap) {
+    if (n_edits > s->cap_edits) {
         HIP_TRY(hipDeviceSynchronize());
         if (s->edits) (void)hipFree(s->edits);
-        if (s->off) (void)hipFree(s->off);
-        if (s->staging) (void)hipHostFree(s->staging);
-        if (s->staging_off) (void)hipHostFree(s->staging_off);
         s->edits = nullptr;
-        s->off = nullptr;
-        s->staging = nullptr;
-        s->staging_off = nullptr;
-        s->cap_edits = s->staging_cap = 0;
+        for (int i = 0; i < 2; ++i) {
+            if (s->staging[i]) (void)hipHostFree(s->staging[i]);
+            s->staging[i] = nullptr;
+            s->staged_rec[i] = false;
+        }
+        s->cap_edits = 0;
         const size_t cap = std::max<size_t>(n_edits, 4096);
         if (hipMalloc((void **)&s->edits, cap * sizeof(vg_edit)) != hipSuccess ||
-            hipMalloc((void **)&s->off, (cap + 1) * sizeof(uint32_t)) != hipSuccess ||
-            hipHostMalloc((void **)&s->staging, cap * sizeof(vg_edit)) != hipSuccess ||
-            hipHostMalloc((void **)&s->staging_off, (cap + 1) * sizeof(uint32_t)) != hipSuccess)
+            hipHostMalloc((void **)&s->staging[0], cap * sizeof(vg_edit)) != hipSuccess ||
+            hipHostMalloc((void **)&s->staging[1], cap * sizeof(vg_edit)) != hipSuccess)
             return fail(VG_E_NOMEM, "edit buffers for %zu edits", cap);
         s->cap_edits = cap;
-        s->staging_cap = cap + 1;
     }
     if (n_pairs > s->cap_pairs) {
         HIP_TRY(hipDeviceSynchronize());
-        void *ptrs[] = {s->pk_in, s->pk_out, s->pv_in, s->pv_out, s->temp};
-        for (void *p : ptrs)
-            if (p) (void)hipFree(p);
-        s->pk_in = s->pk_out = s->pv_in = s->pv_out = nullptr;
-        s->temp = nullptr;
+        if (s->list) (void)hipFree(s->list);
+        s->list = nullptr;
         s->cap_pairs = 0;
         const size_t cap = std::max<size_t>(n_pairs + n_pairs / 4, 1u << 16);
-        bool ok = hipMalloc((void **)&s->pk_in, cap * 4) == hipSuccess &&
-                  hipMalloc((void **)&s->pk_out, cap * 4) == hipSuccess &&
-                  hipMalloc((void **)&s->pv_in, cap * 4) == hipSuccess &&
-                  hipMalloc((void **)&s->pv_out, cap * 4) == hipSuccess;
-        if (!ok) return fail(VG_E_NOMEM, "edit pairs (%zu)", cap);
-        size_t tb = 0;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, s->pk_in, s->pk_out, s->pv_in,
-                                                   s->pv_out, (int)cap, 0, s->key_bits));
-        if (hipMalloc(&s->temp, tb) != hipSuccess) return fail(VG_E_NOMEM, "sort scratch");
-        s->temp_bytes = tb;
+        if (hipMalloc((void **)&s->list, cap * 4) != hipSuccess)
+            return fail(VG_E_NOMEM, "edit pairs (%zu)", cap);
         s->cap_pairs = cap;
     }
     return VG_OK;
@@ -564,56 +652,59 @@ int vg_world_apply_edits(vg_world *w, const vg_edit *edits, uint32_t n, void *st
     if (st) return st;
     hipStream_t hs = (hipStream_t)stream;
     const Dims &d = w->d;
-    // pair counts on the host (exact), validated edits
-    std::vector<uint32_t> off((size_t)n + 1, 0);
-    uint64_t total = 0;
+    // validate, and bound the (brick, edit) pairs by the clipped brick boxes
+    uint64_t bound = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        if (edits[i].radius > kMaxEditRadius)
-            return fail(VG_E_INVALID, "edit %u: radius %u above %d", i, edits[i].radius,
-                        kMaxEditRadius);
-        off[i] = (uint32_t)total;
+        const vg_edit &e = edits[i];
+        if (e.radius > kMaxEditRadius)
+            return fail(VG_E_INVALID, "edit %u: radius %u above %d", i, e.radius, kMaxEditRadius);
+        for (int k = 0; k < 3; ++k)
+            if (e.center[k] < -(1 << 30) || e.center[k] > (1 << 30))
+                return fail(VG_E_INVALID, "edit %u: centre out of range", i);
         EditRange r;
-        if (!edit_range(edits[i], w->dw.origin, w->dw.dims, r)) continue;
-        for (int32_t y = r.lo[1]; y <= r.hi[1]; ++y)
-            for (int32_t z = r.lo[2]; z <= r.hi[2]; ++z)
-                for (int32_t x = r.lo[0]; x <= r.hi[0]; ++x) {
-                    const int32_t wb[3] = {x, y, z};
-                    if (sphere_touches_brick(edits[i], wb)) ++total;
-                }
-        if (total > 0x7FFFFFFFull) return fail(VG_E_INVALID, "too many touched bricks");
+        if (!edit_range(e, w->dw.origin, w->dw.dims, r)) continue;
+        bound += (uint64_t)(r.hi[0] - r.lo[0] + 1) * (uint64_t)(r.hi[1] - r.lo[1] + 1) *
+                 (uint64_t)(r.hi[2] - r.lo[2] + 1);
     }
-    off[n] = (uint32_t)total;
-    st = ensure_edit_capacity(s, n, (size_t)total);
+    if (bound > (1ull << 30))
+        return fail(VG_E_INVALID, "edit batch spans %llu brick slots (at most 2^30): split it",
+                    (unsigned long long)bound);
+    st = ensure_edit_capacity(s, n, (size_t)bound);
     if (st) return st;
     const uint32_t tw = (uint32_t)((d.n_tiles + 31) / 32), cw = (uint32_t)((d.n_chunks + 31) / 32),
                    lw = (uint32_t)((d.n_l4 + 31) / 32);
     const uint32_t mw = std::max(tw, std::max(cw, lw));
     hipLaunchKernelGGL(k_edit_clear, dim3((mw + 255) / 256), dim3(256), 0, hs, s->tile_dirty, tw,
-                       s->chunk_dirty, cw, s->l4_dirty, lw);
+                       s->chunk_dirty, cw, s->l4_dirty, lw, s->tile_n);
     HIP_TRY(hipGetLastError());
-    if (total == 0) return VG_OK;
-    // the staging buffers are reused: the previous batch's copies must have finished
-    HIP_TRY(hipStreamSynchronize(hs));
-    std::memcpy(s->staging, edits, (size_t)n * sizeof(vg_edit));
-    std::memcpy(s->staging_off, off.data(), off.size() * sizeof(uint32_t));
-    HIP_TRY(hipMemcpyAsync(s->edits, s->staging, (size_t)n * sizeof(vg_edit),
+    if (bound == 0) return VG_OK;
+    // staging[slot] is rewritten only once the copy that last read it has finished
+    const int slot = s->slot;
+    s->slot ^= 1;
+    if (s->staged_rec[slot]) HIP_TRY(hipEventSynchronize(s->staged[slot]));
+    std::memcpy(s->staging[slot], edits, (size_t)n * sizeof(vg_edit));
+    HIP_TRY(hipMemcpyAsync(s->edits, s->staging[slot], (size_t)n * sizeof(vg_edit),
                            hipMemcpyHostToDevice, hs));
-    HIP_TRY(hipMemcpyAsync(s->off, s->staging_off, off.size() * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, hs));
-    hipLaunchKernelGGL(k_edit_pairs, dim3((n + 255) / 256), dim3(256), 0, hs, w->dw, s->edits, n,
-                       s->off, s->pk_in, s->pv_in);
+    HIP_TRY(hipEventRecord(s->staged[slot], hs));
+    s->staged_rec[slot] = true;
+    const dim3 eg((n + 3) / 4);
+    hipLaunchKernelGGL(k_edit_enum<false>, eg, dim3(256), 0, hs, w->dw, s->edits, n, s->cnt,
+                       s->end, s->list, s->tile_flag, s->tile_list, s->tile_n);
     HIP_TRY(hipGetLastError());
     size_t tb = s->temp_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(s->temp, tb, s->pk_in, s->pk_out, s->pv_in,
-                                               s->pv_out, (int)total, 0, s->key_bits, hs));
-    const uint32_t chunks = (uint32_t)((total + 63) / 64);
-    hipLaunchKernelGGL(k_edit_bricks, dim3((chunks + 3) / 4), dim3(256), 0, hs, w->dw, s->edits,
-                       s->pk_out, s->pv_out, (uint32_t)total, w->occ, w->mat, w->brick_uni,
-                       s->tile_dirty);
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(s->temp, tb, s->cnt, s->end, (int)d.n_bricks, hs));
+    hipLaunchKernelGGL(k_edit_enum<true>, eg, dim3(256), 0, hs, w->dw, s->edits, n, s->cnt, s->end,
+                       s->list, s->tile_flag, s->tile_list, s->tile_n);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_edit_tiles, dim3((uint32_t)((d.n_tiles + 3) / 4)), dim3(256), 0, hs,
-                       s->tile_dirty, (uint32_t)d.n_tiles, w->occ, w->brick_uni, w->tile,
-                       w->tile_uni, s->chunk_dirty);
+    // touched tiles are at most n_tiles; the list walkers use machine-sized grids
+    const uint32_t tb_grid = (uint32_t)std::min<int64_t>(d.n_tiles, 2048);
+    hipLaunchKernelGGL(k_edit_bricks, dim3(tb_grid), dim3(1024), 0, hs, w->dw, s->edits, s->end,
+                       s->list, s->tile_list, s->tile_n, s->tile_flag, w->occ, w->mat,
+                       w->brick_uni, s->tile_dirty);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_edit_tiles, dim3((uint32_t)std::min<int64_t>((d.n_tiles + 3) / 4, 1024)),
+                       dim3(256), 0, hs, s->tile_dirty, s->tile_list, s->tile_n, w->occ,
+                       w->brick_uni, w->tile, w->tile_uni, s->chunk_dirty);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_edit_chunks, dim3((uint32_t)((d.n_chunks + 3) / 4)), dim3(256), 0, hs,
                        s->chunk_dirty, w->dw, (uint32_t)d.n_chunks, w->tile, w->chunk, s->l4_dirty);
@@ -641,8 +732,9 @@ int vg_frame_dirty_blocks(vg_world *w, const vg_camera *camera, int32_t width, i
     HIP_TRY(hipGetLastError());
     float sd[3] = {0.0f, 0.0f, 0.0f};
     const int shadow = sun != nullptr && host_normalize(sun, sd);
-    hipLaunchKernelGGL(k_edit_blocks, dim3((uint32_t)((w->d.n_tiles + 255) / 256)), dim3(256), 0, hs,
-                       w->dw, s->tile_dirty, (uint32_t)w->d.n_tiles, proj_cam(camera, width, height),
+    hipLaunchKernelGGL(k_edit_blocks, dim3((uint32_t)std::min<int64_t>((w->d.n_tiles + 3) / 4, 1024)),
+                       dim3(256), 0, hs, w->dw, s->tile_dirty, s->tile_list, s->tile_n,
+                       proj_cam(camera, width, height),
                        width, height, nbx, nby, shadow, sd[0], sd[1], sd[2], shadow_tmax, mask,
                        nwords);
     HIP_TRY(hipGetLastError());
