@@ -56,6 +56,22 @@ def cpu_baseline(world, wl, cam, sun, budget_s=12.0):
                       f"rays, R0 = per-voxel DDA of src/raycast.rs, {dt:.1f} s"}
 
 
+TRAFFIC_PROFILE = "profiles/k_frame_persistent_large_1080p_pmc.md"
+
+
+def measured_traffic():
+    """DRAM-side bytes per frame of k_beam + k_frame_persistent on this workload, from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (copy-calibrated); None when the
+    summary is absent. PMC counters cannot be read from inside the timed process itself."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    try:
+        m = re.search(r"^traffic_bytes_per_frame: (\d+)$", open(path).read(), re.M)
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +212,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+            "traffic_source": TRAFFIC_PROFILE if measured_traffic() is not None else None,
             "kernel": "k_beam + k_frame_persistent", "kernel_ms": round(avg_kern_s * 1e3, 4),
             "algorithmic_bytes": int(bytes_per_frame),
         },

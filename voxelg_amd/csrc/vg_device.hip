@@ -517,6 +517,14 @@ __global__ void k_relayout_bricks(const vg_brick *__restrict__ staged,
     bu[bn] = bu_ref[i]; // the staging buffer holds this slab only
 }
 
+// Plain 16-byte-per-lane copy: the calibration kernel of known traffic (bytes read = bytes
+// written = n16 * 16) that HBM counters of the other kernels are scaled by.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                size_t n16) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_relayout_tiles(const uint64_t *__restrict__ tile_ref,
                                  const uint8_t *__restrict__ tu_ref, int64_t n, int32_t tx_n,
                                  int32_t ty_n, int32_t cx_n, int32_t cy_n,
@@ -1044,6 +1052,15 @@ int vg_copy_device(vg_ctx *ctx, void *dst, const void *src, size_t bytes) {
     CTX_DEVICE(ctx);
     if (bytes == 0) return VG_OK;
     if (!dst || !src) return fail(VG_E_INVALID, "null copy pointer");
+    if (bytes % 16 == 0 && (uintptr_t)dst % 16 == 0 && (uintptr_t)src % 16 == 0) {
+        const size_t n16 = bytes / 16;
+        const size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)ctx->cus * 16);
+        hipLaunchKernelGGL(k_copy16, dim3((uint32_t)blocks), dim3(256), 0, 0,
+                           (const uint4 *)src, (uint4 *)dst, n16);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+        return VG_OK;
+    }
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return VG_OK;
 }
