@@ -297,8 +297,12 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
         }
         do {
             if (kind != 0) {
-                bool done = kind == 1 ? walk_iter<COUNT>(r, q, w, ptmax, pflags, plod, h, cnt)
-                                      : walk_iter<COUNT>(r, q, w, stmax, VG_RAY_ANY_HIT, false, h, cnt);
+                // one inlined walk for both kinds (per-lane parameters), so a wave holding
+                // primary and shadow rays at once does not execute two walk bodies
+                const bool prim = kind == 1;
+                bool done = walk_iter<COUNT>(r, q, w, prim ? ptmax : stmax,
+                                             prim ? pflags : (uint32_t)VG_RAY_ANY_HIT, prim && plod,
+                                             h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     if (kind == 1) {
