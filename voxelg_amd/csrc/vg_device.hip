@@ -359,19 +359,17 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
     HitRec h;
     hit_miss(h);
     cache_reset(q);
-    bool exhausted = false;
+    IdQueue iq;
+    queue_reset(iq);
     for (;;) {
         uint64_t act = __ballot(busy);
         int nact = __popcll(act);
-        if (!exhausted && nact < threshold) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
-            base = __shfl(base, 0, 64);
-            if (base >= n) {
-                exhausted = true;
-            } else if (!busy) {
-                id = base + lanes_below(~act);
-                if (id < n) {
+        if (!queue_empty(iq) && nact < threshold) {
+            const uint32_t nid = queue_take(iq, work, n, (uint32_t)(64 - nact), lanes_below(~act),
+                                            !busy, lane);
+            if (!busy) {
+                if (nid != ~0u) {
+                    id = nid;
                     const float4 *rp = reinterpret_cast<const float4 *>(rays + id);
                     const float4 a = rp[0], b = rp[1];
                     const float o[3] = {a.x, a.y, a.z}, din[3] = {b.x, b.y, b.z};
@@ -392,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
             nact = __popcll(__ballot(busy));
         }
         if (nact == 0) {
-            if (exhausted) break;
+            if (queue_empty(iq)) break;
             continue;
         }
         do {
@@ -437,19 +435,16 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
     HitRec h;
     hit_miss(h);
     cache_reset(q);
-    bool exhausted = false;
+    IdQueue iq;
+    queue_reset(iq);
     for (;;) {
         uint64_t act = __ballot(busy);
         int nact = __popcll(act);
-        if (!exhausted && nact < threshold) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
-            base = __shfl(base, 0, 64);
-            if (base >= total) {
-                exhausted = true;
-            } else if (!busy) {
-                const uint32_t id = base + lanes_below(~act);
-                if (id < total) {
+        if (!queue_empty(iq) && nact < threshold) {
+            const uint32_t id = queue_take(iq, work, total, (uint32_t)(64 - nact), lanes_below(~act),
+                                           !busy, lane);
+            if (!busy) {
+                if (id != ~0u) {
                     uint32_t lo = 0, hi = nbat; // last batch with prefix <= id
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -481,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
             nact = __popcll(__ballot(busy));
         }
         if (nact == 0) {
-            if (exhausted) break;
+            if (queue_empty(iq)) break;
             continue;
         }
         do {
@@ -586,7 +581,7 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
                                    const uint32_t *list_n = nullptr) {
     const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
     const uint64_t nblocks = (uint64_t)nbx * nby;
-    // the counter may overshoot the ray count by 64 per wave after it runs out
+    // the counter may overshoot the ray count by one id chunk per wave after it runs out
     if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     if (beam) {
@@ -693,7 +688,7 @@ static int launch_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags
     if (!rays || !hits) return fail(VG_E_INVALID, "null rays/hits");
     if (((uintptr_t)rays & 15u) != 0) return fail(VG_E_INVALID, "rays must be 16-byte aligned");
     if (((uintptr_t)hits & 3u) != 0) return fail(VG_E_INVALID, "hits must be 4-byte aligned");
-    // the counter may overshoot the ray count by 64 per wave after it runs out
+    // the counter may overshoot the ray count by one id chunk per wave after it runs out
     if (n > 0xF0000000ull) return fail(VG_E_INVALID, "too many rays");
     if (w->ctx->schedule == VG_SCHEDULE_PERSISTENT) {
         hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, w->work);

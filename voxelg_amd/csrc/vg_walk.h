@@ -569,4 +569,58 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Wave-local reservation of work ids for the persistent batched-query kernels: a refill
+// hands its idle lanes the next ids of the reservation and takes a new chunk of kIdChunk
+// ids from the global counter (one atomic per chunk, not per refill) only when the
+// reservation runs short. Single-address atomics serialise across the chip, so fewer of
+// them lets waves refill more often. (The frame kernel keeps one atomic per refill: its
+// refills must come from the front of the screen order to stay coherent.) All members are
+// wave-uniform.
+constexpr uint32_t kIdChunk = 128;
+struct IdQueue {
+    uint32_t next, end;
+    bool drained; // the global counter has passed `total`
+};
+
+__device__ __forceinline__ void queue_reset(IdQueue &iq) {
+    iq.next = iq.end = 0u;
+    iq.drained = false;
+}
+
+__device__ __forceinline__ bool queue_empty(const IdQueue &iq) {
+    return iq.drained && iq.next >= iq.end;
+}
+
+// Ids for `need` (<= 64) idle lanes: the idle lane of rank `rank` gets the returned id when
+// `want` is set (~0u: none left). Every lane of the wave must call it together.
+__device__ __forceinline__ uint32_t queue_take(IdQueue &iq, uint32_t *work, uint32_t total,
+                                               uint32_t need, uint32_t rank, bool want,
+                                               uint32_t lane) {
+    const uint32_t avail = iq.end - iq.next;
+    uint32_t b = ~0u;
+    if (avail < need && !iq.drained) {
+        if (lane == 0) b = atomicAdd(work, kIdChunk);
+        b = __shfl(b, 0, 64);
+        if (b >= total) {
+            iq.drained = true;
+            b = ~0u;
+        }
+    }
+    uint32_t id = ~0u;
+    if (want) {
+        if (rank < avail) id = iq.next + rank;
+        else if (b != ~0u) id = b + (rank - avail);
+        if (id >= total) id = ~0u;
+    }
+    if (b != ~0u) {
+        const uint32_t end = total - b < kIdChunk ? total : b + kIdChunk;
+        const uint32_t nx = b + (need - avail);
+        iq.next = nx < end ? nx : end;
+        iq.end = end;
+    } else {
+        iq.next += avail < need ? avail : need;
+    }
+    return id;
+}
+
 } // namespace vg
