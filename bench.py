@@ -59,17 +59,27 @@ def cpu_baseline(world, wl, cam, sun, budget_s=12.0):
 TRAFFIC_PROFILE = "profiles/k_frame_persistent_large_1080p_pmc.md"
 
 
+COUNTERS_PROFILE = "profiles/cache_hit_rates_and_mfma_util_pmc.md"
+
+
+def profile_value(profile, key):
+    """A figure recorded as `key: value` in a committed rocprofv3 --pmc summary (counters
+    cannot be read from inside the timed process itself); None when absent."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), profile)
+    try:
+        m = re.search(r"^" + re.escape(key) + r": ([0-9.eE+-]+)$", open(path).read(), re.M)
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def measured_traffic():
     """DRAM-side bytes per frame of k_beam + k_frame_persistent on this workload, from the
     committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (copy-calibrated); None when the
-    summary is absent. PMC counters cannot be read from inside the timed process itself."""
-    import re
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
-    try:
-        m = re.search(r"^traffic_bytes_per_frame: (\d+)$", open(path).read(), re.M)
-    except OSError:
-        return None
-    return int(m.group(1)) if m else None
+    summary is absent."""
+    v = profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_frame")
+    return None if v is None else int(v)
 
 
 def main():
@@ -214,6 +224,8 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
             "traffic_source": TRAFFIC_PROFILE if measured_traffic() is not None else None,
+            "l2_hit_rate": profile_value(COUNTERS_PROFILE, "l2_hit_rate_frame"),
+            "l2_hit_rate_source": COUNTERS_PROFILE,
             "kernel": "k_beam + k_frame_persistent", "kernel_ms": round(avg_kern_s * 1e3, 4),
             "algorithmic_bytes": int(bytes_per_frame),
         },
@@ -454,6 +466,9 @@ def bench_gi_main(args):
                                "per update) vs plain 256 rays/probe",
                    "world": list(wl.dims)},
         "gi": g,
+        "cache_mfma_util": {"train": profile_value(COUNTERS_PROFILE, "mfma_util_train"),
+                            "infer": profile_value(COUNTERS_PROFILE, "mfma_util_infer"),
+                            "source": COUNTERS_PROFILE},
     }
     print(json.dumps(out), flush=True)
 
