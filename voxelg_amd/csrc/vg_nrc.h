@@ -25,13 +25,18 @@ struct vg_nrc {
     uint32_t *feat;         // training batch encoding: bf16 pair per (sample, level)
     float *dfeat;           // ... its gradient: 32 fp32 per sample
     int32_t fcap;           // samples the training buffers hold
-    // grid-gradient records (samples x levels x 8 corners): entry keys and fp32 pairs, their
-    // sorted copies, the reduce-by-key runs (unique entries, sums, count) and the scratch
-    uint32_t *ck_in, *ck_out, *ck_uniq, *c_runs;
-    float2 *cv_in, *cv_out, *cv_agg;
-    void *ctemp;
+    // grid gradient (samples x levels x 8 corner records): records grouped by entry bucket
+    // (2^gbits entries each) through per-block bucket counts and their exclusive scan, then
+    // summed per bucket in LDS in 64-bit fixed point (exact, so any order gives the same sums)
+    uint32_t *rk;           // record: entry index inside its bucket
+    float2 *rv;             // record: w * d loss / d feature pair
+    uint32_t *hist;         // per (bucket, count block) record counts, scanned in place
+    uint32_t *gmax;         // max |d loss / d feature| of the step (fp32 bits)
+    float2 *gdense;         // dense grid gradient, for vg_nrc_gradients only (lazy)
+    void *ctemp;            // scan scratch
     size_t ctemp_bytes;
-    int key_bits;           // bits of an entry key (the sentinel n_entries included)
+    int gbits;              // log2 entries per bucket
+    int gbuckets;           // levels << (log2_table - gbits)
 };
 
 namespace vg {
@@ -48,9 +53,10 @@ constexpr int kNrcPartial = VG_NRC_MLP_PARAMS + 16; // floats per block partial 
 int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const uint32_t *sel,
                      const uint32_t *n_dev, uint32_t n_max, float4 *out4, float4 *rad,
                      const float4 *pal, hipStream_t stream);
-// Loss gradient of n queries into the partial buffer and the grid gradient, reduced to
-// mlp_g; with update, the optimizer step follows (grid gradient cleared by it).
+// Loss gradient of n queries: the MLP gradient reduced to mlp_g, the grid gradient summed
+// per entry; with update, the optimizer step over both follows; with dense, the grid
+// gradient of every entry is also written there (n_entries float pairs).
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
-                     hipStream_t stream);
+                     hipStream_t stream, float2 *dense = nullptr);
 
 } // namespace vg

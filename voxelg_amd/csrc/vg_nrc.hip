@@ -14,10 +14,11 @@
 // lane encodes hash levels {2g, 2g+1, 8+2g, 9+2g} of its sample.
 // Training stages a layer's input and output gradient (bf16) through LDS to form the
 // weight gradient dW = dY X^T as MFMAs over samples; per-block partial sums are reduced in
-// a second kernel (deterministic). The grid gradient is built without atomics: every
-// (sample, level, corner) contribution becomes an (entry, fp32 pair) record, a stable radix
-// sort (rocPRIM through hipCUB) groups them by entry, a reduce-by-key sums each group, and
-// Adam then updates exactly the entries that received a gradient.
+// a second kernel (deterministic). The grid gradient: every (sample, level, corner)
+// contribution becomes a record that a counting pass (per-block bucket counts, one exclusive
+// scan) places in its bucket of 4096 entries; one block per bucket sums its records in LDS
+// in 64-bit fixed point, which is exact and therefore independent of the record order, and
+// runs Adam on exactly the entries that received a gradient.
 #include "vg_nrc.h"
 
 #include <hipcub/hipcub.hpp>
@@ -333,10 +334,13 @@ __device__ __forceinline__ void back_layer(const u32x4_t *sbf, int lyr, int nks,
 }
 
 // Encoding of a training batch: one thread per (sample, level) writes the level's bf16
-// feature pair, feat[s * 16 + l] (zero for levels beyond the config).
+// feature pair, feat[s * 16 + l] (zero for levels beyond the config); clears the step's
+// gradient maximum.
 __global__ __launch_bounds__(256) void k_nrc_encode(NrcDev m, const vg_nrc_query *__restrict__ q,
-                                                    uint32_t n, uint32_t *__restrict__ feat) {
+                                                    uint32_t n, uint32_t *__restrict__ feat,
+                                                    uint32_t *gmax) {
     const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (t == 0) *gmax = 0u; // the step's gradient maximum is gathered after the backward pass
     const uint32_t s = (uint32_t)(t >> 4), l = (uint32_t)(t & 15u);
     if (s >= n) return;
     float f0 = 0.0f, f1 = 0.0f;
@@ -348,43 +352,158 @@ __global__ __launch_bounds__(256) void k_nrc_encode(NrcDev m, const vg_nrc_query
     feat[t] = pack2(f0, f1);
 }
 
-// Grid-gradient records of one level (blockIdx.y): one thread per sample writes its 8
-// corner contributions (entry, w * d loss / d feature pair) at ((l * n + s) * 8 + corner);
-// samples with no gradient write the sentinel entry n_entries, which sorts last.
-__global__ __launch_bounds__(256) void k_nrc_contrib(NrcDev m, const vg_nrc_query *__restrict__ q,
-                                                     const float2 *__restrict__ dfeat, uint32_t n,
-                                                     uint32_t sentinel, uint32_t *__restrict__ keys,
-                                                     float2 *__restrict__ vals) {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    const int l = (int)blockIdx.y;
-    if (s >= n) return;
-    const size_t base = ((size_t)l * n + s) * 8u;
-    const float2 d = dfeat[(size_t)s * 16u + (uint32_t)l];
-    if (d.x == 0.0f && d.y == 0.0f) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            keys[base + k] = sentinel;
-            vals[base + k] = make_float2(0.0f, 0.0f);
-        }
-        return;
+// ------------------------------------------------------------ grid gradient --
+// Blocks of the record count / scatter passes; each takes a contiguous range of
+// (sample, level) pairs, pair p = s * levels + l.
+constexpr uint32_t kGBlocks = 256;
+
+__device__ __forceinline__ void pair_range(uint32_t pairs, uint32_t &p0, uint32_t &p1) {
+    const uint32_t per = (pairs + kGBlocks - 1) / kGBlocks;
+    p0 = blockIdx.x * per;
+    p1 = p0 + per < pairs ? p0 + per : pairs;
+}
+
+// max |d loss / d feature| over the batch (fp32 bits; non-negative floats order as integers).
+__global__ __launch_bounds__(256) void k_nrc_gmax(const float2 *__restrict__ dfeat, uint32_t n,
+                                                  int levels, uint32_t *gmax) {
+    __shared__ float sm[4];
+    float mx = 0.0f;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n * 16u; i += gridDim.x * 256u) {
+        if ((int)(i & 15u) >= levels) continue;
+        const float2 d = dfeat[i];
+        mx = fmaxf(mx, fmaxf(fabsf(d.x), fabsf(d.y)));
     }
-    const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
-    const float p[3] = {a.x, a.y, a.z};
-    uint32_t idx[8];
-    float w[8];
-    level_corners(m, l, p, idx, w);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        keys[base + k] = idx[k];
-        vals[base + k] = make_float2(w[k] * d.x, w[k] * d.y);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+        atomicMax(gmax, __float_as_uint(mx));
     }
 }
 
-struct Sum2 {
-    __device__ __host__ float2 operator()(const float2 &a, const float2 &b) const {
-        return make_float2(a.x + b.x, a.y + b.y);
+// Record counts per (bucket, block): hist[bucket * kGBlocks + block]. Pairs whose gradient is
+// zero emit no records.
+__global__ __launch_bounds__(256) void k_nrc_gcount(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                    const float2 *__restrict__ dfeat, uint32_t n,
+                                                    int gbits, uint32_t nbuckets,
+                                                    uint32_t *__restrict__ hist) {
+    extern __shared__ uint32_t cnt[];
+    for (uint32_t b = threadIdx.x; b < nbuckets; b += 256u) cnt[b] = 0u;
+    __syncthreads();
+    uint32_t p0, p1;
+    pair_range(n * (uint32_t)m.levels, p0, p1);
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += 256u) {
+        const uint32_t s = p / (uint32_t)m.levels, l = p - s * (uint32_t)m.levels;
+        const float2 d = dfeat[(size_t)s * 16u + l];
+        if (d.x == 0.0f && d.y == 0.0f) continue;
+        const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
+        const float pos[3] = {a.x, a.y, a.z};
+        uint32_t idx[8];
+        float w[8];
+        level_corners(m, (int)l, pos, idx, w);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) atomicAdd(&cnt[idx[k] >> gbits], 1u);
     }
-};
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nbuckets; b += 256u) hist[(size_t)b * kGBlocks + blockIdx.x] = cnt[b];
+}
+
+// The same records, written at their bucket's place: the scanned hist gives every (bucket,
+// block) its first slot; inside a block the slots are taken in any order (the sums below
+// do not depend on it).
+__global__ __launch_bounds__(256) void k_nrc_gscatter(NrcDev m, const vg_nrc_query *__restrict__ q,
+                                                      const float2 *__restrict__ dfeat, uint32_t n,
+                                                      int gbits, uint32_t nbuckets,
+                                                      const uint32_t *__restrict__ hoff,
+                                                      uint32_t *__restrict__ rk,
+                                                      float2 *__restrict__ rv) {
+    extern __shared__ uint32_t cur[];
+    for (uint32_t b = threadIdx.x; b < nbuckets; b += 256u) cur[b] = hoff[(size_t)b * kGBlocks + blockIdx.x];
+    __syncthreads();
+    const uint32_t lmask = (1u << gbits) - 1u;
+    uint32_t p0, p1;
+    pair_range(n * (uint32_t)m.levels, p0, p1);
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += 256u) {
+        const uint32_t s = p / (uint32_t)m.levels, l = p - s * (uint32_t)m.levels;
+        const float2 d = dfeat[(size_t)s * 16u + l];
+        if (d.x == 0.0f && d.y == 0.0f) continue;
+        const float4 a = reinterpret_cast<const float4 *>(q + s)[0];
+        const float pos[3] = {a.x, a.y, a.z};
+        uint32_t idx[8];
+        float w[8];
+        level_corners(m, (int)l, pos, idx, w);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t slot = atomicAdd(&cur[idx[k] >> gbits], 1u);
+            rk[slot] = idx[k] & lmask;
+            rv[slot] = make_float2(w[k] * d.x, w[k] * d.y);
+        }
+    }
+}
+
+__device__ __forceinline__ void adam1(float &p, float &m, float &v, float g, float lr, float b1,
+                                      float b2, float eps, float c1, float c2) {
+    m = b1 * m + (1.0f - b1) * g;
+    v = b2 * v + (1.0f - b2) * (g * g);
+    p -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+// One block per bucket of 2^gbits entries: its records are summed in LDS in 64-bit fixed
+// point (scale 2^K with K = 38 - e, |contribution| <= gmax < 2^e: at most 2^23 records per
+// entry keep every sum below 2^61; integer sums are exact, so the result does not depend on
+// the order of the records). Then, per entry and feature with a non-zero sum: Adam (adam set;
+// the bf16 copy the kernels read is refreshed), or the dense gradient (dense set). A feature
+// whose sum is zero is left alone, like an entry no sample touched.
+__global__ __launch_bounds__(256) void k_nrc_greduce(const uint32_t *__restrict__ hoff,
+                                                     const uint32_t *__restrict__ rk,
+                                                     const float2 *__restrict__ rv, int gbits,
+                                                     const uint32_t *gmax, float2 *__restrict__ p,
+                                                     float2 *__restrict__ mm, float2 *__restrict__ vv,
+                                                     uint32_t *__restrict__ ph,
+                                                     float2 *__restrict__ dense, int adam, float lr,
+                                                     float b1, float b2, float eps,
+                                                     const uint32_t *step) {
+    extern __shared__ unsigned long long acc[]; // 2 per entry
+    const uint32_t ne = 1u << gbits;
+    for (uint32_t i = threadIdx.x; i < 2u * ne; i += 256u) acc[i] = 0ull;
+    __syncthreads();
+    const uint32_t b = blockIdx.x;
+    const uint32_t r0 = hoff[(size_t)b * kGBlocks], r1 = hoff[(size_t)(b + 1u) * kGBlocks];
+    const float gm = __uint_as_float(*gmax);
+    int e = 0;
+    (void)frexpf(gm, &e); // gm < 2^e
+    const int K = 38 - e;
+    const double up = ldexp(1.0, K), down = ldexp(1.0, -K);
+    if (gm > 0.0f && gm <= 3.4e38f) // a non-finite backward pass contributes nothing
+        for (uint32_t r = r0 + threadIdx.x; r < r1; r += 256u) {
+            const uint32_t k = rk[r];
+            const float2 v = rv[r];
+            atomicAdd(&acc[2u * k], (unsigned long long)__double2ll_rn((double)v.x * up));
+            atomicAdd(&acc[2u * k + 1u], (unsigned long long)__double2ll_rn((double)v.y * up));
+        }
+    __syncthreads();
+    float c1 = 1.0f, c2 = 1.0f;
+    if (adam) {
+        const float t = (float)*step;
+        c1 = 1.0f - powf(b1, t);
+        c2 = 1.0f - powf(b2, t);
+    }
+    for (uint32_t i = threadIdx.x; i < ne; i += 256u) {
+        const size_t entry = ((size_t)b << gbits) + i;
+        const float gx = (float)((double)(long long)acc[2u * i] * down);
+        const float gy = (float)((double)(long long)acc[2u * i + 1u] * down);
+        if (dense) dense[entry] = make_float2(gx, gy);
+        if (!adam || (gx == 0.0f && gy == 0.0f)) continue;
+        float2 pp = p[entry], m1 = mm[entry], v1 = vv[entry];
+        if (gx != 0.0f) adam1(pp.x, m1.x, v1.x, gx, lr, b1, b2, eps, c1, c2);
+        if (gy != 0.0f) adam1(pp.y, m1.y, v1.y, gy, lr, b1, b2, eps, c1, c2);
+        p[entry] = pp;
+        mm[entry] = m1;
+        vv[entry] = v1;
+        ph[entry] = pack2(pp.x, pp.y);
+    }
+}
 
 // Forward and backward of a training batch from its encoding (feat): weight-gradient sums
 // per block into `partial`, the encoding gradient per sample into dfeat[s * 32 + input].
@@ -519,38 +638,6 @@ __global__ __launch_bounds__(1024) void k_nrc_reduce(const float *__restrict__ p
     }
 }
 
-__device__ __forceinline__ void adam1(float &p, float &m, float &v, float g, float lr, float b1,
-                                      float b2, float eps, float c1, float c2) {
-    m = b1 * m + (1.0f - b1) * g;
-    v = b2 * v + (1.0f - b2) * (g * g);
-    p -= lr * (m / c1) / (sqrtf(v / c2) + eps);
-}
-
-// Adam over the grid entries that received a gradient (the reduce-by-key runs: unique
-// entries and their summed gradient pairs); a feature whose sum is exactly zero is left
-// alone, like an entry no sample touched. The bf16 copy the kernels read is refreshed.
-__global__ void k_nrc_adam_runs(const uint32_t *__restrict__ ukeys, const float2 *__restrict__ ug,
-                                const uint32_t *n_runs, int64_t n_entries, float2 *__restrict__ p,
-                                float2 *__restrict__ mm, float2 *__restrict__ vv,
-                                uint32_t *__restrict__ ph, float lr, float b1, float b2, float eps,
-                                const uint32_t *step) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n_runs) return;
-    const uint32_t e = ukeys[i];
-    if ((int64_t)e >= n_entries) return; // the sentinel run
-    const float2 g = ug[i];
-    if (g.x == 0.0f && g.y == 0.0f) return;
-    const float t = (float)*step;
-    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
-    float2 pp = p[e], m = mm[e], v = vv[e];
-    if (g.x != 0.0f) adam1(pp.x, m.x, v.x, g.x, lr, b1, b2, eps, c1, c2);
-    if (g.y != 0.0f) adam1(pp.y, m.y, v.y, g.y, lr, b1, b2, eps, c1, c2);
-    p[e] = pp;
-    mm[e] = m;
-    vv[e] = v;
-    ph[e] = pack2(pp.x, pp.y);
-}
-
 // bf16 copy of the whole grid (after an upload or initialisation).
 __global__ void k_nrc_grid_pack(const float2 *__restrict__ p, uint32_t *__restrict__ ph,
                                 int64_t n_entries) {
@@ -635,7 +722,7 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const ui
 }
 
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
-                     hipStream_t stream) {
+                     hipStream_t stream, float2 *dense) {
     if (n == 0) return VG_OK;
     if (n > (uint32_t)m->fcap)
         return fail(VG_E_INVALID, "batch of %u above the training buffers' %d", n, m->fcap);
@@ -643,36 +730,41 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
         return fail(VG_E_INVALID, "batch of %u: too many grid-gradient records", n);
     const NrcDev dv = dev_view(m);
     const uint32_t eblocks = (uint32_t)(((uint64_t)n * 16u + 255u) / 256u);
-    hipLaunchKernelGGL(k_nrc_encode, dim3(eblocks), dim3(256), 0, stream, dv, q, n, m->feat);
+    hipLaunchKernelGGL(k_nrc_encode, dim3(eblocks), dim3(256), 0, stream, dv, q, n, m->feat,
+                       m->gmax);
     HIP_TRY(hipGetLastError());
     const uint64_t tiles = ((uint64_t)n + kNrcTile - 1) / kNrcTile;
     const int blocks = (int)(tiles < (uint64_t)m->partial_blocks ? tiles : m->partial_blocks);
     hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, stream, dv, q, m->feat, t, n,
                        1.0f / (float)n, m->partial, m->dfeat);
     HIP_TRY(hipGetLastError());
-    const size_t nc = (size_t)n * (size_t)m->cfg.levels * 8u;
-    hipLaunchKernelGGL(k_nrc_contrib, dim3((n + 255) / 256, (uint32_t)m->cfg.levels), dim3(256),
-                       0, stream, dv, q, reinterpret_cast<const float2 *>(m->dfeat), n,
-                       (uint32_t)m->n_entries, m->ck_in, m->cv_in);
+    // grid gradient: bucket counts -> scan -> records in bucket order -> per-bucket sums
+    const uint32_t nb = (uint32_t)m->gbuckets;
+    const size_t nh = (size_t)nb * kGBlocks + 1;
+    uint32_t *hoff = m->hist + nh;
+    hipLaunchKernelGGL(k_nrc_gmax, dim3(256), dim3(256), 0, stream,
+                       reinterpret_cast<const float2 *>(m->dfeat), n, m->cfg.levels, m->gmax);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_nrc_gcount, dim3(kGBlocks), dim3(256), nb * sizeof(uint32_t), stream, dv, q,
+                       reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, m->hist);
     HIP_TRY(hipGetLastError());
     size_t tb = m->ctemp_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(m->ctemp, tb, m->ck_in, m->ck_out, m->cv_in,
-                                               m->cv_out, (int)nc, 0, m->key_bits, stream));
-    tb = m->ctemp_bytes;
-    HIP_TRY(hipcub::DeviceReduce::ReduceByKey(m->ctemp, tb, m->ck_out, m->ck_uniq, m->cv_out,
-                                              m->cv_agg, m->c_runs, Sum2(), (int)nc, stream));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(m->ctemp, tb, m->hist, hoff, (int)nh, stream));
+    hipLaunchKernelGGL(k_nrc_gscatter, dim3(kGBlocks), dim3(256), nb * sizeof(uint32_t), stream, dv,
+                       q, reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, hoff, m->rk,
+                       m->rv);
+    HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_nrc_reduce, dim3((VG_NRC_MLP_PARAMS + 1 + 63) / 64), dim3(1024), 0,
                        stream, m->partial, blocks, m->mlp_g, m->loss, m->step, update ? 1 : 0);
     HIP_TRY(hipGetLastError());
+    const vg_nrc_config &c = m->cfg;
+    hipLaunchKernelGGL(k_nrc_greduce, dim3(nb), dim3(256), (2u << m->gbits) * sizeof(uint64_t),
+                       stream, hoff, m->rk, m->rv, m->gbits, m->gmax,
+                       reinterpret_cast<float2 *>(m->grid), reinterpret_cast<float2 *>(m->grid_m),
+                       reinterpret_cast<float2 *>(m->grid_v), m->grid_h, dense, update ? 1 : 0,
+                       c.lr_grid, c.beta1, c.beta2, c.eps, m->step);
+    HIP_TRY(hipGetLastError());
     if (update) {
-        const vg_nrc_config &c = m->cfg;
-        hipLaunchKernelGGL(k_nrc_adam_runs, dim3((uint32_t)((nc + 255) / 256)), dim3(256), 0, stream,
-                           m->ck_uniq, m->cv_agg, m->c_runs, m->n_entries,
-                           reinterpret_cast<float2 *>(m->grid),
-                           reinterpret_cast<float2 *>(m->grid_m),
-                           reinterpret_cast<float2 *>(m->grid_v), m->grid_h, c.lr_grid, c.beta1,
-                           c.beta2, c.eps, m->step);
-        HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(1), dim3(1024), 0, stream, m->mlp, m->mlp_m,
                            m->mlp_v, m->mlp_g, reinterpret_cast<uint16_t *>(m->frag_fwd),
                            reinterpret_cast<uint16_t *>(m->frag_bwd), c.lr_mlp, c.beta1, c.beta2,
@@ -717,43 +809,37 @@ static int check_config(const vg_nrc_config *c) {
     return VG_OK;
 }
 
-static void free_sort(vg_nrc *m) {
-    void *ptrs[] = {m->ck_in, m->ck_out, m->ck_uniq, m->cv_in, m->cv_out, m->cv_agg, m->ctemp};
+static void free_grad(vg_nrc *m) {
+    void *ptrs[] = {m->rk, m->rv, m->ctemp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    m->ck_in = m->ck_out = m->ck_uniq = nullptr;
-    m->cv_in = m->cv_out = m->cv_agg = nullptr;
+    m->rk = nullptr;
+    m->rv = nullptr;
     m->ctemp = nullptr;
     m->ctemp_bytes = 0;
 }
 
 // Training buffers for batches of up to `cap` samples: encoding, its gradient, and the
-// grid-gradient records with their sort / reduce scratch.
+// grid-gradient records.
 static int alloc_train(vg_nrc *m, int32_t cap) {
     (void)hipFree(m->feat);
     (void)hipFree(m->dfeat);
     m->feat = nullptr;
     m->dfeat = nullptr;
-    free_sort(m);
+    free_grad(m);
     m->fcap = 0;
     const size_t nc = (size_t)cap * (size_t)m->cfg.levels * 8u;
     bool ok = hipMalloc((void **)&m->feat, (size_t)cap * 16 * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void **)&m->dfeat, (size_t)cap * 32 * sizeof(float)) == hipSuccess &&
-              hipMalloc((void **)&m->ck_in, nc * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc((void **)&m->ck_out, nc * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc((void **)&m->ck_uniq, nc * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc((void **)&m->cv_in, nc * sizeof(float2)) == hipSuccess &&
-              hipMalloc((void **)&m->cv_out, nc * sizeof(float2)) == hipSuccess &&
-              hipMalloc((void **)&m->cv_agg, nc * sizeof(float2)) == hipSuccess;
+              hipMalloc((void **)&m->rk, nc * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->rv, nc * sizeof(float2)) == hipSuccess;
     if (!ok) return fail(VG_E_NOMEM, "training buffers for %d samples", cap);
-    size_t a = 0, b = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, a, m->ck_in, m->ck_out, m->cv_in,
-                                               m->cv_out, (int)nc, 0, m->key_bits));
-    HIP_TRY(hipcub::DeviceReduce::ReduceByKey(nullptr, b, m->ck_out, m->ck_uniq, m->cv_out,
-                                              m->cv_agg, m->c_runs, Sum2(), (int)nc));
-    m->ctemp_bytes = a > b ? a : b;
-    if (hipMalloc(&m->ctemp, m->ctemp_bytes) != hipSuccess)
-        return fail(VG_E_NOMEM, "sort scratch of %zu bytes", m->ctemp_bytes);
+    const size_t nh = (size_t)m->gbuckets * kGBlocks + 1;
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, m->hist, m->hist + nh, (int)nh));
+    if (hipMalloc(&m->ctemp, tb) != hipSuccess)
+        return fail(VG_E_NOMEM, "scan scratch of %zu bytes", tb);
+    m->ctemp_bytes = tb;
     m->fcap = cap;
     return VG_OK;
 }
@@ -761,8 +847,9 @@ static int alloc_train(vg_nrc *m, int32_t cap) {
 static void free_nrc(vg_nrc *m) {
     void *ptrs[] = {m->mlp,    m->mlp_m,  m->mlp_v,  m->mlp_g,  m->partial, m->frag_fwd,
                     m->frag_bwd, m->grid, m->grid_m, m->grid_v, m->grid_h,  m->step,
-                    m->loss,   m->tq,     m->tt,     m->feat,   m->dfeat,   m->c_runs};
-    free_sort(m);
+                    m->loss,   m->tq,     m->tt,     m->feat,   m->dfeat,   m->hist,
+                    m->gmax,   m->gdense};
+    free_grad(m);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -854,8 +941,8 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
     m->n_grid = 2 * m->n_entries;
     m->partial_blocks = 1024;
     m->tcap = cfg->batch;
-    m->key_bits = 0;
-    while ((1ll << m->key_bits) <= m->n_entries) ++m->key_bits; // entries and the sentinel
+    m->gbits = cfg->log2_table < 12 ? cfg->log2_table : 12;
+    m->gbuckets = cfg->levels << (cfg->log2_table - m->gbits);
     const size_t mb = VG_NRC_MLP_PARAMS * sizeof(float), gb = (size_t)m->n_grid * sizeof(float);
     const size_t fb = (size_t)kNrcFrags * 64 * 16;
     bool ok = hipMalloc((void **)&m->mlp, mb) == hipSuccess &&
@@ -870,7 +957,9 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
               hipMalloc((void **)&m->grid_m, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_v, gb) == hipSuccess &&
               hipMalloc((void **)&m->grid_h, (size_t)m->n_entries * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc((void **)&m->c_runs, 256) == hipSuccess &&
+              hipMalloc((void **)&m->hist,
+                        2 * ((size_t)m->gbuckets * kGBlocks + 1) * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&m->gmax, 256) == hipSuccess &&
               hipMalloc((void **)&m->step, 256) == hipSuccess &&
               hipMalloc((void **)&m->loss, 256) == hipSuccess &&
               hipMalloc((void **)&m->tq, (size_t)m->tcap * sizeof(vg_nrc_query)) == hipSuccess &&
@@ -957,28 +1046,19 @@ int vg_nrc_gradients(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t 
     if (st) return st;
     if (n == 0) return fail(VG_E_INVALID, "empty batch");
     HIP_TRY(hipDeviceSynchronize());
-    st = nrc_train_launch(m, q, reinterpret_cast<const float4 *>(t), n, false, nullptr);
+    if (grid_grad && !m->gdense &&
+        hipMalloc((void **)&m->gdense, (size_t)m->n_entries * sizeof(float2)) != hipSuccess)
+        return fail(VG_E_NOMEM, "dense grid gradient");
+    st = nrc_train_launch(m, q, reinterpret_cast<const float4 *>(t), n, false, nullptr,
+                          grid_grad ? m->gdense : nullptr);
     if (st) return st;
     HIP_TRY(hipDeviceSynchronize());
     if (mlp_grad)
         HIP_TRY(hipMemcpy(mlp_grad, m->mlp_g, VG_NRC_MLP_PARAMS * sizeof(float),
                           hipMemcpyDeviceToHost));
-    if (grid_grad) {
-        uint32_t runs = 0;
-        HIP_TRY(hipMemcpy(&runs, m->c_runs, sizeof(runs), hipMemcpyDeviceToHost));
-        std::vector<uint32_t> keys(runs);
-        std::vector<float2> g(runs);
-        if (runs) {
-            HIP_TRY(hipMemcpy(keys.data(), m->ck_uniq, runs * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(g.data(), m->cv_agg, runs * sizeof(float2), hipMemcpyDeviceToHost));
-        }
-        std::memset(grid_grad, 0, (size_t)m->n_grid * sizeof(float));
-        for (uint32_t i = 0; i < runs; ++i) {
-            if ((int64_t)keys[i] >= m->n_entries) continue;
-            grid_grad[2 * (size_t)keys[i]] = g[i].x;
-            grid_grad[2 * (size_t)keys[i] + 1] = g[i].y;
-        }
-    }
+    if (grid_grad)
+        HIP_TRY(hipMemcpy(grid_grad, m->gdense, (size_t)m->n_grid * sizeof(float),
+                          hipMemcpyDeviceToHost));
     if (loss) HIP_TRY(hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
     HIP_TRY(hipDeviceSynchronize());
     return VG_OK;
