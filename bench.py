@@ -345,15 +345,20 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     p32 = gi.gi_params(sun, rays_per_probe=32, **base)
     stream = ctx.stream()
 
-    def timed(fn, p, n):
-        ev = [(ctx.event(), ctx.event()) for _ in range(n)]
-        for i, (a, b) in enumerate(ev):
+    def timed(fn, p, n, cache=None):
+        # n back-to-back updates between two events on the stream; a learned update's
+        # training step runs on the cache's own stream, so the end event waits for the last
+        # one (vg_nrc_join): the span covers all the work of the n updates
+        a, b = ctx.event(), ctx.event()
+        a.record(stream)
+        for i in range(n):
             p["round"] = 1000 + i
-            a.record(stream)
             fn(p)
-            b.record(stream)
+        if cache is not None:
+            cache.join(stream)
+        b.record(stream)
         stream.synchronize()
-        return float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+        return a.elapsed_ms(b) / n
 
     from voxelg_amd import _lib as L
 
@@ -380,9 +385,9 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     g32.autotune(p32, cache, train=True)  # (its repetitions train the cache further)
     sched_learned = sched()
     learned_ms = timed(lambda p: g32.update_learned(cache, p, train=True, stream=stream), p32,
-                       steps)
+                       steps, cache)
     frozen_ms = timed(lambda p: g32.update_learned(cache, p, train=False, stream=stream), p32,
-                      steps)
+                      steps, cache)
     log(f"gi learned: {learned_ms:.3f} ms/update with training, {frozen_ms:.3f} without "
         f"({sched_learned})")
     p32["round"] = 5000
@@ -530,6 +535,9 @@ def bench_dynamic(args):
         p["round"] = f
         probes.update_learned(cache, p, train=True, stream=stream)
         ev[4].record(stream)
+        if f == n_frames - 1:
+            # the cache's training step overlaps the next frame; the last one is waited for
+            cache.join(stream)
         stream.synchronize()
         dt = time.perf_counter() - t0
         if f >= args.warmup:

@@ -482,6 +482,11 @@ int vg_nrc_infer(vg_nrc *nrc, const vg_nrc_query *queries, uint32_t n, float *ou
 /* One Adam step on n device queries with device targets (r, g, b, weight). */
 int vg_nrc_train_step(vg_nrc *nrc, const vg_nrc_query *queries, const float *target4,
                       uint32_t n, void *stream);
+/* Make `stream` wait for a training step still running on the cache's own stream (the
+ * learned probe update enqueues its step there). The other vg_nrc_* calls and the next
+ * learned update wait by themselves; a caller reading the parameters through its own
+ * kernels calls this first. */
+int vg_nrc_join(vg_nrc *nrc, void *stream);
 /* The loss gradient of n device queries without an update (for inspection): mlp and grid
  * gradients and the loss to the host (any pointer may be NULL). Synchronous. */
 int vg_nrc_gradients(vg_nrc *nrc, const vg_nrc_query *queries, const float *target4,
@@ -496,10 +501,13 @@ int vg_nrc_last_batch(vg_nrc *nrc, vg_nrc_query *queries_host, float *target4_ho
  * bounce tail): gather rays_per_probe rays per updated probe and the sun shadow ray at
  * each hit, then the tail albedo * max(0, cache(hit point, face, direction)) for every hit
  * (one inference launch over the compacted hits), the SH-L1 projection, fold and
- * relocation of vg_probes_update. With VG_NRC_TRAIN the cache then trains on this
- * update's hits: cfg->steps steps of cfg->batch hits drawn with replacement, each
- * target the traced bounce tail (a cosine bounce ray and the sun shadow ray at its hit).
- * params->bounces is not read. */
+ * relocation of vg_probes_update. With VG_NRC_TRAIN the cache then trains: cfg->steps
+ * steps of cfg->batch samples drawn stratified from the previous learned update's hits,
+ * each target the traced bounce tail (a cosine bounce ray and the sun shadow ray at its
+ * hit; the first step's targets are traced inside this update's path launch). With one
+ * step per update the step runs on the cache's own stream, overlapping the caller's next
+ * work: the probe records are complete with this call's work on `stream`, the new
+ * parameters after vg_nrc_join. params->bounces is not read. */
 #define VG_NRC_TRAIN 1u
 int vg_probes_update_learned(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                              uint32_t flags, void *stream);

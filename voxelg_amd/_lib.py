@@ -184,6 +184,7 @@ SIGNATURES = {
     "vg_nrc_set_params": (C.c_int, [_P, _P, _P]),
     "vg_nrc_infer": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "vg_nrc_train_step": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    "vg_nrc_join": (C.c_int, [_P, _P]),
     "vg_nrc_gradients": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, _P]),
     "vg_nrc_stats": (C.c_int, [_P, _I64P, _F32P]),
     "vg_nrc_last_batch": (C.c_int, [_P, _P, _P]),

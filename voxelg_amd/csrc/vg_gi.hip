@@ -886,13 +886,16 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
     const int thr = w->ctx->probe_refill;
     const bool train = nrc && (nflags & VG_NRC_TRAIN) && nrc->cfg.steps > 0;
+    // the path launch writes this update's training batch into the half the previous
+    // update's training step (possibly still running) does not read
+    const int tb = nrc ? nrc->tpar ^ 1 : 0;
     LearnedArgs la;
     la.queries = pr->queries;
     la.half = (uint32_t)pr->ns;
     la.train_n = train ? (uint32_t)nrc->tcap : 0u;
     la.tseed = nrc ? nrc->cfg.seed ^ 0x9E3779B9u : 0u;
-    la.tq = nrc ? nrc->tq : nullptr;
-    la.tt = nrc ? nrc->tt : nullptr;
+    la.tq = nrc ? nrc->tq_base + (size_t)tb * nrc->tcap : nullptr;
+    la.tt = nrc ? nrc->tt_base + (size_t)tb * nrc->tcap : nullptr;
 #define LAUNCH_G(C, B)                                                                         \
     hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
@@ -913,13 +916,28 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
         st = nrc_infer_launch(nrc, pr->queries, pr->ns, pr->ctr + 3, pr->ctr + 1,
                               (uint32_t)n_samples, nullptr, pr->rad, pr->palette, stream);
         if (st) return st;
+        if (train) {
+            nrc->tpar = tb;
+            nrc->tq = la.tq;
+            nrc->tt = la.tt;
+        }
     }
+    // one training step overlaps the next update (it needs nothing the next update's prep and
+    // path launch write); more steps, or a counting run, stay on the caller's stream
+    const bool overlap = train && nrc->cfg.steps == 1 && !count;
+    if (overlap) HIP_TRY(hipEventRecord(nrc->ev_ready, stream));
     const uint32_t wpb = kBlock / 64;
     hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + wpb - 1) / wpb), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, epoch, phase, pr->pwork, pr->dirs, pr->rad, pr->face,
                        pr->probes);
     HIP_TRY(hipGetLastError());
-    if (train) {
+    if (overlap) {
+        HIP_TRY(hipStreamWaitEvent(nrc->tstream, nrc->ev_ready, 0));
+        st = nrc_train_launch(nrc, nrc->tq, nrc->tt, (uint32_t)nrc->tcap, true, nrc->tstream);
+        if (st) return st;
+        HIP_TRY(hipEventRecord(nrc->ev_trained, nrc->tstream));
+        nrc->pending = true;
+    } else if (train) {
         for (int s = 0; s < nrc->cfg.steps; ++s) {
             if (s > 0) { // step 0's targets came with the path launch
                 const uint32_t tseed = nrc->cfg.seed ^ (0x9E3779B9u * (uint32_t)(s + 1));
@@ -1111,7 +1129,9 @@ int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32
     if (st) return st;
     vg_ctx *c = pr->world->ctx;
     return autotune(c, &c->probe_wpc, &c->probe_refill, [&](hipStream_t s) {
-        return probes_update(pr, p, nrc, flags, false, s);
+        const int r = probes_update(pr, p, nrc, flags, false, s);
+        // the timed span includes the update's training step (on the cache's stream)
+        return r == VG_OK && nrc ? nrc_join(nrc, s) : r;
     }, best_ms);
 }
 

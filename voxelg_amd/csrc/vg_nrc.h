@@ -22,6 +22,15 @@ struct vg_nrc {
     vg_nrc_query *tq;       // the learned probe update's last training batch
     float4 *tt;             // ... and its targets
     int32_t tcap;           // entries of tq / tt
+    // Training batches are double-buffered: a learned update's path launch writes one half
+    // while the previous update's training step, still running on tstream, reads the other.
+    vg_nrc_query *tq_base;  // 2 * tcap queries (halves at 0 and tcap)
+    float4 *tt_base;        // 2 * tcap targets
+    int tpar;               // half written by the last learned update
+    hipStream_t tstream;    // the learned update's training step overlaps the next update
+    hipEvent_t ev_ready;    // the update's cache inference has run (parameters may change)
+    hipEvent_t ev_trained;  // the last training step enqueued on tstream has run
+    bool pending;           // ev_trained has been recorded
     uint32_t *feat;         // training batch encoding: bf16 pair per (sample, level)
     float *dfeat;           // ... its gradient: 32 fp32 per sample
     int32_t fcap;           // samples the training buffers hold
@@ -58,5 +67,8 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const ui
 // gradient of every entry is also written there (n_entries float pairs).
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
                      hipStream_t stream, float2 *dense = nullptr);
+// Make `stream` wait for the training step still running on the cache's own stream (work
+// that reads or writes the cache parameters must come after it).
+int nrc_join(vg_nrc *m, hipStream_t stream);
 
 } // namespace vg

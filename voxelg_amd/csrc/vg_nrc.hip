@@ -703,10 +703,17 @@ static int set_dev(const vg_nrc *m) {
     return VG_OK;
 }
 
+int nrc_join(vg_nrc *m, hipStream_t stream) {
+    if (m->pending && stream != m->tstream) HIP_TRY(hipStreamWaitEvent(stream, m->ev_trained, 0));
+    return VG_OK;
+}
+
 int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const uint32_t *sel,
                      const uint32_t *n_dev, uint32_t n_max, float4 *out4, float4 *rad,
                      const float4 *pal, hipStream_t stream) {
     if (n_max == 0) return VG_OK;
+    int st = nrc_join(m, stream);
+    if (st) return st;
     const uint64_t tiles = ((uint64_t)n_max + 63) / 64;
     uint64_t blocks = (tiles + 3) / 4;
     const uint64_t cap = (uint64_t)m->ctx->cus * 8;
@@ -724,6 +731,8 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const ui
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
                      hipStream_t stream, float2 *dense) {
     if (n == 0) return VG_OK;
+    int jst = nrc_join(m, stream);
+    if (jst) return jst;
     if (n > (uint32_t)m->fcap)
         return fail(VG_E_INVALID, "batch of %u above the training buffers' %d", n, m->fcap);
     if (n > (uint32_t)(1u << 31) / (8u * (uint32_t)m->cfg.levels))
@@ -847,11 +856,14 @@ static int alloc_train(vg_nrc *m, int32_t cap) {
 static void free_nrc(vg_nrc *m) {
     void *ptrs[] = {m->mlp,    m->mlp_m,  m->mlp_v,  m->mlp_g,  m->partial, m->frag_fwd,
                     m->frag_bwd, m->grid, m->grid_m, m->grid_v, m->grid_h,  m->step,
-                    m->loss,   m->tq,     m->tt,     m->feat,   m->dfeat,   m->hist,
+                    m->loss,   m->tq_base, m->tt_base, m->feat, m->dfeat,  m->hist,
                     m->gmax,   m->gdense};
     free_grad(m);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (m->ev_ready) (void)hipEventDestroy(m->ev_ready);
+    if (m->ev_trained) (void)hipEventDestroy(m->ev_trained);
+    if (m->tstream) (void)hipStreamDestroy(m->tstream);
 }
 
 // Upload parameters, clear the optimizer, refresh the bf16 copies. Synchronous.
@@ -962,8 +974,13 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
               hipMalloc((void **)&m->gmax, 256) == hipSuccess &&
               hipMalloc((void **)&m->step, 256) == hipSuccess &&
               hipMalloc((void **)&m->loss, 256) == hipSuccess &&
-              hipMalloc((void **)&m->tq, (size_t)m->tcap * sizeof(vg_nrc_query)) == hipSuccess &&
-              hipMalloc((void **)&m->tt, (size_t)m->tcap * sizeof(float4)) == hipSuccess;
+              hipMalloc((void **)&m->tq_base, 2 * (size_t)m->tcap * sizeof(vg_nrc_query)) == hipSuccess &&
+              hipMalloc((void **)&m->tt_base, 2 * (size_t)m->tcap * sizeof(float4)) == hipSuccess &&
+              hipStreamCreateWithFlags(&m->tstream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&m->ev_ready, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&m->ev_trained, hipEventDisableTiming) == hipSuccess;
+    m->tq = m->tq_base;
+    m->tt = m->tt_base;
     if (ok) ok = alloc_train(m, cfg->batch > 65536 ? cfg->batch : 65536) == VG_OK;
     if (!ok) {
         free_nrc(m);
@@ -977,8 +994,8 @@ int vg_nrc_create(vg_ctx *ctx, const vg_nrc_config *cfg, vg_nrc **out) {
     for (int i = 0; i < 3 * 4096; ++i) mlp[i] = unit_sym(cfg->seed, 1, (uint64_t)i) * s_hidden;
     for (int i = 0; i < 3 * 64; ++i) mlp[12288 + i] = unit_sym(cfg->seed, 2, (uint64_t)i) * s_out;
     for (int64_t i = 0; i < m->n_grid; ++i) grid[(size_t)i] = unit_sym(cfg->seed, 3, (uint64_t)i) * cfg->init_grid;
-    HIP_TRY(hipMemset(m->tq, 0, (size_t)m->tcap * sizeof(vg_nrc_query)));
-    HIP_TRY(hipMemset(m->tt, 0, (size_t)m->tcap * sizeof(float4)));
+    HIP_TRY(hipMemset(m->tq_base, 0, 2 * (size_t)m->tcap * sizeof(vg_nrc_query)));
+    HIP_TRY(hipMemset(m->tt_base, 0, 2 * (size_t)m->tcap * sizeof(float4)));
     st = load_params(m, mlp.data(), grid.data());
     if (st) {
         free_nrc(m);
@@ -1031,6 +1048,13 @@ int vg_nrc_infer(vg_nrc *m, const vg_nrc_query *q, uint32_t n, float *out4, void
     if (st) return st;
     return nrc_infer_launch(m, q, 0, nullptr, nullptr, n, reinterpret_cast<float4 *>(out4),
                             nullptr, nullptr, (hipStream_t)stream);
+}
+
+int vg_nrc_join(vg_nrc *m, void *stream) {
+    if (!m) return fail(VG_E_INVALID, "null cache");
+    int st = set_dev(m);
+    if (st) return st;
+    return nrc_join(m, (hipStream_t)stream);
 }
 
 int vg_nrc_train_step(vg_nrc *m, const vg_nrc_query *q, const float *t, uint32_t n, void *stream) {

@@ -124,6 +124,12 @@ class RadianceCache:
         self.ctx.synchronize()
         return self
 
+    def join(self, stream=None) -> "RadianceCache":
+        """Make `stream` wait for a training step still running on the cache's own stream
+        (the learned probe update enqueues its step there)."""
+        check(lib().vg_nrc_join(self._h, getattr(stream, "handle", stream)), "vg_nrc_join")
+        return self
+
     def stats(self):
         """(optimizer steps taken, loss of the last step)."""
         s = C.c_int64()
