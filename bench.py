@@ -394,6 +394,17 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     rays32, nq = g32.update_learned_counts(cache, p32, train=True)
     log(f"gi learned: {learned_ms:.3f} ms/update, rays {rays32}, cache queries {nq}")
 
+    # the probes' consumer: per-pixel indirect light over the headline 1080p frame's hits
+    from voxelg_amd import workloads
+    fw = workloads.LARGE
+    _, fcam = fw.camera()
+    fhits, _ = dw.trace_frame(fcam, fw.width, fw.height, fw.primary_tmax, fw.primary_flags,
+                              fw.sun(), fw.shadow_tmax)
+    find = ctx.alloc(fw.width * fw.height * 16)
+    indirect_ms = timed(lambda p: g32.frame_indirect(p, fcam, fw.width, fw.height, fhits,
+                                                     out=find, stream=stream), p32, steps)
+    log(f"gi: per-pixel probe indirect at {fw.width}x{fw.height}: {indirect_ms:.4f} ms")
+
     # errors of fresh updates against the FP64 reference
     total = g256.total
     slots = np.arange(total)
@@ -437,6 +448,7 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         "err_plain32_2bounce": round(err(rec_plain32), 5),
         "err_ambient_tail32": round(err(rec_ambient), 5),
         "rays_plain": rays256, "rays_learned": rays32, "cache_queries": nq,
+        "indirect_1080p_ms": round(indirect_ms, 4),
         "train_steps": cache.stats()[0], "train_batch": cache.batch,
         "reference": f"FP64 4096 rays/probe 2-bounce, oracle on {threads} host threads, "
                      f"{ref_s:.1f} s",
@@ -482,8 +494,9 @@ def bench_dynamic(args):
     """Config 5: the 256x128x256 world plus 4096 seeded sphere edits per frame (radius 2..12,
     half carve / half fill). One step = one frame: apply the edits on the device (incremental
     pyramid / uniform rebuild), mark the dirtied 8x8 blocks (primary and sun-shadow rays),
-    retrace them at 1920x1080 (primary tmax 512 + shadow 480), then a learned probe update
-    (8x4x8 probes at spacing 32, 32 rays per probe, the cache training online). Every
+    retrace them at 1920x1080 (primary tmax 512 + shadow 480), a learned probe update
+    (8x4x8 probes at spacing 32, 32 rays per probe, the cache training online), then the
+    probes' per-pixel indirect light over the frame's hits (sample_probes). Every
     --check-every frames the incrementally maintained frame is compared bit for bit with the
     oracle's full frame of the same edited world (outside the timed region)."""
     import math
@@ -516,14 +529,15 @@ def bench_dynamic(args):
     cache = nrc.RadianceCache(ctx)
     n_frames = args.warmup + args.steps
     edits = [wl.edits(f, world) for f in range(n_frames)]
-    phases = ("edits", "dirty", "retrace", "probes")
+    phases = ("edits", "dirty", "retrace", "probes", "indirect")
+    ind = ctx.alloc(W * H * 16)
     times = {k: [] for k in phases}
     dirty_frac = []
     checks = []
     wall = 0.0
     nbx, nby = (W + 7) // 8, (H + 7) // 8
     for f in range(n_frames):
-        ev = [ctx.event() for _ in range(5)]
+        ev = [ctx.event() for _ in range(6)]
         t0 = time.perf_counter()
         ev[0].record(stream)
         dw.apply_edits(edits[f], stream)
@@ -535,6 +549,8 @@ def bench_dynamic(args):
         p["round"] = f
         probes.update_learned(cache, p, train=True, stream=stream)
         ev[4].record(stream)
+        probes.frame_indirect(p, cam, W, H, hits, out=ind, stream=stream)
+        ev[5].record(stream)
         if f == n_frames - 1:
             # the cache's training step overlaps the next frame; the last one is waited for
             cache.join(stream)
@@ -562,7 +578,7 @@ def bench_dynamic(args):
         f"{np.mean(dirty_frac):.3f}; cache steps {cache.stats()[0]}")
     out = {
         "metric": "dynamic-world frame ms (4096 sphere edits + incremental rebuild + dirty "
-                  "retrace + learned probe update)",
+                  "retrace + learned probe update + per-pixel probe GI)",
         "value": round(ms_frame, 4), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_frame, 4), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
@@ -570,7 +586,7 @@ def bench_dynamic(args):
         "config": {"workload": "dynamic: 256x128x256 world, 4096 sphere edits per frame (r 2..12, "
                                "50% carve / 50% fill), 1920x1080 primary + sun shadow retrace of "
                                "the dirty 8x8 blocks, learned probe update (8x4x8 probes, 32 "
-                               "rays, online training)",
+                               "rays, online training), per-pixel probe indirect light",
                    "world": list(wl.dims), "resolution": [W, H], "edits_per_frame": wl.n_edits},
         "phases_ms": ms, "dirty_block_fraction": round(float(np.mean(dirty_frac)), 4),
         "checks": checks,

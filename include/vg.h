@@ -497,6 +497,19 @@ int vg_nrc_stats(vg_nrc *nrc, int64_t *steps_out, float *loss_out);
  * aux = the bounce-table index used, and targets (r, g, b, weight). Synchronous. */
 int vg_nrc_last_batch(vg_nrc *nrc, vg_nrc_query *queries_host, float *target4_host);
 
+/* The probe cache's consumer for a traced frame: indirect_light in probe mode
+ * (shaders/rt_gi.wgsl:128-139) over sample_probes (shaders/gi_probes.wgsl:106-179) for every
+ * pixel: at the primary hit (camera origin + dir * t, the hit face's normal, the view
+ * direction), the 8 surrounding probes of the grid (params: the update's grid and spacing)
+ * blended with smoothstepped trilinear weights, the backface weight and the one-tailed
+ * Chebyshev visibility bound of their distance moments, each evaluated as deringed SH-L1
+ * irradiance; times GI_STRENGTH (1) and the fade 1 - smoothstep(54, 90, t). Misses, hits at
+ * t >= 90 and pixels with no gathered probe around get 0. hits: the frame's device records
+ * (vg_trace_frame); out4: device, (r, g, b, 0) per pixel, 16-byte aligned. */
+int vg_frame_indirect(vg_probes *probes, const vg_gi_params *params, const vg_camera *camera_host,
+                      int32_t width, int32_t height, const vg_hit *hits, float *out4,
+                      void *stream);
+
 /* The learned probe update (the reference's cs_gi_probe_update with the cache as the
  * bounce tail): gather rays_per_probe rays per updated probe and the sun shadow ray at
  * each hit, then the tail albedo * max(0, cache(hit point, face, direction)) for every hit

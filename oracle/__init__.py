@@ -123,6 +123,9 @@ def lib():
             "orc_gi_update_learned": (None, [W, P, P, P, P, P, P, C.c_int, U64P]),
             "orc_gi_bounce_tail": (None, [W, P, P, P, P, C.c_uint32, P, U64P]),
             "orc_gi_bounce_tails": (None, [W, P, P, P, C.c_int64, P, P, P, C.c_int]),
+            "orc_sh_irradiance": (None, [P, F32P, F32P]),
+            "orc_sample_probes": (None, [P, I32P, C.c_int32, F32P, F32P, F32P, F32P]),
+            "orc_frame_indirect": (None, [P, I32P, C.c_int32, P, C.c_int32, C.c_int32, P, P]),
             "orc_bf16_round": (C.c_double, [C.c_double]),
             "orc_nrc_hash": (C.c_uint32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
             "orc_nrc_scales": (None, [C.c_int32, C.c_float, C.c_float, P]),
@@ -409,6 +412,43 @@ def gi_sky(params, d):
 
 def gi_bounce_index(seed, rnd, probe, k):
     return int(lib().orc_gi_bounce_index(int(seed) & 0xFFFFFFFF, int(rnd), int(probe), int(k)))
+
+
+def _probe_floats(probes) -> np.ndarray:
+    a = np.ascontiguousarray(probes)
+    if a.dtype != np.float32:
+        a = np.ascontiguousarray(a.view(np.float32))
+    return a.reshape(-1, 48)
+
+
+def sh_irradiance(record, n) -> np.ndarray:
+    """Deringed SH-L1 irradiance / pi of one probe record (48 floats) toward normal n."""
+    rec = _probe_floats(record)
+    out = np.zeros(3, np.float32)
+    lib().orc_sh_irradiance(_p(rec), _f3(n), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def sample_probes(probes, grid, spacing, pw, n, v) -> np.ndarray:
+    """sample_probes at world point pw with normal n and view direction v (fp32)."""
+    rec = _probe_floats(probes)
+    out = np.zeros(3, np.float32)
+    lib().orc_sample_probes(_p(rec), _i3(grid), int(spacing), _f3(pw), _f3(n), _f3(v),
+                            out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def frame_indirect(probes, grid, spacing, camera, width, height, hits: np.ndarray) -> np.ndarray:
+    """indirect_light in probe mode for every pixel of a frame's 20-byte hit records:
+    float32 [height * width, 4]."""
+    rec = _probe_floats(probes)
+    cam = np.ascontiguousarray(camera, np.float32)
+    h = np.ascontiguousarray(hits).view(np.uint8)
+    assert h.size == 20 * width * height
+    out = np.zeros((width * height, 4), np.float32)
+    lib().orc_frame_indirect(_p(rec), _i3(grid), int(spacing), _p(cam), int(width), int(height),
+                             _p(h), _p(out))
+    return out
 
 
 def make_rays(o, d, tmax=np.inf, flags=0) -> np.ndarray:

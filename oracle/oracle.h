@@ -295,7 +295,18 @@ void orc_gi_bounce_tail(const orc_world *w, const orc_gi_params *p, const float 
 void orc_gi_bounce_tails(const orc_world *w, const orc_gi_params *p, const float *btab,
                          const float *palette, int64_t n, const orc_nrc_query *q,
                          const uint32_t *bidx, double *out, int threads);
-/* The judge: for each listed probe slot, an n_dirs-direction estimate (dirs: the
+/* The probe cache's consumer (fp32, the device's operation order): deringed SH-L1
+ * irradiance of a 48-float probe record toward n (shaders/gi_probes.wgsl:86-96);
+ * sample_probes at world point pw, normal n, view direction v (:106-179) over a grid of
+ * records; indirect_light in probe mode (shaders/rt_gi.wgsl:128-139) for every pixel of a
+ * frame's 20-byte hit records, out4: 4 floats per pixel (r, g, b, 0). */
+void orc_sh_irradiance(const float *rec, const float n[3], float out[3]);
+void orc_sample_probes(const float *probes, const int32_t grid[3], int32_t spacing,
+                       const float pw[3], const float n[3], const float v[3], float out[3]);
+void orc_frame_indirect(const float *probes, const int32_t grid[3], int32_t spacing,
+                        const float *camera176, int32_t width, int32_t height,
+                        const uint8_t *hits, float *out4);
+/* The error reference: for each listed probe slot, an n_dirs-direction estimate (dirs: the
  * n_dirs-point Fibonacci set) of its SH-L1 radiance and distance moments from origin
  * origins[3i..3i+2], bounce samples drawn for sample k = 0..n_dirs-1 of `round`. out: 20
  * doubles per probe (r, g, b, dist, dist2 x 4 coefficients). */

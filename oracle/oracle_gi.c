@@ -493,3 +493,122 @@ void orc_gi_bounce_tails(const orc_world *w, const orc_gi_params *p, const float
     for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
     pthread_mutex_destroy(&J.mu);
 }
+
+/* ------------------------------------------------------------- probe sampling --
+ * The probe cache's consumer in fp32, operation for operation as the device evaluates it
+ * (dot products left to right, no contraction): sh_eval_irradiance / sh_eval_scalar /
+ * sample_probes (shaders/gi_probes.wgsl:69-179) and indirect_light in probe mode
+ * (shaders/rt_gi.wgsl:128-139; GI_MAX_T 90 at shaders/raymarch.wgsl:3821, GI_STRENGTH 1).
+ * A probe record is 48 floats in GiProbe order: sh_r 0, sh_g 4, sh_b 8, sh_dist 12,
+ * sh_dist2 16, info 20, staging 24..43, st_info 44. */
+static float dot3f(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+void orc_sh_irradiance(const float *rec, const float n[3], float out[3]) {
+    const float a1 = (float)(0.488603 * 0.666667);
+    const float lv[3] = {n[1], n[2], n[0]};
+    for (int c = 0; c < 3; ++c) {
+        const float *sh = rec + 4 * c;
+        const float l[3] = {sh[1], sh[2], sh[3]};
+        const float dc = sh[0] * 0.282095f;
+        const float lin = dot3f(l, lv) * a1;
+        const float lmax = sqrtf(dot3f(l, l)) * a1;
+        const float q = 0.95f * dc / (lmax > 1e-6f ? lmax : 1e-6f);
+        const float sc = q < 1.0f ? q : 1.0f;
+        const float e = dc + sc * lin;
+        out[c] = e > 0.0f ? e : 0.0f;
+    }
+}
+
+static float sh_scalar_f(const float *c, const float d[3]) {
+    return c[0] * 0.282095f + c[1] * (0.488603f * d[1]) + c[2] * (0.488603f * d[2]) +
+           c[3] * (0.488603f * d[0]);
+}
+
+void orc_sample_probes(const float *probes, const int32_t grid[3], int32_t spacing,
+                       const float pw[3], const float n[3], const float v[3], float out[3]) {
+    float pq[3], f[3];
+    int32_t base[3];
+    const float sp = (float)spacing;
+    for (int k = 0; k < 3; ++k) {
+        pq[k] = (pw[k] + n[k] * 1.0f) - v[k] * 1.0f;
+        const float pg = pq[k] / sp - 0.5f;
+        const float fb = floorf(pg);
+        base[k] = (int32_t)fb;
+        const float fl = pg - fb;
+        f[k] = fl * fl * (3.0f - 2.0f * fl);
+    }
+    float acc[3] = {0.0f, 0.0f, 0.0f}, wsum = 0.0f;
+    for (int i = 0; i < 8; ++i) {
+        const int32_t off[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
+        float w = 1.0f;
+        for (int k = 0; k < 3; ++k) w *= off[k] ? f[k] : 1.0f - f[k];
+        if (w <= 0.0f) continue;
+        int32_t g[3], s[3];
+        for (int k = 0; k < 3; ++k) {
+            g[k] = base[k] + off[k];
+            const int32_t m = g[k] % grid[k];
+            s[k] = m < 0 ? m + grid[k] : m;
+        }
+        const float *rec =
+            probes + 48 * (s[0] + (int64_t)s[1] * grid[0] + (int64_t)s[2] * grid[0] * grid[1]);
+        if (rec[23] < 0.5f || rec[20] != (float)g[0] || rec[21] != (float)g[1] || rec[22] != (float)g[2])
+            continue;
+        float dp[3], dn[3];
+        for (int k = 0; k < 3; ++k) {
+            dp[k] = (((float)g[k] + 0.5f) * sp + rec[45 + k]) - pq[k];
+            dn[k] = dp[k] + n[k] * 0.001f;
+        }
+        const float inv = 1.0f / sqrtf(dot3f(dn, dn));
+        const float d[3] = {dn[0] * inv, dn[1] * inv, dn[2] * inv};
+        float bf = dot3f(d, n);
+        bf = bf > 0.0f ? bf : 0.0f;
+        w = w * (bf * bf + 0.05f);
+        const float dist = sqrtf(dot3f(dp, dp));
+        const float tp[3] = {-d[0], -d[1], -d[2]};
+        const float mu = sh_scalar_f(rec + 12, tp);
+        float delta = dist - (mu + 0.35f);
+        delta = delta > 0.0f ? delta : 0.0f;
+        if (delta > 0.0f) {
+            const float mu2 = sh_scalar_f(rec + 16, tp);
+            float var = mu2 - mu * mu;
+            var = var > 0.25f ? var : 0.25f;
+            float c = var / (var + delta * delta);
+            c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+            w = w * c;
+        }
+        float e[3];
+        orc_sh_irradiance(rec, n, e);
+        for (int k = 0; k < 3; ++k) acc[k] = acc[k] + e[k] * w;
+        wsum = wsum + w;
+    }
+    for (int k = 0; k < 3; ++k) out[k] = wsum <= 1e-4f ? 0.0f : acc[k] / wsum;
+}
+
+static float smooth_f(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+void orc_frame_indirect(const float *probes, const int32_t grid[3], int32_t spacing,
+                        const float *cam, int32_t width, int32_t height, const uint8_t *hits,
+                        float *out4) {
+    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
+        const uint8_t *hr = hits + 20 * pix;
+        float t;
+        memcpy(&t, hr + 12, 4);
+        const uint32_t face = hr[16], hit = hr[18];
+        float *o = out4 + 4 * pix;
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        if (!hit || !(t < 90.0f)) continue;
+        float d[3], pw[3], n[3], e[3];
+        orc_primary_dir(cam, (int32_t)(pix % width), (int32_t)(pix / width), d);
+        for (int k = 0; k < 3; ++k) {
+            pw[k] = cam[k] + d[k] * t;
+            n[k] = face_n(face, k);
+        }
+        orc_sample_probes(probes, grid, spacing, pw, n, d, e);
+        const float s = 1.0f * (1.0f - smooth_f(54.0f, 90.0f, t));
+        for (int k = 0; k < 3; ++k) o[k] = e[k] * s;
+    }
+}

@@ -127,3 +127,60 @@ def test_param_errors(ctx, small):
     q["bounces"] = 3
     with pytest.raises(VGError):
         grid.update(q)
+
+
+def indirect_case(small, p, cam, W, H, pg):
+    hits, _ = small.trace_frame(cam, W, H, 512.0, 0, SUN, 480.0)
+    out = pg.frame_indirect(p, cam, W, H, hits)
+    got = out.numpy(np.float32).reshape(-1, 4)
+    grid = tuple(int(v) for v in p[0]["grid"])
+    ref = oracle.frame_indirect(pg.download(), grid, int(p[0]["spacing"]), cam, W, H,
+                                hits.numpy(np.uint8))
+    return got, ref
+
+
+def test_frame_indirect_matches_oracle(ctx, small, small_world):
+    """The probe cache's per-pixel consumer (indirect_light over sample_probes) equals the
+    oracle's fp32 restatement bit for bit: on records of real updates, and on random records
+    (relocation offsets, distance moments that trigger the Chebyshev bound, ungathered and
+    foreign slots)."""
+    import voxelg_amd as vg
+    p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=32, epochs=1,
+                     update_div=1, bounces=2, flags=1)
+    pg = gi.ProbeGrid(small, p)
+    for r in (0, 1):
+        p["round"] = r
+        pg.update(p)
+    W, H = 192, 108
+    cam = vg.Camera(pos=(128.5, 90.0, 128.5), yaw=0.7, pitch=-0.35).uniform(W, H)
+    print("case: frame indirect over updated probes", flush=True)
+    got, ref = indirect_case(small, p, cam, W, H, pg)
+    assert got.tobytes() == ref.tobytes()
+    assert (got[:, :3] > 0).mean() > 0.2
+    # random records
+    rng = np.random.default_rng(21)
+    rec = np.zeros(pg.total, gi.PROBE_DTYPE)
+    f = rec.view(np.float32).reshape(pg.total, 48)
+    f[:, 0:12] = rng.normal(0.5, 0.6, (pg.total, 12))
+    f[:, 12] = rng.uniform(0.0, 200.0, pg.total)          # mean distance DC
+    f[:, 13:16] = rng.normal(0.0, 20.0, (pg.total, 3))
+    f[:, 16] = f[:, 12] ** 2 * rng.uniform(0.5, 1.5, pg.total)
+    f[:, 17:20] = rng.normal(0.0, 100.0, (pg.total, 3))
+    gx, gy, gz = p[0]["grid"]
+    idx = np.arange(pg.total)
+    f[:, 20], f[:, 21], f[:, 22] = idx % gx, (idx // gx) % gy, idx // (gx * gy)
+    f[:, 23] = rng.choice((0.0, 1.0, 5.0), pg.total, p=(0.1, 0.3, 0.6))
+    f[rng.random(pg.total) < 0.05, 20] += gx             # a foreign world cell
+    f[:, 45:48] = rng.uniform(-6.0, 6.0, (pg.total, 3))   # relocation offsets
+    pg.upload(rec)
+    for cam in (vg.Camera(pos=(128.5, 90.0, 128.5), yaw=0.7, pitch=-0.35).uniform(W, H),
+                vg.Camera(pos=(40.0, 75.0, 200.0), yaw=2.4, pitch=-0.2).uniform(W, H)):
+        print("case: frame indirect over random records", flush=True)
+        got, ref = indirect_case(small, p, cam, W, H, pg)
+        assert got.tobytes() == ref.tobytes()
+        assert (got[:, :3] > 0).mean() > 0.1
+    q = p.copy()
+    q["grid"] = (4, 4, 4)
+    hits, _ = small.trace_frame(cam, W, H, 512.0, 0, SUN, 480.0)
+    with pytest.raises(VGError):
+        pg.frame_indirect(q, cam, W, H, hits)

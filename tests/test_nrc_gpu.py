@@ -193,7 +193,7 @@ def test_training_targets_match_oracle(ctx, small, small_world):
 def test_online_training_gains_over_no_cache(ctx, small, small_world):
     """32 gather rays per probe: the trained cache's bounce tail versus the reference's
     ambient tail (the method without the network) and versus an untrained cache, all
-    judged against the FP64 4096-ray 2-bounce estimate."""
+    measured against the FP64 4096-ray 2-bounce estimate."""
     p, dirs, bt, pal = probe_setup(small)
     ow = oracle.OracleWorld.view(small_world)
     pg = gi.ProbeGrid(small, p)

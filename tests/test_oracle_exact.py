@@ -163,7 +163,7 @@ def test_lod_topmost_material():
     o = np.array([0.5, 101.5, 100.5], np.float32)
     d = np.array([1, 0, 0], np.float32)
     h, _ = ow.trace_r1(o, d, flags=oracle.LOD)
-    # each layer is judged by its lowest set bit only; a turf there drops to the next layer,
+    # each layer is decided by its lowest set bit only; a turf there drops to the next layer,
     # so the answer is layer 2's dirt, not layer 3's grass
     assert h["hit"] and tuple(h["voxel"]) == (440, 101, 100) and h["material"] == 3
     assert h["t"] == np.float32(439.5) and h["face"] == 1

@@ -545,22 +545,6 @@ using namespace vg;
 
 namespace vg {
 
-static CamParams cam_params(const vg_camera *c) {
-    CamParams p;
-    for (int k = 0; k < 3; ++k) {
-        p.origin[k] = c->origin[k];
-        p.forward[k] = c->forward[k];
-        p.right[k] = c->right[k];
-        p.up[k] = c->up[k];
-    }
-    p.tan_half_fov = c->tan_half_fov;
-    p.res[0] = c->resolution[0];
-    p.res[1] = c->resolution[1];
-    p.jitter[0] = c->jitter[0];
-    p.jitter[1] = c->jitter[1];
-    return p;
-}
-
 static int ensure_beam_capacity(vg_world *w, size_t blocks) {
     if (blocks <= w->beam_cap) return VG_OK;
     HIP_TRY(hipDeviceSynchronize());

@@ -843,6 +843,153 @@ __global__ __launch_bounds__(kBlock) void k_nrc_targets(
 }
 
 // --------------------------------------------------------------------- host --
+// ------------------------------------------------------------- probe sampling --
+// The probe cache's consumer (shaders/gi_probes.wgsl:69-179 sh_eval_irradiance, sh_eval_scalar,
+// sample_probes; shaders/rt_gi.wgsl:128-139 indirect_light in probe mode). Operations are
+// plain IEEE fp32 in a fixed order (dot products left to right, no contraction), so the
+// oracle's restatement gives the same bits.
+
+constexpr float kGiMaxT = 90.0f;     // GI_MAX_T (shaders/raymarch.wgsl:3821)
+constexpr float kGiFadeT = 54.0f;    // GI_MAX_T * 0.6, the fade's start (shaders/rt_gi.wgsl:133)
+constexpr float kGiStrength = 1.0f;  // GI_STRENGTH (shaders/rt_gi.wgsl:13)
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ float smoothstep01(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// Deringed cosine-lobe irradiance / pi of a probe's SH-L1 radiance toward normal n.
+__device__ __forceinline__ void sh_irradiance(const float4 *rec, const float n[3], float out[3]) {
+    const float a1 = (float)(0.488603 * 0.666667);
+    const float lv[3] = {n[1], n[2], n[0]}; // stored order (c1y, c1z, c1x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float4 sh = rec[c];
+        const float l[3] = {sh.y, sh.z, sh.w};
+        const float dc = sh.x * 0.282095f;
+        const float lin = dot3(l, lv) * a1;
+        const float lmax = sqrtf(dot3(l, l)) * a1;
+        const float q = 0.95f * dc / (lmax > 1e-6f ? lmax : 1e-6f);
+        const float sc = q < 1.0f ? q : 1.0f;
+        const float e = dc + sc * lin;
+        out[c] = e > 0.0f ? e : 0.0f;
+    }
+}
+
+// SH-L1 scalar field toward unit direction d (distance moments).
+__device__ __forceinline__ float sh_scalar(const float4 c, const float d[3]) {
+    return c.x * 0.282095f + c.y * (0.488603f * d[1]) + c.z * (0.488603f * d[2]) +
+           c.w * (0.488603f * d[0]);
+}
+
+// sample_probes: the 8 probes around p (biased off the surface along n and back along the
+// view direction v), smoothstepped trilinear weights, the backface weight and the
+// one-tailed Chebyshev visibility bound; probes not gathered for that cell are skipped.
+__device__ void sample_probes(const vg_probe *__restrict__ probes, const int32_t grid[3],
+                              int32_t spacing, const float pw[3], const float n[3],
+                              const float v[3], float out[3]) {
+    float pq[3], pg[3], f[3];
+    int32_t base[3];
+    const float sp = (float)spacing;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        pq[k] = (pw[k] + n[k] * 1.0f) - v[k] * 1.0f;
+        pg[k] = pq[k] / sp - 0.5f;
+        const float fb = floorf(pg[k]);
+        base[k] = (int32_t)fb;
+        const float fl = pg[k] - fb;
+        f[k] = fl * fl * (3.0f - 2.0f * fl);
+    }
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    float wsum = 0.0f;
+    for (int i = 0; i < 8; ++i) {
+        const int32_t off[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
+        float w = 1.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) w *= off[k] ? f[k] : 1.0f - f[k];
+        if (w <= 0.0f) continue;
+        int32_t g[3], slot[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            g[k] = base[k] + off[k];
+            int32_t m = g[k] % grid[k];
+            slot[k] = m < 0 ? m + grid[k] : m;
+        }
+        const float4 *rec = reinterpret_cast<const float4 *>(
+            probes + slot[0] + (int64_t)slot[1] * grid[0] + (int64_t)slot[2] * grid[0] * grid[1]);
+        const float4 info = rec[5];
+        if (info.w < 0.5f || info.x != (float)g[0] || info.y != (float)g[1] || info.z != (float)g[2])
+            continue;
+        const float4 st = rec[11];
+        const float rel[3] = {st.y, st.z, st.w};
+        float dp[3], dn[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            dp[k] = (((float)g[k] + 0.5f) * sp + rel[k]) - pq[k];
+            dn[k] = dp[k] + n[k] * 0.001f;
+        }
+        const float inv = 1.0f / sqrtf(dot3(dn, dn));
+        const float d[3] = {dn[0] * inv, dn[1] * inv, dn[2] * inv};
+        float bf = dot3(d, n);
+        bf = bf > 0.0f ? bf : 0.0f;
+        w = w * (bf * bf + 0.05f);
+        const float dist = sqrtf(dot3(dp, dp));
+        const float tp[3] = {-d[0], -d[1], -d[2]};
+        const float mu = sh_scalar(rec[3], tp);
+        float delta = dist - (mu + 0.35f);
+        delta = delta > 0.0f ? delta : 0.0f;
+        if (delta > 0.0f) {
+            const float mu2 = sh_scalar(rec[4], tp);
+            float var = mu2 - mu * mu;
+            var = var > 0.25f ? var : 0.25f;
+            float c = var / (var + delta * delta);
+            c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+            w = w * c;
+        }
+        float e[3];
+        sh_irradiance(rec, n, e);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] = acc[k] + e[k] * w;
+        wsum = wsum + w;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = wsum <= 1e-4f ? 0.0f : acc[k] / wsum;
+}
+
+// indirect_light in probe mode for every pixel of a traced frame: the hit point is
+// camera origin + dir * t (the pixel's primary direction), the normal the hit face's.
+// Misses and hits at t >= GI_MAX_T get zero. out4[pixel] = (r, g, b, 0).
+__global__ __launch_bounds__(kBlock) void k_frame_indirect(const vg_probe *__restrict__ probes,
+                                                           vg_gi_params p, CamParams cam,
+                                                           int32_t width, int32_t height,
+                                                           const vg_hit *__restrict__ hits,
+                                                           float4 *__restrict__ out4) {
+    const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= (int64_t)width * height) return;
+    HitRec h;
+    load_hit(hits + pix, h);
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (h.hit && h.t < kGiMaxT) {
+        float d[3], pw[3], n[3], e[3];
+        primary_dir(cam, (int32_t)(pix % width), (int32_t)(pix / width), d);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            pw[k] = cam.origin[k] + d[k] * h.t;
+            n[k] = face_n(h.face, k);
+        }
+        sample_probes(probes, p.grid, p.spacing, pw, n, d, e);
+        const float fade = 1.0f - smoothstep01(kGiFadeT, kGiMaxT, h.t);
+        const float s = kGiStrength * fade;
+        o = make_float4(e[0] * s, e[1] * s, e[2] * s, 0.0f);
+    }
+    out4[pix] = o;
+}
+
 static int check_params(const vg_probes *pr, const vg_gi_params *p, bool learned) {
     if (!p) return fail(VG_E_INVALID, "null params");
     for (int k = 0; k < 3; ++k)
@@ -1066,6 +1213,28 @@ int vg_probes_destroy(vg_probes *pr) {
     (void)hipDeviceSynchronize();
     free_probes(pr);
     delete pr;
+    return VG_OK;
+}
+
+int vg_frame_indirect(vg_probes *pr, const vg_gi_params *p, const vg_camera *camera,
+                      int32_t width, int32_t height, const vg_hit *hits, float *out4,
+                      void *stream) {
+    if (!pr || !p || !camera) return fail(VG_E_INVALID, "null argument");
+    int st = check_world(pr->world);
+    if (st) return st;
+    for (int k = 0; k < 3; ++k)
+        if (p->grid[k] != pr->grid[k]) return fail(VG_E_INVALID, "params grid differs from the probes'");
+    if (p->spacing <= 0) return fail(VG_E_INVALID, "spacing %d", p->spacing);
+    if (width <= 0 || height <= 0) return fail(VG_E_INVALID, "size %dx%d", width, height);
+    if ((int64_t)width * height > (1ll << 31)) return fail(VG_E_INVALID, "frame too large");
+    if (!hits || !out4) return fail(VG_E_INVALID, "null hits/out");
+    if (((uintptr_t)out4 & 15u) != 0) return fail(VG_E_INVALID, "out4 must be 16-byte aligned");
+    if (((uintptr_t)hits & 3u) != 0) return fail(VG_E_INVALID, "hits must be 4-byte aligned");
+    const int64_t n = (int64_t)width * height;
+    hipLaunchKernelGGL(k_frame_indirect, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, pr->probes, *p, cam_params(camera), width, height, hits,
+                       reinterpret_cast<float4 *>(out4));
+    HIP_TRY(hipGetLastError());
     return VG_OK;
 }
 

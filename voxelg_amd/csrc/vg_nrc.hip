@@ -356,6 +356,8 @@ __global__ __launch_bounds__(256) void k_nrc_encode(NrcDev m, const vg_nrc_query
 // Blocks of the record count / scatter passes; each takes a contiguous range of
 // (sample, level) pairs, pair p = s * levels + l.
 constexpr uint32_t kGBlocks = 256;
+constexpr uint32_t kGThreads = 1024; // count / scatter block size
+constexpr uint32_t kGRThreads = 512; // per-bucket sum block size
 
 __device__ __forceinline__ void pair_range(uint32_t pairs, uint32_t &p0, uint32_t &p1) {
     const uint32_t per = (pairs + kGBlocks - 1) / kGBlocks;
@@ -384,16 +386,16 @@ __global__ __launch_bounds__(256) void k_nrc_gmax(const float2 *__restrict__ dfe
 
 // Record counts per (bucket, block): hist[bucket * kGBlocks + block]. Pairs whose gradient is
 // zero emit no records.
-__global__ __launch_bounds__(256) void k_nrc_gcount(NrcDev m, const vg_nrc_query *__restrict__ q,
+__global__ __launch_bounds__(kGThreads) void k_nrc_gcount(NrcDev m, const vg_nrc_query *__restrict__ q,
                                                     const float2 *__restrict__ dfeat, uint32_t n,
                                                     int gbits, uint32_t nbuckets,
                                                     uint32_t *__restrict__ hist) {
     extern __shared__ uint32_t cnt[];
-    for (uint32_t b = threadIdx.x; b < nbuckets; b += 256u) cnt[b] = 0u;
+    for (uint32_t b = threadIdx.x; b < nbuckets; b += kGThreads) cnt[b] = 0u;
     __syncthreads();
     uint32_t p0, p1;
     pair_range(n * (uint32_t)m.levels, p0, p1);
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += 256u) {
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kGThreads) {
         const uint32_t s = p / (uint32_t)m.levels, l = p - s * (uint32_t)m.levels;
         const float2 d = dfeat[(size_t)s * 16u + l];
         if (d.x == 0.0f && d.y == 0.0f) continue;
@@ -406,25 +408,25 @@ __global__ __launch_bounds__(256) void k_nrc_gcount(NrcDev m, const vg_nrc_query
         for (int k = 0; k < 8; ++k) atomicAdd(&cnt[idx[k] >> gbits], 1u);
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < nbuckets; b += 256u) hist[(size_t)b * kGBlocks + blockIdx.x] = cnt[b];
+    for (uint32_t b = threadIdx.x; b < nbuckets; b += kGThreads) hist[(size_t)b * kGBlocks + blockIdx.x] = cnt[b];
 }
 
 // The same records, written at their bucket's place: the scanned hist gives every (bucket,
 // block) its first slot; inside a block the slots are taken in any order (the sums below
 // do not depend on it).
-__global__ __launch_bounds__(256) void k_nrc_gscatter(NrcDev m, const vg_nrc_query *__restrict__ q,
+__global__ __launch_bounds__(kGThreads) void k_nrc_gscatter(NrcDev m, const vg_nrc_query *__restrict__ q,
                                                       const float2 *__restrict__ dfeat, uint32_t n,
                                                       int gbits, uint32_t nbuckets,
                                                       const uint32_t *__restrict__ hoff,
                                                       uint32_t *__restrict__ rk,
                                                       float2 *__restrict__ rv) {
     extern __shared__ uint32_t cur[];
-    for (uint32_t b = threadIdx.x; b < nbuckets; b += 256u) cur[b] = hoff[(size_t)b * kGBlocks + blockIdx.x];
+    for (uint32_t b = threadIdx.x; b < nbuckets; b += kGThreads) cur[b] = hoff[(size_t)b * kGBlocks + blockIdx.x];
     __syncthreads();
     const uint32_t lmask = (1u << gbits) - 1u;
     uint32_t p0, p1;
     pair_range(n * (uint32_t)m.levels, p0, p1);
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += 256u) {
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kGThreads) {
         const uint32_t s = p / (uint32_t)m.levels, l = p - s * (uint32_t)m.levels;
         const float2 d = dfeat[(size_t)s * 16u + l];
         if (d.x == 0.0f && d.y == 0.0f) continue;
@@ -455,7 +457,7 @@ __device__ __forceinline__ void adam1(float &p, float &m, float &v, float g, flo
 // the order of the records). Then, per entry and feature with a non-zero sum: Adam (adam set;
 // the bf16 copy the kernels read is refreshed), or the dense gradient (dense set). A feature
 // whose sum is zero is left alone, like an entry no sample touched.
-__global__ __launch_bounds__(256) void k_nrc_greduce(const uint32_t *__restrict__ hoff,
+__global__ __launch_bounds__(kGRThreads) void k_nrc_greduce(const uint32_t *__restrict__ hoff,
                                                      const uint32_t *__restrict__ rk,
                                                      const float2 *__restrict__ rv, int gbits,
                                                      const uint32_t *gmax, float2 *__restrict__ p,
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(256) void k_nrc_greduce(const uint32_t *__restrict_
                                                      const uint32_t *step) {
     extern __shared__ unsigned long long acc[]; // 2 per entry
     const uint32_t ne = 1u << gbits;
-    for (uint32_t i = threadIdx.x; i < 2u * ne; i += 256u) acc[i] = 0ull;
+    for (uint32_t i = threadIdx.x; i < 2u * ne; i += kGRThreads) acc[i] = 0ull;
     __syncthreads();
     const uint32_t b = blockIdx.x;
     const uint32_t r0 = hoff[(size_t)b * kGBlocks], r1 = hoff[(size_t)(b + 1u) * kGBlocks];
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(256) void k_nrc_greduce(const uint32_t *__restrict_
     const int K = 38 - e;
     const double up = ldexp(1.0, K), down = ldexp(1.0, -K);
     if (gm > 0.0f && gm <= 3.4e38f) // a non-finite backward pass contributes nothing
-        for (uint32_t r = r0 + threadIdx.x; r < r1; r += 256u) {
+        for (uint32_t r = r0 + threadIdx.x; r < r1; r += kGRThreads) {
             const uint32_t k = rk[r];
             const float2 v = rv[r];
             atomicAdd(&acc[2u * k], (unsigned long long)__double2ll_rn((double)v.x * up));
@@ -489,7 +491,7 @@ __global__ __launch_bounds__(256) void k_nrc_greduce(const uint32_t *__restrict_
         c1 = 1.0f - powf(b1, t);
         c2 = 1.0f - powf(b2, t);
     }
-    for (uint32_t i = threadIdx.x; i < ne; i += 256u) {
+    for (uint32_t i = threadIdx.x; i < ne; i += kGRThreads) {
         const size_t entry = ((size_t)b << gbits) + i;
         const float gx = (float)((double)(long long)acc[2u * i] * down);
         const float gy = (float)((double)(long long)acc[2u * i + 1u] * down);
@@ -754,12 +756,12 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
     hipLaunchKernelGGL(k_nrc_gmax, dim3(256), dim3(256), 0, stream,
                        reinterpret_cast<const float2 *>(m->dfeat), n, m->cfg.levels, m->gmax);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_nrc_gcount, dim3(kGBlocks), dim3(256), nb * sizeof(uint32_t), stream, dv, q,
+    hipLaunchKernelGGL(k_nrc_gcount, dim3(kGBlocks), dim3(kGThreads), nb * sizeof(uint32_t), stream, dv, q,
                        reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, m->hist);
     HIP_TRY(hipGetLastError());
     size_t tb = m->ctemp_bytes;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(m->ctemp, tb, m->hist, hoff, (int)nh, stream));
-    hipLaunchKernelGGL(k_nrc_gscatter, dim3(kGBlocks), dim3(256), nb * sizeof(uint32_t), stream, dv,
+    hipLaunchKernelGGL(k_nrc_gscatter, dim3(kGBlocks), dim3(kGThreads), nb * sizeof(uint32_t), stream, dv,
                        q, reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, hoff, m->rk,
                        m->rv);
     HIP_TRY(hipGetLastError());
@@ -767,7 +769,7 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
                        stream, m->partial, blocks, m->mlp_g, m->loss, m->step, update ? 1 : 0);
     HIP_TRY(hipGetLastError());
     const vg_nrc_config &c = m->cfg;
-    hipLaunchKernelGGL(k_nrc_greduce, dim3(nb), dim3(256), (2u << m->gbits) * sizeof(uint64_t),
+    hipLaunchKernelGGL(k_nrc_greduce, dim3(nb), dim3(kGRThreads), (2u << m->gbits) * sizeof(uint64_t),
                        stream, hoff, m->rk, m->rv, m->gbits, m->gmax,
                        reinterpret_cast<float2 *>(m->grid), reinterpret_cast<float2 *>(m->grid_m),
                        reinterpret_cast<float2 *>(m->grid_v), m->grid_h, dense, update ? 1 : 0,

@@ -49,6 +49,23 @@ inline int check_world(const vg_world *w) {
     return set_device(w);
 }
 
+// The kernels' view of a vg_camera (CameraUniform order, src/camera.rs:25-166).
+inline CamParams cam_params(const vg_camera *c) {
+    CamParams p;
+    for (int k = 0; k < 3; ++k) {
+        p.origin[k] = c->origin[k];
+        p.forward[k] = c->forward[k];
+        p.right[k] = c->right[k];
+        p.up[k] = c->up[k];
+    }
+    p.tan_half_fov = c->tan_half_fov;
+    p.res[0] = c->resolution[0];
+    p.res[1] = c->resolution[1];
+    p.jitter[0] = c->jitter[0];
+    p.jitter[1] = c->jitter[1];
+    return p;
+}
+
 // Host-side normalize_or_zero for the sun direction (same expression as the device).
 inline bool host_normalize(const float in[3], float out[3]) {
     float len = sqrtf(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);

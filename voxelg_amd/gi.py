@@ -192,6 +192,19 @@ class ProbeGrid:
         check(lib().vg_probes_reset(self._h, _stream(stream)), "vg_probes_reset")
         return self
 
+    def frame_indirect(self, params: np.ndarray, camera: np.ndarray, width: int, height: int,
+                       hits, out=None, stream=None):
+        """indirect_light in probe mode (shaders/rt_gi.wgsl:128-139, sample_probes
+        shaders/gi_probes.wgsl:106-179) for every pixel of a traced frame: hits is the
+        frame's device hit buffer; returns a device buffer of (r, g, b, 0) float4 per pixel."""
+        p = np.ascontiguousarray(params, GI_PARAMS_DTYPE)
+        cam = np.ascontiguousarray(camera, np.float32)
+        if out is None:
+            out = self.world._buf(width * height * 16)
+        check(lib().vg_frame_indirect(self._h, ptr(p), ptr(cam), width, height, hits.data_ptr(),
+                                      out.data_ptr(), _stream(stream)), "vg_frame_indirect")
+        return out
+
     def set_palette(self, rgba: np.ndarray) -> "ProbeGrid":
         a = np.ascontiguousarray(rgba, np.float32).reshape(256, 4)
         check(lib().vg_probes_set_palette(self._h, ptr(a)), "vg_probes_set_palette")
