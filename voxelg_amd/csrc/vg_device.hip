@@ -720,7 +720,7 @@ int vg_ctx_create(int device, vg_ctx **out) {
     HIP_TRY(hipSetDevice(device));
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    vg_ctx *c = new (std::nothrow) vg_ctx;
+    vg_ctx *c = new (std::nothrow) vg_ctx();
     if (!c) return fail(VG_E_NOMEM, "ctx");
     c->device = device;
     c->cus = cus > 0 ? cus : 1;
