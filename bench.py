@@ -194,6 +194,22 @@ def main():
         ctx.synchronize()
         log(f"copy calibration: {nbytes} bytes device-to-device")
 
+    # the water secondary rays of the same frame (reflection + refraction of every water
+    # pixel, SKIP_TRANSPARENT, tmax 200), outside the timed region
+    water = (hh["hit"] == 1) & (hh["material"] >= 5) & (hh["material"] <= 12)
+    refl, refr = dw.trace_water_secondary(cam, W, H, hits, stream=stream)
+    wa, wb = ctx.event(), ctx.event()
+    wa.record(stream)
+    for _ in range(10):
+        dw.trace_water_secondary(cam, W, H, hits, refl, refr, stream)
+    wb.record(stream)
+    stream.synchronize()
+    water_ms = wa.elapsed_ms(wb) / 10
+    water_out = {"pixels": int(water.sum()), "rays": 2 * int(water.sum()),
+                 "ms": round(water_ms, 4),
+                 "mrays_per_s": round(2 * int(water.sum()) / water_ms / 1e3, 1)}
+    log(f"water secondary rays: {water_out}")
+
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(world, wl, cam, sun)
@@ -230,6 +246,7 @@ def main():
             "algorithmic_bytes": int(bytes_per_frame),
         },
         "cpu_baseline": base,
+        "water_secondary": water_out,
     }
     print(json.dumps(out), flush=True)
 

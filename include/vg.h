@@ -270,6 +270,16 @@ int vg_trace_frame_blocks(vg_world *world, const vg_camera *camera_host, int32_t
                           int32_t height, float primary_tmax, uint32_t primary_flags,
                           const float sun[3], float shadow_tmax, const uint32_t *mask,
                           vg_hit *hits, uint8_t *occluded, void *stream);
+/* The secondary rays shade_water_top casts at water surfaces (shaders/raymarch.wgsl:
+ * 3534-3667) for a traced frame: for every pixel whose primary hit is water (materials 5..12)
+ * entered through a face, with p = camera origin + d t and n the face normal, the reflection
+ * ray reflect(d, n) from p + n (0.01 + 0.34 graze^2), graze = 1 - clamp(-d . n, 0, 1), and
+ * the refraction ray refract(d, n, 1/1.33) (d on total internal reflection) from p + d 0.001,
+ * each traced as a batched query with VG_RAY_SKIP_TRANSPARENT (trace_no_water) and tmax 200
+ * (SECONDARY_MAX_T). refl_hits / refr_hits: device, one record per pixel (misses elsewhere). */
+int vg_trace_water_secondary(vg_world *world, const vg_camera *camera_host, int32_t width,
+                             int32_t height, const vg_hit *primary_hits, vg_hit *refl_hits,
+                             vg_hit *refr_hits, void *stream);
 /* The world's device state back in the reference layout (any pointer may be NULL; counts
  * from vg_world_counts). Synchronous. */
 int vg_world_download(vg_world *world, vg_brick *bricks, uint64_t *tile_mask,

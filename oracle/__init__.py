@@ -123,6 +123,7 @@ def lib():
             "orc_gi_update_learned": (None, [W, P, P, P, P, P, P, C.c_int, U64P]),
             "orc_gi_bounce_tail": (None, [W, P, P, P, P, C.c_uint32, P, U64P]),
             "orc_gi_bounce_tails": (None, [W, P, P, P, C.c_int64, P, P, P, C.c_int]),
+            "orc_water_rays": (None, [P, C.c_int32, C.c_int32, P, P]),
             "orc_sh_irradiance": (None, [P, F32P, F32P]),
             "orc_sample_probes": (None, [P, I32P, C.c_int32, F32P, F32P, F32P, F32P]),
             "orc_frame_indirect": (None, [P, I32P, C.c_int32, P, C.c_int32, C.c_int32, P, P]),
@@ -287,6 +288,14 @@ class OracleWorld:
                                  cnt, _threads(threads))
         return out, dict(zip(IT_NAMES, (int(c) for c in cnt)))
 
+    def water_secondary(self, camera, width, height, hits: np.ndarray, threads=0):
+        """Reflection and refraction hit records per pixel of a frame's primary hits: the
+        rays of orc_water_rays traced by R1 (misses where the primary hit is not water)."""
+        rays = water_rays(camera, width, height, hits)
+        out, _ = self.trace_rays(rays, threads=threads)
+        out = out.reshape(height * width, 2)
+        return np.ascontiguousarray(out[:, 0]), np.ascontiguousarray(out[:, 1])
+
     def raycast_rays(self, rays: np.ndarray, threads=0) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(len(rays), HIT_DTYPE)
@@ -448,6 +457,16 @@ def frame_indirect(probes, grid, spacing, camera, width, height, hits: np.ndarra
     out = np.zeros((width * height, 4), np.float32)
     lib().orc_frame_indirect(_p(rec), _i3(grid), int(spacing), _p(cam), int(width), int(height),
                              _p(h), _p(out))
+    return out
+
+
+def water_rays(camera, width, height, hits: np.ndarray) -> np.ndarray:
+    """orc_water_rays: two rays per pixel (reflection, refraction) as RAY_DTYPE records."""
+    cam = np.ascontiguousarray(camera, np.float32)
+    h = np.ascontiguousarray(hits).view(np.uint8)
+    assert h.size == 20 * width * height
+    out = np.zeros(2 * width * height, RAY_DTYPE)
+    lib().orc_water_rays(_p(cam), int(width), int(height), _p(h), _p(out))
     return out
 
 

@@ -1303,3 +1303,53 @@ void orc_frame_r0(const orc_world *w, const float *cam, int32_t width, int32_t h
     p.shadow = shadow;
     run_jobs(&p, threads, NULL);
 }
+
+/* The secondary rays shade_water_top casts at a water surface (shaders/raymarch.wgsl:
+ * 3534-3667), fp32 in the device's operation order: for a primary hit of water (5..12)
+ * entered through a face, p = origin + d t, n the face normal; reflection reflect(d, n)
+ * (WGSL: d - 2 dot(n, d) n) from p + n (0.01 + 0.34 graze^2), graze = 1 - clamp(-d . n, 0, 1);
+ * refraction refract(d, n, 1/1.33) (WGSL: k = 1 - eta^2 (1 - dot^2), zero if k < 0, else
+ * eta d - (eta dot + sqrt k) n; d itself when its length is below 0.01) from p + d 0.001.
+ * out: two rays per pixel (reflection, refraction), tmax 200, VG_RAY_SKIP_TRANSPARENT; a
+ * pixel without a water hit gets zero-direction rays (they miss). */
+void orc_water_rays(const float *cam, int32_t width, int32_t height, const uint8_t *hits,
+                    orc_ray *out) {
+    for (int64_t pix = 0; pix < (int64_t)width * height; ++pix) {
+        const uint8_t *hr = hits + 20 * pix;
+        float t;
+        memcpy(&t, hr + 12, 4);
+        const uint32_t face = hr[16], mat = hr[17], hit = hr[18];
+        orc_ray *a = out + 2 * pix, *b = a + 1;
+        memset(a, 0, 2 * sizeof(orc_ray));
+        a->tmax = b->tmax = 200.0f;
+        a->flags = b->flags = 2u;
+        if (!(hit && mat >= 5u && mat <= 12u && face < 6u)) continue;
+        float d[3], n[3], pw[3];
+        orc_primary_dir(cam, (int32_t)(pix % width), (int32_t)(pix / width), d);
+        for (int k = 0; k < 3; ++k) {
+            n[k] = ((int)(face >> 1) == k) ? ((face & 1u) ? -1.0f : 1.0f) : 0.0f;
+            pw[k] = cam[k] + d[k] * t;
+        }
+        const float dn = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+        float c = -d[0] * n[0] + -d[1] * n[1] + -d[2] * n[2];
+        c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+        const float graze = 1.0f - c;
+        const float lift = 0.01f + 0.34f * graze * graze;
+        for (int k = 0; k < 3; ++k) {
+            a->d[k] = d[k] - 2.0f * dn * n[k];
+            a->o[k] = pw[k] + n[k] * lift;
+        }
+        const float eta = (float)(1.0 / 1.33);
+        const float kk = 1.0f - eta * eta * (1.0f - dn * dn);
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if (!(kk < 0.0f)) {
+            const float sk = sqrtf(kk);
+            for (int k = 0; k < 3; ++k) f[k] = eta * d[k] - (eta * dn + sk) * n[k];
+        }
+        const int tir = sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) < 0.01f;
+        for (int k = 0; k < 3; ++k) {
+            b->d[k] = tir ? d[k] : f[k];
+            b->o[k] = pw[k] + d[k] * 0.001f;
+        }
+    }
+}

@@ -161,6 +161,11 @@ void orc_raycast_r0_batch(const orc_world *w, const orc_ray *rays, size_t n, orc
 
 /* primary rays + frame (CameraUniform-compatible 176-byte camera, see src/camera.rs:61-113) */
 void orc_primary_dir(const float *camera176, int32_t px, int32_t py, float out[3]);
+/* The water secondary rays of a frame's primary hits (20-byte records): two rays per pixel
+ * (reflection, refraction; tmax 200, VG_RAY_SKIP_TRANSPARENT), zero directions where the
+ * primary hit is not water entered through a face (shaders/raymarch.wgsl:3534-3667). */
+void orc_water_rays(const float *camera176, int32_t width, int32_t height, const uint8_t *hits,
+                    orc_ray *out);
 void orc_primary_rays(const float *camera176, int32_t w, int32_t h, float tmax, uint32_t flags,
                       orc_ray *out);
 /* One frame: primary (R1, flags) then a sun shadow ray per hit. */

@@ -174,6 +174,7 @@ SIGNATURES = {
     "vg_probes_download": (C.c_int, [_P, _P]),
     "vg_probes_upload": (C.c_int, [_P, _P]),
     "vg_probes_update": (C.c_int, [_P, _P, _P]),
+    "vg_trace_water_secondary": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, _P, _P]),
     "vg_frame_indirect": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P, _P]),
     "vg_probes_update_counts": (C.c_int, [_P, _P, _U64P, _U64P]),
     "vg_nrc_config_default": (C.c_int, [_P]),

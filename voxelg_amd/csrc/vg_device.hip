@@ -335,6 +335,173 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     if (COUNT) flush_counts(cnt, counts);
 }
 
+// ------------------------------------------------------------ water rays --
+// The secondary rays shade_water_top casts at a water surface (shaders/raymarch.wgsl:
+// 3534-3667): at the primary hit p = origin + d t with face normal n, the reflection
+// reflect(d, n) from p + n (0.01 + 0.34 graze^2), graze = 1 - clamp(-d . n, 0, 1), and the
+// refraction refract(d, n, 1/1.33) (d itself on total internal reflection, length < 0.01)
+// from p + d 0.001. Both are traced as batched queries: direction normalize_or_zero'd,
+// VG_RAY_SKIP_TRANSPARENT (trace_no_water), tmax SECONDARY_MAX_T = 200.
+constexpr float kSecondaryMaxT = 200.0f;
+
+__device__ __forceinline__ bool is_water(uint32_t m) { return m >= 5u && m <= 12u; }
+
+__device__ __forceinline__ void water_rays(const float o[3], const float d[3], float t, uint32_t face,
+                                           float ro[3], float rd[3], float fo[3], float fd[3]) {
+    float n[3], pw[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        n[k] = (face < 6u && (int)(face >> 1) == k) ? ((face & 1u) ? -1.0f : 1.0f) : 0.0f;
+        pw[k] = o[k] + d[k] * t;
+    }
+    const float dn = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+    float c = -d[0] * n[0] + -d[1] * n[1] + -d[2] * n[2];
+    c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    const float graze = 1.0f - c;
+    const float lift = 0.01f + 0.34f * graze * graze;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        rd[k] = d[k] - 2.0f * dn * n[k];
+        ro[k] = pw[k] + n[k] * lift;
+    }
+    const float eta = (float)(1.0 / 1.33);
+    const float kk = 1.0f - eta * eta * (1.0f - dn * dn);
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    if (!(kk < 0.0f)) {
+        const float sk = sqrtf(kk);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) f[k] = eta * d[k] - (eta * dn + sk) * n[k];
+    }
+    const bool tir = sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) < 0.01f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        fd[k] = tir ? d[k] : f[k];
+        fo[k] = pw[k] + d[k] * 0.001f;
+    }
+}
+
+// One thread per pixel in 8x8 block order: pixels whose primary hit is water entered through
+// a face are appended to the list (one atomic per wave, so the list keeps runs of
+// neighbouring pixels); every other pixel gets its two miss records here.
+__global__ __launch_bounds__(kBlock) void k_water_compact(int32_t width, int32_t height, uint32_t nbx,
+                                                          uint32_t total_ids,
+                                                          const vg_hit *__restrict__ prim,
+                                                          vg_hit *__restrict__ refl,
+                                                          vg_hit *__restrict__ refr,
+                                                          uint32_t *__restrict__ list,
+                                                          uint32_t *list_n) {
+    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    bool water = false;
+    int64_t pix = 0;
+    if (id < total_ids) {
+        const uint32_t blk = id >> 6, qd = id & 63u;
+        const int32_t px = (int32_t)((blk % nbx) * 8 + (qd & 7));
+        const int32_t py = (int32_t)((blk / nbx) * 8 + (qd >> 3));
+        if (px < width && py < height) {
+            pix = (int64_t)py * width + px;
+            HitRec p0;
+            load_hit(prim + pix, p0);
+            water = p0.hit && is_water(p0.mat) && p0.face < 6u;
+            if (!water) {
+                HitRec m;
+                hit_miss(m);
+                store_hit(refl + pix, m);
+                store_hit(refr + pix, m);
+            }
+        }
+    }
+    const uint64_t b = __ballot(water);
+    if (!b) return; // wave-uniform
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(list_n, (uint32_t)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (water) list[base + lanes_below(b)] = (uint32_t)pix;
+}
+
+// Persistent, ray-compacting over the listed water pixels: a lane traces the pixel's
+// reflection ray, then its refraction ray.
+__global__ __launch_bounds__(kBlock) void k_water_persistent(
+    DevWorld w, CamParams cam, int32_t width, const uint32_t *__restrict__ list,
+    const uint32_t *list_n, const vg_hit *__restrict__ prim, vg_hit *__restrict__ refl,
+    vg_hit *__restrict__ refr, uint32_t *work, int threshold) {
+    const uint32_t total_ids = *list_n;
+    const uint32_t lane = threadIdx.x & 63;
+    const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
+    int kind = 0; // 0 idle, 1 reflection ray, 2 refraction ray
+    int64_t pix = 0;
+    int it = 0;
+    float fo[3] = {0.0f, 0.0f, 0.0f}, fd[3] = {0.0f, 0.0f, 0.0f}; // the pending refraction ray
+    Walk r;
+    WalkCache q;
+    HitRec h;
+    hit_miss(h);
+    cache_reset(q);
+    auto start = [&](const float o[3], const float draw[3], int k) -> bool {
+        float d[3];
+        hit_miss(h);
+        if (!normalize_or_zero(draw, d) || !walk_init(r, w, o, d)) return false;
+        cache_reset(q);
+        it = 0;
+        kind = k;
+        return true;
+    };
+    auto refraction = [&]() {
+        if (!start(fo, fd, 2)) {
+            store_hit(refr + pix, h);
+            kind = 0;
+        }
+    };
+    bool exhausted = false;
+    for (;;) {
+        uint64_t act = __ballot(kind != 0);
+        int nact = __popcll(act);
+        if (!exhausted && nact < threshold) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
+            base = __shfl(base, 0, 64);
+            if (base >= total_ids) {
+                exhausted = true;
+            } else if (kind == 0) {
+                const uint32_t id = base + lanes_below(~act);
+                if (id < total_ids) {
+                    pix = list[id];
+                    HitRec p0;
+                    load_hit(prim + pix, p0);
+                    float d[3], ro[3], rd[3];
+                    primary_dir(cam, (int32_t)(pix % width), (int32_t)(pix / width), d);
+                    water_rays(cam.origin, d, p0.t, p0.face, ro, rd, fo, fd);
+                    if (!start(ro, rd, 1)) {
+                        store_hit(refl + pix, h);
+                        refraction();
+                    }
+                }
+            }
+            nact = __popcll(__ballot(kind != 0));
+        }
+        if (nact == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        do {
+            if (kind != 0) {
+                bool done = walk_iter<false>(r, q, w, kSecondaryMaxT, VG_RAY_SKIP_TRANSPARENT, false,
+                                             h, nullptr);
+                done = done || ++it >= max_iter;
+                if (done) {
+                    if (kind == 1) {
+                        store_hit(refl + pix, h);
+                        refraction();
+                    } else {
+                        store_hit(refr + pix, h);
+                        kind = 0;
+                    }
+                }
+            }
+        } while (__popcll(__ballot(kind != 0)) >= threshold);
+    }
+}
+
 // Persistent, ray-compacting batched query: the same refill discipline over a vg_ray array.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
@@ -565,7 +732,7 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
                                    const uint32_t *list_n = nullptr) {
     const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
     const uint64_t nblocks = (uint64_t)nbx * nby;
-    // the counter may overshoot the ray count by one id chunk per wave after it runs out
+    // the counter may overshoot the ray count by up to 64 per wave after it runs out
     if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     if (beam) {
@@ -703,6 +870,8 @@ static void free_world(vg_world *w) {
                     w->mat,     w->scratch, w->counts_dev, w->work, w->beam_t};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (w->wlist) (void)hipFree(w->wlist);
+    if (w->wlist_n) (void)hipFree(w->wlist_n);
     vg_edit_state_free(w);
 }
 
@@ -913,6 +1082,51 @@ int vg_trace_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t 
                    float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream) {
     return launch_frame(w, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
                         hits, occluded, 1, false, (hipStream_t)stream);
+}
+
+int vg_trace_water_secondary(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                             const vg_hit *primary_hits, vg_hit *refl_hits, vg_hit *refr_hits,
+                             void *stream) {
+    int st = check_world(w);
+    if (st) return st;
+    if (!camera || width <= 0 || height <= 0 || !primary_hits || !refl_hits || !refr_hits)
+        return fail(VG_E_INVALID, "bad water-ray args");
+    if (camera->resolution[0] != (float)width || camera->resolution[1] != (float)height)
+        return fail(VG_E_INVALID, "camera resolution %gx%g does not match %dx%d",
+                    camera->resolution[0], camera->resolution[1], width, height);
+    if ((((uintptr_t)primary_hits | (uintptr_t)refl_hits | (uintptr_t)refr_hits) & 3u) != 0)
+        return fail(VG_E_INVALID, "hit records must be 4-byte aligned");
+    const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
+    const uint64_t nblocks = (uint64_t)nbx * nby;
+    // the counter may overshoot the pixel count by up to 64 per wave after it runs out
+    if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
+    hipStream_t hs = (hipStream_t)stream;
+    const size_t npix = (size_t)width * height;
+    if (npix > w->wlist_cap) {
+        HIP_TRY(hipDeviceSynchronize());
+        if (w->wlist) (void)hipFree(w->wlist);
+        w->wlist = nullptr;
+        w->wlist_cap = 0;
+        if (hipMalloc((void **)&w->wlist, npix * sizeof(uint32_t)) != hipSuccess)
+            return fail(VG_E_NOMEM, "water pixel list");
+        w->wlist_cap = npix;
+    }
+    if (!w->wlist_n && hipMalloc((void **)&w->wlist_n, 256) != hipSuccess)
+        return fail(VG_E_NOMEM, "water pixel count");
+    hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, hs, w->wlist_n);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_water_compact, dim3((uint32_t)((nblocks * 64 + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, hs, width, height, nbx, (uint32_t)(nblocks * 64),
+                       primary_hits, refl_hits, refr_hits, w->wlist, w->wlist_n);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, hs, w->work);
+    HIP_TRY(hipGetLastError());
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
+    hipLaunchKernelGGL(k_water_persistent, grid, block, 0, hs, w->dw, cam_params(camera), width,
+                       w->wlist, w->wlist_n, primary_hits, refl_hits, refr_hits, w->work,
+                       w->ctx->frame_refill);
+    HIP_TRY(hipGetLastError());
+    return VG_OK;
 }
 
 int vg_trace_frame_blocks(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,

@@ -32,6 +32,9 @@ struct vg_world {
     float *beam_t;                   // beam start t per 8x8 pixel block
     size_t beam_cap;                 // entries of beam_t
     vg_edit_state *edit;
+    uint32_t *wlist;                 // water pixels of the last vg_trace_water_secondary
+    uint32_t *wlist_n;               // ... their count (device)
+    size_t wlist_cap;
 };
 
 void vg_edit_state_free(vg_world *w);

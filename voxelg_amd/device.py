@@ -293,6 +293,22 @@ class DeviceWorld:
                                    _stream(stream)), "vg_trace_frame")
         return hits, occluded
 
+    def trace_water_secondary(self, camera: np.ndarray, width: int, height: int, hits,
+                              refl=None, refr=None, stream=None):
+        """The reflection and refraction rays of every water pixel of a traced frame
+        (shade_water_top, shaders/raymarch.wgsl:3534-3667), traced with
+        VG_RAY_SKIP_TRANSPARENT and tmax 200: (refl, refr) device hit buffers, one record
+        per pixel (misses where the primary hit is not water)."""
+        cam = np.ascontiguousarray(camera, np.float32)
+        if refl is None:
+            refl = self._buf(width * height * 20)
+        if refr is None:
+            refr = self._buf(width * height * 20)
+        check(lib().vg_trace_water_secondary(self._h, ptr(cam), width, height, hits.data_ptr(),
+                                             refl.data_ptr(), refr.data_ptr(), _stream(stream)),
+              "vg_trace_water_secondary")
+        return refl, refr
+
     # ------------------------------------------------------------ dynamic --
     def apply_edits(self, edits: np.ndarray, stream=None) -> "DeviceWorld":
         """Sphere edits (EDIT_DTYPE records, applied in order) with the incremental pyramid
