@@ -120,8 +120,8 @@ def main():
     wl = workloads.LARGE
     t0 = time.perf_counter()
     world = wl.world()
-    log(f"world {wl.dims} generated in {time.perf_counter() - t0:.1f} s, "
-        f"solid {world.solid_fraction():.3f}")
+    gen_s = time.perf_counter() - t0
+    log(f"world {wl.dims} generated in {gen_s:.1f} s, solid {world.solid_fraction():.3f}")
     _, cam = wl.camera()
     sun = wl.sun()
     ctx = Context(0)
@@ -210,6 +210,28 @@ def main():
                  "mrays_per_s": round(2 * int(water.sum()) / water_ms / 1e3, 1)}
     log(f"water secondary rays: {water_out}")
 
+    # the same world generated on the device (no host arrays): its time, and the frame it
+    # gives against the uploaded host world's, outside the timed region
+    gw = ctx.world(wl.dims)
+    ga, gb = ctx.event(), ctx.event()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ga.record(stream)
+    gw.generate(wl.seed, (0, 0, 0), stream)
+    gb.record(stream)
+    stream.synchronize()
+    gen_wall = time.perf_counter() - t0
+    ghits, gocc = gw.trace_frame(cam, W, H, wl.primary_tmax, wl.primary_flags, sun,
+                                 wl.shadow_tmax)
+    ctx.synchronize()
+    gen_out = {"ms": round(ga.elapsed_ms(gb), 3), "wall_ms": round(gen_wall * 1e3, 3),
+               "host_generate_s": round(gen_s, 3),
+               "frame_identical": bool(decode_hits(ghits).tobytes() == hh.tobytes() and
+                                       gocc.numpy(np.uint8).tobytes() ==
+                                       occ.numpy(np.uint8).tobytes())}
+    gw.close()
+    log(f"device world generation: {gen_out}")
+
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(world, wl, cam, sun)
@@ -247,6 +269,7 @@ def main():
         },
         "cpu_baseline": base,
         "water_secondary": water_out,
+        "device_world_generation": gen_out,
     }
     print(json.dumps(out), flush=True)
 

@@ -280,6 +280,12 @@ int vg_trace_frame_blocks(vg_world *world, const vg_camera *camera_host, int32_t
 int vg_trace_water_secondary(vg_world *world, const vg_camera *camera_host, int32_t width,
                              int32_t height, const vg_hit *primary_hits, vg_hit *refl_hits,
                              vg_hit *refr_hits, void *stream);
+/* The benchmark terrain (vg_generate_terrain's seeded fBm heights, caves, materials and water
+ * line) generated directly in the world's device layout with its pyramid and uniform tables:
+ * the same bytes as vg_generate_terrain + vg_build_pyramid + vg_world_upload at `origin`, no
+ * host arrays (1.07e9 voxels of the 2048x256x2048 world never cross PCIe). Enqueued on
+ * `stream`; the world counts as uploaded once this returns. */
+int vg_world_generate(vg_world *world, uint64_t seed, const int32_t origin[3], void *stream);
 /* The world's device state back in the reference layout (any pointer may be NULL; counts
  * from vg_world_counts). Synchronous. */
 int vg_world_download(vg_world *world, vg_brick *bricks, uint64_t *tile_mask,

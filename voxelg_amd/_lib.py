@@ -126,6 +126,7 @@ SIGNATURES = {
     "vg_world_create": (C.c_int, [_P, _I32P, C.POINTER(_P)]),
     "vg_world_destroy": (C.c_int, [_P]),
     "vg_world_upload": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _I32P]),
+    "vg_world_generate": (C.c_int, [_P, C.c_uint64, _I32P, _P]),
     "vg_trace_rays": (C.c_int, [_P, _P, C.c_size_t, C.c_uint32, _P, _P]),
     "vg_trace_primary": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _P, _P]),
     "vg_trace_shadow": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _F32P, C.c_float, _P, _P]),

@@ -521,6 +521,147 @@ __global__ void k_download_tiles(const uint64_t *__restrict__ tile, const uint8_
     tu_out[ti] = tu[tn];
 }
 
+// ------------------------------------------------------------- generation --
+// The benchmark terrain generated on the device: the same bytes as vg_generate_terrain +
+// vg_build_pyramid (src/voxel.rs:1163-1232 noise and heights, the generator's column rule),
+// fp32 in the host's operation order (value noise with smoothstep weights, 5-octave fBm,
+// u32 wrapping hashes; no contraction).
+__device__ __forceinline__ float gen_hash2(int32_t x, int32_t z) {
+    uint32_t h = (uint32_t)x * 0x9E3779B1u + (uint32_t)z * 0x85EBCA77u;
+    h *= 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    return (float)(h & 0xFFFFFFu) / (float)0xFFFFFF * 2.0f - 1.0f;
+}
+__device__ __forceinline__ float gen_hash3(int32_t x, int32_t y, int32_t z) {
+    uint32_t h = (uint32_t)x * 0x9E3779B1u + (uint32_t)y * 0x85EBCA77u + (uint32_t)z * 0xC2B2AE3Du;
+    h *= 0xD2B74407u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    return (float)(h & 0xFFFFFFu) / (float)0xFFFFFF * 2.0f - 1.0f;
+}
+__device__ __forceinline__ float gen_smooth(float t) { return t * t * (3.0f - 2.0f * t); }
+__device__ __forceinline__ float gen_lerp(float a, float b, float w) { return a * (1.0f - w) + b * w; }
+__device__ __forceinline__ float gen_noise2(float x, float z) {
+    const int32_t xi = (int32_t)floorf(x), zi = (int32_t)floorf(z);
+    const float xf = gen_smooth(x - (float)xi), zf = gen_smooth(z - (float)zi);
+    const float a = gen_lerp(gen_hash2(xi, zi), gen_hash2(xi + 1, zi), xf);
+    const float b = gen_lerp(gen_hash2(xi, zi + 1), gen_hash2(xi + 1, zi + 1), xf);
+    return gen_lerp(a, b, zf);
+}
+__device__ __forceinline__ float gen_noise3(float x, float y, float z) {
+    const int32_t xi = (int32_t)floorf(x), yi = (int32_t)floorf(y), zi = (int32_t)floorf(z);
+    const float xf = gen_smooth(x - (float)xi), yf = gen_smooth(y - (float)yi),
+                zf = gen_smooth(z - (float)zi);
+    const float a = gen_lerp(gen_hash3(xi, yi, zi), gen_hash3(xi + 1, yi, zi), xf);
+    const float b = gen_lerp(gen_hash3(xi, yi + 1, zi), gen_hash3(xi + 1, yi + 1, zi), xf);
+    const float c = gen_lerp(gen_hash3(xi, yi, zi + 1), gen_hash3(xi + 1, yi, zi + 1), xf);
+    const float d = gen_lerp(gen_hash3(xi, yi + 1, zi + 1), gen_hash3(xi + 1, yi + 1, zi + 1), xf);
+    return gen_lerp(gen_lerp(a, b, yf), gen_lerp(c, d, yf), zf);
+}
+__device__ __forceinline__ int32_t gen_height(float sx, float sz, float top, int32_t x, int32_t z) {
+    float total = 0.0f, amp = 1.0f, freq = 1.0f, max_amp = 0.0f;
+    const float fx = ((float)x + sx) * 0.007f, fz = ((float)z + sz) * 0.007f;
+    for (int i = 0; i < 5; ++i) {
+        total += gen_noise2(fx * freq, fz * freq) * amp;
+        max_amp += amp;
+        amp *= 0.5f;
+        freq *= 2.0f;
+    }
+    float h = 64.0f + 40.0f * (total / max_amp);
+    h = fminf(fmaxf(h, 2.0f), top);
+    return (int32_t)h;
+}
+
+// One wave per slot brick column (bx, bz), one lane per voxel of a brick: every brick of the
+// column from y = 0 up is written whole (occupancy from the ballot, the material bytes, the
+// uniform byte).
+__global__ __launch_bounds__(256) void k_gen_bricks(DevWorld w, float sx, float sz, uint32_t ncols,
+                                                    uint64_t *__restrict__ occ,
+                                                    uint8_t *__restrict__ mat,
+                                                    uint8_t *__restrict__ bu) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t col = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (col >= ncols) return; // wave-uniform
+    const int32_t nbx = w.dims[0] >> 2;
+    const int32_t bx = (int32_t)(col % (uint32_t)nbx), bz = (int32_t)(col / (uint32_t)nbx);
+    const int32_t cx = (int32_t)(lane & 3u), cz = (int32_t)((lane >> 2) & 3u), vy = (int32_t)(lane >> 4);
+    const int32_t wx = slot_to_world(bx * 4 + cx, w.origin[0], w.dims[0]);
+    const int32_t wz = slot_to_world(bz * 4 + cz, w.origin[2], w.dims[2]);
+    const int32_t Y = w.dims[1];
+    const int32_t h = gen_height(sx, sz, (float)(Y - 1), wx, wz);
+    const int32_t ytop = min(max(h, 55), Y - 1);
+    for (int32_t by = 0; by < (Y >> 2); ++by) {
+        const int32_t y = by * 4 + vy;
+        uint32_t m = 0;
+        if (y <= ytop) {
+            if (y > h) {
+                m = 12u; // water L8 in open air below y = 56
+            } else if (!(y > 4 && y + 3 < h &&
+                         gen_noise3((float)wx * 0.045f, (float)y * 0.085f, (float)wz * 0.045f) > 0.6f)) {
+                m = y == h ? (h <= 57 ? 1u : 2u) : (y + 4 >= h ? 3u : 4u);
+            }
+        }
+        const int32_t sb[3] = {bx, by, bz};
+        const uint32_t bn = brick_nested(w, sb);
+        const uint64_t o = __ballot(m != 0u);
+        const uint32_t m0 = __shfl(m, 0, 64);
+        const bool same = __ballot(m == m0) == ~0ull;
+        mat[(size_t)bn * 64u + lane] = (uint8_t)m;
+        if (lane == 0) {
+            occ[bn] = o;
+            bu[bn] = (o == ~0ull && same && m0 != 0u) ? (uint8_t)m0 : (uint8_t)0;
+        }
+    }
+}
+
+// The pyramid of a whole world, bottom up: one wave per nested tile (word = ballot of its
+// non-empty bricks, uniform byte when all 64 bricks share one uniform material), one wave
+// per chunk, one thread per L4 cell.
+__global__ __launch_bounds__(256) void k_gen_tiles(uint32_t n_tiles, const uint64_t *__restrict__ occ,
+                                                   const uint8_t *__restrict__ bu,
+                                                   uint64_t *__restrict__ tile, uint8_t *__restrict__ tu) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tn = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (tn >= n_tiles) return; // wave-uniform
+    const uint32_t bn = tn * 64u + lane;
+    const uint64_t word = __ballot(occ[bn] != 0ull);
+    const uint32_t b = bu[bn];
+    const uint32_t b0 = __shfl(b, 0, 64);
+    const bool uni = b0 != 0u && __ballot(b == b0) == ~0ull;
+    if (lane == 0) {
+        tile[tn] = word;
+        tu[tn] = uni ? (uint8_t)b0 : (uint8_t)0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gen_chunks(uint32_t n_chunks, const uint64_t *__restrict__ tile,
+                                                    uint64_t *__restrict__ chunk) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t ci = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (ci >= n_chunks) return; // wave-uniform
+    const uint64_t word = __ballot(tile[ci * 64u + lane] != 0ull);
+    if (lane == 0) chunk[ci] = word;
+}
+
+__global__ void k_gen_l4(DevWorld w, uint32_t n_l4, const uint64_t *__restrict__ chunk,
+                         uint64_t *__restrict__ l4) {
+    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= n_l4) return;
+    const uint32_t lx = li % (uint32_t)w.l4dim[0];
+    const uint32_t ly = (li / (uint32_t)w.l4dim[0]) % (uint32_t)w.l4dim[1];
+    const uint32_t lz = li / ((uint32_t)w.l4dim[0] * (uint32_t)w.l4dim[1]);
+    uint64_t word = 0;
+    for (int b = 0; b < 64; ++b) {
+        const uint32_t cx = lx * 4u + (uint32_t)(b & 3), cy = ly * 4u + (uint32_t)(b >> 4),
+                       cz = lz * 4u + (uint32_t)((b >> 2) & 3);
+        if (cx >= (uint32_t)w.cdim[0] || cy >= (uint32_t)w.cdim[1] || cz >= (uint32_t)w.cdim[2]) continue;
+        const uint32_t ci = cx + cy * (uint32_t)w.cdim[0] + cz * (uint32_t)w.cdim[0] * (uint32_t)w.cdim[1];
+        if (chunk[ci] != 0ull) word |= 1ull << b;
+    }
+    l4[li] = word;
+}
+
 // --------------------------------------------------------------------- host --
 static void free_edit_state(vg_edit_state *s) {
     void *dev[] = {s->edits, s->cnt, s->end, s->list, s->temp, s->tile_flag, s->tile_list,
@@ -794,6 +935,45 @@ int vg_world_download(vg_world *w, vg_brick *bricks, uint64_t *tile_mask, uint64
         (void)hipFree(tmp);
         if (e != hipSuccess) return fail(VG_E_DEVICE, "brick download: %s", hipGetErrorString(e));
     }
+    return VG_OK;
+}
+
+int vg_world_generate(vg_world *w, uint64_t seed, const int32_t origin[3], void *stream) {
+    if (!w || !origin) return fail(VG_E_INVALID, "null world/origin");
+    const Dims &d = w->d;
+    for (int k = 0; k < 3; ++k)
+        if (pos_mod(origin[k], d.v[k]) != 0 && d.v[k] % 256 != 0)
+            return fail(VG_E_INVALID,
+                        "origin[%d]=%d folds the window but dims[%d]=%d is not a multiple of 256", k,
+                        origin[k], k, d.v[k]);
+    int st = set_device(w);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize()); // kernels on any stream may still read the old world
+    hipStream_t hs = (hipStream_t)stream;
+    w->uploaded = false;
+    w->dw.fold = 0;
+    for (int k = 0; k < 3; ++k) {
+        w->dw.origin[k] = origin[k];
+        w->dw.ro_mod[k] = pos_mod(origin[k], d.v[k]);
+        if (w->dw.ro_mod[k]) w->dw.fold = 1;
+    }
+    // seed_offset_xz (src/voxel.rs:1163-1168), as the host generator
+    const float sx = (float)(int32_t)(uint32_t)(seed >> 32) * 0.01734f;
+    const float sz = (float)(int32_t)(uint32_t)(seed & 0xFFFFFFFFu) * 0.02153f;
+    const uint32_t ncols = (uint32_t)((d.v[0] >> 2) * (d.v[2] >> 2));
+    hipLaunchKernelGGL(k_gen_bricks, dim3((ncols + 3) / 4), dim3(256), 0, hs, w->dw, sx, sz, ncols,
+                       w->occ, w->mat, w->brick_uni);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_gen_tiles, dim3((uint32_t)((d.n_tiles + 3) / 4)), dim3(256), 0, hs,
+                       (uint32_t)d.n_tiles, w->occ, w->brick_uni, w->tile, w->tile_uni);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_gen_chunks, dim3((uint32_t)((d.n_chunks + 3) / 4)), dim3(256), 0, hs,
+                       (uint32_t)d.n_chunks, w->tile, w->chunk);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_gen_l4, dim3((uint32_t)((d.n_l4 + 255) / 256)), dim3(256), 0, hs, w->dw,
+                       (uint32_t)d.n_l4, w->chunk, w->l4);
+    HIP_TRY(hipGetLastError());
+    w->uploaded = true;
     return VG_OK;
 }
 

@@ -234,6 +234,14 @@ class DeviceWorld:
         self.origin = tuple(world.origin)
         return self
 
+    def generate(self, seed: int, origin=(0, 0, 0), stream=None) -> "DeviceWorld":
+        """The benchmark terrain generated on the device (the bytes World.generate +
+        upload would give), pyramid and uniform tables included."""
+        check(lib().vg_world_generate(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF, i32x3(origin),
+                                      _stream(stream)), "vg_world_generate")
+        self.origin = tuple(origin)
+        return self
+
     # ------------------------------------------------------------ queries --
     def _buf(self, nbytes):
         return self.ctx.alloc(max(int(nbytes), 1))
