@@ -58,8 +58,78 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
         dmax[k] = hi;
     }
     const int lane = threadIdx.x & 63;
-    float t0 = 0.0f;
-    for (int st = 0; st < kBeamMaxSteps && t0 < tmax; ++st) {
+    // Packed rounds: 8 consecutive steps at once, 8 lanes each, while every box fits 8 tiles
+    // (the common case: a block's box is 1-2 tiles wide per axis); the first step that stops
+    // the march decides, exactly as the one-step loop below would. A step whose box needs
+    // more than 8 tiles hands over to that loop.
+    int st = 0;
+    {
+        const int g = lane >> 3, j = lane & 7;
+        for (;;) {
+            const int s = st + g;
+            const float ts0 = (float)s * kBeamStep, ts1 = (float)(s + 1) * kBeamStep;
+            // 0 test, 1 stop before (loop bound), 2 exit (box left the window), 3 too wide
+            // (stop), 4 more than 8 tiles (hand over)
+            int kind = 0;
+            int32_t lo[3], n[3];
+            int32_t total = 1;
+            if (s >= kBeamMaxSteps || !(ts0 < tmax)) {
+                kind = 1;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float a = c.origin[k] + dmin[k] * ts0, b = c.origin[k] + dmin[k] * ts1;
+                    float cc = c.origin[k] + dmax[k] * ts0, e = c.origin[k] + dmax[k] * ts1;
+                    float mn = fsel_min(fsel_min(a, b), fsel_min(cc, e)) - 1.0f;
+                    float mx = fsel_max(fsel_max(a, b), fsel_max(cc, e)) + 1.0f;
+                    int32_t tl = (int32_t)floorf(mn * 0.0625f), th = (int32_t)floorf(mx * 0.0625f);
+                    const int32_t wl = w.origin[k] >> 4, wh = (w.origin[k] + w.dims[k] - 1) >> 4;
+                    tl = tl < wl ? wl : tl;
+                    th = th > wh ? wh : th;
+                    lo[k] = tl;
+                    n[k] = th >= tl ? th - tl + 1 : 0;
+                    total *= n[k];
+                }
+                if (total == 0) kind = 2;
+                else if (n[0] > 4 || n[1] > 4 || n[2] > 4) kind = 3;
+                else if (total > 8) kind = 4;
+            }
+            bool occ = false;
+            const bool tests = kind == 0 && j < total;
+            if (tests) {
+                const int32_t tw[3] = {lo[0] + j % n[0], lo[1] + (j / n[0]) % n[1],
+                                       lo[2] + j / (n[0] * n[1])};
+                int32_t sl[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    int32_t v = tw[k] * 16 - w.origin[k] + w.ro_mod[k];
+                    v = v < 0 ? v + w.dims[k] : v;
+                    sl[k] = v >= w.dims[k] ? v - w.dims[k] : v;
+                }
+                const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                                    (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+                occ = (w.chunk[ci] >> cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3)) & 1;
+            }
+            // the first step (group) that stops the march
+            const uint64_t stop = __ballot(kind != 0 || occ);
+            if (!stop) {
+                if (COUNT && tests) cnt[VG_IT_BEAM_TEST]++;
+                st += 8;
+                continue;
+            }
+            const int gs = (__ffsll((long long)stop) - 1) >> 3;
+            const int kind_s = __shfl(kind, gs * 8, 64);
+            if (COUNT && tests && (g < gs || (g == gs && kind_s == 0))) cnt[VG_IT_BEAM_TEST]++;
+            const float t_s = (float)(st + gs) * kBeamStep;
+            if (kind_s == 4) {
+                st += gs; // this step needs the wide loop
+                break;
+            }
+            return kind_s == 2 ? t_s + kBeamStep : t_s;
+        }
+    }
+    float t0 = (float)st * kBeamStep;
+    for (; st < kBeamMaxSteps && t0 < tmax; ++st) {
         const float t1 = (float)(st + 1) * kBeamStep;
         int32_t lo[3], n[3];
         int32_t total = 1;
