@@ -404,7 +404,8 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
 
     def sched():
         return {"waves_per_cu": ctx.get_option(L.VG_OPT_PROBE_WAVES_PER_CU),
-                "refill_below": ctx.get_option(L.VG_OPT_PROBE_REFILL)}
+                "refill_below": ctx.get_option(L.VG_OPT_PROBE_REFILL),
+                "ids_per_atomic": ctx.get_option(L.VG_OPT_PROBE_CHUNK)}
 
     g256 = gi.ProbeGrid(dw, p256)
     rays256, it256 = g256.update_counts(p256)

@@ -310,7 +310,8 @@ enum {
     VG_OPT_RAYS_REFILL = 5,
     VG_OPT_CU_COUNT = 6, /* read only */
     VG_OPT_PROBE_WAVES_PER_CU = 7, /* probe path kernels (vg_probes_update*) */
-    VG_OPT_PROBE_REFILL = 8
+    VG_OPT_PROBE_REFILL = 8,
+    VG_OPT_PROBE_CHUNK = 9 /* probe path work ids taken per atomic: 0 (what a refill lacks) or 16..1024 */
 };
 int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
 int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);

@@ -970,6 +970,7 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->rays_refill = 32;
     c->probe_wpc = 8;
     c->probe_refill = 8;
+    c->probe_chunk = 0;
     *out = c;
     return VG_OK;
 }
@@ -1423,6 +1424,11 @@ int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
          : option == VG_OPT_RAYS_REFILL ? ctx->rays_refill
                                         : ctx->probe_refill) = value;
         return VG_OK;
+    case VG_OPT_PROBE_CHUNK:
+        if (value != 0 && (value < 16 || value > 1024))
+            return fail(VG_E_INVALID, "probe id chunk %d: 0 or 16..1024", value);
+        ctx->probe_chunk = value;
+        return VG_OK;
     default:
         return fail(VG_E_INVALID, "option %d cannot be set", option);
     }
@@ -1438,6 +1444,7 @@ int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
     case VG_OPT_RAYS_REFILL: *value = ctx->rays_refill; return VG_OK;
     case VG_OPT_PROBE_WAVES_PER_CU: *value = ctx->probe_wpc; return VG_OK;
     case VG_OPT_PROBE_REFILL: *value = ctx->probe_refill; return VG_OK;
+    case VG_OPT_PROBE_CHUNK: *value = ctx->probe_chunk; return VG_OK;
     case VG_OPT_CU_COUNT: *value = ctx->cus; return VG_OK;
     default: return fail(VG_E_INVALID, "unknown option %d", option);
     }

@@ -35,6 +35,12 @@ struct vg_probes {
     vg_nrc_query *queries;  // learned update: compacted gather hits, two halves of `ns` entries
     size_t ns;              // samples per update at most (total probes x rays)
     uint32_t *ctr;          // device counters, see kGiCtrs
+    // Heavy paths (walk iterations >= a threshold) of the last update, traced first by the
+    // next one, which skips them in the natural order: two halves, alternating per update
+    uint32_t *heavy;        // sample ids: 2 * hcap
+    uint32_t *hmask;        // one bit per sample: 2 * hwords
+    uint32_t hcap, hwords;
+    int hpar;               // half the next update writes
     unsigned long long *counts; // VG_IT_COUNT walk classes + 3 ray kinds
 };
 
@@ -42,10 +48,12 @@ namespace vg {
 
 constexpr uint32_t kProbeNew = 1u, kProbeBuried = 2u;
 constexpr int kGiCounters = VG_IT_COUNT + 3;
-// Device counters of a probe grid: [0] path work and [1] this update's cache queries (one
-// 64-bit word), [2] the previous learned update's query count, [3] the query-list half this
-// update writes, [4 + s - 1] work of training step s >= 1 (vg_nrc_config.steps <= 64).
-constexpr int kGiCtrs = 4 + 64;
+// Device counters of a probe grid: [0] path work, [1] this update's cache queries, [2] the
+// previous learned update's query count, [3] the query-list half this update writes,
+// [4 + s - 1] work of training step s >= 1 (vg_nrc_config.steps <= 64), [kCtrHeavy] this
+// update's heavy paths, [kCtrHeavy + 1] the previous update's.
+constexpr int kCtrHeavy = 4 + 64;
+constexpr int kGiCtrs = kCtrHeavy + 2;
 
 // ----------------------------------------------------------------- helpers --
 __device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
@@ -193,14 +201,18 @@ __device__ __forceinline__ void probe_cell(const vg_gi_params &p, const DevWorld
 // ----------------------------------------------------------------- kernels --
 __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params p, uint32_t n_upd,
                                                       uint32_t phase, const vg_probe *probes,
-                                                      float4 *pwork, uint32_t *ctr, int learned) {
+                                                      float4 *pwork, uint32_t *ctr, int learned,
+                                                      uint32_t *hmask_cur, uint32_t hwords) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j == 0) { // the kernels enqueued next consume the counters
-        const uint32_t prev = ctr[1], par = ctr[3];
+        const uint32_t prev = ctr[1], par = ctr[3], hprev = ctr[kCtrHeavy];
         for (int k = 0; k < kGiCtrs; ++k) ctr[k] = 0u;
         ctr[2] = learned ? prev : 0u;
         ctr[3] = learned ? (par ^ 1u) : par;
+        ctr[kCtrHeavy + 1] = hprev;
     }
+    // this update's heavy-path bits start clear
+    for (uint32_t k = j; k < hwords; k += gridDim.x * kBlock) hmask_cur[k] = 0u;
     if (j >= n_upd) return;
     const int64_t idx = (int64_t)j * p.update_div + phase;
     int32_t g[3];
@@ -222,8 +234,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params 
     pwork[j] = make_float4(o[0], o[1], o[2], __uint_as_float(fl));
 }
 
-// Path stages of a lane.
-enum : int { kIdle = 0, kGather = 1, kShadowX = 2, kBounce = 3, kShadowY = 4 };
+// Path stages of a lane: kNew holds a freshly taken work id whose first ray is not set up yet;
+// the walking stages are the kinds of ray the lane traces.
+enum : int { kIdle = 0, kNew = 1, kGather = 2, kShadowX = 3, kBounce = 4, kShadowY = 5 };
 
 // Learned-mode arguments of the path kernel.
 struct LearnedArgs {
@@ -235,23 +248,62 @@ struct LearnedArgs {
     float4 *tt;            // ... and targets (rgb, weight)
 };
 
+// Longest-first order: the previous update's heavy samples (their ids, and one bit per
+// sample) are traced first and skipped in the natural order; this update's are recorded.
+// Any order gives the same results; this one starts the long paths early, so the launch
+// does not end on a tail of them.
+constexpr uint32_t kHeavyIters = 24;  // gather + sun shadow paths (about the top 1 %)
+constexpr uint32_t kHeavyIters2 = 48; // 2-bounce paths
+struct HeavyArgs {
+    const uint32_t *prev_ids, *prev_mask;
+    uint32_t *cur_ids, *cur_mask;
+    uint32_t cap;    // ids each list holds (a longer list is cut: the rest stay in order)
+    uint32_t thresh; // walk iterations that make a path heavy
+};
+
+// Path state that must outlive a walk, one slot per thread in LDS (structure of arrays, so a
+// wave's accesses are bank-conflict free). Only the walk itself stays in registers while the
+// lanes march, which keeps the kernel at four waves per SIMD.
+struct PathSlots {
+    float so[3][kBlock]; // secondary origin: off the gather hit (training: off the query point)
+    float tX[kBlock];    // gather hit distance
+    float ndlX[kBlock];  // max(0, n . sun) at the gather hit
+    float visX[kBlock];  // its sun visibility (2-bounce paths keep it across the bounce)
+    float ndlY[kBlock];  // max(0, n . sun) at the bounce hit
+    uint32_t pc[kBlock]; // walk iterations of the lane's sample path so far (~0u: training)
+    uint32_t fm[kBlock]; // gather face | gather material << 8 | bounce material << 16
+    uint32_t tq[kBlock]; // training lane: index of its query in the previous hit list
+    uint32_t tb[kBlock]; // training lane: bounce-table index
+};
+
+// Learned mode: finished gather hits wait in a per-wave LDS ring of cache queries and go to
+// the global list 64 at a time (one atomic per run of 64, not per refill).
+constexpr uint32_t kQStage = 128;
+template <uint32_t CAP> struct QueryStage {
+    float4 q[kBlock / 64][CAP][2];
+};
+
 template <bool COUNT, int BOUNCES>
 __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
     const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
     const float4 *__restrict__ btab, const float4 *__restrict__ pal, float4 *__restrict__ rad,
-    uint8_t *__restrict__ face_out, LearnedArgs la, uint32_t *ctr, int threshold,
-    unsigned long long *counts) {
+    uint8_t *__restrict__ face_out, LearnedArgs la, HeavyArgs ha, uint32_t *ctr, int threshold,
+    uint32_t chunk, unsigned long long *counts) {
     // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail;
     // the lanes' kBounce / kShadowY stages then trace training targets)
+    __shared__ PathSlots sl;
+    __shared__ QueryStage<BOUNCES == 3 ? kQStage : 1u> qs;
     const uint32_t lane = threadIdx.x & 63;
+    const uint32_t me = threadIdx.x;
+    const uint32_t wv = threadIdx.x >> 6;
     const uint32_t par = BOUNCES == 3 ? ctr[3] : 0u;
     vg_nrc_query *qcur = la.queries + (size_t)par * la.half;
     const vg_nrc_query *qprev = la.queries + (size_t)(par ^ 1u) * la.half;
     const uint32_t nprev = BOUNCES == 3 ? ctr[2] : 0u;
     const uint32_t train_n = BOUNCES == 3 ? la.train_n : 0u;
-    const uint32_t total_ids = n_samples + train_n;
-    uint32_t tqi = 0, tbidx = 0; // training lane: query index and bounce-table index
+    const uint32_t n_heavy = ctr[kCtrHeavy + 1] < ha.cap ? ctr[kCtrHeavy + 1] : ha.cap;
+    const uint32_t total_ids = train_n + n_heavy + n_samples;
     uint32_t cnt[VG_IT_COUNT];
     uint32_t nray[3] = {0, 0, 0};
     if (COUNT)
@@ -260,52 +312,23 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
     const uint32_t N = (uint32_t)p.rays_per_probe;
     int stage = kIdle;
-    uint32_t sid = 0;
-    float tX = 0.0f, ndlX = 0.0f, visX = 0.0f, ndlY = 0.0f;
-    uint32_t fX = VG_FACE_NONE, mX = 0, mY = 0;
-    float so[3] = {0.0f, 0.0f, 0.0f}; // secondary origin off the gather hit
-    float lb[3] = {0.0f, 0.0f, 0.0f}; // radiance arriving at the gather hit from its bounce
+    uint32_t sid = 0; // sample id (training lane: batch index; kNew: the raw work id)
     int it = 0;
     Walk r;
     WalkCache q;
     HitRec h;
     hit_miss(h);
     cache_reset(q);
-    bool exhausted = false;
-    bool emit = false; // learned: this lane holds a cache query to append
-    float eX[3] = {0.0f, 0.0f, 0.0f}, eD[3] = {0.0f, 0.0f, 0.0f};
-    uint32_t eF = 0, eS = 0;
+    IdQueue iq;
+    queue_reset(iq);
+    bool emit = false;  // learned: this lane's finished gather hit still has to be listed
+    bool heavy = false; // this lane's finished sample path still has to be recorded heavy
+    uint32_t qhead = 0, qn = 0; // the wave's staged queries: ring start and count
 
-    // Finish the sample: radiance toward the probe and its visible distance.
-    auto finish_hit = [&]() {
-        const float4 a = pal[mX];
-        const float sx = ndlX * visX;
-        float4 out;
-        out.x = a.x * (p.sun_color[0] * sx + lb[0]);
-        out.y = a.y * (p.sun_color[1] * sx + lb[1]);
-        out.z = a.z * (p.sun_color[2] * sx + lb[2]);
-        out.w = tX;
-        rad[sid] = out;
-        face_out[sid] = (uint8_t)fX;
-        stage = kIdle;
-    };
-    auto tail = [&]() {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) lb[k] = p.ambient[k] * p.ambient_tail;
-    };
-    auto start = [&](const float o[3], const float d[3], int st) -> bool {
-        hit_miss(h);
-        if (!walk_init(r, w, o, d)) return false;
-        cache_reset(q);
-        it = 0;
-        stage = st;
-        if (COUNT) nray[st == kGather ? 0 : (st == kBounce ? 2 : 1)]++;
-        return true;
-    };
-    // the gather direction and origin of the current sample
-    auto gather_ray = [&](float o[3], float d[3]) {
-        const float4 po = pwork[sid / N];
-        const float4 dd = dirs[(sid % N) * (uint32_t)p.epochs + epoch];
+    // the gather direction and origin of sample s
+    auto gather_ray = [&](uint32_t s, float o[3], float d[3]) {
+        const float4 po = pwork[s / N];
+        const float4 dd = dirs[(s % N) * (uint32_t)p.epochs + epoch];
         o[0] = po.x;
         o[1] = po.y;
         o[2] = po.z;
@@ -313,247 +336,288 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         d[1] = dd.y;
         d[2] = dd.z;
     };
-    auto bounce_ray = [&](float d[3]) {
-        const uint32_t j = sid / N;
-        const uint32_t probe = j * (uint32_t)p.update_div + phase;
-        bounce_dir(btab[bounce_index(p.seed, p.round, probe, sid % N)], fX, d);
+    auto sky_sample = [&](const float d[3]) {
+        float sky[3];
+        sky_color(p, d, sky);
+        rad[sid] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky, sky[2] * p.gi_sky, p.gi_dist);
+        face_out[sid] = (uint8_t)VG_FACE_NONE;
     };
-    // After the shadow ray at the gather hit (or none): bounce or tail, then finish.
-    auto after_x = [&]() {
-        if (BOUNCES == 3) {
-            float o[3], d[3];
-            gather_ray(o, d);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                eX[k] = o[k] + d[k] * tX; // the hit point (the secondary origin before its offset)
-                eD[k] = d[k];
-                lb[k] = 0.0f;
-            }
-            eF = fX | (mX << 8);
-            eS = sid;
-            emit = true;
-            finish_hit();
-            return;
-        }
-        if (BOUNCES == 2) {
-            float d[3];
-            bounce_ray(d);
-            if (start(so, d, kBounce)) return;
-            float sky[3];
-            sky_color(p, d, sky);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) lb[k] = sky[k] * p.gi_sky;
-        } else {
-            tail();
-        }
-        finish_hit();
+    // Finish a sample with a gather hit: radiance toward the probe, given the light lb
+    // arriving at the hit from its bounce, and the visible distance.
+    auto finish_hit = [&](const float lb[3]) {
+        const uint32_t fm = sl.fm[me];
+        const float4 a = pal[(fm >> 8) & 0xFFu];
+        const float sx = sl.ndlX[me] * sl.visX[me];
+        rad[sid] = make_float4(a.x * (p.sun_color[0] * sx + lb[0]),
+                               a.y * (p.sun_color[1] * sx + lb[1]),
+                               a.z * (p.sun_color[2] * sx + lb[2]), sl.tX[me]);
+        face_out[sid] = (uint8_t)(fm & 0xFFu);
     };
-    auto after_y = [&](float visY) {
-        const float4 a = pal[mY];
-        const float sy = ndlY * visY;
+    // light leaving the bounce hit toward the gather hit (sun through its shadow ray + ambient)
+    auto bounce_light = [&](float visY, float out[3]) {
+        const float4 a = pal[(sl.fm[me] >> 16) & 0xFFu];
+        const float sy = sl.ndlY[me] * visY;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            lb[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
-        finish_hit();
+            out[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
     };
-
-    // Learned: a finished hit stays pending in its (idle) lane until the wave's next refill,
-    // whose one 64-bit atomic on ctr[0..1] takes both the new work (low word) and the query
-    // slots (high word); the last pending hits go with the wave's exit.
-    auto put_query = [&](uint32_t slot) {
-        float4 *dst = reinterpret_cast<float4 *>(qcur + slot);
-        dst[0] = make_float4(eX[0], eX[1], eX[2], __uint_as_float(eF));
-        dst[1] = make_float4(eD[0], eD[1], eD[2], __uint_as_float(eS));
-        emit = false;
+    auto bounce_ray = [&](float d[3]) { // plain 2-bounce: the cosine bounce of sample sid
+        const uint32_t probe = (sid / N) * (uint32_t)p.update_div + phase;
+        bounce_dir(btab[bounce_index(p.seed, p.round, probe, sid % N)], sl.fm[me] & 0xFFu, d);
     };
-    unsigned long long *ctr64 = reinterpret_cast<unsigned long long *>(ctr);
-    // Training lanes (learned): sid holds the batch index, fX the query's face, so the
-    // secondary origin; the query is re-read from the previous list when written out.
-    auto train_dir = [&](float d[3]) { bounce_dir(btab[tbidx], fX, d); };
-    auto train_write = [&](const float lb3[3]) {
-        const float4 *src = reinterpret_cast<const float4 *>(qprev + tqi);
+    auto train_dir = [&](float d[3]) { bounce_dir(btab[sl.tb[me]], sl.fm[me] & 0xFFu, d); };
+    // a training target: the previous list's query, its bounce-table index, the traced light
+    auto train_write = [&](const float t[3]) {
+        const float4 *src = reinterpret_cast<const float4 *>(qprev + sl.tq[me]);
         const float4 a = src[0], b = src[1];
         float4 *dst = reinterpret_cast<float4 *>(la.tq + sid);
-        dst[0] = make_float4(a.x, a.y, a.z, __uint_as_float(fX));
-        dst[1] = make_float4(b.x, b.y, b.z, __uint_as_float(tbidx));
-        la.tt[sid] = make_float4(lb3[0], lb3[1], lb3[2], 1.0f);
-        stage = kIdle;
+        dst[0] = make_float4(a.x, a.y, a.z, __uint_as_float(sl.fm[me] & 0xFFu));
+        dst[1] = make_float4(b.x, b.y, b.z, __uint_as_float(sl.tb[me]));
+        la.tt[sid] = make_float4(t[0], t[1], t[2], 1.0f);
     };
-    auto train_sky = [&]() {
-        float d[3], sky[3], t[3];
-        train_dir(d);
-        sky_color(p, d, sky);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) t[k] = sky[k] * p.gi_sky;
-        train_write(t);
-    };
-    auto train_hit = [&](float visY) {
-        const float4 a = pal[mY];
-        const float sy = ndlY * visY;
-        float t[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            t[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
-        train_write(t);
-    };
-    auto train_start = [&](uint32_t j) {
-        sid = j;
-        if (nprev == 0u) {
-            float4 *dst = reinterpret_cast<float4 *>(la.tq + j);
-            dst[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)VG_FACE_NONE));
-            dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            la.tt[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            return;
+    // Learned: a finished hit stays pending in its (idle) lane until the wave's next refill
+    // (or its exit), which stages the wave's pending hits in LDS; full runs of 64 (at exit:
+    // the rest) are appended to the list with one atomic on ctr[1]. The query (hit point,
+    // gather direction, face | material << 8, sample id) is rebuilt from the lane's slot.
+    auto stage_queries = [&](bool final) {
+        const uint64_t em = __ballot(emit);
+        if (emit) {
+            float o[3], d[3];
+            gather_ray(sid, o, d);
+            const float t = sl.tX[me];
+            const uint32_t k = (qhead + qn + lanes_below(em)) & (kQStage - 1u);
+            qs.q[wv][k][0] = make_float4(o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t,
+                                         __uint_as_float(sl.fm[me] & 0xFFFFu));
+            qs.q[wv][k][1] = make_float4(d[0], d[1], d[2], __uint_as_float(sid));
+            emit = false;
         }
-        train_pick(la.tseed, p.round, j, train_n, nprev, tqi, tbidx);
-        const float4 a = reinterpret_cast<const float4 *>(qprev + tqi)[0];
-        fX = __float_as_uint(a.w) & 0xFFu;
-        const float qp[3] = {a.x, a.y, a.z};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) so[k] = qp[k] + face_n(fX, k) * 0.03f;
-        float d[3];
-        train_dir(d);
-        if (!start(so, d, kBounce)) train_sky();
+        qn += (uint32_t)__popcll(em);
+        while (qn >= 64u || (final && qn > 0u)) {
+            const uint32_t m = qn < 64u ? qn : 64u;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(ctr + 1, m);
+            base = __shfl(base, 0, 64);
+            if (lane < m) {
+                const uint32_t k = (qhead + lane) & (kQStage - 1u);
+                float4 *dst = reinterpret_cast<float4 *>(qcur + base + lane);
+                dst[0] = qs.q[wv][k][0];
+                dst[1] = qs.q[wv][k][1];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            qhead = (qhead + m) & (kQStage - 1u);
+            qn -= m;
+        }
     };
+
+    // Record the wave's finished heavy sample paths (at a refill or the exit, before the
+    // lanes take new ids): a slot in this update's list, then the sample's bit.
+    auto flush_heavy = [&]() {
+        const uint64_t hm = __ballot(heavy);
+        if (hm) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(ctr + kCtrHeavy, (uint32_t)__popcll(hm));
+            base = __shfl(base, 0, 64);
+            if (heavy) {
+                const uint32_t k = base + lanes_below(hm);
+                if (k < ha.cap) {
+                    ha.cur_ids[k] = sid;
+                    atomicOr(ha.cur_mask + (sid >> 5), 1u << (sid & 31u));
+                }
+                heavy = false;
+            }
+        }
+    };
+
+    // The lane's ray of stage `ev` has ended (h holds its result; kNew: a new work id):
+    // settle what it decides and set up the next ray of the path, if any. A ray that cannot
+    // start (it misses the world window) ends at once without a hit, exactly as a walk that
+    // finds nothing, so every ray setup goes through the one walk_init below.
+    auto transition = [&]() {
+        int ev = stage;
+        if (ev >= kGather) {
+            const uint32_t c = sl.pc[me];
+            if (c != ~0u) sl.pc[me] = c + (uint32_t)it;
+        }
+        for (;;) {
+            float o[3], d[3];
+            int ns = kIdle;
+            bool after_x = false;
+            if (ev == kNew) {
+                if (BOUNCES == 3 && sid < train_n) {
+                    sl.pc[me] = ~0u;
+                    if (nprev == 0u) { // no hits listed yet: a zero-weight target
+                        float4 *dst = reinterpret_cast<float4 *>(la.tq + sid);
+                        dst[0] = make_float4(0.0f, 0.0f, 0.0f,
+                                             __uint_as_float((uint32_t)VG_FACE_NONE));
+                        dst[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        la.tt[sid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    } else {
+                        uint32_t tqi, tbidx;
+                        train_pick(la.tseed, p.round, sid, train_n, nprev, tqi, tbidx);
+                        const float4 a = reinterpret_cast<const float4 *>(qprev + tqi)[0];
+                        const uint32_t f = __float_as_uint(a.w) & 0xFFu;
+                        sl.tq[me] = tqi;
+                        sl.tb[me] = tbidx;
+                        sl.fm[me] = f;
+                        o[0] = a.x + face_n(f, 0) * 0.03f;
+                        o[1] = a.y + face_n(f, 1) * 0.03f;
+                        o[2] = a.z + face_n(f, 2) * 0.03f;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) sl.so[k][me] = o[k];
+                        bounce_dir(btab[tbidx], f, d);
+                        ns = kBounce;
+                    }
+                } else {
+                    // the previous update's heavy samples first, then the others in order
+                    const uint32_t k = sid - train_n;
+                    bool take;
+                    if (k < n_heavy) {
+                        sid = ha.prev_ids[k];
+                        take = sid < n_samples;
+                    } else {
+                        sid = k - n_heavy;
+                        take = !((ha.prev_mask[sid >> 5] >> (sid & 31u)) & 1u);
+                    }
+                    sl.pc[me] = 0u;
+                    if (take && !(__float_as_uint(pwork[sid / N].w) & kProbeBuried)) {
+                        gather_ray(sid, o, d);
+                        ns = kGather;
+                    }
+                }
+            } else if (ev == kGather) {
+                float go[3], gd[3];
+                gather_ray(sid, go, gd);
+                if (!h.hit) {
+                    sky_sample(gd);
+                } else {
+                    const uint32_t f = h.face;
+                    const float ndl = face_ndl(f, p.sun);
+                    sl.tX[me] = h.t;
+                    sl.fm[me] = f | (h.mat << 8);
+                    sl.ndlX[me] = ndl;
+                    gi_offset_origin(go, gd, h.t, f, o);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) sl.so[k][me] = o[k];
+                    if (ndl > 0.0f) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) d[k] = p.sun[k];
+                        ns = kShadowX;
+                    } else {
+                        sl.visX[me] = 0.0f;
+                        after_x = true;
+                    }
+                }
+            } else if (ev == kShadowX) {
+                sl.visX[me] = h.hit ? 0.0f : 1.0f;
+                after_x = true;
+            } else if (ev == kBounce) {
+                // the bounce of a 2-bounce sample, or a learned lane's training path
+                if (BOUNCES == 3) train_dir(d);
+                else bounce_ray(d);
+                if (!h.hit) {
+                    float sky[3], t[3];
+                    sky_color(p, d, sky);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) t[k] = sky[k] * p.gi_sky;
+                    if (BOUNCES == 3) train_write(t);
+                    else finish_hit(t);
+                } else {
+                    const uint32_t f = h.face;
+                    const float ndl = face_ndl(f, p.sun);
+                    sl.fm[me] = (sl.fm[me] & 0xFFFFu) | (h.mat << 16);
+                    sl.ndlY[me] = ndl;
+                    if (ndl > 0.0f) {
+                        const float bo[3] = {sl.so[0][me], sl.so[1][me], sl.so[2][me]};
+                        gi_offset_origin(bo, d, h.t, f, o);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) d[k] = p.sun[k];
+                        ns = kShadowY;
+                    } else {
+                        float t[3];
+                        bounce_light(0.0f, t);
+                        if (BOUNCES == 3) train_write(t);
+                        else finish_hit(t);
+                    }
+                }
+            } else { // kShadowY
+                float t[3];
+                bounce_light(h.hit ? 0.0f : 1.0f, t);
+                if (BOUNCES == 3) train_write(t);
+                else finish_hit(t);
+            }
+            if (after_x) {
+                if (BOUNCES == 3) { // the cache supplies the bounce: list the hit
+                    const float lb[3] = {0.0f, 0.0f, 0.0f};
+                    finish_hit(lb);
+                    emit = true;
+                } else if (BOUNCES == 2) {
+                    bounce_ray(d);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) o[k] = sl.so[k][me];
+                    ns = kBounce;
+                } else {
+                    float lb[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) lb[k] = p.ambient[k] * p.ambient_tail;
+                    finish_hit(lb);
+                }
+            }
+            if (ns == kIdle) {
+                const uint32_t c = sl.pc[me];
+                heavy = c != ~0u && c >= ha.thresh;
+                stage = kIdle;
+                return;
+            }
+            hit_miss(h);
+            if (walk_init(r, w, o, d)) {
+                cache_reset(q);
+                it = 0;
+                stage = ns;
+                if (COUNT) nray[ns == kGather ? 0 : (ns == kBounce ? 2 : 1)]++;
+                return;
+            }
+            ev = ns; // could not start: ended without a hit
+        }
+    };
+
+    // Work ids come from ctr[0], `chunk` per atomic (samples of a few probes; 0: what the
+    // refill lacks), so a wave refills its idle lanes from its own reservation; training ids
+    // come first.
     for (;;) {
         uint64_t act = __ballot(stage != kIdle);
         int nact = __popcll(act);
-        if (!exhausted && nact < threshold) {
-            uint32_t base = 0;
-            if (BOUNCES == 3) {
-                const uint64_t em = __ballot(emit);
-                unsigned long long old = 0;
-                if (lane == 0)
-                    old = atomicAdd(ctr64, ((unsigned long long)__popcll(em) << 32) |
-                                               (unsigned long long)(uint32_t)(64 - nact));
-                old = __shfl(old, 0, 64);
-                base = (uint32_t)old;
-                if (emit) put_query((uint32_t)(old >> 32) + lanes_below(em));
-            } else {
-                if (lane == 0) base = atomicAdd(ctr, (uint32_t)(64 - nact));
-                base = __shfl(base, 0, 64);
-            }
-            if (base >= total_ids) {
-                exhausted = true;
-            } else if (stage == kIdle) {
-                const uint32_t id = base + lanes_below(~act);
-                if (id < train_n) {
-                    train_start(id);
-                } else if (id < total_ids) {
-                    sid = id - train_n;
-                    const uint32_t fl = __float_as_uint(pwork[sid / N].w);
-                    if (!(fl & kProbeBuried)) {
-                        float o[3], d[3];
-                        gather_ray(o, d);
-                        if (!start(o, d, kGather)) {
-                            float sky[3];
-                            sky_color(p, d, sky);
-                            rad[sid] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky,
-                                                   sky[2] * p.gi_sky, p.gi_dist);
-                            face_out[sid] = (uint8_t)VG_FACE_NONE;
-                        }
-                    }
-                }
+        if (!queue_empty(iq) && nact < threshold) {
+            if (BOUNCES == 3) stage_queries(false);
+            flush_heavy();
+            const uint32_t id = queue_take(iq, ctr, total_ids, (uint32_t)(64 - nact),
+                                           lanes_below(~act), stage == kIdle, lane, chunk);
+            if (id != ~0u) {
+                sid = id;
+                stage = kNew;
             }
             nact = __popcll(__ballot(stage != kIdle));
         }
         if (nact == 0) {
-            if (exhausted) {
-                if (BOUNCES == 3) {
-                    const uint64_t em = __ballot(emit);
-                    if (em) {
-                        unsigned long long old = 0;
-                        if (lane == 0)
-                            old = atomicAdd(ctr64, (unsigned long long)__popcll(em) << 32);
-                        old = __shfl(old, 0, 64);
-                        if (emit) put_query((uint32_t)(old >> 32) + lanes_below(em));
-                    }
-                }
+            if (queue_empty(iq)) {
+                if (BOUNCES == 3) stage_queries(true);
+                flush_heavy();
                 break;
             }
             continue;
         }
         do {
-            if (stage != kIdle) {
+            bool ended = stage == kNew;
+            if (stage >= kGather) {
                 const bool shadow = stage == kShadowX || stage == kShadowY;
-                bool done = walk_iter<COUNT>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
-                                             shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
-                                             false, h, cnt);
-                done = done || ++it >= max_iter;
-                if (done) {
-                    if (stage == kGather) {
-                        float o[3], d[3];
-                        gather_ray(o, d);
-                        if (!h.hit) {
-                            float sky[3];
-                            sky_color(p, d, sky);
-                            rad[sid] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky,
-                                                   sky[2] * p.gi_sky, p.gi_dist);
-                            face_out[sid] = (uint8_t)VG_FACE_NONE;
-                            stage = kIdle;
-                        } else {
-                            tX = h.t;
-                            fX = h.face;
-                            mX = h.mat;
-                            ndlX = face_ndl(fX, p.sun);
-                            visX = 0.0f;
-                            gi_offset_origin(o, d, tX, fX, so);
-                            if (ndlX > 0.0f) {
-                                if (!start(so, p.sun, kShadowX)) {
-                                    visX = 1.0f;
-                                    after_x();
-                                }
-                            } else {
-                                after_x();
-                            }
-                        }
-                    } else if (stage == kShadowX) {
-                        visX = h.hit ? 0.0f : 1.0f;
-                        after_x();
-                    } else if (BOUNCES == 3 && stage == kBounce) {
-                        if (!h.hit) {
-                            train_sky();
-                        } else {
-                            float d[3], yo[3];
-                            train_dir(d);
-                            const uint32_t fY = h.face;
-                            mY = h.mat;
-                            ndlY = face_ndl(fY, p.sun);
-                            gi_offset_origin(so, d, h.t, fY, yo);
-                            if (ndlY > 0.0f) {
-                                if (!start(yo, p.sun, kShadowY)) train_hit(1.0f);
-                            } else {
-                                train_hit(0.0f);
-                            }
-                        }
-                    } else if (BOUNCES == 3) { // kShadowY of a training lane
-                        train_hit(h.hit ? 0.0f : 1.0f);
-                    } else if (stage == kBounce) {
-                        if (!h.hit) {
-                            float d[3], sky[3];
-                            bounce_ray(d);
-                            sky_color(p, d, sky);
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) lb[k] = sky[k] * p.gi_sky;
-                            finish_hit();
-                        } else {
-                            float d[3], yo[3];
-                            bounce_ray(d);
-                            const uint32_t fY = h.face;
-                            mY = h.mat;
-                            ndlY = face_ndl(fY, p.sun);
-                            gi_offset_origin(so, d, h.t, fY, yo);
-                            if (ndlY > 0.0f) {
-                                if (!start(yo, p.sun, kShadowY)) after_y(1.0f);
-                            } else {
-                                after_y(0.0f);
-                            }
-                        }
-                    } else { // kShadowY
-                        after_y(h.hit ? 0.0f : 1.0f);
-                    }
-                }
+                bool done = walk_iter<COUNT, true>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                                                   shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
+                                                   false, h, cnt);
+                ended = done || ++it >= max_iter;
             }
+            if (ended) transition();
         } while (__popcll(__ballot(stage != kIdle)) >= threshold);
     }
     if (COUNT) {
@@ -1027,8 +1091,19 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     if (n_upd == 0) return VG_OK;
     const uint64_t n_samples = (uint64_t)n_upd * (uint64_t)pr->rays;
     unsigned long long *cd = count ? pr->counts : nullptr;
+    // heavy-path lists: this update writes half hp and reads the other (the previous update's)
+    const int hp = pr->hpar;
+    pr->hpar ^= 1;
+    HeavyArgs ha;
+    ha.cur_ids = pr->heavy + (size_t)hp * pr->hcap;
+    ha.prev_ids = pr->heavy + (size_t)(hp ^ 1) * pr->hcap;
+    ha.cur_mask = pr->hmask + (size_t)hp * pr->hwords;
+    ha.prev_mask = pr->hmask + (size_t)(hp ^ 1) * pr->hwords;
+    ha.cap = pr->hcap;
+    ha.thresh = (nrc || p->bounces != 2) ? kHeavyIters : kHeavyIters2;
     hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
-                       w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0);
+                       w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0,
+                       ha.cur_mask, pr->hwords);
     HIP_TRY(hipGetLastError());
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
     const int thr = w->ctx->probe_refill;
@@ -1046,7 +1121,8 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
 #define LAUNCH_G(C, B)                                                                         \
     hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
-                       pr->palette, pr->rad, pr->face, la, pr->ctr, thr, cd)
+                       pr->palette, pr->rad, pr->face, la, ha, pr->ctr, thr,                    \
+                       (uint32_t)w->ctx->probe_chunk, cd)
     const int mode = nrc ? 3 : p->bounces;
     if (count) {
         if (mode == 3) LAUNCH_G(true, 3);
@@ -1132,8 +1208,8 @@ static int probes_update_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nr
 }
 
 static void free_probes(vg_probes *p) {
-    void *ptrs[] = {p->probes, p->dirs, p->bounce, p->palette, p->pwork,
-                    p->rad,    p->face, p->queries, p->ctr, p->counts};
+    void *ptrs[] = {p->probes, p->dirs,    p->bounce, p->palette, p->pwork, p->rad,
+                    p->face,   p->queries, p->ctr,    p->counts,  p->heavy, p->hmask};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -1168,6 +1244,8 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
     pr->total = total;
     const size_t nd = (size_t)p->rays_per_probe * p->epochs;
     const size_t ns = (size_t)total * p->rays_per_probe;
+    pr->hwords = (uint32_t)((ns + 31) / 32);
+    pr->hcap = (uint32_t)(ns / 32 > 4096 ? ns / 32 : 4096);
     bool ok = hipMalloc((void **)&pr->probes, (size_t)total * sizeof(vg_probe)) == hipSuccess &&
               hipMalloc((void **)&pr->dirs, nd * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->bounce, VG_GI_BOUNCE_TABLE * sizeof(float4)) == hipSuccess &&
@@ -1177,6 +1255,10 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
               hipMalloc((void **)&pr->face, ns) == hipSuccess &&
               hipMalloc((void **)&pr->queries, 2 * ns * sizeof(vg_nrc_query)) == hipSuccess &&
               hipMalloc((void **)&pr->ctr, kGiCtrs * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&pr->heavy, 2 * (size_t)pr->hcap * sizeof(uint32_t)) ==
+                  hipSuccess &&
+              hipMalloc((void **)&pr->hmask, 2 * (size_t)pr->hwords * sizeof(uint32_t)) ==
+                  hipSuccess &&
               hipMalloc((void **)&pr->counts, kGiCounters * sizeof(unsigned long long)) ==
                   hipSuccess;
     if (!ok) {
@@ -1197,6 +1279,7 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
         e = hipMemcpy(pr->palette, pal.data(), pal.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(pr->probes, 0, (size_t)total * sizeof(vg_probe));
     if (e == hipSuccess) e = hipMemset(pr->ctr, 0, kGiCtrs * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(pr->hmask, 0, 2 * (size_t)pr->hwords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_probes(pr);
@@ -1301,7 +1384,7 @@ int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32
         const int r = probes_update(pr, p, nrc, flags, false, s);
         // the timed span includes the update's training step (on the cache's stream)
         return r == VG_OK && nrc ? nrc_join(nrc, s) : r;
-    }, best_ms);
+    }, best_ms, &c->probe_chunk);
 }
 
 int vg_probes_update_learned_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc,

@@ -323,8 +323,12 @@ __device__ __forceinline__ void cache_reset(WalkCache &q) {
 // shaders/raymarch.wgsl:4626-4775, plus exact 2x2x2 groups of empty children). The words of
 // the current L4 cell, chunk, tile and brick stay in registers until the walk leaves them.
 // Returns true when the walk has ended (h then holds the hit, or stays a miss). COUNT
-// tallies iteration classes for algorithmic-byte accounting.
-template <bool COUNT>
+// tallies iteration classes for algorithmic-byte accounting. With SPEC the L4, chunk, tile
+// and brick words of the current voxel are all fetched up front (every level's index is a
+// function of the voxel coordinates alone and every level is stored dense, so each fetch is
+// in bounds): one memory round trip per iteration instead of a chain of dependent ones, for
+// kernels whose rays are too incoherent to hide that chain. The decisions are unchanged.
+template <bool COUNT, bool SPEC = false>
 __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld &w, float tmax,
                                           uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
     if (r.t_cur > tmax) return true;
@@ -338,6 +342,29 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld 
             int32_t s = r.l[k] + w.ro_mod[k];
             sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
         }
+    }
+    if (SPEC) {
+        const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
+                             (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
+        const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
+                            (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+        const uint32_t tn = ci * 64u + (uint32_t)cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3,
+                                                      (sl[2] >> 4) & 3);
+        const uint32_t bn = tn * 64u + (uint32_t)cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3,
+                                                      (sl[2] >> 2) & 3);
+        // unconditional loads, all consumed by one empty asm statement: the compiler can
+        // neither sink them into the branches that use them nor wait between them
+        q.l4w = w.l4[l4i];
+        q.cw = w.chunk[ci];
+        q.tw = w.tile[tn];
+        q.tu = w.tile_uni[tn];
+        q.ow = w.occ[bn];
+        q.bu = w.brick_uni[bn];
+        asm volatile("" ::"v"(q.l4w), "v"(q.cw), "v"(q.tw), "v"(q.tu), "v"(q.ow), "v"(q.bu));
+        q.cur_l4 = l4i;
+        q.cur_c = ci;
+        q.cur_t = tn;
+        q.cur_b = bn;
     }
     int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
     do {
@@ -569,10 +596,10 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Wave-local reservation of work ids for the persistent batched-query kernels: a refill
-// hands its idle lanes the next ids of the reservation and takes a new chunk of kIdChunk
-// ids from the global counter (one atomic per chunk, not per refill) only when the
-// reservation runs short. Single-address atomics serialise across the chip, so fewer of
+// Wave-local reservation of work ids for the persistent batched-query and probe path
+// kernels: a refill hands its idle lanes the next ids of the reservation and takes a new
+// chunk of ids (kIdChunk, or a tuned size) from the global counter (one atomic per chunk,
+// not per refill) only when the reservation runs short. Single-address atomics serialise across the chip, so fewer of
 // them lets waves refill more often. (The frame kernel keeps one atomic per refill: its
 // refills must come from the front of the screen order to stay coherent.) All members are
 // wave-uniform.
@@ -592,14 +619,17 @@ __device__ __forceinline__ bool queue_empty(const IdQueue &iq) {
 }
 
 // Ids for `need` (<= 64) idle lanes: the idle lane of rank `rank` gets the returned id when
-// `want` is set (~0u: none left). Every lane of the wave must call it together.
+// `want` is set (~0u: none left). A short reservation takes max(chunk, shortfall) ids with
+// one atomic (chunk 0: exactly what the refill lacks, no reservation left over). Every lane
+// of the wave must call it together.
 __device__ __forceinline__ uint32_t queue_take(IdQueue &iq, uint32_t *work, uint32_t total,
                                                uint32_t need, uint32_t rank, bool want,
-                                               uint32_t lane) {
+                                               uint32_t lane, uint32_t chunk = kIdChunk) {
     const uint32_t avail = iq.end - iq.next;
-    uint32_t b = ~0u;
+    uint32_t b = ~0u, g = 0u;
     if (avail < need && !iq.drained) {
-        if (lane == 0) b = atomicAdd(work, kIdChunk);
+        g = need - avail > chunk ? need - avail : chunk;
+        if (lane == 0) b = atomicAdd(work, g);
         b = __shfl(b, 0, 64);
         if (b >= total) {
             iq.drained = true;
@@ -613,7 +643,7 @@ __device__ __forceinline__ uint32_t queue_take(IdQueue &iq, uint32_t *work, uint
         if (id >= total) id = ~0u;
     }
     if (b != ~0u) {
-        const uint32_t end = total - b < kIdChunk ? total : b + kIdChunk;
+        const uint32_t end = total - b < g ? total : b + g;
         const uint32_t nx = b + (need - avail);
         iq.next = nx < end ? nx : end;
         iq.end = end;

@@ -15,6 +15,7 @@ struct vg_ctx {
     int rays_refill;
     int probe_wpc;       // persistent probe path kernel: waves per CU
     int probe_refill;
+    int probe_chunk;     // ... work ids per atomic (0: exactly what a refill lacks)
 };
 
 struct vg_edit_state; // sphere-edit buffers (vg_edit.hip), created on first use
@@ -78,10 +79,14 @@ inline bool host_normalize(const float in[3], float out[3]) {
     return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
 }
 
+// With chunk_opt, the work ids taken per atomic are a third dimension of the sweep.
 template <class L>
-inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms) {
+inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms,
+                    int *chunk_opt = nullptr) {
     static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
     static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
+    static const int kChunk[] = {0, 64, 128, 256};
+    const int n_chunk = chunk_opt ? 4 : 1;
     const int reps = 3;
     hipStream_t stream;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -91,34 +96,35 @@ inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *b
     const int old_schedule = c->schedule;
     c->schedule = VG_SCHEDULE_PERSISTENT;
     float best = INFINITY;
-    int bw = *wpc_opt, br = *refill_opt, st = VG_OK;
-    for (int wpc : kWpc) {
-        for (int rf : kRefill) {
-            *wpc_opt = wpc;
-            *refill_opt = rf;
-            st = launch(stream);
-            if (st) break;
-            float ms_min = INFINITY;
-            for (int i = 0; i < reps && st == VG_OK; ++i) {
-                float ms = 0.0f;
-                if (hipEventRecord(e0, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
-                if (!st) st = launch(stream);
-                if (!st && hipEventRecord(e1, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
-                if (!st && hipEventSynchronize(e1) != hipSuccess) st = fail(VG_E_DEVICE, "sync");
-                if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = fail(VG_E_DEVICE, "elapsed");
-                ms_min = ms < ms_min ? ms : ms_min;
-            }
-            if (st) break;
-            if (ms_min < best) {
-                best = ms_min;
-                bw = wpc;
-                br = rf;
-            }
+    int bw = *wpc_opt, br = *refill_opt, bc = chunk_opt ? *chunk_opt : 0, st = VG_OK;
+    for (int ci = 0; ci < n_chunk * 49 && st == VG_OK; ++ci) {
+        const int wpc = kWpc[ci % 7], rf = kRefill[(ci / 7) % 7];
+        *wpc_opt = wpc;
+        *refill_opt = rf;
+        if (chunk_opt) *chunk_opt = kChunk[ci / 49];
+        st = launch(stream);
+        if (st) break;
+        float ms_min = INFINITY;
+        for (int i = 0; i < reps && st == VG_OK; ++i) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
+            if (!st) st = launch(stream);
+            if (!st && hipEventRecord(e1, stream) != hipSuccess) st = fail(VG_E_DEVICE, "event");
+            if (!st && hipEventSynchronize(e1) != hipSuccess) st = fail(VG_E_DEVICE, "sync");
+            if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = fail(VG_E_DEVICE, "elapsed");
+            ms_min = ms < ms_min ? ms : ms_min;
         }
         if (st) break;
+        if (ms_min < best) {
+            best = ms_min;
+            bw = wpc;
+            br = rf;
+            if (chunk_opt) bc = *chunk_opt;
+        }
     }
     *wpc_opt = bw;
     *refill_opt = br;
+    if (chunk_opt) *chunk_opt = bc;
     c->schedule = old_schedule;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
