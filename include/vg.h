@@ -541,9 +541,11 @@ int vg_frame_indirect(vg_probes *probes, const vg_gi_params *params, const vg_ca
 #define VG_NRC_TRAIN 1u
 int vg_probes_update_learned(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                              uint32_t flags, void *stream);
-/* Times every (waves per CU, refill) pair of the probe path kernels on this very update
- * (repeated; nrc may be NULL for the plain update, with VG_NRC_TRAIN the cache trains on
- * every repetition) and keeps the fastest in the context (VG_OPT_PROBE_*). */
+/* Times every schedule of the probe path kernels (waves per CU, refill threshold, work ids
+ * per atomic) on this very update (repeated; nrc may be NULL for the plain update, with
+ * VG_NRC_TRAIN the cache trains on every repetition, timed as two back-to-back updates so
+ * the training step's overlap with the next update counts) and keeps the fastest in the
+ * context (VG_OPT_PROBE_*). */
 int vg_autotune_probes(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                        uint32_t flags, float *best_ms);
 /* The same, reporting rays traced (gather, shadow, training bounce) and the cache queries

@@ -5,11 +5,12 @@
 //   k_probe_paths   persistent, ray-compacting: one lane per gather sample runs the whole
 //                   path (gather ray -> sun shadow ray at the hit -> optional cosine bounce
 //                   ray -> sun shadow ray at the bounce hit) through the exact walk, refilling
-//                   idle lanes like the frame kernel, and writes the sample's radiance and
-//                   visible distance.
-//   k_probe_reduce  one wave per updated probe: SH-L1 projection of its samples in a fixed
-//                   lane order (deterministic), the nearest hit for relocation, the staging
-//                   sum and the hysteresis fold of complete cycles.
+//                   idle lanes like the frame kernel, and writes the sample's gather face
+//                   and, for a hit, its radiance and visible distance.
+//   k_probe_reduce  a group of lanes per updated probe: the sky radiance of samples without
+//                   a gather hit, SH-L1 projection of its samples in a fixed lane order
+//                   (deterministic), the nearest hit for relocation, the staging sum and the
+//                   hysteresis fold of complete cycles.
 // Samples of one probe have consecutive ids, so a wave's gather rays share their origin.
 // The learned update (vg_probes_update_learned) runs the path kernel without the bounce:
 // every gather hit is appended to a compacted list of cache queries, one inference launch
@@ -30,7 +31,7 @@ struct vg_probes {
     float4 *bounce;         // VG_GI_BOUNCE_TABLE cosine-hemisphere directions
     float4 *palette;        // 256 albedo entries
     float4 *pwork;          // per updated probe: ray origin xyz, w = flags bits
-    float4 *rad;            // per sample: radiance rgb, visible distance
+    float4 *rad;            // per sample with a gather hit: radiance rgb, visible distance
     uint8_t *face;          // per sample: face code of the gather hit (VG_FACE_NONE: none)
     vg_nrc_query *queries;  // learned update: compacted gather hits, two halves of `ns` entries
     size_t ns;              // samples per update at most (total probes x rays)
@@ -336,12 +337,6 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         d[1] = dd.y;
         d[2] = dd.z;
     };
-    auto sky_sample = [&](const float d[3]) {
-        float sky[3];
-        sky_color(p, d, sky);
-        rad[sid] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky, sky[2] * p.gi_sky, p.gi_dist);
-        face_out[sid] = (uint8_t)VG_FACE_NONE;
-    };
     // Finish a sample with a gather hit: radiance toward the probe, given the light lb
     // arriving at the hit from its bounce, and the visible distance.
     auto finish_hit = [&](const float lb[3]) {
@@ -488,11 +483,12 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     }
                 }
             } else if (ev == kGather) {
-                float go[3], gd[3];
-                gather_ray(sid, go, gd);
                 if (!h.hit) {
-                    sky_sample(gd);
+                    // the sky seen along the gather ray: k_probe_reduce evaluates it
+                    face_out[sid] = (uint8_t)VG_FACE_NONE;
                 } else {
+                    float go[3], gd[3];
+                    gather_ray(sid, go, gd);
                     const uint32_t f = h.face;
                     const float ndl = face_ndl(f, p.sun);
                     sl.tX[me] = h.t;
@@ -632,127 +628,131 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     }
 }
 
-// One wave per updated probe: project its samples, fold, relocate, write the record.
+// A group of L lanes per updated probe (L = the sample count rounded up to a power of two,
+// 8..64; 64 / L probes per wave): project its samples onto SH-L1 (each lane a fixed strided
+// subset, then a butterfly over the group: a fixed order, so deterministic), find the
+// nearest hit for relocation, then fold and relocate the 48-float record with one field
+// (or a few) per lane.
 __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_params p, uint32_t n_upd,
-                                                        uint32_t epoch, uint32_t phase,
+                                                        uint32_t epoch, uint32_t phase, uint32_t L,
                                                         const float4 *__restrict__ pwork,
                                                         const float4 *__restrict__ dirs,
                                                         const float4 *__restrict__ rad,
                                                         const uint8_t *__restrict__ face,
                                                         vg_probe *probes) {
+    __shared__ float sums[kBlock / 8][20]; // per group: the 20 projected sums
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t j = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (j >= n_upd) return; // wave-uniform
-    const int64_t idx = (int64_t)j * p.update_div + phase;
+    const uint32_t gl = lane & (L - 1u);                 // lane within the group
+    const uint32_t grp = threadIdx.x / L;                // group within the block
+    const uint32_t j = blockIdx.x * (kBlock / L) + grp;  // updated probe
+    const bool live = j < n_upd;
+    const uint32_t jj = live ? j : 0u;
+    const int64_t idx = (int64_t)jj * p.update_div + phase;
     const uint32_t N = (uint32_t)p.rays_per_probe;
-    const float4 pw = pwork[j];
-    const uint32_t fl = __float_as_uint(pw.w);
-    int32_t g[3];
-    probe_cell(p, w, idx, g);
-    float4 *rec = reinterpret_cast<float4 *>(probes + idx);
-    if (fl & kProbeBuried) {
-        if (lane < 12) rec[lane] = lane == 5 ? make_float4((float)g[0], (float)g[1], (float)g[2], 0.0f)
-                                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
+    const uint32_t fl = __float_as_uint(pwork[jj].w);
+    const bool buried = (fl & kProbeBuried) != 0;
     const float wpp = 12.566371f / (float)N; // 4 pi / N (uniform-sphere MC weight)
-    float c[5][4];
+    float c[20];
 #pragma unroll
-    for (int a = 0; a < 5; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) c[a][b] = 0.0f;
+    for (int a = 0; a < 20; ++a) c[a] = 0.0f;
     float near_t = 1.0e30f;
     uint32_t near_k = 0xFFFFFFFFu;
-    for (uint32_t k = lane; k < N; k += 64) {
-        const uint32_t s = j * N + k;
-        const float4 v = rad[s];
-        const float4 dd = dirs[k * (uint32_t)p.epochs + epoch];
-        const float bs[4] = {0.282095f, 0.488603f * dd.y, 0.488603f * dd.z, 0.488603f * dd.x};
-        const float ws[5] = {wpp * v.x, wpp * v.y, wpp * v.z, wpp * v.w, wpp * v.w * v.w};
+    if (live && !buried) {
+        for (uint32_t k = gl; k < N; k += L) {
+            const uint32_t s = jj * N + k;
+            const float4 dd = dirs[k * (uint32_t)p.epochs + epoch];
+            const uint32_t fs = face[s];
+            float4 v;
+            if (fs == VG_FACE_NONE) { // no gather hit: the sky along the ray, at gi_dist
+                const float d3[3] = {dd.x, dd.y, dd.z};
+                float sky[3];
+                sky_color(p, d3, sky);
+                v = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky, sky[2] * p.gi_sky, p.gi_dist);
+            } else {
+                v = rad[s];
+            }
+            const float bs[4] = {0.282095f, 0.488603f * dd.y, 0.488603f * dd.z, 0.488603f * dd.x};
+            const float ws[5] = {wpp * v.x, wpp * v.y, wpp * v.z, wpp * v.w, wpp * v.w * v.w};
 #pragma unroll
-        for (int a = 0; a < 5; ++a)
+            for (int a = 0; a < 5; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) c[a][b] += ws[a] * bs[b];
-        if (face[s] != VG_FACE_NONE && v.w < near_t) {
-            near_t = v.w;
-            near_k = k;
+                for (int b = 0; b < 4; ++b) c[a * 4 + b] += ws[a] * bs[b];
+            if (fs != VG_FACE_NONE && v.w < near_t) {
+                near_t = v.w;
+                near_k = k;
+            }
         }
     }
+    // butterfly over the group (groups are whole: every lane of a group takes the same path)
+    for (uint32_t off = L >> 1; off > 0; off >>= 1) {
 #pragma unroll
-    for (int a = 0; a < 5; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            float v = c[a][b];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            c[a][b] = v;
-        }
-    // nearest hit, ties to the lowest sample (the reference's ascending loop with <)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float ot = __shfl_xor(near_t, off, 64);
-        const uint32_t ok = __shfl_xor(near_k, off, 64);
+        for (int a = 0; a < 20; ++a) c[a] += __shfl_xor(c[a], (int)off, 64);
+        // nearest hit, ties to the lowest sample (the reference's ascending loop with <)
+        const float ot = __shfl_xor(near_t, (int)off, 64);
+        const uint32_t ok = __shfl_xor(near_k, (int)off, 64);
         if (ot < near_t || (ot == near_t && ok < near_k)) {
             near_t = ot;
             near_k = ok;
         }
     }
-    if (lane != 0) return;
-    float n[3] = {0.0f, 0.0f, 0.0f};
-    if (near_k != 0xFFFFFFFFu) {
-        const uint32_t f = face[j * N + near_k];
+    if (gl == 0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) n[k] = face_n(f, k);
+        for (int a = 0; a < 20; ++a) sums[grp][a] = c[a];
     }
-    float4 pr[12];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) pr[i] = rec[i];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (!live) return;
+    int32_t g[3];
+    probe_cell(p, w, idx, g);
+    float *rec = reinterpret_cast<float *>(probes + idx);
+    if (buried) {
+        for (uint32_t f = gl; f < 48; f += L) rec[f] = f >= 20 && f < 23 ? (float)g[f - 20] : 0.0f;
+        return;
+    }
     const bool is_new = (fl & kProbeNew) != 0;
-    const float off_prev[3] = {is_new ? 0.0f : pr[11].y, is_new ? 0.0f : pr[11].z,
-                               is_new ? 0.0f : pr[11].w};
-    auto v4 = [](const float *a) { return make_float4(a[0], a[1], a[2], a[3]); };
-    if (is_new) {
-        pr[0] = v4(c[0]);
-        pr[1] = v4(c[1]);
-        pr[2] = v4(c[2]);
-        pr[3] = v4(c[3]);
-        pr[4] = v4(c[4]);
+    const bool fold = !is_new && epoch == (uint32_t)p.epochs - 1u;
+    // staging count after this update (the fold divides by it, then clears it)
+    const float cnt1 = is_new ? 0.0f : rec[44] + 1.0f;
+    float n3[3] = {0.0f, 0.0f, 0.0f};
+    if (near_k != 0xFFFFFFFFu) {
+        const uint32_t f = face[jj * N + near_k];
 #pragma unroll
-        for (int i = 6; i < 12; ++i) pr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    } else {
-        float *st = &pr[6].x;
-#pragma unroll
-        for (int a = 0; a < 5; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) st[a * 4 + b] = st[a * 4 + b] + c[a][b];
-        pr[11].x = pr[11].x + 1.0f;
-        if (epoch == (uint32_t)p.epochs - 1u) {
-            const float cnt = pr[11].x > 1.0f ? pr[11].x : 1.0f;
-            const float h = p.hysteresis;
-            float *sh = &pr[0].x;
-#pragma unroll
-            for (int i = 0; i < 20; ++i) sh[i] = (st[i] / cnt) * (1.0f - h) + sh[i] * h;
-#pragma unroll
-            for (int i = 6; i < 12; ++i) pr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+        for (int k = 0; k < 3; ++k) n3[k] = face_n(f, k);
     }
-    // relocation (shaders/gi_probes.wgsl:381-392)
-    float off[3] = {off_prev[0], off_prev[1], off_prev[2]};
-    if (near_t < 1.2f) {
-        const float m = (1.2f - near_t) < 0.35f ? (1.2f - near_t) : 0.35f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float v = off[k] + n[k] * m;
-            off[k] = v < -3.5f ? -3.5f : (v > 3.5f ? 3.5f : v);
+    for (uint32_t f = gl; f < 48; f += L) {
+        float out;
+        if (f < 20) {
+            const float cf = sums[grp][f];
+            if (is_new) {
+                out = cf;
+            } else if (fold) {
+                const float cnt = cnt1 > 1.0f ? cnt1 : 1.0f;
+                const float h = p.hysteresis;
+                out = ((rec[24 + f] + cf) / cnt) * (1.0f - h) + rec[f] * h;
+            } else {
+                out = rec[f];
+            }
+        } else if (f < 24) {
+            out = f < 23 ? (float)g[f - 20] : 1.0f;
+        } else if (f < 44) {
+            out = (is_new || fold) ? 0.0f : rec[f] + sums[grp][f - 24];
+        } else if (f == 44) {
+            out = (is_new || fold) ? 0.0f : cnt1;
+        } else {
+            // relocation (shaders/gi_probes.wgsl:381-392)
+            const int k = (int)f - 45;
+            float o = is_new ? 0.0f : rec[f];
+            if (near_t < 1.2f) {
+                const float m = (1.2f - near_t) < 0.35f ? (1.2f - near_t) : 0.35f;
+                const float v = o + n3[k] * m;
+                o = v < -3.5f ? -3.5f : (v > 3.5f ? 3.5f : v);
+            } else if (near_t > 2.5f) {
+                o = o * 0.98f;
+            }
+            out = o;
         }
-    } else if (near_t > 2.5f) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) off[k] = off[k] * 0.98f;
+        rec[f] = out;
     }
-    pr[11] = make_float4(pr[11].x, off[0], off[1], off[2]);
-    pr[5] = make_float4((float)g[0], (float)g[1], (float)g[2], 1.0f);
-#pragma unroll
-    for (int i = 0; i < 12; ++i) rec[i] = pr[i];
 }
 
 // Training targets of the learned update's steps after the first (the first step's are
@@ -1149,10 +1149,12 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     // path launch write); more steps, or a counting run, stay on the caller's stream
     const bool overlap = train && nrc->cfg.steps == 1 && !count;
     if (overlap) HIP_TRY(hipEventRecord(nrc->ev_ready, stream));
-    const uint32_t wpb = kBlock / 64;
-    hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + wpb - 1) / wpb), dim3(kBlock), 0, stream,
-                       w->dw, *p, n_upd, epoch, phase, pr->pwork, pr->dirs, pr->rad, pr->face,
-                       pr->probes);
+    uint32_t lpp = 8; // lanes per probe in the reduction: the sample count's power of two
+    while (lpp < 64 && lpp < (uint32_t)pr->rays) lpp <<= 1;
+    const uint32_t ppb = kBlock / lpp;
+    hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + ppb - 1) / ppb), dim3(kBlock), 0, stream,
+                       w->dw, *p, n_upd, epoch, phase, lpp, pr->pwork, pr->dirs, pr->rad,
+                       pr->face, pr->probes);
     HIP_TRY(hipGetLastError());
     if (overlap) {
         HIP_TRY(hipStreamWaitEvent(nrc->tstream, nrc->ev_ready, 0));
@@ -1380,11 +1382,17 @@ int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32
     int st = check_world(pr->world);
     if (st) return st;
     vg_ctx *c = pr->world->ctx;
-    return autotune(c, &c->probe_wpc, &c->probe_refill, [&](hipStream_t s) {
-        const int r = probes_update(pr, p, nrc, flags, false, s);
-        // the timed span includes the update's training step (on the cache's stream)
+    // With training, a learned update's step overlaps the next update, so a candidate is
+    // timed on two updates back to back (the steady state) and the span closes on the
+    // second one's step; the reported time is per update.
+    const bool pair = nrc && (flags & VG_NRC_TRAIN) && nrc->cfg.steps == 1;
+    st = autotune(c, &c->probe_wpc, &c->probe_refill, [&](hipStream_t s) {
+        int r = probes_update(pr, p, nrc, flags, false, s);
+        if (r == VG_OK && pair) r = probes_update(pr, p, nrc, flags, false, s);
         return r == VG_OK && nrc ? nrc_join(nrc, s) : r;
     }, best_ms, &c->probe_chunk);
+    if (st == VG_OK && pair && best_ms) *best_ms *= 0.5f;
+    return st;
 }
 
 int vg_probes_update_learned_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc,
