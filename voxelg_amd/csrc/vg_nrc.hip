@@ -365,25 +365,6 @@ __device__ __forceinline__ void pair_range(uint32_t pairs, uint32_t &p0, uint32_
     p1 = p0 + per < pairs ? p0 + per : pairs;
 }
 
-// max |d loss / d feature| over the batch (fp32 bits; non-negative floats order as integers).
-__global__ __launch_bounds__(256) void k_nrc_gmax(const float2 *__restrict__ dfeat, uint32_t n,
-                                                  int levels, uint32_t *gmax) {
-    __shared__ float sm[4];
-    float mx = 0.0f;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n * 16u; i += gridDim.x * 256u) {
-        if ((int)(i & 15u) >= levels) continue;
-        const float2 d = dfeat[i];
-        mx = fmaxf(mx, fmaxf(fabsf(d.x), fabsf(d.y)));
-    }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        mx = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
-        atomicMax(gmax, __float_as_uint(mx));
-    }
-}
-
 // Record counts per (bucket, block): hist[bucket * kGBlocks + block]. Pairs whose gradient is
 // zero emit no records.
 __global__ __launch_bounds__(kGThreads) void k_nrc_gcount(NrcDev m, const vg_nrc_query *__restrict__ q,
@@ -508,24 +489,26 @@ __global__ __launch_bounds__(kGRThreads) void k_nrc_greduce(const uint32_t *__re
 }
 
 // Forward and backward of a training batch from its encoding (feat): weight-gradient sums
-// per block into `partial`, the encoding gradient per sample into dfeat[s * 32 + input].
+// per block into `partial`, the encoding gradient per sample into dfeat[s * 32 + input], and
+// the largest |gradient| of the used levels' inputs into gmax (fp32 bits; zeroed by
+// k_nrc_encode): the grid gradient's fixed-point scale.
 __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query *__restrict__ q,
                                                    const uint32_t *__restrict__ feat,
                                                    const float4 *__restrict__ tgt, uint32_t n,
                                                    float inv_n, float *__restrict__ partial,
-                                                   float *__restrict__ dfeat) {
+                                                   float *__restrict__ dfeat, uint32_t *gmax) {
     __shared__ u32x4_t sf[kNrcFrags * 64];
     __shared__ u32x4_t sbw[kNrcFrags * 64];
     __shared__ __align__(16) uint16_t sX[kNrcWidth * kSRow];
     __shared__ __align__(16) uint16_t sD[kNrcWidth * kSRow];
-    __shared__ float sl[4];
+    __shared__ float sl[4], sm[4];
     load_frags(sf, m.frag_fwd);
     load_frags(sbw, m.frag_bwd);
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const int wave = threadIdx.x >> 6;
     float *part = partial + (size_t)blockIdx.x * kNrcPartial;
-    float loss = 0.0f;
+    float loss = 0.0f, gmx = 0.0f;
     bool first = true;
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * kNrcTile < n; tile += gridDim.x) {
         u32x4_t X[4][2][4];
@@ -603,15 +586,29 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query 
             float4 *dst = reinterpret_cast<float4 *>(dfeat + (size_t)s * 32u);
             dst[g] = make_float4(ga[0][sb][0], ga[0][sb][1], ga[0][sb][2], ga[0][sb][3]);
             dst[4 + g] = make_float4(ga[1][sb][0], ga[1][sb][1], ga[1][sb][2], ga[1][sb][3]);
+            // inputs 4g + k (levels 2g, 2g + 1) and 16 + 4g + k (levels 8 + 2g, 9 + 2g)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (2 * g + (k >> 1) < m.levels) gmx = fmaxf(gmx, fabsf(ga[0][sb][k]));
+                if (8 + 2 * g + (k >> 1) < m.levels) gmx = fmaxf(gmx, fabsf(ga[1][sb][k]));
+            }
         }
         first = false;
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
-    if (lane == 0) sl[wave] = loss;
+    for (int off = 32; off > 0; off >>= 1) {
+        loss += __shfl_xor(loss, off, 64);
+        gmx = fmaxf(gmx, __shfl_xor(gmx, off, 64));
+    }
+    if (lane == 0) {
+        sl[wave] = loss;
+        sm[wave] = gmx;
+    }
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         part[VG_NRC_MLP_PARAMS] = ((sl[0] + sl[1]) + (sl[2] + sl[3])) * 0.5f * inv_n;
+        atomicMax(gmax, __float_as_uint(fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]))));
+    }
 }
 
 // Sum of the block partials: the MLP gradient and the loss; with `update`, one more step.
@@ -688,6 +685,43 @@ __global__ __launch_bounds__(1024) void k_nrc_adam_mlp(float *__restrict__ p, fl
     }
 }
 
+// One optimizer step of the MLP, one thread per parameter: Adam, then the new weight's bf16
+// into its one place in each fragment array (the inverse of k_nrc_adam_mlp's packing map;
+// the fragment slots no parameter maps to hold zeros from the last full packing).
+__global__ __launch_bounds__(256) void k_nrc_adam_pack(float *__restrict__ p, float *__restrict__ mm,
+                                                       float *__restrict__ vv,
+                                                       const float *__restrict__ g,
+                                                       uint16_t *__restrict__ ffwd,
+                                                       uint16_t *__restrict__ fbwd, float lr,
+                                                       float b1, float b2, float eps,
+                                                       const uint32_t *step) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= VG_NRC_MLP_PARAMS) return;
+    const float t = (float)*step;
+    const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+    float pp = p[i], m = mm[i], v = vv[i];
+    adam1(pp, m, v, g[i], lr, b1, b2, eps, c1, c2);
+    p[i] = pp;
+    mm[i] = m;
+    vv[i] = v;
+    const uint16_t hb = (uint16_t)bf16_bits(pp);
+    // fragment element e = f * 512 + lane * 8 + j with f = lyr * 8 + blk * 2 + ks,
+    // lane = gg * 16 + r, phi = 32 ks + 16 (j >> 2) + 4 gg + (j & 3) (k_nrc_adam_mlp)
+    auto slot = [](int lyr, int blk, int r, int phi) {
+        const int ks = phi >> 5, gg = (phi >> 2) & 3, j = ((phi >> 4) & 1) * 4 + (phi & 3);
+        return ((lyr * 8 + blk * 2 + ks) * 64 + gg * 16 + r) * 8 + j;
+    };
+    if (i < 12288) {
+        const int lyr = i >> 12, row = (i >> 6) & 63, col = i & 63;
+        ffwd[slot(lyr, row >> 4, row & 15, col)] = hb; // wf = p[lyr][16 blk + r][phi]
+        fbwd[slot(lyr, col >> 4, col & 15, row)] = hb; // wb = p[lyr][phi][16 blk + r]
+    } else {
+        const int i3 = i - 12288, row = i3 >> 6, col = i3 & 63; // row < 16
+        ffwd[slot(3, 0, row, col)] = hb;                 // blk 0: p[3][r][phi]
+        fbwd[slot(3, col >> 4, col & 15, row)] = hb;     // phi < 16: p[3][phi][16 blk + r]
+    }
+}
+
 // ---------------------------------------------------------------------- host --
 static NrcDev dev_view(const vg_nrc *m) {
     NrcDev d;
@@ -747,15 +781,12 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
     const uint64_t tiles = ((uint64_t)n + kNrcTile - 1) / kNrcTile;
     const int blocks = (int)(tiles < (uint64_t)m->partial_blocks ? tiles : m->partial_blocks);
     hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, stream, dv, q, m->feat, t, n,
-                       1.0f / (float)n, m->partial, m->dfeat);
+                       1.0f / (float)n, m->partial, m->dfeat, m->gmax);
     HIP_TRY(hipGetLastError());
     // grid gradient: bucket counts -> scan -> records in bucket order -> per-bucket sums
     const uint32_t nb = (uint32_t)m->gbuckets;
     const size_t nh = (size_t)nb * kGBlocks + 1;
     uint32_t *hoff = m->hist + nh;
-    hipLaunchKernelGGL(k_nrc_gmax, dim3(256), dim3(256), 0, stream,
-                       reinterpret_cast<const float2 *>(m->dfeat), n, m->cfg.levels, m->gmax);
-    HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_nrc_gcount, dim3(kGBlocks), dim3(kGThreads), nb * sizeof(uint32_t), stream, dv, q,
                        reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, m->hist);
     HIP_TRY(hipGetLastError());
@@ -776,10 +807,11 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
                        c.lr_grid, c.beta1, c.beta2, c.eps, m->step);
     HIP_TRY(hipGetLastError());
     if (update) {
-        hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(1), dim3(1024), 0, stream, m->mlp, m->mlp_m,
-                           m->mlp_v, m->mlp_g, reinterpret_cast<uint16_t *>(m->frag_fwd),
+        hipLaunchKernelGGL(k_nrc_adam_pack, dim3((VG_NRC_MLP_PARAMS + 255) / 256), dim3(256), 0,
+                           stream, m->mlp, m->mlp_m, m->mlp_v, m->mlp_g,
+                           reinterpret_cast<uint16_t *>(m->frag_fwd),
                            reinterpret_cast<uint16_t *>(m->frag_bwd), c.lr_mlp, c.beta1, c.beta2,
-                           c.eps, m->step, 1);
+                           c.eps, m->step);
         HIP_TRY(hipGetLastError());
     }
     return VG_OK;
