@@ -129,6 +129,41 @@ def test_param_errors(ctx, small):
         grid.update(q)
 
 
+@pytest.mark.parametrize("bounces", [1, 2])
+def test_schedules_give_the_same_records(ctx, small, small_world, bounces):
+    """The probe path kernel's schedule (waves per CU, refill threshold, work ids per atomic)
+    and its longest-first order (each update traces the previous update's heavy paths first
+    and skips them in the natural order) change only the order of the work: every schedule
+    gives the oracle's records, round after round."""
+    from voxelg_amd import _lib as L
+    keep = [ctx.get_option(o) for o in (L.VG_OPT_PROBE_WAVES_PER_CU, L.VG_OPT_PROBE_REFILL,
+                                         L.VG_OPT_PROBE_CHUNK)]
+    try:
+        for wpc, refill, chunk in ((8, 4, 0), (16, 48, 16), (4, 60, 128), (24, 32, 1024)):
+            ctx.set_option(L.VG_OPT_PROBE_WAVES_PER_CU, wpc)
+            ctx.set_option(L.VG_OPT_PROBE_REFILL, refill)
+            ctx.set_option(L.VG_OPT_PROBE_CHUNK, chunk)
+            print(f"case: schedule {wpc} waves/CU, refill {refill}, chunk {chunk}", flush=True)
+            p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=64, epochs=1,
+                             update_div=1, bounces=bounces)
+            run_rounds(ctx, small, small_world, p, range(3))
+    finally:
+        for o, v in zip((L.VG_OPT_PROBE_WAVES_PER_CU, L.VG_OPT_PROBE_REFILL,
+                         L.VG_OPT_PROBE_CHUNK), keep):
+            ctx.set_option(o, v)
+
+
+def test_schedule_option_errors(ctx):
+    from voxelg_amd import _lib as L
+    for bad in (-1, 1, 8, 15, 1025):
+        with pytest.raises(VGError):
+            ctx.set_option(L.VG_OPT_PROBE_CHUNK, bad)
+    keep = ctx.get_option(L.VG_OPT_PROBE_CHUNK)
+    ctx.set_option(L.VG_OPT_PROBE_CHUNK, 256)
+    assert ctx.get_option(L.VG_OPT_PROBE_CHUNK) == 256
+    ctx.set_option(L.VG_OPT_PROBE_CHUNK, keep)
+
+
 def indirect_case(small, p, cam, W, H, pg):
     hits, _ = small.trace_frame(cam, W, H, 512.0, 0, SUN, 480.0)
     out = pg.frame_indirect(p, cam, W, H, hits)
