@@ -555,8 +555,9 @@ __global__ __launch_bounds__(kBlock) void k_water_persistent(
         }
         do {
             if (kind != 0) {
-                bool done = walk_iter<false>(r, q, w, kSecondaryMaxT, VG_RAY_SKIP_TRANSPARENT, false,
-                                             h, nullptr);
+                // incoherent rays: fetch every level's word of a step at once
+                bool done = walk_iter<false, true>(r, q, w, kSecondaryMaxT,
+                                                   VG_RAY_SKIP_TRANSPARENT, false, h, nullptr);
                 done = done || ++it >= max_iter;
                 if (done) {
                     if (kind == 1) {
@@ -632,7 +633,8 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
         }
         do {
             if (busy) {
-                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
+                // incoherent rays: fetch every level's word of a step at once
+                bool done = walk_iter<COUNT, true>(r, q, w, tmax, f, lod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     store_hit(hits + id, h);
@@ -718,7 +720,8 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
         }
         do {
             if (busy) {
-                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
+                // incoherent rays: fetch every level's word of a step at once
+                bool done = walk_iter<COUNT, true>(r, q, w, tmax, f, lod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     store_hit(out, h);

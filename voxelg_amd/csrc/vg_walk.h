@@ -327,7 +327,9 @@ __device__ __forceinline__ void cache_reset(WalkCache &q) {
 // and brick words of the current voxel are all fetched up front (every level's index is a
 // function of the voxel coordinates alone and every level is stored dense, so each fetch is
 // in bounds): one memory round trip per iteration instead of a chain of dependent ones, for
-// kernels whose rays are too incoherent to hide that chain. The decisions are unchanged.
+// kernels whose rays are too incoherent to hide that chain (batched queries, probe paths,
+// water secondary rays: 1.5-5 % faster; the coherent frame kernel is issue-bound and 9 %
+// slower with it). The decisions are unchanged.
 template <bool COUNT, bool SPEC = false>
 __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld &w, float tmax,
                                           uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
