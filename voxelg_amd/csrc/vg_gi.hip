@@ -266,6 +266,8 @@ struct HeavyArgs {
 // wave's accesses are bank-conflict free). Only the walk itself stays in registers while the
 // lanes march, which keeps the kernel at four waves per SIMD.
 struct PathSlots {
+    float go[3][kBlock]; // gather ray origin (the probe's) and direction of the lane's sample
+    float gd[3][kBlock];
     float so[3][kBlock]; // secondary origin: off the gather hit (training: off the query point)
     float tX[kBlock];    // gather hit distance
     float ndlX[kBlock];  // max(0, n . sun) at the gather hit
@@ -295,6 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     // the lanes' kBounce / kShadowY stages then trace training targets)
     __shared__ PathSlots sl;
     __shared__ QueryStage<BOUNCES == 3 ? kQStage : 1u> qs;
+    __shared__ float4 spal[256]; // the albedo palette
+    for (uint32_t i = threadIdx.x; i < 256u; i += kBlock) spal[i] = pal[i];
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t me = threadIdx.x;
     const uint32_t wv = threadIdx.x >> 6;
@@ -326,22 +331,11 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     bool heavy = false; // this lane's finished sample path still has to be recorded heavy
     uint32_t qhead = 0, qn = 0; // the wave's staged queries: ring start and count
 
-    // the gather direction and origin of sample s
-    auto gather_ray = [&](uint32_t s, float o[3], float d[3]) {
-        const float4 po = pwork[s / N];
-        const float4 dd = dirs[(s % N) * (uint32_t)p.epochs + epoch];
-        o[0] = po.x;
-        o[1] = po.y;
-        o[2] = po.z;
-        d[0] = dd.x;
-        d[1] = dd.y;
-        d[2] = dd.z;
-    };
     // Finish a sample with a gather hit: radiance toward the probe, given the light lb
     // arriving at the hit from its bounce, and the visible distance.
     auto finish_hit = [&](const float lb[3]) {
         const uint32_t fm = sl.fm[me];
-        const float4 a = pal[(fm >> 8) & 0xFFu];
+        const float4 a = spal[(fm >> 8) & 0xFFu];
         const float sx = sl.ndlX[me] * sl.visX[me];
         rad[sid] = make_float4(a.x * (p.sun_color[0] * sx + lb[0]),
                                a.y * (p.sun_color[1] * sx + lb[1]),
@@ -350,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     };
     // light leaving the bounce hit toward the gather hit (sun through its shadow ray + ambient)
     auto bounce_light = [&](float visY, float out[3]) {
-        const float4 a = pal[(sl.fm[me] >> 16) & 0xFFu];
+        const float4 a = spal[(sl.fm[me] >> 16) & 0xFFu];
         const float sy = sl.ndlY[me] * visY;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
@@ -377,8 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     auto stage_queries = [&](bool final) {
         const uint64_t em = __ballot(emit);
         if (emit) {
-            float o[3], d[3];
-            gather_ray(sid, o, d);
+            const float o[3] = {sl.go[0][me], sl.go[1][me], sl.go[2][me]};
+            const float d[3] = {sl.gd[0][me], sl.gd[1][me], sl.gd[2][me]};
             const float t = sl.tX[me];
             const uint32_t k = (qhead + qn + lanes_below(em)) & (kQStage - 1u);
             qs.q[wv][k][0] = make_float4(o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t,
@@ -477,8 +471,22 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         take = !((ha.prev_mask[sid >> 5] >> (sid & 31u)) & 1u);
                     }
                     sl.pc[me] = 0u;
-                    if (take && !(__float_as_uint(pwork[sid / N].w) & kProbeBuried)) {
-                        gather_ray(sid, o, d);
+                    // the probe's origin and flags and the direction load together
+                    const uint32_t sc = take ? sid : 0u;
+                    const float4 po = pwork[sc / N];
+                    const float4 dd = dirs[(sc % N) * (uint32_t)p.epochs + epoch];
+                    if (take && !(__float_as_uint(po.w) & kProbeBuried)) {
+                        o[0] = po.x;
+                        o[1] = po.y;
+                        o[2] = po.z;
+                        d[0] = dd.x;
+                        d[1] = dd.y;
+                        d[2] = dd.z;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            sl.go[k][me] = o[k];
+                            sl.gd[k][me] = d[k];
+                        }
                         ns = kGather;
                     }
                 }
@@ -487,8 +495,8 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     // the sky seen along the gather ray: k_probe_reduce evaluates it
                     face_out[sid] = (uint8_t)VG_FACE_NONE;
                 } else {
-                    float go[3], gd[3];
-                    gather_ray(sid, go, gd);
+                    const float go[3] = {sl.go[0][me], sl.go[1][me], sl.go[2][me]};
+                    const float gd[3] = {sl.gd[0][me], sl.gd[1][me], sl.gd[2][me]};
                     const uint32_t f = h.face;
                     const float ndl = face_ndl(f, p.sun);
                     sl.tX[me] = h.t;
