@@ -123,15 +123,43 @@ __device__ __forceinline__ u32x4_t input_extras(int g, const float d[3], uint32_
 
 // Input k-steps of lane group g for one sample (phi order over the logical inputs): the
 // first holds hash levels 2g, 2g+1, 8+2g, 9+2g as bf16 feature pairs.
+// The 32 gathers of the four levels are issued together and consumed once (an empty asm
+// statement keeps the compiler from waiting between levels); a level beyond the config
+// reads entry 0 with zero weights, which gives exactly the zero feature.
 __device__ __forceinline__ void build_input(const NrcDev &m, int g, const float p[3],
                                             const float d[3], uint32_t face, u32x4_t &x0,
                                             u32x4_t &x1) {
-    float a0, a1, b0, b1, c0, c1, e0, e1;
-    level_feat(m, 2 * g, p, a0, a1);
-    level_feat(m, 2 * g + 1, p, b0, b1);
-    level_feat(m, 8 + 2 * g, p, c0, c1);
-    level_feat(m, 9 + 2 * g, p, e0, e1);
-    x0 = u32x4_t{pack2(a0, a1), pack2(b0, b1), pack2(c0, c1), pack2(e0, e1)};
+    const int ls[4] = {2 * g, 2 * g + 1, 8 + 2 * g, 9 + 2 * g};
+    uint32_t v[4][8];
+    float w[4][8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t idx[8];
+        const bool used = ls[k] < m.levels;
+        level_corners(m, used ? ls[k] : 0, p, idx, w[k]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (!used) w[k][q] = 0.0f;
+            v[k][q] = m.grid_h[used ? idx[q] : 0u];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        asm volatile("" ::"v"(v[k][0]), "v"(v[k][1]), "v"(v[k][2]), "v"(v[k][3]), "v"(v[k][4]),
+                     "v"(v[k][5]), "v"(v[k][6]), "v"(v[k][7]));
+    float f[4][2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[k][0] = 0.0f;
+        f[k][1] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            f[k][0] = __builtin_fmaf(w[k][q], bf_lo(v[k][q]), f[k][0]);
+            f[k][1] = __builtin_fmaf(w[k][q], bf_hi(v[k][q]), f[k][1]);
+        }
+    }
+    x0 = u32x4_t{pack2(f[0][0], f[0][1]), pack2(f[1][0], f[1][1]), pack2(f[2][0], f[2][1]),
+                 pack2(f[3][0], f[3][1])};
     x1 = input_extras(g, d, face);
 }
 
