@@ -427,10 +427,12 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     sched_learned = sched()
     learned_ms = timed(lambda p: g32.update_learned(cache, p, train=True, stream=stream), p32,
                        steps, cache)
+    g32.autotune(p32, cache, train=False)  # the inference-only update has its own schedule
+    sched_frozen = sched()
     frozen_ms = timed(lambda p: g32.update_learned(cache, p, train=False, stream=stream), p32,
                       steps, cache)
-    log(f"gi learned: {learned_ms:.3f} ms/update with training, {frozen_ms:.3f} without "
-        f"({sched_learned})")
+    log(f"gi learned: {learned_ms:.3f} ms/update with training ({sched_learned}), "
+        f"{frozen_ms:.3f} without ({sched_frozen})")
     p32["round"] = 5000
     rays32, nq = g32.update_learned_counts(cache, p32, train=True)
     log(f"gi learned: {learned_ms:.3f} ms/update, rays {rays32}, cache queries {nq}")
@@ -485,6 +487,7 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         "learned_inference_only_ms": round(frozen_ms, 4),
         "speedup_inference_only": round(plain_ms / frozen_ms, 3),
         "schedule_plain": sched_plain, "schedule_learned": sched_learned,
+        "schedule_inference_only": sched_frozen,
         "err_plain256": round(err(rec_plain), 5), "err_learned32": round(err(rec_learned), 5),
         "err_plain32_2bounce": round(err(rec_plain32), 5),
         "err_ambient_tail32": round(err(rec_ambient), 5),
