@@ -99,7 +99,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs the ray-query path on a single "
-              "MI355X (rays are independent; see DESIGN.md for how frames would shard).",
+              "MI355X (rays are independent: frames would shard by screen blocks, probe updates by "
+              "probe slabs).",
               file=sys.stderr)
         sys.exit(2)
 
