@@ -98,8 +98,8 @@ def main():
                          "of known bytes, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build runs the ray-query path on a single "
-              "MI355X (rays are independent: frames would shard by screen blocks, probe updates by "
+        print("bench.py measures one GPU only: the ray-query path runs on a single MI355X "
+              "(rays are independent: frames would shard by screen blocks, probe updates by "
               "probe slabs).",
               file=sys.stderr)
         sys.exit(2)
