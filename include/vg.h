@@ -42,7 +42,7 @@ enum {
     VG_E_INVALID = -1,     /* bad argument (null pointer, bad dims, size mismatch) */
     VG_E_NOMEM = -2,       /* host or device allocation failed */
     VG_E_DEVICE = -3,      /* HIP runtime / launch error */
-    VG_E_UNSUPPORTED = -4, /* valid request this build does not serve */
+    VG_E_UNSUPPORTED = -4, /* valid request the library does not serve */
     VG_E_STATE = -5        /* world not uploaded, or wrong context */
 };
 
