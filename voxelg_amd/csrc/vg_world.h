@@ -85,8 +85,8 @@ inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *b
                     int *chunk_opt = nullptr) {
     static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
     static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
-    static const int kChunk[] = {0, 64, 128, 256};
-    const int n_chunk = chunk_opt ? 4 : 1;
+    static const int kChunk[] = {0, 16, 32, 64, 128, 256};
+    const int n_chunk = chunk_opt ? 6 : 1;
     const int reps = 3;
     hipStream_t stream;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
