@@ -492,6 +492,142 @@ def algorithmic_bytes(it_counts: dict) -> int:
     return int(sum(it_counts[n] * b for n, b in zip(IT_NAMES, IT_BYTES)))
 
 
+# ------------------------------------------------- parity against R0 (checker) --
+def normalize_or_zero_v(d) -> np.ndarray:
+    """glam's normalize_or_zero over rows of fp32 directions: d * (1 / sqrt(x x + y y + z z)),
+    zero where that reciprocal is not finite and positive (src/raycast.rs:24)."""
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        ln = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+        rcp = np.float32(1.0) / ln
+    ok = np.isfinite(rcp) & (rcp > 0)
+    return np.where(ok[:, None], d * rcp[:, None], np.float32(0.0)).astype(np.float32)
+
+
+def ray_box_entry_v(o, d, v) -> np.ndarray:
+    """src/accel.rs:578-586 in fp32, one row per ray: the entry t of voxel v along o + t d."""
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    vmin = np.asarray(v).reshape(-1, 3).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        inv = np.where(np.abs(d) < 1e-8, np.float32(1e30), np.float32(1.0) / d).astype(np.float32)
+        t0 = (vmin - o) * inv
+        t1 = (vmin + np.float32(1.0) - o) * inv
+    return np.maximum(np.max(np.minimum(t0, t1), axis=1), np.float32(0.0)).astype(np.float32)
+
+
+def lattice_touch_v(p, eps=0.02) -> np.ndarray:
+    """src/accel.rs:588-599 per row: how many coordinates of p lie on a voxel-lattice plane."""
+    p = np.asarray(p, np.float64).reshape(-1, 3)
+    return np.sum(np.abs(p - np.round(p)) < eps, axis=1)
+
+
+def classify_vs_r0(eyes, dirs, cand: np.ndarray, ref: np.ndarray, tmax=None) -> dict:
+    """Hits of a candidate traversal against the reference CPU traversal R0 (orc_raycast_r0,
+    src/raycast.rs:21-141) by the rule of the reference's accel_matches_cpu_raycast test
+    (src/accel.rs:655-718), for every ray at once.
+
+    eyes: ray origins (one, or one per ray); dirs: the rays' unit directions (after
+    normalize_or_zero); cand, ref: HIT_DTYPE records. A hit point is eye + t dir, with R0's t
+    the entry t of its voxel (ray_box_entry). It is grazing when two of its coordinates lie
+    within 0.02 of a lattice plane (lattice_touch >= 2): an edge or corner, where the
+    accumulated t of R0 and an exact walk may order two plane crossings differently.
+      agree           both hit the same voxel (clean, or a tie when R0's point grazes)
+      ties            agreeing grazing hits + disagreements where either point grazes
+      clean_disagree  other disagreements (one side hits, the other misses, or two voxels)
+      tmax_ties       disagreements where exactly one side hits within 1e-3 (relative, at
+                      least 1e-3 absolute) of the ray's tmax: R0 compares its accumulated t
+                      against tmax, an exact walk its exact t; counted apart, not clean
+    On clean agreements the face and material must match and |t - t_R0| is measured, t_R0
+    being the t R0 reports."""
+    dirs = np.asarray(dirs, np.float32).reshape(-1, 3)
+    n = len(dirs)
+    eyes = np.broadcast_to(np.asarray(eyes, np.float32).reshape(-1, 3), (n, 3))
+    ch = cand["hit"] != 0
+    rh = ref["hit"] != 0
+    t_ref = ray_box_entry_v(eyes, dirs, ref["voxel"])
+    cpu_p = eyes + t_ref[:, None] * dirs
+    cand_p = eyes + cand["t"].astype(np.float32)[:, None] * dirs
+    cpu_graze = rh & (lattice_touch_v(cpu_p) >= 2)
+    cand_graze = ch & (lattice_touch_v(cand_p) >= 2)
+    same = ch & rh & np.all(cand["voxel"] == ref["voxel"], axis=1)
+    both_miss = ~ch & ~rh
+    agree_clean = same & ~cpu_graze
+    agree_tie = same & cpu_graze
+    dis = ~both_miss & ~same
+    dis_tie = dis & (cpu_graze | cand_graze)
+    dis_clean = dis & ~(cpu_graze | cand_graze)
+    tmax_tie = np.zeros(n, bool)
+    if tmax is not None:
+        tm = np.broadcast_to(np.asarray(tmax, np.float64).reshape(-1), (n,))
+        tol = np.maximum(1e-3, 1e-3 * np.abs(tm))
+        one_side = ch ^ rh
+        t_side = np.where(ch, cand["t"].astype(np.float64), t_ref.astype(np.float64))
+        tmax_tie = dis_clean & one_side & (np.abs(t_side - tm) <= tol)
+        dis_clean = dis_clean & ~tmax_tie
+    # against the t R0 reports: its ray_box_entry along its own normalize_or_zero of the
+    # direction, which may differ from the given unit direction in the last bit
+    dt = np.abs(cand["t"].astype(np.float64) - ref["t"].astype(np.float64))
+    face_bad = agree_clean & (cand["face"] != ref["face"])
+    mat_bad = agree_clean & (cand["material"] != ref["material"])
+    return {
+        "rays": int(n), "both_miss": int(both_miss.sum()),
+        "agree": int(same.sum()), "agree_clean": int(agree_clean.sum()),
+        "ties": int(agree_tie.sum() + dis_tie.sum()), "agree_ties": int(agree_tie.sum()),
+        "disagree_ties": int(dis_tie.sum()), "tmax_ties": int(tmax_tie.sum()),
+        "clean_disagree": int(dis_clean.sum()),
+        "face_mismatch": int(face_bad.sum()), "material_mismatch": int(mat_bad.sum()),
+        "max_dt": float(dt[agree_clean].max()) if agree_clean.any() else 0.0,
+        "max_dt_all_agree": float(dt[same].max()) if same.any() else 0.0,
+        "bad_index": np.flatnonzero(dis_clean | face_bad | mat_bad)[:16].tolist(),
+    }
+
+
+def shadow_origins(eye, dirs, hits: np.ndarray) -> np.ndarray:
+    """orc_shadow_origin per hit record: p_hit + n * 0.001 (shaders/raymarch.wgsl:3955)."""
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = len(dirs)
+    eyes = np.ascontiguousarray(np.broadcast_to(np.asarray(eye, np.float32).reshape(-1, 3),
+                                                (n, 3)))
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    out = np.zeros((n, 3), np.float32)
+    F32P = C.POINTER(C.c_float)
+    L = lib()
+    base_e, base_d, base_o = eyes.ctypes.data, dirs.ctypes.data, out.ctypes.data
+    for i in range(n):
+        L.orc_shadow_origin(C.cast(base_e + 12 * i, F32P), C.cast(base_d + 12 * i, F32P),
+                            hits.ctypes.data + 20 * i, C.cast(base_o + 12 * i, F32P))
+    return out
+
+
+def classify_shadows_vs_r0(ow: "OracleWorld", eye, dirs, prim: np.ndarray, occ, occ_r0,
+                           sun, shadow_tmax, rows_mask) -> dict:
+    """Shadow bytes of a candidate frame against R0's shadow bytes, on the rays of rows_mask
+    whose primary hits agree with R0 on a clean hit with the same t (so both traced the same
+    shadow ray). A disagreeing shadow ray is traced again by R0 and by the exact walk (any-hit)
+    and classified like a primary hit (classify_vs_r0)."""
+    occ = np.asarray(occ).reshape(-1).astype(bool)
+    occ_r0 = np.asarray(occ_r0).reshape(-1).astype(bool)
+    idx = np.flatnonzero(rows_mask & (prim["hit"] != 0))
+    dis = idx[occ[idx] != occ_r0[idx]]
+    out = {"rays": int(len(idx)), "agree": int(len(idx) - len(dis)), "ties": 0,
+           "clean_disagree": 0}
+    if len(dis):
+        so = shadow_origins(eye, np.asarray(dirs, np.float32).reshape(-1, 3)[dis], prim[dis])
+        sd = normalize_or_zero_v(np.asarray(sun, np.float32).reshape(1, 3))[0]
+        a = np.zeros(len(dis), HIT_DTYPE)
+        b = np.zeros(len(dis), HIT_DTYPE)
+        for k in range(len(dis)):
+            a[k], _ = ow.trace_r1(so[k], sd, tmax=shadow_tmax, flags=ANY_HIT)
+            b[k] = ow.raycast_r0(so[k], sd, tmax=shadow_tmax)
+        c = classify_vs_r0(so, np.broadcast_to(sd, so.shape), a, b, tmax=shadow_tmax)
+        # a re-traced pair that agrees (or misses on both sides) leaves the frame's
+        # disagreement unexplained: only traversal ties explain one
+        out["ties"] = c["disagree_ties"] + c["tmax_ties"]
+        out["clean_disagree"] = int(len(dis)) - out["ties"]
+    return out
+
+
 # noise / generator helpers
 def hash2(x, z):
     return lib().orc_hash2(int(x), int(z))

@@ -1038,6 +1038,7 @@ typedef struct job_t {
     const float *cam;
     int32_t width, height, row0, row1;
     float ptmax, stmax, sun[3];
+    float sun_in[3]; /* R0 frame: the sun direction as given (raycast normalizes it) */
     uint8_t *shadow;
     uint32_t *iters; /* optional: per pixel (primary, shadow) iteration counts */
 } job_t;
@@ -1108,15 +1109,23 @@ void orc_raycast_r0_batch(const orc_world *w, const orc_ray *rays, size_t n, orc
 
 /* Camera floats (src/camera.rs:61-113): origin 0..2, forward 4..6, right 8..10, up 12..14,
  * tan_half_fov 15, resolution 16..17, world_origin (i32) 20..22, jitter 24..25. */
-void orc_primary_dir(const float *cam, int32_t px, int32_t py, float out[3]) {
-    /* uv = (pixel + 0.5 + jitter) / resolution (raymarch.wgsl:600); ray_dir_uv (:430-438) */
+/* The pinhole vector before normalization: forward + right * ndc.x * tan(fov/2) * aspect +
+ * up * ndc.y * tan(fov/2), ndc = (2u - 1, 1 - 2v), uv = (pixel + 0.5 + jitter) / resolution
+ * (raymarch.wgsl:600; ray_dir_uv :430-438). */
+void orc_primary_vec(const float *cam, int32_t px, int32_t py, float r[3]) {
     float u = ((float)px + 0.5f + cam[24]) / cam[16];
     float v = ((float)py + 0.5f + cam[25]) / cam[17];
     float nx = u * 2.0f - 1.0f, ny = 1.0f - v * 2.0f;
     float aspect = cam[16] / cam[17];
     float a = nx * cam[15] * aspect, b = ny * cam[15];
-    float r[3];
     for (int k = 0; k < 3; ++k) r[k] = cam[4 + k] + cam[8 + k] * a + cam[12 + k] * b;
+}
+
+/* The primary ray direction: the pinhole vector normalized as glam's normalize_or_zero does
+ * (v * (1 / sqrt(v . v))), so raycast() given the pinhole vector traces this very direction. */
+void orc_primary_dir(const float *cam, int32_t px, int32_t py, float out[3]) {
+    float r[3];
+    orc_primary_vec(cam, px, py, r);
     float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     float rcp = 1.0f / len;
     out[0] = r[0] * rcp;
@@ -1232,8 +1241,15 @@ static void frame_rows(job_t *j) {
                 uint64_t *tc = ((x & 7) == 0 && (y & 7) == 0) ? &c[ORC_IT_BEAM_TEST] : NULL;
                 tb = orc_beam_t(j->w, cam, x >> 3, y >> 3, j->ptmax, tc);
             }
-            if (j->kind == 2) trace_core_from(j->w, cam, dir, j->ptmax, j->flags, tb, h, c);
-            else orc_raycast_r0(j->w, cam, dir, j->ptmax, h);
+            if (j->kind == 2) {
+                trace_core_from(j->w, cam, dir, j->ptmax, j->flags, tb, h, c);
+            } else {
+                /* raycast() normalizes its direction itself: given the pinhole vector it
+                 * traces exactly the primary direction above */
+                float pv[3];
+                orc_primary_vec(cam, x, y, pv);
+                orc_raycast_r0(j->w, cam, pv, j->ptmax, h);
+            }
             uint32_t it_primary = 0;
             for (int k = 0; k < ORC_IT_COUNT; ++k) it_primary += (uint32_t)c[k];
             uint8_t occ = 0;
@@ -1244,7 +1260,7 @@ static void frame_rows(job_t *j) {
                 if (j->kind == 2)
                     trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT | (j->flags & ORC_NO_GROUPS),
                                &sh, c);
-                else orc_raycast_r0(j->w, so, j->sun, j->stmax, &sh);
+                else orc_raycast_r0(j->w, so, j->sun_in, j->stmax, &sh); /* normalizes it */
                 occ = sh.hit;
             }
             j->shadow[i] = occ;
@@ -1298,6 +1314,7 @@ void orc_frame_r0(const orc_world *w, const float *cam, int32_t width, int32_t h
     p.row1 = row1;
     p.ptmax = primary_tmax;
     p.stmax = shadow_tmax;
+    for (int k = 0; k < 3; ++k) p.sun_in[k] = sun[k];
     if (!normalize_or_zero(sun, p.sun)) p.sun[0] = p.sun[1] = p.sun[2] = 0.0f;
     p.out = hits;
     p.shadow = shadow;

@@ -160,6 +160,7 @@ void orc_raycast_r0_batch(const orc_world *w, const orc_ray *rays, size_t n, orc
                           int threads);
 
 /* primary rays + frame (CameraUniform-compatible 176-byte camera, see src/camera.rs:61-113) */
+void orc_primary_vec(const float *camera176, int32_t px, int32_t py, float out[3]);
 void orc_primary_dir(const float *camera176, int32_t px, int32_t py, float out[3]);
 /* The water secondary rays of a frame's primary hits (20-byte records): two rays per pixel
  * (reflection, refraction; tmax 200, VG_RAY_SKIP_TRANSPARENT), zero directions where the
@@ -173,7 +174,9 @@ void orc_frame(const orc_world *w, const float *camera176, int32_t width, int32_
                float primary_tmax, uint32_t primary_flags, const float sun[3], float shadow_tmax,
                int32_t row0, int32_t row1, orc_hit *hits, uint8_t *shadow, int threads,
                uint64_t counts[ORC_IT_COUNT], uint32_t *iters /* may be NULL */);
-/* Same frame with R0 as the traversal (the CPU baseline). */
+/* Same frame with R0 as the traversal (the CPU baseline): raycast() is given the pinhole
+ * vector and the sun direction as they are and normalizes each itself, so it traces the
+ * very rays of orc_frame (and of the GPU frame). */
 void orc_frame_r0(const orc_world *w, const float *camera176, int32_t width, int32_t height,
                   float primary_tmax, const float sun[3], float shadow_tmax, int32_t row0,
                   int32_t row1, orc_hit *hits, uint8_t *shadow, int threads);

@@ -152,3 +152,23 @@ def roof_samples():
             under.append(24 + m < hx < 44 - m and 24 + m < hz < 44 - m)
             open_.append(hx < 24 - m or hx > 44 + m or hz < 24 - m or hz > 44 + m)
     return np.array(origins, np.float32), sun, np.array(under), np.array(open_)
+
+
+def assert_north_star(c, what):
+    """A classify_vs_r0 result meets the north-star criterion: hits match R0 on voxel, face
+    and material, t within 1e-4 voxels, and the only disagreements are edge ties
+    (lattice_touch >= 2) or tmax-boundary ties, rare ones."""
+    assert c["clean_disagree"] == 0, (what, c)
+    assert c["face_mismatch"] == 0 and c["material_mismatch"] == 0, (what, c)
+    assert c["max_dt"] <= 1e-4, (what, c)
+    assert c["disagree_ties"] + c["tmax_ties"] <= 1e-4 * c["rays"] + 16, (what, c)
+    print(what, {k: v for k, v in c.items() if k != "bad_index"}, flush=True)
+
+
+def same_shadow_ray(cand, ref, eye, dirs, sel):
+    """Rays of sel whose primary hits agree with R0's on voxel, face and entry t: both sides
+    then trace the same shadow ray (an edge tie entered through another face would move the
+    shadow origin)."""
+    return sel & (cand["hit"] != 0) & (ref["hit"] != 0) & \
+        np.all(cand["voxel"] == ref["voxel"], axis=1) & (cand["face"] == ref["face"]) & \
+        (cand["t"] == ref["t"])

@@ -65,21 +65,32 @@ def test_edits_match_oracle_world(ctx, origin):
     assert not np.array_equal(ref.bricks, world.bricks)
 
 
-def test_dirty_retrace_is_bit_exact(ctx):
+@pytest.mark.parametrize("flags, pos, yaw, tmax", [
+    (0, (128.5, 90.0, 128.5), 0.7, 512.0),
+    # the beam prologue over the block list (beam t indexed by block id)
+    (vg.VG_RAY_LOD | vg.VG_RAY_BEAM, (128.5, 90.0, 128.5), 0.7, 700.0),
+    # brick / tile LOD past t 400 / 520: a camera outside the 256-voxel window looking in
+    (vg.VG_RAY_LOD, (-150.5, 110.0, -140.5), 0.393, 700.0)])
+def test_dirty_retrace_is_bit_exact(ctx, flags, pos, yaw, tmax):
     wl = workloads.DYNAMIC
     world = wl.world()
     ref = oracle.OracleWorld.copy_of(world)
     dw = ctx.world(wl.dims).upload(world)
     W, H = 640, 360
-    cam = vg.Camera(pos=(128.5, 90.0, 128.5), yaw=0.7, pitch=-0.35).uniform(W, H)
+    cam = vg.Camera(pos=pos, yaw=yaw, pitch=-0.35).uniform(W, H)
     sun = vg.sun_direction(math.radians(35.0), 2.1)
-    hits, occ = dw.trace_frame(cam, W, H, 512.0, 0, sun, 480.0)
+    hits, occ = dw.trace_frame(cam, W, H, tmax, flags, sun, 480.0)
+    if flags & vg.VG_RAY_LOD and pos[0] < 0:
+        _, _, c0 = ref.frame(cam, W, H, tmax, flags, sun, 480.0)
+        assert c0["brick_lod"] > 0 or c0["tile_lod"] > 0, c0
     nbx, nby = (W + 7) // 8, (H + 7) // 8
     rng = np.random.default_rng(5)
     for f, n in enumerate((1, 16, 4096)):
         if n == 1:  # a small carve at a voxel the frame sees: a partial mask
-            h0 = decode_hits(hits).reshape(H, W)[H // 2, (3 * W) // 4]
-            assert h0["hit"] == 1
+            hh = decode_hits(hits).reshape(H, W)
+            ys, xs = np.nonzero(hh["hit"] == 1)
+            k = np.argmin(np.abs(ys - H // 2) + np.abs(xs - (3 * W) // 4))
+            h0 = hh[ys[k], xs[k]]
             edits = np.zeros(1, EDIT_DTYPE)
             edits[0] = (tuple(int(v) for v in h0["voxel"]), 2, 0, (0, 0))
         else:
@@ -89,10 +100,10 @@ def test_dirty_retrace_is_bit_exact(ctx):
         dw.apply_edits(edits)
         ref.apply_spheres(edits)
         mask = dw.dirty_blocks(cam, W, H, sun, 480.0)
-        dw.trace_frame_blocks(cam, W, H, 512.0, 0, sun, 480.0, mask, hits, occ)
+        dw.trace_frame_blocks(cam, W, H, tmax, flags, sun, 480.0, mask, hits, occ)
         ctx.synchronize()
         got = decode_hits(hits)
-        exp, exp_occ, _ = ref.frame(cam, W, H, 512.0, 0, sun, 480.0)
+        exp, exp_occ, _ = ref.frame(cam, W, H, tmax, flags, sun, 480.0)
         assert got.tobytes() == exp.tobytes(), f"frame {f}: hits differ"
         assert np.array_equal(occ.numpy(np.uint8), exp_occ), f"frame {f}: shadows differ"
         m = mask.numpy(np.uint32)

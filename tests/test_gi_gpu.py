@@ -219,3 +219,7 @@ def test_frame_indirect_matches_oracle(ctx, small, small_world):
     hits, _ = small.trace_frame(cam, W, H, 512.0, 0, SUN, 480.0)
     with pytest.raises(VGError):
         pg.frame_indirect(q, cam, W, H, hits)
+    # the pixel rays come from the camera: a resolution other than the frame's is refused
+    bad = vg.Camera(pos=(128.5, 90.0, 128.5), yaw=0.7, pitch=-0.35).uniform(W // 2, H)
+    with pytest.raises(VGError):
+        pg.frame_indirect(p, bad, W, H, hits)

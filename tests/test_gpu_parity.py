@@ -299,7 +299,49 @@ def test_edge_cases_and_errors(ctx, small, small_world):
         small.trace_primary(cam, 640, 360, 512.0)
 
 
-def test_large_frame_sampled_rows_bit_exact(ctx):
+def test_small_frame_vs_r0_full(ctx, small, small_world):
+    # the north-star criterion on the 1280x720 frame of the 256x128x256 world (primary tmax
+    # 512, LOD off) plus its sun shadow rays: the GPU's hits against the reference CPU
+    # traversal R0 (src/raycast.rs:21-141) traced over the same rays, every pixel
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    print("case: small frame vs R0", flush=True)
+    hits, occ = small.trace_frame(cam, wl.width, wl.height, wl.primary_tmax, 0, sun, 480.0)
+    ctx.synchronize()
+    got, occ = gpu_hits(hits), occ.numpy(np.uint8)
+    ow = oracle.OracleWorld.view(small_world)
+    r0, s0 = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, 480.0, threads=THREADS)
+    dirs = oracle.primary_dirs(cam, wl.width, wl.height)
+    c = oracle.classify_vs_r0(cam[:3], dirs, got, r0, tmax=wl.primary_tmax)
+    scenes.assert_north_star(c, "small frame GPU vs R0")
+    assert c["agree_clean"] > 0.85 * r0["hit"].sum()
+    alln = np.ones(len(got), bool)
+    cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, got, occ, s0, sun, 480.0,
+                                       scenes.same_shadow_ray(got, r0, cam[:3], dirs, alln))
+    print("small frame shadows GPU vs R0", cs, flush=True)
+    assert cs["clean_disagree"] == 0 and cs["rays"] > 0.8 * c["agree"], cs
+
+
+def test_uneven_batches_vs_r0_full(ctx, small, small_world):
+    # one whole step of the 96 uneven batches against R0 (which normalizes each direction
+    # itself, as vg_trace_batches does)
+    rays, hbuf, bs = batched_setup(ctx, small, small_world)
+    print(f"case: {bs.n} uneven batches vs R0", flush=True)
+    small.trace_batches(bs)
+    ctx.synchronize()
+    got = gpu_hits(hbuf)
+    r0 = oracle.OracleWorld.view(small_world).raycast_rays(rays, THREADS)
+    c = oracle.classify_vs_r0(rays["o"], oracle.normalize_or_zero_v(rays["d"]), got, r0,
+                              tmax=rays["tmax"])
+    scenes.assert_north_star(c, "uneven batches GPU vs R0")
+    assert c["agree_clean"] > 0.8 * r0["hit"].sum()
+
+
+def test_large_frame_bit_exact_and_vs_r0(ctx):
+    # the headline frame (2048x256x2048 world, 1920x1080, LOD 400 / 520, beam, sun shadow per
+    # hit): every pixel equals the oracle's R1 bit for bit, and against R0 the hits before
+    # t 400 (where LOD cannot have fired) and their shadow rays meet the north-star criterion
     wl = workloads.LARGE
     print("case: large world generate", flush=True)
     world = wl.world()
@@ -310,24 +352,35 @@ def test_large_frame_sampled_rows_bit_exact(ctx):
     hits, occ = dw.trace_frame(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags, sun,
                                wl.shadow_tmax)
     ctx.synchronize()
-    got = gpu_hits(hits).reshape(wl.height, wl.width)
-    occ = occ.numpy(np.uint8).reshape(wl.height, wl.width)
+    flat = gpu_hits(hits)
+    occ = occ.numpy(np.uint8)
     ow = oracle.OracleWorld.view(world)
-    for r0 in range(0, wl.height, wl.height // 12):
-        ref, ref_occ, _ = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags,
-                                   sun, wl.shadow_tmax, rows=(r0, r0 + 2), threads=THREADS)
-        assert_same(got[r0:r0 + 2].reshape(-1), ref, f"large rows {r0}")
-        assert np.array_equal(occ[r0:r0 + 2].reshape(-1), ref_occ)
+    print("case: large frame oracle R1", flush=True)
+    ref, ref_occ, _ = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags,
+                               sun, wl.shadow_tmax, threads=THREADS)
+    assert_same(flat, ref, "large frame")
+    assert np.array_equal(occ, ref_occ)
     # the beam prologue is a pure acceleration: the whole frame without it is identical
     assert wl.primary_flags & vg.VG_RAY_BEAM
     h2, o2 = dw.trace_frame(cam, wl.width, wl.height, wl.primary_tmax,
                             wl.primary_flags & ~vg.VG_RAY_BEAM, sun, wl.shadow_tmax)
     ctx.synchronize()
-    assert gpu_hits(h2).tobytes() == got.tobytes()
-    assert np.array_equal(o2.numpy(np.uint8).reshape(wl.height, wl.width), occ)
-    flat = got.reshape(-1)
+    assert gpu_hits(h2).tobytes() == flat.tobytes()
+    assert np.array_equal(o2.numpy(np.uint8), occ)
     h = flat["hit"] == 1
     assert h.mean() > 0.3
     assert np.all(flat["t"][h] <= wl.primary_tmax) and np.all(flat["t"][h] >= 0)
-    assert np.all(occ.reshape(-1)[~h] == 0)
+    assert np.all(occ[~h] == 0)
     dw.close()
+    print("case: large frame R0", flush=True)
+    r0, s0 = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, wl.shadow_tmax,
+                         threads=THREADS)
+    dirs = oracle.primary_dirs(cam, wl.width, wl.height)
+    sel = (flat["hit"] == 0) | (flat["t"] < 400.0)
+    c = oracle.classify_vs_r0(cam[:3], dirs[sel], flat[sel], r0[sel], tmax=wl.primary_tmax)
+    scenes.assert_north_star(c, "large frame GPU vs R0 (t < 400)")
+    assert c["agree_clean"] > 0.5 * sel.sum()
+    cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, flat, occ, s0, sun, wl.shadow_tmax,
+                                       scenes.same_shadow_ray(flat, r0, cam[:3], dirs, sel))
+    print("large frame shadows GPU vs R0", cs, flush=True)
+    assert cs["clean_disagree"] == 0 and cs["rays"] > 0.8 * c["agree"], cs

@@ -127,6 +127,30 @@ def test_train_step_is_adam_on_the_gradient(cache12):
     assert cache12.stats()[0] == 2
 
 
+def test_non_finite_batch_changes_nothing(cache12):
+    # one NaN (or infinite) target makes the backward pass non-finite: the step then changes
+    # neither the grid nor the MLP (its gradient maximum, the grid's fixed-point scale, is
+    # +inf), and the next finite batch trains as usual
+    rng = np.random.default_rng(14)
+    mlp, grid = rand_params(rng, cache12)
+    q = rand_queries(rng, 1024, hi=120.0)
+    t = rng.uniform(0.0, 1.0, (1024, 4)).astype(np.float32)
+    t[:, 3] = 1.0
+    for bad in (np.nan, np.inf):
+        cache12.set_params(mlp, grid)
+        tb = t.copy()
+        tb[517, 1] = bad
+        case(f"nrc train step with a {bad} target")
+        cache12.train_step(q, tb)
+        m1, g1 = cache12.params()
+        np.testing.assert_array_equal(m1, mlp)
+        np.testing.assert_array_equal(g1, grid)
+        assert not np.isfinite(cache12.stats()[1])
+    cache12.train_step(q, t)
+    m2, g2 = cache12.params()
+    assert np.isfinite(m2).all() and np.isfinite(g2).all() and not np.array_equal(g2, grid)
+
+
 def probe_setup(small, grid=(8, 4, 8), rays=32):
     p = gi.gi_params(SUN, grid=grid, spacing=32, rays_per_probe=rays, epochs=1, update_div=1,
                      bounces=2, flags=1)

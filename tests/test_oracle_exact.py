@@ -118,6 +118,60 @@ def test_r1_matches_r0_on_clean_primary_hits(small_world):
     assert clean > 0.9 * r1["hit"].sum()
     agree = (r1["hit"] == r0["hit"]) & np.all(r1["voxel"] == r0["voxel"], axis=1)
     assert agree.mean() > 0.99
+    # the vectorised classifier agrees with the per-ray one it replaces at scale
+    c = oracle.classify_vs_r0(cam[:3], dirs, r1, r0, tmax=wl.primary_tmax)
+    assert c["agree_clean"] == clean and c["clean_disagree"] == 0
+
+
+def test_r1_vs_r0_small_frame_full(small_world):
+    # the 1280x720 frame of the 256x128x256 world, every pixel
+    wl = workloads.SMALL
+    _, cam = wl.camera()
+    ow = oracle.OracleWorld.view(small_world)
+    r1, _, _ = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, 0, None, 0.0)
+    r0, _ = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, None, 0.0)
+    dirs = oracle.primary_dirs(cam, wl.width, wl.height)
+    c = oracle.classify_vs_r0(cam[:3], dirs, r1, r0, tmax=wl.primary_tmax)
+    scenes.assert_north_star(c, "small frame R1 vs R0")
+    assert c["agree_clean"] > 0.85 * r0["hit"].sum()
+
+
+def test_r1_vs_r0_batched_step_full(small_world):
+    # one whole step of the 96 uneven batches (primary, shadow any-hit and AO rays, tmax in
+    # {32, 128, 512, 1024}); R0 normalizes every direction itself, as the GPU entry does
+    rays, _ = workloads.BATCHED.rays(small_world)
+    ow = oracle.OracleWorld.view(small_world)
+    r1, _ = ow.trace_rays(rays)
+    r0 = ow.raycast_rays(rays)
+    d = oracle.normalize_or_zero_v(rays["d"])
+    c = oracle.classify_vs_r0(rays["o"], d, r1, r0, tmax=rays["tmax"])
+    scenes.assert_north_star(c, "batched step R1 vs R0")
+    assert c["agree_clean"] > 0.8 * r0["hit"].sum()
+
+
+def test_r1_vs_r0_large_frame_rows():
+    # rows of the 1920x1080 frame of the 2048x256x2048 world with brick / tile LOD past
+    # 400 / 520: compared where the LOD frame equals the exact walk (no hit at t >= 400),
+    # plus the shadow rays of the agreeing hits
+    wl = workloads.LARGE
+    world = wl.world()
+    _, cam = wl.camera()
+    sun = wl.sun()
+    ow = oracle.OracleWorld.view(world)
+    rows = (300, 340)  # past the horizon's far LOD hits and down to near ground
+    r1, s1, _ = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, wl.primary_flags, sun,
+                         wl.shadow_tmax, rows)
+    r0, s0 = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, wl.shadow_tmax, rows)
+    dirs = oracle.primary_dirs(cam, wl.width, wl.height)[rows[0] * wl.width:rows[1] * wl.width]
+    sel = (r1["hit"] == 0) | (r1["t"] < 400.0)
+    c = oracle.classify_vs_r0(cam[:3], dirs[sel], r1[sel], r0[sel], tmax=wl.primary_tmax)
+    scenes.assert_north_star(c, "large frame rows R1 vs R0")
+    assert c["agree_clean"] > 1000 and (~sel).sum() > 1000
+    same_ray = scenes.same_shadow_ray(r1, r0, cam[:3], dirs, sel)
+    cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, r1, s1, s0, sun, wl.shadow_tmax,
+                                       same_ray)
+    print("large frame rows shadows", cs)
+    assert cs["clean_disagree"] == 0 and cs["rays"] > 1000, cs
 
 
 def test_phase1_leaves_solid_start():

@@ -1320,6 +1320,10 @@ int vg_frame_indirect(vg_probes *pr, const vg_gi_params *p, const vg_camera *cam
     if (p->spacing <= 0) return fail(VG_E_INVALID, "spacing %d", p->spacing);
     if (width <= 0 || height <= 0) return fail(VG_E_INVALID, "size %dx%d", width, height);
     if ((int64_t)width * height > (1ll << 31)) return fail(VG_E_INVALID, "frame too large");
+    // the pixel rays are rebuilt from the camera: its resolution is the frame's
+    if (camera->resolution[0] != (float)width || camera->resolution[1] != (float)height)
+        return fail(VG_E_INVALID, "camera resolution %gx%g does not match %dx%d",
+                    camera->resolution[0], camera->resolution[1], width, height);
     if (!hits || !out4) return fail(VG_E_INVALID, "null hits/out");
     if (((uintptr_t)out4 & 15u) != 0) return fail(VG_E_INVALID, "out4 must be 16-byte aligned");
     if (((uintptr_t)hits & 3u) != 0) return fail(VG_E_INVALID, "hits must be 4-byte aligned");

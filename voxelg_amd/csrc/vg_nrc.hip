@@ -54,6 +54,11 @@ __device__ __forceinline__ uint32_t pack2(float a, float b) {
 __device__ __forceinline__ float bf_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
 __device__ __forceinline__ float relu(float v) { return v > 0.0f ? v : 0.0f; }
+// running max of |x|, +inf once any x is not finite
+__device__ __forceinline__ float gmax_acc(float g, float x) {
+    const float a = fabsf(x);
+    return a <= 3.402823466e38f ? fmaxf(g, a) : INFINITY;
+}
 
 __device__ __forceinline__ f32x4_t mfma(u32x4_t a, u32x4_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
@@ -581,6 +586,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query 
                     const float r = acc[0][sb][i] - tr[sb][i];
                     e = tw[sb] * r;
                     loss += e * r;
+                    gmx = fabsf(e) <= 3.402823466e38f ? gmx : INFINITY;
                 }
                 dy[i] = e * inv_n;
             }
@@ -615,10 +621,12 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDev m, const vg_nrc_query 
             dst[g] = make_float4(ga[0][sb][0], ga[0][sb][1], ga[0][sb][2], ga[0][sb][3]);
             dst[4 + g] = make_float4(ga[1][sb][0], ga[1][sb][1], ga[1][sb][2], ga[1][sb][3]);
             // inputs 4g + k (levels 2g, 2g + 1) and 16 + 4g + k (levels 8 + 2g, 9 + 2g)
+            // a non-finite gradient makes the maximum +inf (fmaxf alone would drop a NaN),
+            // and a step whose maximum is not finite changes no parameter
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                if (2 * g + (k >> 1) < m.levels) gmx = fmaxf(gmx, fabsf(ga[0][sb][k]));
-                if (8 + 2 * g + (k >> 1) < m.levels) gmx = fmaxf(gmx, fabsf(ga[1][sb][k]));
+                if (2 * g + (k >> 1) < m.levels) gmx = gmax_acc(gmx, ga[0][sb][k]);
+                if (8 + 2 * g + (k >> 1) < m.levels) gmx = gmax_acc(gmx, ga[1][sb][k]);
             }
         }
         first = false;
@@ -722,9 +730,11 @@ __global__ __launch_bounds__(256) void k_nrc_adam_pack(float *__restrict__ p, fl
                                                        uint16_t *__restrict__ ffwd,
                                                        uint16_t *__restrict__ fbwd, float lr,
                                                        float b1, float b2, float eps,
-                                                       const uint32_t *step) {
+                                                       const uint32_t *step, const uint32_t *gmax) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= VG_NRC_MLP_PARAMS) return;
+    // a non-finite backward pass (the step's gradient maximum is +inf) changes nothing
+    if (!(__uint_as_float(*gmax) <= 3.402823466e38f)) return;
     const float t = (float)*step;
     const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
     float pp = p[i], m = mm[i], v = vv[i];
@@ -839,7 +849,7 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
                            stream, m->mlp, m->mlp_m, m->mlp_v, m->mlp_g,
                            reinterpret_cast<uint16_t *>(m->frag_fwd),
                            reinterpret_cast<uint16_t *>(m->frag_bwd), c.lr_mlp, c.beta1, c.beta2,
-                           c.eps, m->step);
+                           c.eps, m->step, m->gmax);
         HIP_TRY(hipGetLastError());
     }
     return VG_OK;
