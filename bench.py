@@ -22,38 +22,106 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+# value: the frame's Mrays/s; the "gi" object holds the probe-update half (ms per update)
+METRIC = "Mrays/s at 1080p primary+shadow; GI probe ms at \u22642% irradiance err vs plain"
+GI_ERR_LIMIT = 0.02
+
+
+def gi_threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(world, wl, cam, sun, budget_s=12.0):
-    """The oracle's R0 (src/raycast.rs restated, per-voxel DDA) on a bounded sample of the
-    same frame: evenly spaced image rows, every host thread, primary + shadow rays."""
+def cpu_baseline(world, wl, cam, sun, gpu_hits, gpu_occ, row_budget_s=2.0):
+    """The reference's CPU traversal on this host, after benches/trace.rs:24-41 (warm-up, then
+    repeated timed runs): R0 (src/raycast.rs:21-141 restated, the per-voxel DDA) and R1 (the
+    exact hierarchical walk the GPU reproduces), each from a -O3 -march=native build of the
+    oracle made on this host, on every host thread this process may use (at most 16, one
+    GPU's share), rows interleaved over the threads. Timed on every k-th row of the frame (k
+    sized so one R0 run takes about row_budget_s): one warm-up, then the median of 5 runs.
+    Before the timing, R0 and R1 trace the whole frame once: the GPU frame is compared with
+    both (the parity report)."""
     import numpy as np
 
     import oracle
+    L, build = oracle.timing_lib()
+    cpu = oracle.host_cpu()
+    threads = max(1, min(16, cpu["usable_cpus"]))
     ow = oracle.OracleWorld.view(world)
-    threads = min(os.cpu_count() or 1, 256)
-    # calibrate on 2 rows, then size the sample to about budget_s of CPU time
+    W, H = wl.width, wl.height
+    args = (cam, W, H, wl.primary_tmax)
     t0 = time.perf_counter()
-    h, s = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, wl.shadow_tmax,
-                       rows=(wl.height // 2, wl.height // 2 + 2), threads=threads)
-    dt = time.perf_counter() - t0
-    n_rows = int(max(4, min(wl.height, 2 * budget_s / max(dt, 1e-3))))
-    step = max(1, wl.height // n_rows)
-    rows = list(range(step // 2, wl.height, step))[:n_rows]
-    rays = 0
+    r0, s0 = ow.frame_r0(*args, sun, wl.shadow_tmax, threads=threads, L=L)
+    full_r0_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    for r in rows:
-        h, s = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, wl.shadow_tmax,
-                           rows=(r, r + 1), threads=threads)
-        rays += wl.width + int(h["hit"].sum())
-    dt = time.perf_counter() - t0
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(rows)} of {wl.height} rows (every {step}th), {rays} primary+shadow "
-                      f"rays, R0 = per-voxel DDA of src/raycast.rs, {dt:.1f} s"}
+    r1, s1, _ = ow.frame(*args, wl.primary_flags, sun, wl.shadow_tmax, threads=threads, L=L)
+    full_r1_s = time.perf_counter() - t0
+    parity = parity_report(wl, cam, sun, ow, gpu_hits, gpu_occ, r0, s0, r1, s1)
+
+    step = max(1, int(np.ceil(full_r0_s / row_budget_s)))
+    rows = (step // 2, H, step)
+    n_rows = len(range(*rows))
+
+    def timed(fn):
+        fn()  # warm-up
+        ts, rays = [], 0
+        for _ in range(5):
+            t0 = time.perf_counter()
+            h = fn()
+            ts.append(time.perf_counter() - t0)
+            rays = n_rows * W + int(h["hit"].sum())
+        return rays, float(np.median(ts)), ts
+
+    n0, med0, ts0 = timed(lambda: ow.frame_r0(*args, sun, wl.shadow_tmax, rows, threads,
+                                             L=L)[0])
+    n1, med1, ts1 = timed(lambda: ow.frame(*args, wl.primary_flags, sun, wl.shadow_tmax, rows,
+                                          threads, L=L)[0])
+    return {"value": n0 / med0 / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{n_rows} of {H} rows (every {step}th), {n0} primary+shadow rays per run; "
+                      f"1 warm-up + median of 5 runs ({', '.join(f'{t:.2f}' for t in ts0)} s); "
+                      f"R0 = src/raycast.rs per-voxel DDA, one raycast() per ray",
+            "r1_value": n1 / med1 / 1e6,
+            "r1_sample": f"same rows, median of 5 runs ({', '.join(f'{t:.3f}' for t in ts1)} s); "
+                         f"R1 = the exact hierarchical walk (LOD 400/520, as the GPU frame)",
+            "build": build, "threads": threads, "cpu_model": cpu["model"],
+            "physical_cores": cpu["physical_cores"], "logical_cpus": cpu["logical_cpus"],
+            "usable_cpus": cpu["usable_cpus"],
+            "full_frame_s": {"r0": round(full_r0_s, 2), "r1": round(full_r1_s, 2)}}, parity
+
+
+def parity_report(wl, cam, sun, ow, got, occ, r0, s0, r1, s1):
+    """The GPU frame against the CPU traversals over every pixel: bit for bit against R1 (the
+    exact walk with the same LOD), and against R0 (src/raycast.rs) by the rule of the
+    reference's accel_matches_cpu_raycast test (src/accel.rs:655-718) where the LOD frame
+    equals the exact walk (rays without a hit at t >= 400), plus the shadow rays of the hits
+    that agree on voxel, face and t."""
+    import numpy as np
+
+    import oracle
+    dirs = oracle.primary_dirs(cam, wl.width, wl.height)
+    occ = np.asarray(occ).reshape(-1)
+    r1_same = got.tobytes() == r1.tobytes() and np.array_equal(occ, s1)
+    sel = (got["hit"] == 0) | (got["t"] < 400.0)
+    c = oracle.classify_vs_r0(cam[:3], dirs[sel], got[sel], r0[sel], tmax=wl.primary_tmax)
+    same = sel & (got["hit"] != 0) & (r0["hit"] != 0) & \
+        np.all(got["voxel"] == r0["voxel"], axis=1) & (got["face"] == r0["face"]) & \
+        (got["t"] == r0["t"])
+    cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, got, occ, s0, sun, wl.shadow_tmax,
+                                       same)
+    return {"vs_r1_bit_exact": bool(r1_same), "rays": int(len(got)),
+            "vs_r0": {"compared": c["rays"], "outside_lod_region": int((~sel).sum()),
+                      "agree": c["agree"], "agree_clean": c["agree_clean"],
+                      "ties": c["ties"], "edge_tie_disagreements": c["disagree_ties"],
+                      "tmax_ties": c["tmax_ties"], "clean_disagree": c["clean_disagree"],
+                      "face_mismatch": c["face_mismatch"],
+                      "material_mismatch": c["material_mismatch"], "max_dt": c["max_dt"],
+                      "shadow": cs}}
 
 
 TRAFFIC_PROFILE = "profiles/k_frame_persistent_large_1080p_pmc.md"
@@ -93,6 +161,8 @@ def main():
                     help="dynamic: bit-exact oracle check of the frame every N frames")
     ap.add_argument("--gi-train-updates", type=int, default=150,
                     help="learned GI: online-training updates before the timed ones")
+    ap.add_argument("--gi-steps", type=int, default=10, help="GI: timed probe updates per mode")
+    ap.add_argument("--no-gi", action="store_true", help="frame: skip the GI probe half")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (a plain copy kernel "
                          "of known bytes, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -233,13 +303,21 @@ def main():
     gw.close()
     log(f"device world generation: {gen_out}")
 
-    base = None
+    # the metric's second half: the GI probe update on the same world, learned 32-ray (cache
+    # trained online) against plain 256-ray probes, errors against the FP64 reference
+    gi_out = None
+    if not args.no_gi:
+        gi_out = gi_probe_bench(ctx, dw, world, sun, args.gi_steps, args.gi_train_updates,
+                                gi_threads())
+
+    base = parity = None
     if not args.no_cpu_baseline:
-        base = cpu_baseline(world, wl, cam, sun)
+        base, parity = cpu_baseline(world, wl, cam, sun, hh, occ.numpy(np.uint8))
         log(f"cpu baseline: {base}")
+        log(f"parity: {parity}")
 
     out = {
-        "metric": "Mrays/s at 1080p primary+shadow",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -269,6 +347,8 @@ def main():
             "algorithmic_bytes": int(bytes_per_frame),
         },
         "cpu_baseline": base,
+        "parity": parity,
+        "gi": gi_out,
         "water_secondary": water_out,
         "device_world_generation": gen_out,
     }
@@ -330,21 +410,37 @@ def bench_batched(args):
         f"{value:.0f} Mrays/s, {n_hit} hits")
     base = None
     if not args.no_cpu_baseline:
+        # R0 (one raycast() per ray) and R1 over whole steps, the host-native -O3 build, every
+        # usable host thread (at most 16): one warm-up, then the median of 5 steps
         import oracle
+        L, build = oracle.timing_lib()
+        cpu = oracle.host_cpu()
+        threads = gi_threads()
         ow = oracle.OracleWorld.view(world)
-        threads = min(os.cpu_count() or 1, 256)
-        # whole steps, repeated until about 10 s of CPU work
-        t0 = time.perf_counter()
-        reps = 0
-        while reps < 1 or time.perf_counter() - t0 < 10.0:
-            ow.raycast_rays(rays, threads)
-            reps += 1
-        dt = time.perf_counter() - t0
-        base = {"value": len(rays) * reps / dt / 1e6, "unit": "Mrays/s", "cores": threads,
+
+        def med5(fn):
+            fn()
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return float(np.median(ts)), ts
+        m0, ts0 = med5(lambda: ow.raycast_rays(rays, threads, L=L))
+        m1, ts1 = med5(lambda: ow.trace_rays(rays, 0, threads, L=L))
+        r0 = ow.raycast_rays(rays, threads, L=L)
+        c = oracle.classify_vs_r0(rays["o"], oracle.normalize_or_zero_v(rays["d"]), hits, r0,
+                                  tmax=rays["tmax"])
+        base = {"value": len(rays) / m0 / 1e6, "unit": "Mrays/s", "cores": threads,
                 "kind": "port",
-                "sample": f"{reps} full steps ({len(rays) * reps} rays), R0 = per-voxel DDA of "
-                          f"src/raycast.rs, {dt:.1f} s"}
+                "sample": f"whole steps ({len(rays)} rays), 1 warm-up + median of 5 "
+                          f"({', '.join(f'{t:.2f}' for t in ts0)} s), R0 = src/raycast.rs "
+                          f"per-voxel DDA",
+                "r1_value": len(rays) / m1 / 1e6, "build": build, "cpu_model": cpu["model"],
+                "physical_cores": cpu["physical_cores"], "usable_cpus": cpu["usable_cpus"]}
+        parity = {"vs_r0": {k: v for k, v in c.items() if k != "bad_index"}}
         log(f"cpu baseline: {base}")
+        log(f"parity: {parity}")
     out = {
         "metric": "Mrays/s of uneven batched ray queries",
         "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -362,6 +458,7 @@ def bench_batched(args):
                      "kernel_ms": round(float(np.mean(kern_ms)), 4),
                      "algorithmic_bytes": int(bytes_per_step)},
         "cpu_baseline": base,
+        "parity": parity if base is not None else None,
     }
     print(json.dumps(out), flush=True)
 
@@ -386,10 +483,15 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     p32 = gi.gi_params(sun, rays_per_probe=32, **base)
     stream = ctx.stream()
 
-    def timed(fn, p, n, cache=None):
+    kernel_ms = {}
+
+    def timed(fn, p, n, cache=None, grid_obj=None, name=None):
         # n back-to-back updates between two events on the stream; a learned update's
         # training step runs on the cache's own stream, so the end event waits for the last
-        # one (vg_nrc_join): the span covers all the work of the n updates
+        # one (vg_nrc_join): the span covers all the work of the n updates. With grid_obj,
+        # the grid also records each launch's time (events on the same stream).
+        if grid_obj is not None:
+            grid_obj.set_timing(True)
         a, b = ctx.event(), ctx.event()
         a.record(stream)
         for i in range(n):
@@ -399,6 +501,9 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
             cache.join(stream)
         b.record(stream)
         stream.synchronize()
+        if grid_obj is not None:
+            kernel_ms[name] = grid_obj.timing()
+            grid_obj.set_timing(False)
         return a.elapsed_ms(b) / n
 
     from voxelg_amd import _lib as L
@@ -412,7 +517,8 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     rays256, it256 = g256.update_counts(p256)
     g256.autotune(p256)
     sched_plain = sched()
-    plain_ms = timed(lambda p: g256.update(p, stream), p256, steps)
+    plain_ms = timed(lambda p: g256.update(p, stream), p256, steps, grid_obj=g256,
+                     name="plain256")
     log(f"gi plain: {plain_ms:.3f} ms/update ({sched_plain}), rays {rays256}")
 
     cache = nrc.RadianceCache(ctx)
@@ -427,16 +533,34 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     g32.autotune(p32, cache, train=True)  # (its repetitions train the cache further)
     sched_learned = sched()
     learned_ms = timed(lambda p: g32.update_learned(cache, p, train=True, stream=stream), p32,
-                       steps, cache)
+                       steps, cache, g32, "learned32")
     g32.autotune(p32, cache, train=False)  # the inference-only update has its own schedule
     sched_frozen = sched()
     frozen_ms = timed(lambda p: g32.update_learned(cache, p, train=False, stream=stream), p32,
-                      steps, cache)
+                      steps, cache, g32, "learned32_inference_only")
     log(f"gi learned: {learned_ms:.3f} ms/update with training ({sched_learned}), "
         f"{frozen_ms:.3f} without ({sched_frozen})")
+    # the cheapest plain setting beside them: 32 rays per probe, the 2-bounce traced tail
+    gp32 = gi.ProbeGrid(dw, p32)
+    gp32.autotune(p32)
+    sched_plain32 = sched()
+    plain32_ms = timed(lambda p: gp32.update(p, stream), p32, steps, grid_obj=gp32,
+                       name="plain32")
+    log(f"gi plain 32 rays: {plain32_ms:.3f} ms/update ({sched_plain32})")
+    gp32.close()
     p32["round"] = 5000
-    rays32, nq = g32.update_learned_counts(cache, p32, train=True)
-    log(f"gi learned: {learned_ms:.3f} ms/update, rays {rays32}, cache queries {nq}")
+    rays32, nq, it32 = g32.update_learned_counts(cache, p32, train=True, classes=True)
+    log(f"gi learned: {learned_ms:.3f} ms/update, rays {rays32}, cache queries {nq}, "
+        f"walk classes {it32}; launch ms {kernel_ms}")
+    # roofline of the learned update's dominant launch, the path kernel: the pyramid words its
+    # walks read (iteration classes) + per gather sample its probe origin, direction and face
+    # byte (33 B) + per cache query its hit record, radiance and list entry (17 + 32 B) + per
+    # training sample its query, table entry and target (32 + 16 + 48 B)
+    from voxelg_amd._lib import algorithmic_bytes
+    path_bytes = (algorithmic_bytes(it32) + 33 * rays32["gather"] + 49 * nq +
+                  96 * rays32["bounce"])
+    path_ms = kernel_ms["learned32"]["paths"]
+    path_gbs = path_bytes / (path_ms / 1e3) / 1e9
 
     # the probes' consumer: per-pixel indirect light over the headline 1080p frame's hits
     from voxelg_amd import workloads
@@ -481,18 +605,39 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     def err(rec):
         return float(gi.irradiance_error(gi.sh_coefficients(rec)[live], ref).mean())
 
+    errs = {"plain256": err(rec_plain), "learned32": err(rec_learned),
+            "plain32": err(rec_plain32), "ambient32": err(rec_ambient)}
+    times = {"plain256": plain_ms, "plain32": plain32_ms}
+    under = {k: v for k, v in times.items() if errs[k] <= GI_ERR_LIMIT}
+    fastest = min(under, key=under.get) if under else None
+    ok = errs["learned32"] <= GI_ERR_LIMIT
     out = {
+        "value": round(learned_ms, 4), "unit": "ms/update", "higher_is_better": False,
+        "err_limit": GI_ERR_LIMIT, "learned_meets_err_limit": bool(ok),
+        "speedup_vs_plain256": round(plain_ms / learned_ms, 3) if ok else None,
+        "target_speedup": 4.0,
+        "fastest_plain_under_limit": fastest,
+        "speedup_vs_fastest_plain_under_limit":
+            round(under[fastest] / learned_ms, 3) if (ok and fastest) else None,
         "probes": int(total), "live_probes": int(live.sum()), "grid": list(grid), "spacing": 32,
         "plain_ms": round(plain_ms, 4), "learned_ms": round(learned_ms, 4),
+        "plain32_ms": round(plain32_ms, 4),
         "speedup": round(plain_ms / learned_ms, 3),
         "learned_inference_only_ms": round(frozen_ms, 4),
         "speedup_inference_only": round(plain_ms / frozen_ms, 3),
-        "schedule_plain": sched_plain, "schedule_learned": sched_learned,
+        "schedule_plain": sched_plain, "schedule_plain32": sched_plain32,
+        "schedule_learned": sched_learned,
         "schedule_inference_only": sched_frozen,
         "err_plain256": round(err(rec_plain), 5), "err_learned32": round(err(rec_learned), 5),
         "err_plain32_2bounce": round(err(rec_plain32), 5),
         "err_ambient_tail32": round(err(rec_ambient), 5),
         "rays_plain": rays256, "rays_learned": rays32, "cache_queries": nq,
+        "launch_ms": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv
+                          for kk, vv in v.items()} for k, v in kernel_ms.items()},
+        "roofline": {"bound": "hbm", "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_probe_paths<false, 3> (learned update's path launch)",
+                     "kernel_ms": round(path_ms, 4), "algorithmic_bytes": int(path_bytes)},
         "indirect_1080p_ms": round(indirect_ms, 4),
         "train_steps": cache.stats()[0], "train_batch": cache.batch,
         "reference": f"FP64 4096 rays/probe 2-bounce, oracle on {threads} host threads, "
@@ -514,8 +659,7 @@ def bench_gi_main(args):
     sun = wl.sun()
     ctx = Context(0)
     dw = ctx.world(wl.dims).upload(world)
-    threads = min(os.cpu_count() or 1, 16)
-    g = gi_probe_bench(ctx, dw, world, sun, args.steps, args.gi_train_updates, threads)
+    g = gi_probe_bench(ctx, dw, world, sun, args.steps, args.gi_train_updates, gi_threads())
     out = {
         "metric": "GI probe ms at <=2% irradiance err vs plain",
         "value": g["learned_ms"], "unit": "ms/update", "n_gpus": 1, "steps": args.steps,

@@ -548,11 +548,20 @@ int vg_probes_update_learned(vg_probes *probes, const vg_gi_params *params, vg_n
  * context (VG_OPT_PROBE_*). */
 int vg_autotune_probes(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                        uint32_t flags, float *best_ms);
-/* The same, reporting rays traced (gather, shadow, training bounce) and the cache queries
- * (compacted gather hits). Synchronous. */
+/* The same, reporting rays traced (gather, shadow, training bounce), the cache queries
+ * (compacted gather hits) and the path walks' iteration-class totals (may be NULL).
+ * Synchronous. */
 int vg_probes_update_learned_counts(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                                     uint32_t flags, uint64_t rays_host[3],
-                                    uint64_t *queries_host);
+                                    uint64_t *queries_host, uint64_t counts_host[VG_IT_COUNT]);
+/* Per-launch timing of the probe updates that follow, with HIP events on each update's
+ * stream (no counterpart in the reference; for measurement): on != 0 starts a record of up to
+ * 64 updates, 0 stops and frees the events. vg_probes_timing waits for the recorded updates
+ * and returns the mean milliseconds per update of [0] the prep launch, [1] the path kernel,
+ * [2] the cache inference (0 in plain updates), [3] the reduction, and how many updates were
+ * recorded, then starts a new record. */
+int vg_probes_set_timing(vg_probes *probes, int on);
+int vg_probes_timing(vg_probes *probes, float ms_out[4], int32_t *n_out);
 
 /* raycast (src/raycast.rs:21) over a world on the device: one ray, host in and out.
  * Returns VG_OK with hit_host->hit = 0/1. Synchronous. */

@@ -76,78 +76,148 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _load(path):
+    """The oracle library at path with every entry point's signature set."""
+    L = C.CDLL(path)
+    P, I32P, U64P, F32P = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(C.c_float)
+    W = C.POINTER(OrcWorld)
+    sig = {
+        "orc_counts": (None, [I32P, C.POINTER(C.c_int64)]),
+        "orc_brick_uniform_of": (C.c_uint8, [P]),
+        "orc_rebuild_all_masks": (None, [W]),
+        "orc_rebuild_all_uniform": (None, [W]),
+        "orc_set_voxel": (None, [W, C.c_int32, C.c_int32, C.c_int32, C.c_uint8]),
+        "orc_apply_spheres": (None, [W, P, C.c_int64]),
+        "orc_material_at_world": (C.c_uint8, [W, C.c_int32, C.c_int32, C.c_int32]),
+        "orc_hash2": (C.c_float, [C.c_int32, C.c_int32]),
+        "orc_hash3": (C.c_float, [C.c_int32, C.c_int32, C.c_int32]),
+        "orc_value_noise_2d": (C.c_float, [C.c_float, C.c_float]),
+        "orc_value_noise_3d": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+        "orc_fbm_2d": (C.c_float, [C.c_float, C.c_float, C.c_uint32]),
+        "orc_seed_offset_xz": (None, [C.c_uint64, F32P]),
+        "orc_terrain_height": (C.c_int32, [I32P, C.c_uint64, C.c_int32, C.c_int32]),
+        "orc_generate": (None, [W, C.c_uint64]),
+        "orc_raycast_r0": (None, [W, F32P, F32P, C.c_float, P]),
+        "orc_trace_r1": (None, [W, P, C.c_uint32, P, U64P]),
+        "orc_trace_flat": (None, [W, P, C.c_uint32, P]),
+        "orc_trace_brute": (None, [W, P, P, I32P]),
+        "orc_trace_r1_batch": (None, [W, P, C.c_size_t, C.c_uint32, P, U64P, C.c_int]),
+        "orc_raycast_r0_batch": (None, [W, P, C.c_size_t, P, C.c_int]),
+        "orc_primary_dir": (None, [P, C.c_int32, C.c_int32, F32P]),
+        "orc_primary_rays": (None, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, P]),
+        "orc_frame": (None, [W, P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P,
+                             C.c_float, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int, U64P,
+                             P]),
+        "orc_frame_r0": (None, [W, P, C.c_int32, C.c_int32, C.c_float, F32P, C.c_float,
+                                C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int]),
+        "orc_shadow_origin": (None, [F32P, F32P, P, F32P]),
+        "orc_trace_dir": (C.c_int, [W, F32P, F32P, C.c_float, C.c_uint32, P, U64P]),
+        "orc_gi_sky": (None, [P, F32P, C.POINTER(C.c_double)]),
+        "orc_gi_bounce_index": (C.c_uint32, [C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32]),
+        "orc_gi_sample": (None, [W, P, P, P, F32P, F32P, C.c_uint32, C.c_uint32,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_uint8), F32P, U64P]),
+        "orc_gi_update": (None, [W, P, P, P, P, P, C.c_int, U64P]),
+        "orc_gi_reference": (None, [W, P, C.c_int32, P, P, P, P, P, C.c_int64, P, C.c_int]),
+        "orc_gi_update_learned": (None, [W, P, P, P, P, P, P, C.c_int, U64P]),
+        "orc_gi_bounce_tail": (None, [W, P, P, P, P, C.c_uint32, P, U64P]),
+        "orc_gi_bounce_tails": (None, [W, P, P, P, C.c_int64, P, P, P, C.c_int]),
+        "orc_water_rays": (None, [P, C.c_int32, C.c_int32, P, P]),
+        "orc_sh_irradiance": (None, [P, F32P, F32P]),
+        "orc_sample_probes": (None, [P, I32P, C.c_int32, F32P, F32P, F32P, F32P]),
+        "orc_frame_indirect": (None, [P, I32P, C.c_int32, P, C.c_int32, C.c_int32, P, P]),
+        "orc_bf16_round": (C.c_double, [C.c_double]),
+        "orc_nrc_hash": (C.c_uint32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+        "orc_nrc_scales": (None, [C.c_int32, C.c_float, C.c_float, P]),
+        "orc_nrc_encode": (None, [P, P, P, C.c_int, P]),
+        "orc_nrc_forward": (None, [P, P, P, C.c_int64, P, C.c_int, P]),
+        "orc_nrc_loss": (C.c_double, [P, P, P, C.c_int64, P, P, C.c_int]),
+        "orc_nrc_loss_grad": (C.c_double, [P, P, P, C.c_int64, P, P, C.c_int, P, P]),
+        "orc_nrc_adam": (None, [C.c_int64, P, P, P, P, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_int64, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P, I32P, U64P, F32P = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(C.c_float)
-        W = C.POINTER(OrcWorld)
-        sig = {
-            "orc_counts": (None, [I32P, C.POINTER(C.c_int64)]),
-            "orc_brick_uniform_of": (C.c_uint8, [P]),
-            "orc_rebuild_all_masks": (None, [W]),
-            "orc_rebuild_all_uniform": (None, [W]),
-            "orc_set_voxel": (None, [W, C.c_int32, C.c_int32, C.c_int32, C.c_uint8]),
-            "orc_apply_spheres": (None, [W, P, C.c_int64]),
-            "orc_material_at_world": (C.c_uint8, [W, C.c_int32, C.c_int32, C.c_int32]),
-            "orc_hash2": (C.c_float, [C.c_int32, C.c_int32]),
-            "orc_hash3": (C.c_float, [C.c_int32, C.c_int32, C.c_int32]),
-            "orc_value_noise_2d": (C.c_float, [C.c_float, C.c_float]),
-            "orc_value_noise_3d": (C.c_float, [C.c_float, C.c_float, C.c_float]),
-            "orc_fbm_2d": (C.c_float, [C.c_float, C.c_float, C.c_uint32]),
-            "orc_seed_offset_xz": (None, [C.c_uint64, F32P]),
-            "orc_terrain_height": (C.c_int32, [I32P, C.c_uint64, C.c_int32, C.c_int32]),
-            "orc_generate": (None, [W, C.c_uint64]),
-            "orc_raycast_r0": (None, [W, F32P, F32P, C.c_float, P]),
-            "orc_trace_r1": (None, [W, P, C.c_uint32, P, U64P]),
-            "orc_trace_flat": (None, [W, P, C.c_uint32, P]),
-            "orc_trace_brute": (None, [W, P, P, I32P]),
-            "orc_trace_r1_batch": (None, [W, P, C.c_size_t, C.c_uint32, P, U64P, C.c_int]),
-            "orc_raycast_r0_batch": (None, [W, P, C.c_size_t, P, C.c_int]),
-            "orc_primary_dir": (None, [P, C.c_int32, C.c_int32, F32P]),
-            "orc_primary_rays": (None, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, P]),
-            "orc_frame": (None, [W, P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, F32P,
-                                 C.c_float, C.c_int32, C.c_int32, P, P, C.c_int, U64P, P]),
-            "orc_frame_r0": (None, [W, P, C.c_int32, C.c_int32, C.c_float, F32P, C.c_float,
-                                    C.c_int32, C.c_int32, P, P, C.c_int]),
-            "orc_shadow_origin": (None, [F32P, F32P, P, F32P]),
-            "orc_trace_dir": (C.c_int, [W, F32P, F32P, C.c_float, C.c_uint32, P, U64P]),
-            "orc_gi_sky": (None, [P, F32P, C.POINTER(C.c_double)]),
-            "orc_gi_bounce_index": (C.c_uint32, [C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32]),
-            "orc_gi_sample": (None, [W, P, P, P, F32P, F32P, C.c_uint32, C.c_uint32,
-                                     C.POINTER(C.c_double), C.POINTER(C.c_uint8), F32P, U64P]),
-            "orc_gi_update": (None, [W, P, P, P, P, P, C.c_int, U64P]),
-            "orc_gi_reference": (None, [W, P, C.c_int32, P, P, P, P, P, C.c_int64, P, C.c_int]),
-            "orc_gi_update_learned": (None, [W, P, P, P, P, P, P, C.c_int, U64P]),
-            "orc_gi_bounce_tail": (None, [W, P, P, P, P, C.c_uint32, P, U64P]),
-            "orc_gi_bounce_tails": (None, [W, P, P, P, C.c_int64, P, P, P, C.c_int]),
-            "orc_water_rays": (None, [P, C.c_int32, C.c_int32, P, P]),
-            "orc_sh_irradiance": (None, [P, F32P, F32P]),
-            "orc_sample_probes": (None, [P, I32P, C.c_int32, F32P, F32P, F32P, F32P]),
-            "orc_frame_indirect": (None, [P, I32P, C.c_int32, P, C.c_int32, C.c_int32, P, P]),
-            "orc_bf16_round": (C.c_double, [C.c_double]),
-            "orc_nrc_hash": (C.c_uint32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
-            "orc_nrc_scales": (None, [C.c_int32, C.c_float, C.c_float, P]),
-            "orc_nrc_encode": (None, [P, P, P, C.c_int, P]),
-            "orc_nrc_forward": (None, [P, P, P, C.c_int64, P, C.c_int, P]),
-            "orc_nrc_loss": (C.c_double, [P, P, P, C.c_int64, P, P, C.c_int]),
-            "orc_nrc_loss_grad": (C.c_double, [P, P, P, C.c_int64, P, P, C.c_int, P, P]),
-            "orc_nrc_adam": (None, [C.c_int64, P, P, P, P, C.c_double, C.c_double, C.c_double,
-                                    C.c_double, C.c_int64, C.c_int]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+_timing = None
+TIMING_FLAGS = ["-O3", "-march=native", "-std=c11", "-fPIC", "-ffp-contract=off",
+                "-fno-fast-math", "-pthread", "-D_GNU_SOURCE"]
+
+
+def timing_lib():
+    """(library, description): the same sources built for timing on the host that runs
+    this (gcc -O3 -march=native, contraction off: the results are those of the test build),
+    into a private temporary directory; the test build (-O2) when no compiler is present."""
+    global _timing
+    if _timing is None:
+        import shutil
+        import tempfile
+        cc = shutil.which("gcc") or shutil.which("cc")
+        if cc:
+            out = os.path.join(tempfile.mkdtemp(prefix="oracle_timing_"), "liboracle_timing.so")
+            src = [os.path.join(_HERE, f) for f in ("oracle.c", "oracle_gi.c", "oracle_nrc.c")]
+            r = subprocess.run([cc] + TIMING_FLAGS + ["-shared", "-o", out] + src + ["-lm"],
+                               capture_output=True, text=True)
+            if r.returncode == 0:
+                _timing = (_load(out), os.path.basename(cc) + " " + " ".join(TIMING_FLAGS[:2]))
+        if _timing is None:
+            _timing = (lib(), "test build (gcc -O2)")
+    return _timing
+
+
+def host_cpu() -> dict:
+    """The host's CPU model, its physical cores and logical CPUs (Linux /proc/cpuinfo),
+    and the CPUs this process may run on."""
+    model, cores = "unknown", set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k:
+                if core is not None:
+                    cores.add((phys, core))
+                phys = core = None
+        if core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "physical_cores": len(cores) or None,
+            "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
 def _threads(threads: int) -> int:
     """Worker threads: the request, else the host's cores capped at 16 (one GPU's CPU share)."""
     return int(threads) if threads else min(16, os.cpu_count() or 1)
+
+
+def _rows(rows, height):
+    if rows is None:
+        return 0, int(height), 1
+    r0, r1 = int(rows[0]), int(rows[1])
+    return r0, r1, int(rows[2]) if len(rows) > 2 else 1
 
 
 def _f3(v):
@@ -280,11 +350,12 @@ class OracleWorld:
             ties[i] = t.value
         return out, ties
 
-    def trace_rays(self, rays: np.ndarray, flags=0, threads=0):
+    def trace_rays(self, rays: np.ndarray, flags=0, threads=0, L=None):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(len(rays), HIT_DTYPE)
         cnt = (C.c_uint64 * IT_COUNT)()
-        lib().orc_trace_r1_batch(C.byref(self.cstruct()), _p(rays), len(rays), flags, _p(out),
+        (L or lib()).orc_trace_r1_batch(C.byref(self.cstruct()), _p(rays), len(rays), flags,
+                                        _p(out),
                                  cnt, _threads(threads))
         return out, dict(zip(IT_NAMES, (int(c) for c in cnt)))
 
@@ -296,41 +367,43 @@ class OracleWorld:
         out = out.reshape(height * width, 2)
         return np.ascontiguousarray(out[:, 0]), np.ascontiguousarray(out[:, 1])
 
-    def raycast_rays(self, rays: np.ndarray, threads=0) -> np.ndarray:
+    def raycast_rays(self, rays: np.ndarray, threads=0, L=None) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(len(rays), HIT_DTYPE)
-        lib().orc_raycast_r0_batch(C.byref(self.cstruct()), _p(rays), len(rays), _p(out),
+        (L or lib()).orc_raycast_r0_batch(C.byref(self.cstruct()), _p(rays), len(rays), _p(out),
                                    _threads(threads))
         return out
 
     def frame(self, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
-              rows=None, threads=0, iters=False):
-        """R1 frame: hits (+ shadow bytes) for rows [r0, r1) of the image; with iters=True
-        also the per-pixel (primary, shadow) iteration counts."""
-        r0, r1 = rows if rows is not None else (0, height)
-        n = (r1 - r0) * width
+              rows=None, threads=0, iters=False, L=None):
+        """R1 frame: hits (+ shadow bytes) for rows (r0, r1) or (r0, r1, step) of the image,
+        i.e. r0, r0 + step, ... below r1; with iters=True also the per-pixel (primary,
+        shadow) iteration counts. L: another build of the oracle library."""
+        r0, r1, st = _rows(rows, height)
+        n = len(range(r0, r1, st)) * width
         hits = np.zeros(n, HIT_DTYPE)
         shadow = np.zeros(n, np.uint8)
         it = np.zeros((n, 2), np.uint32) if iters else None
         cnt = (C.c_uint64 * IT_COUNT)()
         cam = np.ascontiguousarray(camera, np.float32)
-        lib().orc_frame(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
-                        primary_flags, _f3(sun if sun is not None else (0, 0, 0)),
-                        shadow_tmax, r0, r1, _p(hits), _p(shadow), _threads(threads), cnt,
-                        _p(it) if iters else None)
+        (L or lib()).orc_frame(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
+                               primary_flags, _f3(sun if sun is not None else (0, 0, 0)),
+                               shadow_tmax, r0, r1, st, _p(hits), _p(shadow), _threads(threads),
+                               cnt, _p(it) if iters else None)
         out = (hits, shadow, dict(zip(IT_NAMES, (int(c) for c in cnt))))
         return out + (it,) if iters else out
 
     def frame_r0(self, camera, width, height, primary_tmax, sun, shadow_tmax, rows=None,
-                 threads=0):
-        r0, r1 = rows if rows is not None else (0, height)
-        n = (r1 - r0) * width
+                 threads=0, L=None):
+        """The same frame traced by R0 (raycast() of src/raycast.rs per pixel, then per hit)."""
+        r0, r1, st = _rows(rows, height)
+        n = len(range(r0, r1, st)) * width
         hits = np.zeros(n, HIT_DTYPE)
         shadow = np.zeros(n, np.uint8)
         cam = np.ascontiguousarray(camera, np.float32)
-        lib().orc_frame_r0(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
-                           _f3(sun if sun is not None else (0, 0, 0)), shadow_tmax, r0, r1,
-                           _p(hits), _p(shadow), _threads(threads))
+        (L or lib()).orc_frame_r0(C.byref(self.cstruct()), _p(cam), width, height, primary_tmax,
+                                  _f3(sun if sun is not None else (0, 0, 0)), shadow_tmax, r0,
+                                  r1, st, _p(hits), _p(shadow), _threads(threads))
         return hits, shadow
 
     # ---------------------------------------------------------- GI probes --

@@ -1036,7 +1036,7 @@ typedef struct job_t {
     uint64_t counts[ORC_IT_COUNT];
     /* frame */
     const float *cam;
-    int32_t width, height, row0, row1;
+    int32_t width, height, row0, row1, row_step; /* rows row0, row0 + step, ... < row1 */
     float ptmax, stmax, sun[3];
     float sun_in[3]; /* R0 frame: the sun direction as given (raycast normalizes it) */
     uint8_t *shadow;
@@ -1228,9 +1228,13 @@ float orc_beam_t(const orc_world *w, const float *cam, int32_t bx, int32_t by, f
 
 static void frame_rows(job_t *j) {
     const float *cam = j->cam;
-    for (int32_t y = j->row0 + j->tid; y < j->row1; y += j->nthreads) {
+    const int32_t step = j->row_step > 0 ? j->row_step : 1;
+    const int32_t nrows = j->row1 > j->row0 ? (j->row1 - j->row0 + step - 1) / step : 0;
+    /* the selected rows interleaved over the threads */
+    for (int32_t k = j->tid; k < nrows; k += j->nthreads) {
+        const int32_t y = j->row0 + k * step;
         for (int32_t x = 0; x < j->width; ++x) {
-            size_t i = (size_t)(y - j->row0) * j->width + x;
+            size_t i = (size_t)k * j->width + x;
             float dir[3];
             orc_primary_dir(cam, x, y, dir);
             orc_hit *h = &j->out[i];
@@ -1279,8 +1283,8 @@ static void frame_rows(job_t *j) {
 
 void orc_frame(const orc_world *w, const float *cam, int32_t width, int32_t height,
                float primary_tmax, uint32_t primary_flags, const float sun[3], float shadow_tmax,
-               int32_t row0, int32_t row1, orc_hit *hits, uint8_t *shadow, int threads,
-               uint64_t counts[ORC_IT_COUNT], uint32_t *iters) {
+               int32_t row0, int32_t row1, int32_t row_step, orc_hit *hits, uint8_t *shadow,
+               int threads, uint64_t counts[ORC_IT_COUNT], uint32_t *iters) {
     job_t p;
     memset(&p, 0, sizeof(p));
     p.kind = 2;
@@ -1290,6 +1294,7 @@ void orc_frame(const orc_world *w, const float *cam, int32_t width, int32_t heig
     p.height = height;
     p.row0 = row0;
     p.row1 = row1;
+    p.row_step = row_step;
     p.ptmax = primary_tmax;
     p.stmax = shadow_tmax;
     p.flags = primary_flags;
@@ -1302,7 +1307,7 @@ void orc_frame(const orc_world *w, const float *cam, int32_t width, int32_t heig
 
 void orc_frame_r0(const orc_world *w, const float *cam, int32_t width, int32_t height,
                   float primary_tmax, const float sun[3], float shadow_tmax, int32_t row0,
-                  int32_t row1, orc_hit *hits, uint8_t *shadow, int threads) {
+                  int32_t row1, int32_t row_step, orc_hit *hits, uint8_t *shadow, int threads) {
     job_t p;
     memset(&p, 0, sizeof(p));
     p.kind = 3;
@@ -1312,6 +1317,7 @@ void orc_frame_r0(const orc_world *w, const float *cam, int32_t width, int32_t h
     p.height = height;
     p.row0 = row0;
     p.row1 = row1;
+    p.row_step = row_step;
     p.ptmax = primary_tmax;
     p.stmax = shadow_tmax;
     for (int k = 0; k < 3; ++k) p.sun_in[k] = sun[k];

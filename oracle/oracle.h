@@ -169,17 +169,19 @@ void orc_water_rays(const float *camera176, int32_t width, int32_t height, const
                     orc_ray *out);
 void orc_primary_rays(const float *camera176, int32_t w, int32_t h, float tmax, uint32_t flags,
                       orc_ray *out);
-/* One frame: primary (R1, flags) then a sun shadow ray per hit. */
+/* One frame: primary (R1, flags) then a sun shadow ray per hit, over the rows row0,
+ * row0 + row_step, ... below row1 (hits and shadow hold those rows in order), the rows
+ * interleaved over the threads. */
 void orc_frame(const orc_world *w, const float *camera176, int32_t width, int32_t height,
                float primary_tmax, uint32_t primary_flags, const float sun[3], float shadow_tmax,
-               int32_t row0, int32_t row1, orc_hit *hits, uint8_t *shadow, int threads,
-               uint64_t counts[ORC_IT_COUNT], uint32_t *iters /* may be NULL */);
+               int32_t row0, int32_t row1, int32_t row_step, orc_hit *hits, uint8_t *shadow,
+               int threads, uint64_t counts[ORC_IT_COUNT], uint32_t *iters /* may be NULL */);
 /* Same frame with R0 as the traversal (the CPU baseline): raycast() is given the pinhole
  * vector and the sun direction as they are and normalizes each itself, so it traces the
  * very rays of orc_frame (and of the GPU frame). */
 void orc_frame_r0(const orc_world *w, const float *camera176, int32_t width, int32_t height,
                   float primary_tmax, const float sun[3], float shadow_tmax, int32_t row0,
-                  int32_t row1, orc_hit *hits, uint8_t *shadow, int threads);
+                  int32_t row1, int32_t row_step, orc_hit *hits, uint8_t *shadow, int threads);
 void orc_shadow_origin(const float o[3], const float d[3], const orc_hit *h, float out[3]);
 /* Beam prologue start t of the 8x8 pixel block (bx, by); tests counts tested tiles. */
 float orc_beam_t(const orc_world *w, const float *camera176, int32_t bx, int32_t by, float tmax,

@@ -12,17 +12,32 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP) device")
 
 
+def _stale(lib, sources):
+    """The library is missing or older than one of its sources."""
+    if not os.path.exists(lib):
+        return True
+    t = os.path.getmtime(lib)
+    return any(os.path.getmtime(f) > t for f in sources if os.path.exists(f))
+
+
 def pytest_sessionstart(session):
-    # An incremental build (the Makefiles' dependencies: nothing to do when the libraries
-    # are newer than their sources), so a test never runs a library older than its sources.
-    # Where no compiler is present the libraries shipped beside the sources are used.
-    import shutil
+    # Rebuild a library that is missing or older than its sources, so a test never runs a
+    # library older than the code it checks; an up-to-date library is loaded as it is (its
+    # object files need not be present).
+    import glob
+    csrc = os.path.join(ROOT, "voxelg_amd", "csrc")
+    engine = glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "*.h")) + \
+        glob.glob(os.path.join(csrc, "*.cpp")) + [os.path.join(csrc, "Makefile"),
+                                                  os.path.join(ROOT, "include", "vg.h")]
+    odir = os.path.join(ROOT, "oracle")
+    checker = glob.glob(os.path.join(odir, "*.c")) + glob.glob(os.path.join(odir, "*.h")) + \
+        [os.path.join(odir, "Makefile")]
     import subprocess
-    for d, tool in ((os.path.join(ROOT, "voxelg_amd", "csrc"), "/opt/rocm/bin/hipcc"),
-                    (os.path.join(ROOT, "oracle"), "gcc")):
-        if shutil.which("make") and (os.path.exists(tool) or shutil.which(tool)):
-            subprocess.run(["make", "-s", "-C", d, "-j", str(min(16, os.cpu_count() or 1))],
-                           check=True)
+    jobs = str(min(16, os.cpu_count() or 1))
+    if _stale(os.path.join(ROOT, "voxelg_amd", "libvoxelg_amd.so"), engine):
+        subprocess.run(["make", "-s", "-C", csrc, "-j", jobs], check=True)
+    if _stale(os.path.join(odir, "liboracle.so"), checker):
+        subprocess.run(["make", "-s", "-C", odir], check=True)
 
 
 @pytest.fixture(scope="session")

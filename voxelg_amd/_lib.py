@@ -193,7 +193,10 @@ SIGNATURES = {
     "vg_nrc_stats": (C.c_int, [_P, _I64P, _F32P]),
     "vg_nrc_last_batch": (C.c_int, [_P, _P, _P]),
     "vg_probes_update_learned": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
-    "vg_probes_update_learned_counts": (C.c_int, [_P, _P, _P, C.c_uint32, _U64P, _U64P]),
+    "vg_probes_update_learned_counts": (C.c_int, [_P, _P, _P, C.c_uint32, _U64P, _U64P,
+                                                  _U64P]),
+    "vg_probes_set_timing": (C.c_int, [_P, C.c_int]),
+    "vg_probes_timing": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "vg_autotune_probes": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float)]),
     "vg_world_apply_edits": (C.c_int, [_P, _P, C.c_uint32, _P]),
     "vg_frame_mask_words": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),

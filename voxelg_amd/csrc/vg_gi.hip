@@ -42,6 +42,10 @@ struct vg_probes {
     uint32_t *hmask;        // one bit per sample: 2 * hwords
     uint32_t hcap, hwords;
     int hpar;               // half the next update writes
+    // per-launch timing (vg_probes_set_timing): 5 events per recorded update, bracketing prep,
+    // the path kernel, the cache inference and the reduction on the update's stream
+    hipEvent_t *tev;        // kTimingCap * 5 events (null: timing off)
+    int tn;                 // updates recorded
     unsigned long long *counts; // VG_IT_COUNT walk classes + 3 ray kinds
 };
 
@@ -54,6 +58,7 @@ constexpr int kGiCounters = VG_IT_COUNT + 3;
 // [4 + s - 1] work of training step s >= 1 (vg_nrc_config.steps <= 64), [kCtrHeavy] this
 // update's heavy paths, [kCtrHeavy + 1] the previous update's.
 constexpr int kCtrHeavy = 4 + 64;
+constexpr int kTimingCap = 64; // updates one timing record holds
 constexpr int kGiCtrs = kCtrHeavy + 2;
 
 // ----------------------------------------------------------------- helpers --
@@ -1109,10 +1114,16 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     ha.prev_mask = pr->hmask + (size_t)(hp ^ 1) * pr->hwords;
     ha.cap = pr->hcap;
     ha.thresh = (nrc || p->bounces != 2) ? kHeavyIters : kHeavyIters2;
+    hipEvent_t *tev = (pr->tev && pr->tn < kTimingCap && !count) ? pr->tev + 5 * pr->tn : nullptr;
+    if (tev) {
+        pr->tn++;
+        HIP_TRY(hipEventRecord(tev[0], stream));
+    }
     hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0,
                        ha.cur_mask, pr->hwords);
     HIP_TRY(hipGetLastError());
+    if (tev) HIP_TRY(hipEventRecord(tev[1], stream));
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
     const int thr = w->ctx->probe_refill;
     const bool train = nrc && (nflags & VG_NRC_TRAIN) && nrc->cfg.steps > 0;
@@ -1143,6 +1154,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     }
 #undef LAUNCH_G
     HIP_TRY(hipGetLastError());
+    if (tev) HIP_TRY(hipEventRecord(tev[2], stream));
     if (nrc) {
         st = nrc_infer_launch(nrc, pr->queries, pr->ns, pr->ctr + 3, pr->ctr + 1,
                               (uint32_t)n_samples, nullptr, pr->rad, pr->palette, stream);
@@ -1157,6 +1169,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     // path launch write); more steps, or a counting run, stay on the caller's stream
     const bool overlap = train && nrc->cfg.steps == 1 && !count;
     if (overlap) HIP_TRY(hipEventRecord(nrc->ev_ready, stream));
+    if (tev) HIP_TRY(hipEventRecord(tev[3], stream));
     uint32_t lpp = 8; // lanes per probe in the reduction: the sample count's power of two
     while (lpp < 64 && lpp < (uint32_t)pr->rays) lpp <<= 1;
     const uint32_t ppb = kBlock / lpp;
@@ -1164,6 +1177,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
                        w->dw, *p, n_upd, epoch, phase, lpp, pr->pwork, pr->dirs, pr->rad,
                        pr->face, pr->probes);
     HIP_TRY(hipGetLastError());
+    if (tev) HIP_TRY(hipEventRecord(tev[4], stream));
     if (overlap) {
         HIP_TRY(hipStreamWaitEvent(nrc->tstream, nrc->ev_ready, 0));
         st = nrc_train_launch(nrc, nrc->tq, nrc->tt, (uint32_t)nrc->tcap, true, nrc->tstream);
@@ -1217,11 +1231,21 @@ static int probes_update_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nr
     return VG_OK;
 }
 
+static void free_timing(vg_probes *p) {
+    if (!p->tev) return;
+    for (int i = 0; i < 5 * kTimingCap; ++i)
+        if (p->tev[i]) (void)hipEventDestroy(p->tev[i]);
+    delete[] p->tev;
+    p->tev = nullptr;
+    p->tn = 0;
+}
+
 static void free_probes(vg_probes *p) {
     void *ptrs[] = {p->probes, p->dirs,    p->bounce, p->palette, p->pwork, p->rad,
                     p->face,   p->queries, p->ctr,    p->counts,  p->heavy, p->hmask};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    free_timing(p);
 }
 
 } // namespace vg
@@ -1409,9 +1433,52 @@ int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32
 
 int vg_probes_update_learned_counts(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc,
                                     uint32_t flags, uint64_t rays_host[3],
-                                    uint64_t *queries_host) {
+                                    uint64_t *queries_host, uint64_t counts_host[VG_IT_COUNT]) {
     if (!nrc) return fail(VG_E_INVALID, "null cache");
-    return probes_update_counts(pr, p, nrc, flags, rays_host, nullptr, queries_host);
+    return probes_update_counts(pr, p, nrc, flags, rays_host, counts_host, queries_host);
+}
+
+int vg_probes_set_timing(vg_probes *pr, int on) {
+    if (!pr) return fail(VG_E_INVALID, "null probes");
+    int st = set_device(pr->world);
+    if (st) return st;
+    if (!on) {
+        HIP_TRY(hipDeviceSynchronize());
+        free_timing(pr);
+        return VG_OK;
+    }
+    if (!pr->tev) {
+        pr->tev = new hipEvent_t[5 * kTimingCap]();
+        for (int i = 0; i < 5 * kTimingCap; ++i)
+            if (hipEventCreate(&pr->tev[i]) != hipSuccess) {
+                free_timing(pr);
+                return fail(VG_E_DEVICE, "timing events");
+            }
+    }
+    pr->tn = 0;
+    return VG_OK;
+}
+
+int vg_probes_timing(vg_probes *pr, float ms_out[4], int32_t *n_out) {
+    if (!pr || !ms_out) return fail(VG_E_INVALID, "null argument");
+    if (!pr->tev) return fail(VG_E_INVALID, "timing is off (vg_probes_set_timing)");
+    int st = set_device(pr->world);
+    if (st) return st;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int n = pr->tn;
+    for (int u = 0; u < n; ++u) {
+        hipEvent_t *e = pr->tev + 5 * u;
+        HIP_TRY(hipEventSynchronize(e[4]));
+        for (int k = 0; k < 4; ++k) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    for (int k = 0; k < 4; ++k) ms_out[k] = n ? (float)(acc[k] / n) : 0.0f;
+    if (n_out) *n_out = n;
+    pr->tn = 0;
+    return VG_OK;
 }
 
 int vg_probes_update_counts(vg_probes *pr, const vg_gi_params *p, uint64_t rays_host[3],

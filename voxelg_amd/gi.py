@@ -157,16 +157,33 @@ class ProbeGrid:
                                              _stream(stream)), "vg_probes_update_learned")
         return self
 
-    def update_learned_counts(self, cache, params: np.ndarray, train: bool = True):
-        """One learned update reporting (rays dict: gather/shadow/bounce, cache queries)."""
+    def update_learned_counts(self, cache, params: np.ndarray, train: bool = True,
+                              classes: bool = False):
+        """One learned update reporting (rays dict: gather/shadow/bounce, cache queries[,
+        walk iteration classes of the path kernel])."""
         p = np.ascontiguousarray(params, GI_PARAMS_DTYPE)
         rays = (C.c_uint64 * 3)()
         nq = C.c_uint64()
+        cnt = (C.c_uint64 * VG_IT_COUNT)()
         check(lib().vg_probes_update_learned_counts(self._h, ptr(p), cache._h, 1 if train else 0,
-                                                    rays, C.byref(nq)),
+                                                    rays, C.byref(nq), cnt),
               "vg_probes_update_learned_counts")
-        return (dict(zip(("gather", "shadow", "bounce"), (int(r) for r in rays))),
-                int(nq.value))
+        out = (dict(zip(("gather", "shadow", "bounce"), (int(r) for r in rays))), int(nq.value))
+        return out + (dict(zip(IT_NAMES, (int(c) for c in cnt))),) if classes else out
+
+    def set_timing(self, on: bool = True) -> "ProbeGrid":
+        """Record per-launch times of the updates that follow (HIP events on their stream)."""
+        check(lib().vg_probes_set_timing(self._h, 1 if on else 0), "vg_probes_set_timing")
+        return self
+
+    def timing(self) -> dict:
+        """Mean ms per recorded update of each launch (prep, paths, infer, reduce) and the
+        number of updates recorded; starts a new record. Synchronous."""
+        ms = (C.c_float * 4)()
+        n = C.c_int32()
+        check(lib().vg_probes_timing(self._h, ms, C.byref(n)), "vg_probes_timing")
+        return dict(zip(("prep", "paths", "infer", "reduce"), (float(v) for v in ms)),
+                    updates=int(n.value))
 
     def autotune(self, params: np.ndarray, cache=None, train: bool = False) -> float:
         """Pick the probe path kernels' schedule (waves per CU, refill) on this update; the
