@@ -225,6 +225,19 @@ int vg_trace_frame(vg_world *world, const vg_camera *camera_host, int32_t width,
                    float primary_tmax, uint32_t primary_flags, const float sun[3],
                    float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream);
 
+/* vg_trace_frame for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It has no counterpart in the reference.
+ * It calls no hipMalloc / hipFree: its scratch (the ray counter and the beam start t of up to
+ * 512 x 512 8x8 pixel blocks, 1 MiB) is a static array per device that the library holds until
+ * the process ends, its address taken once per device by vg_ctx_create. Frames above 262 144
+ * blocks (4096 x 4096 pixels) return VG_E_INVALID. All calls on one device share that array,
+ * which keeps the last call's contents (each call resets what must start at zero): calls that
+ * overlap on one device race; serialize them on one stream. */
+int vg_trace_frame_noalloc(vg_world *world, const vg_camera *camera_host, int32_t width,
+                           int32_t height, float primary_tmax, uint32_t primary_flags,
+                           const float sun[3], float shadow_tmax, vg_hit *hits,
+                           uint8_t *occluded, void *stream);
+
 /* The same frame, also reporting the walk's iteration-class totals (VG_IT_*) into
  * counts_host; synchronous. Used for algorithmic-byte accounting, not for timing. */
 int vg_trace_frame_counts(vg_world *world, const vg_camera *camera_host, int32_t width,

@@ -133,6 +133,8 @@ SIGNATURES = {
     "vg_trace_shadow": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _F32P, C.c_float, _P, _P]),
     "vg_trace_frame": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _F32P,
                                  C.c_float, _P, _P, _P]),
+    "vg_trace_frame_noalloc": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
+                                         _F32P, C.c_float, _P, _P, _P]),
     "vg_trace_frame_counts": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
                                         _F32P, C.c_float, _P, _P, _U64P]),
     "vg_trace_rays_counts": (C.c_int, [_P, _P, C.c_size_t, C.c_uint32, _P, _U64P]),

@@ -17,6 +17,7 @@
 // correctly rounded divide and sqrt), recomputes every plane-crossing t from integer
 // coordinates, and skips empty cells with an integer-exact exit computation, so its
 // result equals a one-voxel-at-a-time walk bit for bit.
+#include <mutex>
 #include <vector>
 
 #include "vg_world.h"
@@ -797,29 +798,27 @@ static int ensure_beam_capacity(vg_world *w, size_t blocks) {
     return VG_OK;
 }
 
-static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t width, int32_t height,
-                                   float ptmax, uint32_t pflags, const float s[3], float stmax,
-                                   vg_hit *hits, uint8_t *occluded, bool shadow, bool beam,
-                                   bool count, hipStream_t stream,
-                                   const uint32_t *list = nullptr,
-                                   const uint32_t *list_n = nullptr) {
+// The persistent frame launch over caller-provided scratch: `work` (the ray counter, reset
+// by the first kernel) and `beam_t` (one float per 8x8 block). Allocates nothing.
+static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t width,
+                                      int32_t height, float ptmax, uint32_t pflags,
+                                      const float s[3], float stmax, vg_hit *hits,
+                                      uint8_t *occluded, bool shadow, bool beam, bool count,
+                                      hipStream_t stream, uint32_t *work, float *beam_t,
+                                      const uint32_t *list, const uint32_t *list_n) {
     const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
     const uint64_t nblocks = (uint64_t)nbx * nby;
-    // the counter may overshoot the ray count by up to 64 per wave after it runs out
-    if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     if (beam) {
-        int st = ensure_beam_capacity(w, (size_t)nblocks);
-        if (st) return st;
         dim3 g((uint32_t)((nblocks + 3) / 4));
         if (count)
             hipLaunchKernelGGL((k_beam<true>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
-                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, list, list_n, cd);
+                               (uint32_t)nblocks, ptmax, beam_t, work, list, list_n, cd);
         else
             hipLaunchKernelGGL((k_beam<false>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
-                               (uint32_t)nblocks, ptmax, w->beam_t, w->work, list, list_n, cd);
+                               (uint32_t)nblocks, ptmax, beam_t, work, list, list_n, cd);
     } else {
-        hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, w->work);
+        hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, work);
     }
     HIP_TRY(hipGetLastError());
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
@@ -827,8 +826,8 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
     const int thr = w->ctx->frame_refill;
 #define LAUNCH_P(C, SH, B)                                                                     \
     hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, cp, width, \
-                       height, nbx, total, ptmax, pflags, s[0], s[1], s[2], stmax, w->beam_t,  \
-                       hits, occluded, w->work, thr, list, list_n, cd)
+                       height, nbx, total, ptmax, pflags, s[0], s[1], s[2], stmax, beam_t,     \
+                       hits, occluded, work, thr, list, list_n, cd)
     if (count) {
         if (shadow) {
             if (beam) LAUNCH_P(true, true, true);
@@ -851,10 +850,65 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
     return VG_OK;
 }
 
+static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t width, int32_t height,
+                                   float ptmax, uint32_t pflags, const float s[3], float stmax,
+                                   vg_hit *hits, uint8_t *occluded, bool shadow, bool beam,
+                                   bool count, hipStream_t stream,
+                                   const uint32_t *list = nullptr,
+                                   const uint32_t *list_n = nullptr) {
+    const uint64_t nblocks = (uint64_t)((width + 7) / 8) * (uint64_t)((height + 7) / 8);
+    // the counter may overshoot the ray count by up to 64 per wave after it runs out
+    if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
+    if (beam) {
+        int st = ensure_beam_capacity(w, (size_t)nblocks);
+        if (st) return st;
+    }
+    return launch_frame_persistent_on(w, cp, width, height, ptmax, pflags, s, stmax, hits,
+                                      occluded, shadow, beam, count, stream, w->work, w->beam_t,
+                                      list, list_n);
+}
+
+// Scratch of the no-allocation frame entry point (vg_trace_frame_noalloc): one array per
+// device, `scratch`, holding the ray counter at offset 0 and the beam start t of up to
+// kNoAllocBlocks 8x8 pixel blocks from offset 256. Kernels get its device address, taken once
+// per device with hipGetSymbolAddress (the name itself is a host-side stand-in). It keeps the
+// last call's contents; the first kernel of every call resets the counter.
+constexpr size_t kNoAllocBlocks = 512 * 512; // 8x8 blocks of a 4096x4096 image
+constexpr size_t kNoAllocBytes = 256 + kNoAllocBlocks * sizeof(float);
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+struct NoAllocScratch {
+    uint32_t *work;
+    float *beam_t;
+};
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on the current device `device` (taken on the first call
+// for that device, under a lock).
+static int noalloc_scratch(int device, NoAllocScratch *out) {
+    if (device < 0 || device >= kMaxDevices) return fail(VG_E_INVALID, "device %d", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VG_E_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    uint8_t *base = static_cast<uint8_t *>(g_scratch_ptr[device]);
+    out->work = reinterpret_cast<uint32_t *>(base);
+    out->beam_t = reinterpret_cast<float *>(base + 256);
+    return VG_OK;
+}
+
 static int launch_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
                         float ptmax, uint32_t pflags, const float sun[3], float stmax,
                         vg_hit *hits, uint8_t *occluded, int mode, bool count,
-                        hipStream_t stream) {
+                        hipStream_t stream, const NoAllocScratch *na = nullptr) {
     int st = check_world(w);
     if (st) return st;
     if (!camera || width <= 0 || height <= 0 || !hits) return fail(VG_E_INVALID, "bad frame args");
@@ -869,9 +923,14 @@ static int launch_frame(vg_world *w, const vg_camera *camera, int32_t width, int
     CamParams cp = cam_params(camera);
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     const bool beam = (pflags & VG_RAY_BEAM) != 0;
-    if (w->ctx->schedule == VG_SCHEDULE_PERSISTENT && mode != 2)
+    if (w->ctx->schedule == VG_SCHEDULE_PERSISTENT && mode != 2) {
+        if (na) // the caller checked the block count against kNoAllocBlocks
+            return launch_frame_persistent_on(w, cp, width, height, ptmax, pflags, s, stmax,
+                                              hits, occluded, mode == 1, beam, count, stream,
+                                              na->work, na->beam_t, nullptr, nullptr);
         return launch_frame_persistent(w, cp, width, height, ptmax, pflags, s, stmax, hits,
                                        occluded, mode == 1, beam, count, stream);
+    }
     const uint32_t nbx = (uint32_t)((width + 15) / 16), nby = (uint32_t)((height + 15) / 16);
     const uint32_t nb = nbx * nby;
     dim3 grid(nb), block(kBlock);
@@ -974,6 +1033,12 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->probe_wpc = 8;
     c->probe_refill = 8;
     c->probe_chunk = 0;
+    NoAllocScratch na; // the no-allocation entry point's scratch address on this device
+    int st = noalloc_scratch(device, &na);
+    if (st) {
+        delete c;
+        return st;
+    }
     *out = c;
     return VG_OK;
 }
@@ -1156,6 +1221,25 @@ int vg_trace_frame(vg_world *w, const vg_camera *camera, int32_t width, int32_t 
                    float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream) {
     return launch_frame(w, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
                         hits, occluded, 1, false, (hipStream_t)stream);
+}
+
+int vg_trace_frame_noalloc(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
+                           float primary_tmax, uint32_t primary_flags, const float sun[3],
+                           float shadow_tmax, vg_hit *hits, uint8_t *occluded, void *stream) {
+    int st = check_world(w);
+    if (st) return st;
+    if (width <= 0 || height <= 0) return fail(VG_E_INVALID, "bad frame size %dx%d", width, height);
+    const uint64_t nblocks = (uint64_t)((width + 7) / 8) * (uint64_t)((height + 7) / 8);
+    if (nblocks > kNoAllocBlocks)
+        return fail(VG_E_INVALID,
+                    "%dx%d frame: %llu 8x8 blocks above the no-allocation scratch's %zu "
+                    "(4096x4096 pixels); use vg_trace_frame",
+                    width, height, (unsigned long long)nblocks, kNoAllocBlocks);
+    NoAllocScratch na;
+    st = noalloc_scratch(w->ctx->device, &na);
+    if (st) return st;
+    return launch_frame(w, camera, width, height, primary_tmax, primary_flags, sun, shadow_tmax,
+                        hits, occluded, 1, false, (hipStream_t)stream, &na);
 }
 
 int vg_trace_water_secondary(vg_world *w, const vg_camera *camera, int32_t width, int32_t height,
