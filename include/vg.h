@@ -307,11 +307,13 @@ int vg_world_download(vg_world *world, vg_brick *bricks, uint64_t *tile_mask,
 
 /* ------------------------------------------------------------- scheduling -- */
 /* Traces run as persistent, ray-compacting kernels by default: a machine-sized grid whose
- * waves refill idle lanes from a global ray counter whenever fewer than `refill` of their
- * 64 lanes hold a live ray (a primary hit turns into that lane's shadow ray at once).
+ * waves refill idle lanes from a ray counter whenever fewer than `refill` of their 64 lanes
+ * hold a live ray (a primary hit turns into that lane's shadow ray at once). The frame
+ * kernel's 8x8 blocks are cut into VG_OPT_FRAME_STRIPES contiguous stripes, each with its
+ * own counter; a wave starts on a stripe of its XCD and moves on when it is used up.
  * VG_SCHEDULE_STATIC launches one thread per ray instead. The options live in the context;
- * the autotuners time every (waves per CU, refill) pair on the caller's workload and keep
- * the fastest. A world's persistent launches share one counter: launches on one world must
+ * the autotuners time every (waves per CU, refill[, stripes]) candidate on the caller's
+ * workload and keep the fastest. A world's persistent launches share one counter: launches on one world must
  * not run concurrently (use one stream per world, or synchronize). */
 #define VG_SCHEDULE_PERSISTENT 0
 #define VG_SCHEDULE_STATIC 1
@@ -324,7 +326,9 @@ enum {
     VG_OPT_CU_COUNT = 6, /* read only */
     VG_OPT_PROBE_WAVES_PER_CU = 7, /* probe path kernels (vg_probes_update*) */
     VG_OPT_PROBE_REFILL = 8,
-    VG_OPT_PROBE_CHUNK = 9 /* probe path work ids taken per atomic: 0 (what a refill lacks) or 16..1024 */
+    VG_OPT_PROBE_CHUNK = 9, /* probe path work ids taken per atomic: 0 (what a refill lacks) or 16..1024 */
+    VG_OPT_FRAME_STRIPES = 10 /* persistent frame kernel: contiguous stripes of 8x8 blocks, each
+                                 with its own work counter (1..64) */
 };
 int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
 int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);

@@ -180,7 +180,8 @@ def test_kat_roof_shadow_and_empty(ctx):
     dw.close()
 
 
-@pytest.mark.parametrize("schedule", ["static", (8, 1), (16, 32), (32, 64), (4, 16)])
+@pytest.mark.parametrize("schedule", ["static", (8, 1), (16, 32), (32, 64), (4, 16),
+                                      (16, 4, 1), (12, 8, 7), (20, 16, 64)])
 def test_schedules_agree(ctx, small, small_world, schedule):
     # the one-thread-per-ray kernels and the persistent ray-compacting kernels at several
     # (waves per CU, refill threshold) settings return the same bytes as the oracle
@@ -192,6 +193,9 @@ def test_schedules_agree(ctx, small, small_world, schedule):
         for o, v in ((L.VG_OPT_FRAME_WAVES_PER_CU, schedule[0]), (L.VG_OPT_FRAME_REFILL, schedule[1]),
                      (L.VG_OPT_RAYS_WAVES_PER_CU, schedule[0]), (L.VG_OPT_RAYS_REFILL, schedule[1])):
             ctx.set_option(o, v)
+        # work stripes of the frame kernel (1: one counter; 64: more stripes than the small
+        # frame's rows of blocks hold evenly)
+        ctx.set_option(L.VG_OPT_FRAME_STRIPES, schedule[2] if len(schedule) > 2 else 4)
     try:
         wl = workloads.SMALL
         _, cam = wl.camera()
@@ -215,6 +219,7 @@ def test_schedules_agree(ctx, small, small_world, schedule):
             ctx.set_option(o, 16)
         for o in (L.VG_OPT_FRAME_REFILL, L.VG_OPT_RAYS_REFILL):
             ctx.set_option(o, 32)
+        ctx.set_option(L.VG_OPT_FRAME_STRIPES, 4)
 
 
 def test_autotune_keeps_results(ctx, small, small_world):

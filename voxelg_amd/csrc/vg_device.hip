@@ -261,9 +261,47 @@ __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams ca
 }
 
 
+// Striped work counters of the persistent frame kernel: stripe c's counter is work[c *
+// kStripeWords], one 128-byte line each.
+constexpr uint32_t kStripeWords = 32;
+constexpr uint32_t kMaxStripes = 64;
+constexpr size_t kWorkBytes = kMaxStripes * kStripeWords * sizeof(uint32_t);
+
+// Ids [base, lim) of stripe sc for `need` idle lanes (one atomic on the stripe's counter); a
+// wave whose stripe is used up moves on to the next stripe, and base = ~0u once it has found
+// every stripe used up. Wave-uniform: every lane calls it together. (The counters are the
+// only shared state: flags read with agent-scope atomic loads beside them cost the kernel
+// ~30 % on this part, measured.)
+__device__ __forceinline__ void stripe_take(uint32_t *work, uint32_t stripes, uint32_t nblk,
+                                            uint32_t sblk, uint32_t need, uint32_t lane,
+                                            uint32_t &sc, uint32_t &base, uint32_t &lim) {
+    // stripe c: blocks [c sblk, min((c + 1) sblk, nblk)), sblk = ceil(nblk / stripes)
+    base = ~0u;
+    for (uint32_t tries = 0; tries < stripes; ++tries) {
+        const uint32_t b0 = sc * sblk < nblk ? sc * sblk : nblk;
+        const uint32_t b1 = b0 + sblk < nblk ? b0 + sblk : nblk;
+        const uint32_t s0 = b0 * 64u, s1 = b1 * 64u;
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(work + sc * kStripeWords, need);
+        // wave-uniform from here on: scalar registers and branches
+        b = __builtin_amdgcn_readfirstlane(__shfl(b, 0, 64));
+        if (b < s1 - s0) {
+            base = s0 + b;
+            lim = s1 - base < need ? s1 : base + need;
+            return;
+        }
+        sc = sc + 1u < stripes ? sc + 1u : 0u;
+    }
+}
+
+// Zero the striped counters (one thread per stripe).
+__device__ __forceinline__ void stripe_reset(uint32_t *work, uint32_t t) {
+    if (t < kMaxStripes) work[t * kStripeWords] = 0u;
+}
+
 // Beam pre-pass of the persistent frame: one wave per 8x8 pixel block writes the block's
-// exact start t (beam_start). Its first thread also zeroes the frame's work counter, which
-// the persistent kernel enqueued after it on the same stream consumes.
+// exact start t (beam_start). Its first workgroup also zeroes the frame's striped work
+// counters, which the persistent kernel enqueued after it on the same stream consumes.
 // With a block list (list / list_n on the device), the waves take the listed blocks only.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint32_t nbx,
@@ -272,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint
                                                  const uint32_t *__restrict__ list,
                                                  const uint32_t *list_n,
                                                  unsigned long long *counts) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *work = 0u;
+    if (blockIdx.x == 0) stripe_reset(work, threadIdx.x);
     const uint32_t wi = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (wi >= (list ? *list_n : nblocks)) return; // whole waves leave together
     const uint32_t blk = list ? list[wi] : wi;
@@ -292,6 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint
 
 __global__ void k_reset_work(uint32_t *work) { *work = 0u; }
 
+__global__ void k_reset_stripes(uint32_t *work) { stripe_reset(work, threadIdx.x); }
+
 // Persistent, ray-compacting frame kernel. Every wave keeps its 64 lanes busy: whenever
 // fewer than `threshold` lanes hold a live ray, the wave takes as many new primary rays as
 // it has idle lanes from one global counter (one atomic per wave) and hands them out by
@@ -303,11 +343,19 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     DevWorld w, CamParams cam, int32_t width, int32_t height, uint32_t nbx, uint32_t total_ids,
     float ptmax, uint32_t pflags, float sx, float sy, float sz, float stmax,
     const float *__restrict__ beam_t, vg_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
-    uint32_t *work, int threshold, const uint32_t *__restrict__ list, const uint32_t *list_n,
-    unsigned long long *counts) {
+    uint32_t *work, int threshold, uint32_t stripes, const uint32_t *__restrict__ list,
+    const uint32_t *list_n, unsigned long long *counts) {
     // with a block list, ray ids run over the listed 8x8 blocks only
     const uint32_t lane = threadIdx.x & 63;
     if (list) total_ids = *list_n * 64u;
+    // striped work queue: the 8x8 blocks are cut into `stripes` contiguous stripes, each with
+    // its own counter; a wave starts on a stripe of its XCD (workgroups go round-robin over
+    // the 8 XCDs) and moves on to the next open stripe when its own is used up
+    const uint32_t nblk = total_ids / 64u;
+    const uint32_t sblk = (nblk + stripes - 1u) / stripes;
+    const uint32_t per_xcd = stripes >= 8u ? stripes / 8u : 1u;
+    uint32_t sc = stripes >= 8u ? (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u) % per_xcd
+                                : blockIdx.x % stripes;
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
 #pragma unroll
@@ -329,14 +377,13 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
         uint64_t act = __ballot(kind != 0);
         int nact = __popcll(act);
         if (!exhausted && nact < threshold) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(work, (uint32_t)(64 - nact));
-            base = __shfl(base, 0, 64);
-            if (base >= total_ids) {
+            uint32_t base = ~0u, lim = 0u; // ids [base, lim) for the idle lanes
+            stripe_take(work, stripes, nblk, sblk, 64u - (uint32_t)nact, lane, sc, base, lim);
+            if (base == ~0u) {
                 exhausted = true;
             } else if (kind == 0) {
                 const uint32_t id = base + lanes_below(~act);
-                if (id < total_ids) {
+                if (id < lim) {
                     const uint32_t blk = list ? list[id >> 6] : id >> 6, qd = id & 63u;
                     px = (int32_t)((blk % nbx) * 8 + (qd & 7));
                     py = (int32_t)((blk / nbx) * 8 + (qd >> 3));
@@ -818,16 +865,17 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
             hipLaunchKernelGGL((k_beam<false>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
                                (uint32_t)nblocks, ptmax, beam_t, work, list, list_n, cd);
     } else {
-        hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, work);
+        hipLaunchKernelGGL(k_reset_stripes, dim3(1), dim3(kMaxStripes), 0, stream, work);
     }
     HIP_TRY(hipGetLastError());
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
     const uint32_t total = (uint32_t)(nblocks * 64);
     const int thr = w->ctx->frame_refill;
+    const uint32_t stripes = (uint32_t)w->ctx->frame_stripes;
 #define LAUNCH_P(C, SH, B)                                                                     \
     hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, cp, width, \
                        height, nbx, total, ptmax, pflags, s[0], s[1], s[2], stmax, beam_t,     \
-                       hits, occluded, work, thr, list, list_n, cd)
+                       hits, occluded, work, thr, stripes, list, list_n, cd)
     if (count) {
         if (shadow) {
             if (beam) LAUNCH_P(true, true, true);
@@ -869,12 +917,12 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
 }
 
 // Scratch of the no-allocation frame entry point (vg_trace_frame_noalloc): one array per
-// device, `scratch`, holding the ray counter at offset 0 and the beam start t of up to
-// kNoAllocBlocks 8x8 pixel blocks from offset 256. Kernels get its device address, taken once
+// device, `scratch`, holding the striped ray counters at offset 0 and the beam start t of up
+// to kNoAllocBlocks 8x8 pixel blocks from offset kWorkBytes. Kernels get its device address, taken once
 // per device with hipGetSymbolAddress (the name itself is a host-side stand-in). It keeps the
 // last call's contents; the first kernel of every call resets the counter.
 constexpr size_t kNoAllocBlocks = 512 * 512; // 8x8 blocks of a 4096x4096 image
-constexpr size_t kNoAllocBytes = 256 + kNoAllocBlocks * sizeof(float);
+constexpr size_t kNoAllocBytes = kWorkBytes + kNoAllocBlocks * sizeof(float);
 constexpr int kMaxDevices = 64;
 static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
@@ -901,7 +949,7 @@ static int noalloc_scratch(int device, NoAllocScratch *out) {
     }
     uint8_t *base = static_cast<uint8_t *>(g_scratch_ptr[device]);
     out->work = reinterpret_cast<uint32_t *>(base);
-    out->beam_t = reinterpret_cast<float *>(base + 256);
+    out->beam_t = reinterpret_cast<float *>(base + kWorkBytes);
     return VG_OK;
 }
 
@@ -1028,6 +1076,7 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->schedule = VG_SCHEDULE_PERSISTENT;
     c->frame_wpc = 16;
     c->frame_refill = 32;
+    c->frame_stripes = 4;
     c->rays_wpc = 16;
     c->rays_refill = 32;
     c->probe_wpc = 8;
@@ -1072,7 +1121,7 @@ int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
         {(void **)&w->mat, (size_t)d.n_bricks * 64},
         {(void **)&w->scratch, 256},
         {(void **)&w->counts_dev, VG_IT_COUNT * sizeof(unsigned long long)},
-        {(void **)&w->work, 256},
+        {(void **)&w->work, kWorkBytes},
         {(void **)&w->beam_t, (size_t)kBeamDefaultBlocks * sizeof(float)},
     };
     for (auto &a : allocs) {
@@ -1511,6 +1560,11 @@ int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
          : option == VG_OPT_RAYS_REFILL ? ctx->rays_refill
                                         : ctx->probe_refill) = value;
         return VG_OK;
+    case VG_OPT_FRAME_STRIPES:
+        if (value < 1 || value > (int)kMaxStripes)
+            return fail(VG_E_INVALID, "frame stripes %d not in 1..%u", value, kMaxStripes);
+        ctx->frame_stripes = value;
+        return VG_OK;
     case VG_OPT_PROBE_CHUNK:
         if (value != 0 && (value < 16 || value > 1024))
             return fail(VG_E_INVALID, "probe id chunk %d: 0 or 16..1024", value);
@@ -1527,6 +1581,7 @@ int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
     case VG_OPT_SCHEDULE: *value = ctx->schedule; return VG_OK;
     case VG_OPT_FRAME_WAVES_PER_CU: *value = ctx->frame_wpc; return VG_OK;
     case VG_OPT_FRAME_REFILL: *value = ctx->frame_refill; return VG_OK;
+    case VG_OPT_FRAME_STRIPES: *value = ctx->frame_stripes; return VG_OK;
     case VG_OPT_RAYS_WAVES_PER_CU: *value = ctx->rays_wpc; return VG_OK;
     case VG_OPT_RAYS_REFILL: *value = ctx->rays_refill; return VG_OK;
     case VG_OPT_PROBE_WAVES_PER_CU: *value = ctx->probe_wpc; return VG_OK;
@@ -1543,10 +1598,19 @@ int vg_autotune_frame(vg_world *w, const vg_camera *camera, int32_t width, int32
     int st = check_world(w);
     if (st) return st;
     vg_ctx *c = w->ctx;
-    return autotune(c, &c->frame_wpc, &c->frame_refill, [&](hipStream_t s) {
-        return vg_trace_frame(w, camera, width, height, primary_tmax, primary_flags, sun,
-                              shadow_tmax, hits, occluded, (void *)s);
-    }, best_ms);
+    // the frame kernel: waves per CU x refill threshold x work stripes (the coherent frame
+    // refills best from a few lanes, measured; wider refill thresholds only lose)
+    static const int kWpc[] = {8, 12, 16, 20, 24};
+    static const int kRefill[] = {4, 8, 16, 32};
+    static const int kStripes[] = {1, 2, 4, 8, 16};
+    return autotune_lists(c, &c->frame_wpc, TuneList{kWpc, 5}, &c->frame_refill,
+                          TuneList{kRefill, 4}, &c->frame_stripes, TuneList{kStripes, 5},
+                          [&](hipStream_t s) {
+                              return vg_trace_frame(w, camera, width, height, primary_tmax,
+                                                    primary_flags, sun, shadow_tmax, hits,
+                                                    occluded, (void *)s);
+                          },
+                          best_ms);
 }
 
 int vg_autotune_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,

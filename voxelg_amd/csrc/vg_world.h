@@ -11,6 +11,7 @@ struct vg_ctx {
     int schedule;        // VG_SCHEDULE_*
     int frame_wpc;       // persistent frame kernel: waves per CU
     int frame_refill;    // ... refill when fewer lanes of a wave are busy
+    int frame_stripes;   // ... work stripes, each with its own counter (1..64)
     int rays_wpc;        // persistent batched kernel: waves per CU
     int rays_refill;
     int probe_wpc;       // persistent probe path kernel: waves per CU
@@ -79,14 +80,20 @@ inline bool host_normalize(const float in[3], float out[3]) {
     return !(out[0] == 0.0f && out[1] == 0.0f && out[2] == 0.0f);
 }
 
-// With chunk_opt, the work ids taken per atomic are a third dimension of the sweep.
+// Time `launch` (bound to its arguments) over every (waves per CU, refill threshold[, third
+// option]) of the candidate lists and keep the fastest in the options: each candidate runs
+// once to warm up, then `reps` times between events on one stream; its score is the fastest
+// repetition. With a third option (work ids per atomic of the probe kernels, work stripes of
+// the frame kernel) its candidates form a third dimension of the sweep.
+struct TuneList {
+    const int *v;
+    int n;
+};
+
 template <class L>
-inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms,
-                    int *chunk_opt = nullptr) {
-    static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
-    static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
-    static const int kChunk[] = {0, 16, 32, 64, 128, 256};
-    const int n_chunk = chunk_opt ? 6 : 1;
+inline int autotune_lists(vg_ctx *c, int *wpc_opt, TuneList wl, int *refill_opt, TuneList rl,
+                          int *third_opt, TuneList tl, L launch, float *best_ms) {
+    const int n_third = third_opt ? tl.n : 1;
     const int reps = 3;
     hipStream_t stream;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -96,12 +103,13 @@ inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *b
     const int old_schedule = c->schedule;
     c->schedule = VG_SCHEDULE_PERSISTENT;
     float best = INFINITY;
-    int bw = *wpc_opt, br = *refill_opt, bc = chunk_opt ? *chunk_opt : 0, st = VG_OK;
-    for (int ci = 0; ci < n_chunk * 49 && st == VG_OK; ++ci) {
-        const int wpc = kWpc[ci % 7], rf = kRefill[(ci / 7) % 7];
+    int bw = *wpc_opt, br = *refill_opt, bt = third_opt ? *third_opt : 0, st = VG_OK;
+    const int total = wl.n * rl.n * n_third;
+    for (int ci = 0; ci < total && st == VG_OK; ++ci) {
+        const int wpc = wl.v[ci % wl.n], rf = rl.v[(ci / wl.n) % rl.n];
         *wpc_opt = wpc;
         *refill_opt = rf;
-        if (chunk_opt) *chunk_opt = kChunk[ci / 49];
+        if (third_opt) *third_opt = tl.v[ci / (wl.n * rl.n)];
         st = launch(stream);
         if (st) break;
         float ms_min = INFINITY;
@@ -119,12 +127,12 @@ inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *b
             best = ms_min;
             bw = wpc;
             br = rf;
-            if (chunk_opt) bc = *chunk_opt;
+            if (third_opt) bt = *third_opt;
         }
     }
     *wpc_opt = bw;
     *refill_opt = br;
-    if (chunk_opt) *chunk_opt = bc;
+    if (third_opt) *third_opt = bt;
     c->schedule = old_schedule;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -132,6 +140,18 @@ inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *b
     if (st) return st;
     if (best_ms) *best_ms = best;
     return VG_OK;
+}
+
+// The batched-query and probe sweeps: waves per CU {4..32} x refill {4..48}, and with chunk_opt
+// the probe kernels' work ids per atomic {0, 16..256}.
+template <class L>
+inline int autotune(vg_ctx *c, int *wpc_opt, int *refill_opt, L launch, float *best_ms,
+                    int *chunk_opt = nullptr) {
+    static const int kWpc[] = {4, 8, 12, 16, 20, 24, 32};
+    static const int kRefill[] = {4, 8, 12, 16, 24, 32, 48};
+    static const int kChunk[] = {0, 16, 32, 64, 128, 256};
+    return autotune_lists(c, wpc_opt, TuneList{kWpc, 7}, refill_opt, TuneList{kRefill, 7},
+                          chunk_opt, TuneList{kChunk, 6}, launch, best_ms);
 }
 
 
