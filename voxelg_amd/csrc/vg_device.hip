@@ -265,7 +265,25 @@ __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams ca
 // kStripeWords], one 128-byte line each.
 constexpr uint32_t kStripeWords = 32;
 constexpr uint32_t kMaxStripes = 64;
-constexpr size_t kWorkBytes = kMaxStripes * kStripeWords * sizeof(uint32_t);
+constexpr size_t kStripeBytes = kMaxStripes * kStripeWords * sizeof(uint32_t);
+
+// Parameters of a frame launch that only the persistent kernel's refills and shadow-ray
+// set-ups read. The launch's first kernel (k_beam or k_frame_setup) writes them from its own
+// arguments to device memory after the counters, and the persistent kernel loads them where
+// it needs them, so they do not hold scalar registers through the walk loop (held there,
+// they spilled the world's pointers into vector lanes, reloaded on every pyramid fetch).
+struct FrameParams {
+    CamParams cam;
+    float sun[3];
+    int32_t width, height;
+    uint32_t nbx;
+};
+constexpr size_t kWorkBytes = kStripeBytes + 256; // counters, then the FrameParams
+static_assert(sizeof(FrameParams) <= 256, "FrameParams must fit its slot");
+
+__device__ __forceinline__ FrameParams *frame_params(uint32_t *work) {
+    return reinterpret_cast<FrameParams *>(reinterpret_cast<uint8_t *>(work) + kStripeBytes);
+}
 
 // Ids [base, lim) of stripe sc for `need` idle lanes (one atomic on the stripe's counter); a
 // wave whose stripe is used up moves on to the next stripe, and base = ~0u once it has found
@@ -304,13 +322,18 @@ __device__ __forceinline__ void stripe_reset(uint32_t *work, uint32_t t) {
 // counters, which the persistent kernel enqueued after it on the same stream consumes.
 // With a block list (list / list_n on the device), the waves take the listed blocks only.
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint32_t nbx,
-                                                 uint32_t nblocks, float ptmax,
-                                                 float *__restrict__ beam_t, uint32_t *work,
+__global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, FrameParams fp, uint32_t nblocks,
+                                                 float ptmax, float *__restrict__ beam_t,
+                                                 uint32_t *work,
                                                  const uint32_t *__restrict__ list,
                                                  const uint32_t *list_n,
                                                  unsigned long long *counts) {
-    if (blockIdx.x == 0) stripe_reset(work, threadIdx.x);
+    if (blockIdx.x == 0) {
+        stripe_reset(work, threadIdx.x);
+        if (threadIdx.x == 0) *frame_params(work) = fp;
+    }
+    const CamParams &cam = fp.cam;
+    const uint32_t nbx = fp.nbx;
     const uint32_t wi = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (wi >= (list ? *list_n : nblocks)) return; // whole waves leave together
     const uint32_t blk = list ? list[wi] : wi;
@@ -330,7 +353,11 @@ __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, CamParams cam, uint
 
 __global__ void k_reset_work(uint32_t *work) { *work = 0u; }
 
-__global__ void k_reset_stripes(uint32_t *work) { stripe_reset(work, threadIdx.x); }
+// The frame launch's first kernel without the beam: counters and FrameParams.
+__global__ void k_frame_setup(uint32_t *work, FrameParams fp) {
+    stripe_reset(work, threadIdx.x);
+    if (threadIdx.x == 0) *frame_params(work) = fp;
+}
 
 // Persistent, ray-compacting frame kernel. Every wave keeps its 64 lanes busy: whenever
 // fewer than `threshold` lanes hold a live ray, the wave takes as many new primary rays as
@@ -340,11 +367,11 @@ __global__ void k_reset_stripes(uint32_t *work) { stripe_reset(work, threadIdx.x
 // (waves per CU chosen by the autotuner), not to the image.
 template <bool COUNT, bool SHADOW, bool BEAM>
 __global__ __launch_bounds__(kBlock) void k_frame_persistent(
-    DevWorld w, CamParams cam, int32_t width, int32_t height, uint32_t nbx, uint32_t total_ids,
-    float ptmax, uint32_t pflags, float sx, float sy, float sz, float stmax,
+    DevWorld w, uint32_t total_ids, float ptmax, uint32_t pflags, float stmax,
     const float *__restrict__ beam_t, vg_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
     uint32_t *work, int threshold, uint32_t stripes, const uint32_t *__restrict__ list,
     const uint32_t *list_n, unsigned long long *counts) {
+    const FrameParams *__restrict__ fpp = frame_params(work);
     // with a block list, ray ids run over the listed 8x8 blocks only
     const uint32_t lane = threadIdx.x & 63;
     if (list) total_ids = *list_n * 64u;
@@ -384,11 +411,14 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
             } else if (kind == 0) {
                 const uint32_t id = base + lanes_below(~act);
                 if (id < lim) {
+                    const uint32_t nbx = fpp->nbx;
+                    const int32_t width = fpp->width, height = fpp->height;
                     const uint32_t blk = list ? list[id >> 6] : id >> 6, qd = id & 63u;
                     px = (int32_t)((blk % nbx) * 8 + (qd & 7));
                     py = (int32_t)((blk / nbx) * 8 + (qd >> 3));
                     if (px < width && py < height) {
                         pix = (int64_t)py * width + px;
+                        const CamParams cam = fpp->cam;
                         float d[3];
                         primary_dir(cam, px, py, d);
                         hit_miss(h);
@@ -429,7 +459,8 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                         if (SHADOW) {
                             bool started = false;
                             if (h.hit) {
-                                float d[3], so[3], sd[3] = {sx, sy, sz};
+                                const CamParams cam = fpp->cam;
+                                float d[3], so[3], sd[3] = {fpp->sun[0], fpp->sun[1], fpp->sun[2]};
                                 primary_dir(cam, px, py, d);
                                 shadow_origin(cam.origin, d, h, so);
                                 hit_miss(h);
@@ -856,16 +887,22 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
     const uint64_t nblocks = (uint64_t)nbx * nby;
     unsigned long long *cd = count ? w->counts_dev : nullptr;
+    FrameParams fp;
+    fp.cam = cp;
+    for (int k = 0; k < 3; ++k) fp.sun[k] = s[k];
+    fp.width = width;
+    fp.height = height;
+    fp.nbx = nbx;
     if (beam) {
         dim3 g((uint32_t)((nblocks + 3) / 4));
         if (count)
-            hipLaunchKernelGGL((k_beam<true>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
+            hipLaunchKernelGGL((k_beam<true>), g, dim3(kBlock), 0, stream, w->dw, fp,
                                (uint32_t)nblocks, ptmax, beam_t, work, list, list_n, cd);
         else
-            hipLaunchKernelGGL((k_beam<false>), g, dim3(kBlock), 0, stream, w->dw, cp, nbx,
+            hipLaunchKernelGGL((k_beam<false>), g, dim3(kBlock), 0, stream, w->dw, fp,
                                (uint32_t)nblocks, ptmax, beam_t, work, list, list_n, cd);
     } else {
-        hipLaunchKernelGGL(k_reset_stripes, dim3(1), dim3(kMaxStripes), 0, stream, work);
+        hipLaunchKernelGGL(k_frame_setup, dim3(1), dim3(kMaxStripes), 0, stream, work, fp);
     }
     HIP_TRY(hipGetLastError());
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
@@ -873,9 +910,9 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     const int thr = w->ctx->frame_refill;
     const uint32_t stripes = (uint32_t)w->ctx->frame_stripes;
 #define LAUNCH_P(C, SH, B)                                                                     \
-    hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, cp, width, \
-                       height, nbx, total, ptmax, pflags, s[0], s[1], s[2], stmax, beam_t,     \
-                       hits, occluded, work, thr, stripes, list, list_n, cd)
+    hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, total,     \
+                       ptmax, pflags, stmax, beam_t, hits, occluded, work, thr, stripes, list,  \
+                       list_n, cd)
     if (count) {
         if (shadow) {
             if (beam) LAUNCH_P(true, true, true);
@@ -917,10 +954,11 @@ static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t wid
 }
 
 // Scratch of the no-allocation frame entry point (vg_trace_frame_noalloc): one array per
-// device, `scratch`, holding the striped ray counters at offset 0 and the beam start t of up
-// to kNoAllocBlocks 8x8 pixel blocks from offset kWorkBytes. Kernels get its device address, taken once
-// per device with hipGetSymbolAddress (the name itself is a host-side stand-in). It keeps the
-// last call's contents; the first kernel of every call resets the counter.
+// device, `scratch`, holding the striped ray counters and the launch parameters (kWorkBytes)
+// and from there the beam start t of up to kNoAllocBlocks 8x8 pixel blocks. Kernels get its
+// device address, taken once per device with hipGetSymbolAddress (the name itself is a
+// host-side stand-in). It keeps the last call's contents; the first kernel of every call
+// resets the counters and writes the parameters.
 constexpr size_t kNoAllocBlocks = 512 * 512; // 8x8 blocks of a 4096x4096 image
 constexpr size_t kNoAllocBytes = kWorkBytes + kNoAllocBlocks * sizeof(float);
 constexpr int kMaxDevices = 64;
