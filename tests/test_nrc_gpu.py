@@ -255,3 +255,55 @@ def test_online_training_gains_over_no_cache(ctx, small, small_world):
           f"untrained cache {e_untrained:.4f} (last loss {loss:.5f})", flush=True)
     assert e_trained < 0.75 * e_ambient
     assert e_trained < 0.75 * e_untrained
+
+
+def test_learned_update_error_config1_world(ctx, small, small_world):
+    """Config 4's update on the config-1 world (8x4x8 probes at spacing 32, 32 gather rays per
+    probe, the cache trained online on 65 536 samples per update), against the FP64 4096-ray
+    2-bounce estimate. The 2 % limit is a property of the ray count here: the plain update
+    tracing both bounces with the same 32 directions is 2.9 % off (64 rays: 1.9 %), so the
+    32-ray quadrature alone exceeds it on this world (on the 2048x256x2048 world bench.py
+    gates both updates at 2 %). What the cache must do is lose nothing against tracing the
+    bounce it replaces, and beat the ambient tail by far; plain 256 rays meets 2 %."""
+    p, dirs, bt, pal = probe_setup(small)
+    ow = oracle.OracleWorld.view(small_world)
+    pg = gi.ProbeGrid(small, p)
+    total = pg.total
+    slots = np.arange(total)
+    origins = gi.probe_origins(np.zeros(total, gi.PROBE_DTYPE), p, slots)
+    pref = p.copy()
+    pref["rays_per_probe"] = 4096
+    ref = ow.gi_reference(pref, gi.directions(4096), bt, pal, slots, origins, threads=THREADS)
+
+    def err(grid):
+        rec = grid.download()
+        live = rec["info"][:, 3] == 1
+        assert live.sum() > 100
+        return float(gi.irradiance_error(gi.sh_coefficients(rec)[live], ref[live]).mean())
+
+    cache = nrc.RadianceCache(ctx)  # default configuration: 65 536 samples per update
+    assert cache.batch == 65536
+    case("learned update: 150 online-training updates")
+    for r in range(150):
+        p["round"] = r
+        pg.update_learned(cache, p, train=True)
+    p["round"] = 7000
+    pg.reset().update_learned(cache, p, train=True)
+    ctx.synchronize()
+    e_learned = err(pg)
+
+    def plain(rays, bounces):
+        q = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=rays, epochs=1,
+                         update_div=1, bounces=bounces, flags=1)
+        q["round"] = 7000
+        g = gi.ProbeGrid(small, q)
+        g.update(q)
+        ctx.synchronize()
+        return err(g)
+
+    e32, e_ambient, e256 = plain(32, 2), plain(32, 1), plain(256, 2)
+    print(f"config-1 world: learned 32-ray {e_learned:.4f}, plain 32-ray 2-bounce {e32:.4f}, "
+          f"32-ray ambient tail {e_ambient:.4f}, plain 256-ray {e256:.4f}", flush=True)
+    assert e_learned <= 1.02 * e32, (e_learned, e32)
+    assert e_learned < 0.5 * e_ambient
+    assert e256 <= 0.02
