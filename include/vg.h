@@ -359,9 +359,13 @@ int vg_autotune_batches(vg_world *world, const vg_batch *batches, const uint32_t
                         uint32_t n_batches, uint32_t total, float *best_ms);
 
 /* ----------------------------------------------------------- HIP graphs -- */
-/* Capture the launches that vg_trace_* calls enqueue on `stream` (not the null stream)
- * between begin and end into an executable graph, then replay it with one call. Nothing in
- * a captured sequence allocates or memsets: counters are reset by kernels. */
+/* Capture the launches that vg_trace_*, vg_probes_update and vg_probes_update_learned calls
+ * enqueue on `stream` (not the null stream) between begin and end into an executable graph,
+ * then replay it with one call. Nothing in a captured sequence allocates or memsets:
+ * counters are reset by kernels, and per-call alternations (double-buffered lists) follow
+ * device-side counts. A learned update with VG_NRC_TRAIN captured this way joins its
+ * training step into the graph (no overlap with the next replay); join the cache
+ * (vg_nrc_join) and synchronise before beginning such a capture. */
 typedef struct vg_graph vg_graph;
 int vg_graph_begin(vg_ctx *ctx, void *stream);
 int vg_graph_end(vg_ctx *ctx, void *stream, vg_graph **out);

@@ -307,3 +307,75 @@ def test_learned_update_error_config1_world(ctx, small, small_world):
     assert e_learned <= 1.02 * e32, (e_learned, e32)
     assert e_learned < 0.5 * e_ambient
     assert e256 <= 0.02
+
+
+def test_probe_updates_replay_from_graphs(ctx, small, small_world):
+    """Plain and learned probe updates captured into HIP graphs: every replay (eager work
+    between them) gives the records of the same update run eagerly from the same state; a
+    captured training update trains the cache once per replay, its records coming from the
+    parameters before its own step."""
+    from voxelg_amd.device import Graph
+    p, dirs, bt, pal = probe_setup(small)
+    p["round"] = 3
+    rng = np.random.default_rng(21)
+    stream = ctx.stream()
+    other = oracle.make_rays(rng.uniform(0, 200, (20000, 3)), rng.normal(size=(20000, 3)))
+
+    def same_records(a, b, what):
+        assert a.tobytes() == b.tobytes(), what
+
+    # plain 2-bounce update
+    pe, pgr = gi.ProbeGrid(small, p), gi.ProbeGrid(small, p)
+    gp = Graph(ctx, stream, lambda s: pgr.update(p, stream=s))
+    for rep in range(3):
+        case(f"plain update graph replay {rep}")
+        pe.reset()
+        pgr.reset()
+        ctx.synchronize()
+        pe.update(p)
+        small.trace_rays(ctx.rays(other))  # eager work between replays
+        ctx.synchronize()
+        gp.launch(stream)
+        stream.synchronize()
+        same_records(pgr.download(), pe.download(), f"plain replay {rep}")
+    # learned, inference only
+    cache = nrc.RadianceCache(ctx, nrc.default_config(batch=4096))
+    mlp, grid = rand_params(rng, cache, 0.05, 0.05)
+    cache.set_params(mlp, grid)
+    le, lg = gi.ProbeGrid(small, p), gi.ProbeGrid(small, p)
+    gl = Graph(ctx, stream, lambda s: lg.update_learned(cache, p, train=False, stream=s))
+    for rep in range(3):
+        case(f"learned update graph replay {rep}")
+        le.reset()
+        lg.reset()
+        ctx.synchronize()
+        le.update_learned(cache, p, train=False)
+        small.trace_rays(ctx.rays(other))
+        ctx.synchronize()
+        gl.launch(stream)
+        stream.synchronize()
+        same_records(lg.download(), le.download(), f"learned replay {rep}")
+    # learned with training: the step is joined into the graph
+    train_cache = nrc.RadianceCache(ctx, nrc.default_config(batch=4096))
+    train_cache.set_params(mlp, grid)
+    twin_cache = nrc.RadianceCache(ctx, nrc.default_config(batch=4096))
+    lt, twin = gi.ProbeGrid(small, p), gi.ProbeGrid(small, p)
+    lt.update_learned(train_cache, p, train=True)  # a first hit list to draw targets from
+    train_cache.join()
+    ctx.synchronize()
+    steps0 = train_cache.stats()[0]
+    gt = Graph(ctx, stream, lambda s: lt.update_learned(train_cache, p, train=True, stream=s))
+    for rep in range(3):
+        case(f"training update graph replay {rep}")
+        m0, g0 = train_cache.params()
+        twin_cache.set_params(m0, g0)
+        twin.upload(lt.download())
+        ctx.synchronize()
+        gt.launch(stream)
+        stream.synchronize()
+        twin.update_learned(twin_cache, p, train=False)
+        ctx.synchronize()
+        same_records(lt.download(), twin.download(), f"training replay {rep}")
+        m1, g1 = train_cache.params()
+        assert train_cache.stats()[0] == steps0 + rep + 1
+        assert np.isfinite(m1).all() and np.isfinite(g1).all() and not np.array_equal(g1, g0)

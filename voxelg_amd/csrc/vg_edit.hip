@@ -25,7 +25,7 @@
 //           when a box straddles the camera plane).
 #include "vg_world.h"
 
-#include <hipcub/hipcub.hpp>
+#include "vg_scan.h"
 
 #include <algorithm>
 #include <vector>
@@ -699,9 +699,7 @@ static int edit_state(vg_world *w, vg_edit_state **out) {
                      hipMemset(s->cnt, 0, (size_t)d.n_bricks * 4) == hipSuccess;
         for (int i = 0; i < 2 && ok; ++i)
             ok = hipEventCreateWithFlags(&s->staged[i], hipEventDisableTiming) == hipSuccess;
-        size_t tb = 0;
-        if (ok) ok = hipcub::DeviceScan::InclusiveSum(nullptr, tb, s->cnt, s->end, (int)d.n_bricks) ==
-                     hipSuccess;
+        const size_t tb = (size_t)scan_scratch_words((uint32_t)d.n_bricks) * sizeof(uint32_t);
         if (ok) ok = hipMalloc(&s->temp, tb) == hipSuccess;
         if (!ok) {
             free_edit_state(s);
@@ -832,8 +830,8 @@ int vg_world_apply_edits(vg_world *w, const vg_edit *edits, uint32_t n, void *st
     hipLaunchKernelGGL(k_edit_enum<false>, eg, dim3(256), 0, hs, w->dw, s->edits, n, s->cnt,
                        s->end, s->list, s->tile_flag, s->tile_list, s->tile_n);
     HIP_TRY(hipGetLastError());
-    size_t tb = s->temp_bytes;
-    HIP_TRY(hipcub::DeviceScan::InclusiveSum(s->temp, tb, s->cnt, s->end, (int)d.n_bricks, hs));
+    HIP_TRY(scan_u32(s->cnt, s->end, (uint32_t)d.n_bricks, static_cast<uint32_t *>(s->temp), true,
+                     hs));
     hipLaunchKernelGGL(k_edit_enum<true>, eg, dim3(256), 0, hs, w->dw, s->edits, n, s->cnt, s->end,
                        s->list, s->tile_flag, s->tile_list, s->tile_n);
     HIP_TRY(hipGetLastError());

@@ -41,7 +41,6 @@ struct vg_probes {
     uint32_t *heavy;        // sample ids: 2 * hcap
     uint32_t *hmask;        // one bit per sample: 2 * hwords
     uint32_t hcap, hwords;
-    int hpar;               // half the next update writes
     // per-launch timing (vg_probes_set_timing): 5 events per recorded update, bracketing prep,
     // the path kernel, the cache inference and the reduction on the update's stream
     hipEvent_t *tev;        // kTimingCap * 5 events (null: timing off)
@@ -56,10 +55,13 @@ constexpr int kGiCounters = VG_IT_COUNT + 3;
 // Device counters of a probe grid: [0] path work, [1] this update's cache queries, [2] the
 // previous learned update's query count, [3] the query-list half this update writes,
 // [4 + s - 1] work of training step s >= 1 (vg_nrc_config.steps <= 64), [kCtrHeavy] this
-// update's heavy paths, [kCtrHeavy + 1] the previous update's.
+// update's heavy paths, [kCtrHeavy + 1] the previous update's, [kCtrUpd] the number of updates
+// so far (its parity picks the heavy-list halves; the reduction, the update's last launch,
+// counts it up, so no host-side state changes per update and a graph may capture one).
 constexpr int kCtrHeavy = 4 + 64;
+constexpr int kCtrUpd = kCtrHeavy + 2;
 constexpr int kTimingCap = 64; // updates one timing record holds
-constexpr int kGiCtrs = kCtrHeavy + 2;
+constexpr int kGiCtrs = kCtrUpd + 1;
 
 // ----------------------------------------------------------------- helpers --
 __device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
@@ -208,16 +210,17 @@ __device__ __forceinline__ void probe_cell(const vg_gi_params &p, const DevWorld
 __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params p, uint32_t n_upd,
                                                       uint32_t phase, const vg_probe *probes,
                                                       float4 *pwork, uint32_t *ctr, int learned,
-                                                      uint32_t *hmask_cur, uint32_t hwords) {
+                                                      uint32_t *hmask, uint32_t hwords) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j == 0) { // the kernels enqueued next consume the counters
         const uint32_t prev = ctr[1], par = ctr[3], hprev = ctr[kCtrHeavy];
-        for (int k = 0; k < kGiCtrs; ++k) ctr[k] = 0u;
+        for (int k = 0; k < kCtrUpd; ++k) ctr[k] = 0u;
         ctr[2] = learned ? prev : 0u;
         ctr[3] = learned ? (par ^ 1u) : par;
         ctr[kCtrHeavy + 1] = hprev;
     }
     // this update's heavy-path bits start clear
+    uint32_t *hmask_cur = hmask + (size_t)(ctr[kCtrUpd] & 1u) * hwords;
     for (uint32_t k = j; k < hwords; k += gridDim.x * kBlock) hmask_cur[k] = 0u;
     if (j >= n_upd) return;
     const int64_t idx = (int64_t)j * p.update_div + phase;
@@ -261,9 +264,10 @@ struct LearnedArgs {
 constexpr uint32_t kHeavyIters = 24;  // gather + sun shadow paths (about the top 1 %)
 constexpr uint32_t kHeavyIters2 = 48; // 2-bounce paths
 struct HeavyArgs {
-    const uint32_t *prev_ids, *prev_mask;
-    uint32_t *cur_ids, *cur_mask;
+    uint32_t *ids;   // two halves of `cap` ids; half (updates so far) & 1 is this update's
+    uint32_t *mask;  // two halves of `words` words
     uint32_t cap;    // ids each list holds (a longer list is cut: the rest stay in order)
+    uint32_t words;
     uint32_t thresh; // walk iterations that make a path heavy
 };
 
@@ -314,6 +318,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     const uint32_t nprev = BOUNCES == 3 ? ctr[2] : 0u;
     const uint32_t train_n = BOUNCES == 3 ? la.train_n : 0u;
     const uint32_t n_heavy = ctr[kCtrHeavy + 1] < ha.cap ? ctr[kCtrHeavy + 1] : ha.cap;
+    const uint32_t hp = ctr[kCtrUpd] & 1u; // this update's half of the heavy lists
+    uint32_t *cur_ids = ha.ids + (size_t)hp * ha.cap, *cur_mask = ha.mask + (size_t)hp * ha.words;
+    const uint32_t *prev_ids = ha.ids + (size_t)(hp ^ 1u) * ha.cap;
+    const uint32_t *prev_mask = ha.mask + (size_t)(hp ^ 1u) * ha.words;
     const uint32_t total_ids = train_n + n_heavy + n_samples;
     uint32_t cnt[VG_IT_COUNT];
     uint32_t nray[3] = {0, 0, 0};
@@ -417,8 +425,8 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             if (heavy) {
                 const uint32_t k = base + lanes_below(hm);
                 if (k < ha.cap) {
-                    ha.cur_ids[k] = sid;
-                    atomicOr(ha.cur_mask + (sid >> 5), 1u << (sid & 31u));
+                    cur_ids[k] = sid;
+                    atomicOr(cur_mask + (sid >> 5), 1u << (sid & 31u));
                 }
                 heavy = false;
             }
@@ -469,11 +477,11 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     const uint32_t k = sid - train_n;
                     bool take;
                     if (k < n_heavy) {
-                        sid = ha.prev_ids[k];
+                        sid = prev_ids[k];
                         take = sid < n_samples;
                     } else {
                         sid = k - n_heavy;
-                        take = !((ha.prev_mask[sid >> 5] >> (sid & 31u)) & 1u);
+                        take = !((prev_mask[sid >> 5] >> (sid & 31u)) & 1u);
                     }
                     sl.pc[me] = 0u;
                     // the probe's origin and flags and the direction load together
@@ -652,8 +660,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
                                                         const float4 *__restrict__ dirs,
                                                         const float4 *__restrict__ rad,
                                                         const uint8_t *__restrict__ face,
-                                                        vg_probe *probes) {
+                                                        vg_probe *probes, uint32_t *ctr) {
     __shared__ float sums[kBlock / 8][20]; // per group: the 20 projected sums
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[kCtrUpd] += 1u; // the update's last launch
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t gl = lane & (L - 1u);                 // lane within the group
     const uint32_t grp = threadIdx.x / L;                // group within the block
@@ -1104,14 +1113,12 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     if (n_upd == 0) return VG_OK;
     const uint64_t n_samples = (uint64_t)n_upd * (uint64_t)pr->rays;
     unsigned long long *cd = count ? pr->counts : nullptr;
-    // heavy-path lists: this update writes half hp and reads the other (the previous update's)
-    const int hp = pr->hpar;
-    pr->hpar ^= 1;
+    // heavy-path lists: this update writes one half and reads the other (the previous
+    // update's); the device's update count picks them
     HeavyArgs ha;
-    ha.cur_ids = pr->heavy + (size_t)hp * pr->hcap;
-    ha.prev_ids = pr->heavy + (size_t)(hp ^ 1) * pr->hcap;
-    ha.cur_mask = pr->hmask + (size_t)hp * pr->hwords;
-    ha.prev_mask = pr->hmask + (size_t)(hp ^ 1) * pr->hwords;
+    ha.ids = pr->heavy;
+    ha.mask = pr->hmask;
+    ha.words = pr->hwords;
     ha.cap = pr->hcap;
     ha.thresh = (nrc || p->bounces != 2) ? kHeavyIters : kHeavyIters2;
     hipEvent_t *tev = (pr->tev && pr->tn < kTimingCap && !count) ? pr->tev + 5 * pr->tn : nullptr;
@@ -1121,7 +1128,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     }
     hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0,
-                       ha.cur_mask, pr->hwords);
+                       pr->hmask, pr->hwords);
     HIP_TRY(hipGetLastError());
     if (tev) HIP_TRY(hipEventRecord(tev[1], stream));
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
@@ -1175,7 +1182,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     const uint32_t ppb = kBlock / lpp;
     hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + ppb - 1) / ppb), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, epoch, phase, lpp, pr->pwork, pr->dirs, pr->rad,
-                       pr->face, pr->probes);
+                       pr->face, pr->probes, pr->ctr);
     HIP_TRY(hipGetLastError());
     if (tev) HIP_TRY(hipEventRecord(tev[4], stream));
     if (overlap) {
@@ -1183,7 +1190,14 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
         st = nrc_train_launch(nrc, nrc->tq, nrc->tt, (uint32_t)nrc->tcap, true, nrc->tstream);
         if (st) return st;
         HIP_TRY(hipEventRecord(nrc->ev_trained, nrc->tstream));
-        nrc->pending = true;
+        if (nrc_capturing(stream)) {
+            // under graph capture the step is joined back into the captured stream, so the
+            // graph holds the whole update (and nothing outside it waits for the step)
+            HIP_TRY(hipStreamWaitEvent(stream, nrc->ev_trained, 0));
+            nrc->pending = false;
+        } else {
+            nrc->pending = true;
+        }
     } else if (train) {
         for (int s = 0; s < nrc->cfg.steps; ++s) {
             if (s > 0) { // step 0's targets came with the path launch

@@ -68,7 +68,10 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const ui
 int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t n, bool update,
                      hipStream_t stream, float2 *dense = nullptr);
 // Make `stream` wait for the training step still running on the cache's own stream (work
-// that reads or writes the cache parameters must come after it).
+// that reads or writes the cache parameters must come after it). Under graph capture it
+// does not wait for a step enqueued before the capture began.
 int nrc_join(vg_nrc *m, hipStream_t stream);
+// `stream` is being captured into a HIP graph.
+bool nrc_capturing(hipStream_t stream);
 
 } // namespace vg

@@ -16,12 +16,12 @@
 // weight gradient dW = dY X^T as MFMAs over samples; per-block partial sums are reduced in
 // a second kernel (deterministic). The grid gradient: every (sample, level, corner)
 // contribution becomes a record that a counting pass (per-block bucket counts, one exclusive
-// scan) places in its bucket of 4096 entries; one block per bucket sums its records in LDS
+// scan, vg_scan.h) places in its bucket of 4096 entries; one block per bucket sums its records in LDS
 // in 64-bit fixed point, which is exact and therefore independent of the record order, and
 // runs Adam on exactly the entries that received a gradient.
 #include "vg_nrc.h"
 
-#include <hipcub/hipcub.hpp>
+#include "vg_scan.h"
 
 #include <vector>
 
@@ -777,8 +777,17 @@ static int set_dev(const vg_nrc *m) {
     return VG_OK;
 }
 
+bool nrc_capturing(hipStream_t stream) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return stream && hipStreamIsCapturing(stream, &cs) == hipSuccess &&
+           cs == hipStreamCaptureStatusActive;
+}
+
 int nrc_join(vg_nrc *m, hipStream_t stream) {
-    if (m->pending && stream != m->tstream) HIP_TRY(hipStreamWaitEvent(stream, m->ev_trained, 0));
+    // A capture does not wait for a step enqueued before it began (the caller joins and
+    // synchronises first); steps enqueued inside a capture are joined where they are made.
+    if (m->pending && stream != m->tstream && !nrc_capturing(stream))
+        HIP_TRY(hipStreamWaitEvent(stream, m->ev_trained, 0));
     return VG_OK;
 }
 
@@ -828,8 +837,8 @@ int nrc_train_launch(vg_nrc *m, const vg_nrc_query *q, const float4 *t, uint32_t
     hipLaunchKernelGGL(k_nrc_gcount, dim3(kGBlocks), dim3(kGThreads), nb * sizeof(uint32_t), stream, dv, q,
                        reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, m->hist);
     HIP_TRY(hipGetLastError());
-    size_t tb = m->ctemp_bytes;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(m->ctemp, tb, m->hist, hoff, (int)nh, stream));
+    HIP_TRY(scan_u32(m->hist, hoff, (uint32_t)nh, static_cast<uint32_t *>(m->ctemp), false,
+                     stream));
     hipLaunchKernelGGL(k_nrc_gscatter, dim3(kGBlocks), dim3(kGThreads), nb * sizeof(uint32_t), stream, dv,
                        q, reinterpret_cast<const float2 *>(m->dfeat), n, m->gbits, nb, hoff, m->rk,
                        m->rv);
@@ -916,8 +925,7 @@ static int alloc_train(vg_nrc *m, int32_t cap) {
               hipMalloc((void **)&m->rv, nc * sizeof(float2)) == hipSuccess;
     if (!ok) return fail(VG_E_NOMEM, "training buffers for %d samples", cap);
     const size_t nh = (size_t)m->gbuckets * kGBlocks + 1;
-    size_t tb = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, m->hist, m->hist + nh, (int)nh));
+    const size_t tb = (size_t)scan_scratch_words((uint32_t)nh) * sizeof(uint32_t);
     if (hipMalloc(&m->ctemp, tb) != hipSuccess)
         return fail(VG_E_NOMEM, "scan scratch of %zu bytes", tb);
     m->ctemp_bytes = tb;
