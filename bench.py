@@ -209,14 +209,16 @@ def main():
         dw.trace_frame(cam, W, H, wl.primary_tmax, wl.primary_flags, sun, wl.shadow_tmax, hits,
                        occ, stream)
 
-    # schedule autotuning (outside the timed region): waves per CU x refill threshold x work
-    # stripes of the persistent frame kernel, timed on this very frame
+    # schedule autotuning (outside the timed region): pixel-block shape x waves per CU x refill
+    # threshold x work stripes of the persistent frame kernel, timed on this very frame
     from voxelg_amd import _lib as L
     tuned_ms = dw.autotune_frame(cam, W, H, wl.primary_tmax, wl.primary_flags, sun,
                                  wl.shadow_tmax)
     sched = {"waves_per_cu": ctx.get_option(L.VG_OPT_FRAME_WAVES_PER_CU),
              "refill_below": ctx.get_option(L.VG_OPT_FRAME_REFILL),
              "stripes": ctx.get_option(L.VG_OPT_FRAME_STRIPES),
+             "block": [ctx.get_option(L.VG_OPT_FRAME_BLOCK_W),
+                       64 // ctx.get_option(L.VG_OPT_FRAME_BLOCK_W)],
              "cus": ctx.get_option(L.VG_OPT_CU_COUNT)}
     log(f"autotuned persistent frame kernel: {sched} ({tuned_ms:.3f} ms)")
 

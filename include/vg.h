@@ -327,8 +327,11 @@ enum {
     VG_OPT_PROBE_WAVES_PER_CU = 7, /* probe path kernels (vg_probes_update*) */
     VG_OPT_PROBE_REFILL = 8,
     VG_OPT_PROBE_CHUNK = 9, /* probe path work ids taken per atomic: 0 (what a refill lacks) or 16..1024 */
-    VG_OPT_FRAME_STRIPES = 10 /* persistent frame kernel: contiguous stripes of 8x8 blocks, each
-                                 with its own work counter (1..64) */
+    VG_OPT_FRAME_STRIPES = 10, /* persistent frame kernel: contiguous stripes of pixel blocks,
+                                  each with its own work counter (1..64) */
+    VG_OPT_FRAME_BLOCK_W = 11  /* persistent frame kernel: pixel-block width 4, 8, 16 or 32 (the
+                                  block is one wave's 64 rays: height 64 / width; the beam
+                                  prologue marches one box per block) */
 };
 int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
 int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);

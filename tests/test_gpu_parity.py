@@ -181,7 +181,8 @@ def test_kat_roof_shadow_and_empty(ctx):
 
 
 @pytest.mark.parametrize("schedule", ["static", (8, 1), (16, 32), (32, 64), (4, 16),
-                                      (16, 4, 1), (12, 8, 7), (20, 16, 64)])
+                                      (16, 4, 1), (12, 8, 7), (20, 16, 64), (16, 4, 4, 4),
+                                      (16, 8, 2, 16), (12, 4, 8, 32)])
 def test_schedules_agree(ctx, small, small_world, schedule):
     # the one-thread-per-ray kernels and the persistent ray-compacting kernels at several
     # (waves per CU, refill threshold) settings return the same bytes as the oracle
@@ -196,6 +197,8 @@ def test_schedules_agree(ctx, small, small_world, schedule):
         # work stripes of the frame kernel (1: one counter; 64: more stripes than the small
         # frame's rows of blocks hold evenly)
         ctx.set_option(L.VG_OPT_FRAME_STRIPES, schedule[2] if len(schedule) > 2 else 4)
+        # pixel-block shape (width x 64 / width): the beam marches one box per block
+        ctx.set_option(L.VG_OPT_FRAME_BLOCK_W, schedule[3] if len(schedule) > 3 else 8)
     try:
         wl = workloads.SMALL
         _, cam = wl.camera()
@@ -207,7 +210,8 @@ def test_schedules_agree(ctx, small, small_world, schedule):
             cam, wl.width, wl.height, 700.0, flags, sun, 480.0, threads=THREADS)
         assert_same(gpu_hits(hits), ref, f"frame {schedule}")
         assert np.array_equal(occ.numpy(np.uint8), ref_occ)
-        assert c == rc
+        if ctx.get_option(L.VG_OPT_FRAME_BLOCK_W) == 8 or schedule == "static":
+            assert c == rc  # the oracle's beam marches 8x8 blocks
         rays = mixed_rays(np.random.default_rng(9), 100_000, small_world.dims)
         got, c = small.trace_rays_counts(ctx.rays(rays))
         ref, rc = oracle.OracleWorld.view(small_world).trace_rays(rays, 0, THREADS)
@@ -220,6 +224,7 @@ def test_schedules_agree(ctx, small, small_world, schedule):
         for o in (L.VG_OPT_FRAME_REFILL, L.VG_OPT_RAYS_REFILL):
             ctx.set_option(o, 32)
         ctx.set_option(L.VG_OPT_FRAME_STRIPES, 4)
+        ctx.set_option(L.VG_OPT_FRAME_BLOCK_W, 8)
 
 
 def test_autotune_keeps_results(ctx, small, small_world):
@@ -228,18 +233,30 @@ def test_autotune_keeps_results(ctx, small, small_world):
     _, cam = wl.camera()
     sun = workloads.LARGE.sun()
     flags = vg.VG_RAY_LOD | vg.VG_RAY_BEAM
-    ms = small.autotune_frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
-    assert 0.0 < ms < 100.0
-    assert 1 <= ctx.get_option(L.VG_OPT_FRAME_WAVES_PER_CU) <= 64
-    assert 1 <= ctx.get_option(L.VG_OPT_FRAME_REFILL) <= 64
-    assert ctx.get_option(L.VG_OPT_CU_COUNT) >= 1
-    hits, occ = small.trace_frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
-    ref, ref_occ, _ = oracle.OracleWorld.view(small_world).frame(
-        cam, wl.width, wl.height, 700.0, flags, sun, 480.0, threads=THREADS)
-    ctx.synchronize()
-    assert_same(gpu_hits(hits), ref, "frame after autotune")
-    with pytest.raises(Exception):
-        ctx.set_option(L.VG_OPT_FRAME_REFILL, 0)
+    opts = (L.VG_OPT_FRAME_WAVES_PER_CU, L.VG_OPT_FRAME_REFILL, L.VG_OPT_FRAME_STRIPES,
+            L.VG_OPT_FRAME_BLOCK_W)
+    before = [ctx.get_option(o) for o in opts]
+    try:
+        ms = small.autotune_frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+        assert 0.0 < ms < 100.0
+        assert 1 <= ctx.get_option(L.VG_OPT_FRAME_WAVES_PER_CU) <= 64
+        assert 1 <= ctx.get_option(L.VG_OPT_FRAME_REFILL) <= 64
+        assert 1 <= ctx.get_option(L.VG_OPT_FRAME_STRIPES) <= 64
+        assert ctx.get_option(L.VG_OPT_FRAME_BLOCK_W) in (4, 8, 16, 32)
+        assert ctx.get_option(L.VG_OPT_CU_COUNT) >= 1
+        hits, occ = small.trace_frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
+        ref, ref_occ, _ = oracle.OracleWorld.view(small_world).frame(
+            cam, wl.width, wl.height, 700.0, flags, sun, 480.0, threads=THREADS)
+        ctx.synchronize()
+        assert_same(gpu_hits(hits), ref, "frame after autotune")
+        assert np.array_equal(occ.numpy(np.uint8), ref_occ)
+        with pytest.raises(Exception):
+            ctx.set_option(L.VG_OPT_FRAME_REFILL, 0)
+        with pytest.raises(Exception):
+            ctx.set_option(L.VG_OPT_FRAME_BLOCK_W, 12)
+    finally:
+        for o, v in zip(opts, before):
+            ctx.set_option(o, v)
 
 
 def batched_setup(ctx, small, small_world):

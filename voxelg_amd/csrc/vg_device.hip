@@ -276,7 +276,8 @@ struct FrameParams {
     CamParams cam;
     float sun[3];
     int32_t width, height;
-    uint32_t nbx;
+    uint32_t nbx;   // pixel blocks per image row
+    uint32_t bws;   // log2 of the block width: blocks of 2^bws x 64 / 2^bws pixels (a wave)
 };
 constexpr size_t kWorkBytes = kStripeBytes + 256; // counters, then the FrameParams
 static_assert(sizeof(FrameParams) <= 256, "FrameParams must fit its slot");
@@ -333,13 +334,13 @@ __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, FrameParams fp, uin
         if (threadIdx.x == 0) *frame_params(work) = fp;
     }
     const CamParams &cam = fp.cam;
-    const uint32_t nbx = fp.nbx;
+    const uint32_t nbx = fp.nbx, bws = fp.bws;
     const uint32_t wi = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (wi >= (list ? *list_n : nblocks)) return; // whole waves leave together
     const uint32_t blk = list ? list[wi] : wi;
     const uint32_t lane = threadIdx.x & 63;
-    const int32_t px = (int32_t)((blk % nbx) * 8 + (lane & 7));
-    const int32_t py = (int32_t)((blk / nbx) * 8 + (lane >> 3));
+    const int32_t px = (int32_t)(((blk % nbx) << bws) + (lane & ((1u << bws) - 1u)));
+    const int32_t py = (int32_t)((blk / nbx) * (64u >> bws) + (lane >> bws));
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
 #pragma unroll
@@ -411,11 +412,11 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
             } else if (kind == 0) {
                 const uint32_t id = base + lanes_below(~act);
                 if (id < lim) {
-                    const uint32_t nbx = fpp->nbx;
+                    const uint32_t nbx = fpp->nbx, bws = fpp->bws;
                     const int32_t width = fpp->width, height = fpp->height;
                     const uint32_t blk = list ? list[id >> 6] : id >> 6, qd = id & 63u;
-                    px = (int32_t)((blk % nbx) * 8 + (qd & 7));
-                    py = (int32_t)((blk / nbx) * 8 + (qd >> 3));
+                    px = (int32_t)(((blk % nbx) << bws) + (qd & ((1u << bws) - 1u)));
+                    py = (int32_t)((blk / nbx) * (64u >> bws) + (qd >> bws));
                     if (px < width && py < height) {
                         pix = (int64_t)py * width + px;
                         const CamParams cam = fpp->cam;
@@ -884,7 +885,10 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
                                       uint8_t *occluded, bool shadow, bool beam, bool count,
                                       hipStream_t stream, uint32_t *work, float *beam_t,
                                       const uint32_t *list, const uint32_t *list_n) {
-    const uint32_t nbx = (uint32_t)((width + 7) / 8), nby = (uint32_t)((height + 7) / 8);
+    // a block list names 8x8 blocks (vg_frame_dirty_blocks); else the context's block shape
+    const uint32_t bws = list ? 3u : (uint32_t)w->ctx->frame_bws;
+    const uint32_t bw = 1u << bws, bh = 64u >> bws;
+    const uint32_t nbx = ((uint32_t)width + bw - 1) / bw, nby = ((uint32_t)height + bh - 1) / bh;
     const uint64_t nblocks = (uint64_t)nbx * nby;
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     FrameParams fp;
@@ -893,6 +897,7 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     fp.width = width;
     fp.height = height;
     fp.nbx = nbx;
+    fp.bws = bws;
     if (beam) {
         dim3 g((uint32_t)((nblocks + 3) / 4));
         if (count)
@@ -935,13 +940,21 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     return VG_OK;
 }
 
+// Pixel blocks (one wave's 64 rays each) of a width x height frame: the context's block
+// shape, or 8x8 for a block list (the dirty-block masks name 8x8 blocks).
+static uint64_t frame_blocks(const vg_ctx *c, int32_t width, int32_t height, bool list) {
+    const uint32_t bws = list ? 3u : (uint32_t)c->frame_bws;
+    const uint64_t bw = 1u << bws, bh = 64u >> bws;
+    return (((uint64_t)width + bw - 1) / bw) * (((uint64_t)height + bh - 1) / bh);
+}
+
 static int launch_frame_persistent(vg_world *w, const CamParams &cp, int32_t width, int32_t height,
                                    float ptmax, uint32_t pflags, const float s[3], float stmax,
                                    vg_hit *hits, uint8_t *occluded, bool shadow, bool beam,
                                    bool count, hipStream_t stream,
                                    const uint32_t *list = nullptr,
                                    const uint32_t *list_n = nullptr) {
-    const uint64_t nblocks = (uint64_t)((width + 7) / 8) * (uint64_t)((height + 7) / 8);
+    const uint64_t nblocks = frame_blocks(w->ctx, width, height, list != nullptr);
     // the counter may overshoot the ray count by up to 64 per wave after it runs out
     if (nblocks * 64 > 0xF0000000ull) return fail(VG_E_INVALID, "image too large");
     if (beam) {
@@ -1115,6 +1128,7 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->frame_wpc = 16;
     c->frame_refill = 32;
     c->frame_stripes = 4;
+    c->frame_bws = 3;
     c->rays_wpc = 16;
     c->rays_refill = 32;
     c->probe_wpc = 8;
@@ -1316,10 +1330,10 @@ int vg_trace_frame_noalloc(vg_world *w, const vg_camera *camera, int32_t width, 
     int st = check_world(w);
     if (st) return st;
     if (width <= 0 || height <= 0) return fail(VG_E_INVALID, "bad frame size %dx%d", width, height);
-    const uint64_t nblocks = (uint64_t)((width + 7) / 8) * (uint64_t)((height + 7) / 8);
+    const uint64_t nblocks = frame_blocks(w->ctx, width, height, false);
     if (nblocks > kNoAllocBlocks)
         return fail(VG_E_INVALID,
-                    "%dx%d frame: %llu 8x8 blocks above the no-allocation scratch's %zu "
+                    "%dx%d frame: %llu pixel blocks above the no-allocation scratch's %zu "
                     "(4096x4096 pixels); use vg_trace_frame",
                     width, height, (unsigned long long)nblocks, kNoAllocBlocks);
     NoAllocScratch na;
@@ -1598,6 +1612,11 @@ int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
          : option == VG_OPT_RAYS_REFILL ? ctx->rays_refill
                                         : ctx->probe_refill) = value;
         return VG_OK;
+    case VG_OPT_FRAME_BLOCK_W:
+        if (value != 4 && value != 8 && value != 16 && value != 32)
+            return fail(VG_E_INVALID, "frame block width %d: 4, 8, 16 or 32", value);
+        ctx->frame_bws = value == 4 ? 2 : value == 8 ? 3 : value == 16 ? 4 : 5;
+        return VG_OK;
     case VG_OPT_FRAME_STRIPES:
         if (value < 1 || value > (int)kMaxStripes)
             return fail(VG_E_INVALID, "frame stripes %d not in 1..%u", value, kMaxStripes);
@@ -1620,6 +1639,7 @@ int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
     case VG_OPT_FRAME_WAVES_PER_CU: *value = ctx->frame_wpc; return VG_OK;
     case VG_OPT_FRAME_REFILL: *value = ctx->frame_refill; return VG_OK;
     case VG_OPT_FRAME_STRIPES: *value = ctx->frame_stripes; return VG_OK;
+    case VG_OPT_FRAME_BLOCK_W: *value = 1 << ctx->frame_bws; return VG_OK;
     case VG_OPT_RAYS_WAVES_PER_CU: *value = ctx->rays_wpc; return VG_OK;
     case VG_OPT_RAYS_REFILL: *value = ctx->rays_refill; return VG_OK;
     case VG_OPT_PROBE_WAVES_PER_CU: *value = ctx->probe_wpc; return VG_OK;
@@ -1636,19 +1656,50 @@ int vg_autotune_frame(vg_world *w, const vg_camera *camera, int32_t width, int32
     int st = check_world(w);
     if (st) return st;
     vg_ctx *c = w->ctx;
-    // the frame kernel: waves per CU x refill threshold x work stripes (the coherent frame
-    // refills best from a few lanes, measured; wider refill thresholds only lose)
-    static const int kWpc[] = {8, 12, 16, 20, 24};
-    static const int kRefill[] = {4, 8, 16, 32};
-    static const int kStripes[] = {1, 2, 4, 8, 16};
-    return autotune_lists(c, &c->frame_wpc, TuneList{kWpc, 5}, &c->frame_refill,
-                          TuneList{kRefill, 4}, &c->frame_stripes, TuneList{kStripes, 5},
-                          [&](hipStream_t s) {
-                              return vg_trace_frame(w, camera, width, height, primary_tmax,
-                                                    primary_flags, sun, shadow_tmax, hits,
-                                                    occluded, (void *)s);
-                          },
-                          best_ms);
+    // the frame kernel: pixel-block shape x waves per CU x refill threshold x work stripes
+    // (the coherent frame refills best from a few lanes, measured; wider refill thresholds
+    // only lose)
+    static const int kWpc[] = {12, 16, 20, 24};
+    static const int kRefill[] = {4, 8, 16};
+    static const int kStripes[] = {1, 2, 4, 8};
+    static const int kBws[] = {3, 2, 4, 5}; // block widths 8, 4, 16, 32
+    float best = INFINITY;
+    const int ow = c->frame_wpc, orf = c->frame_refill, os = c->frame_stripes, ob = c->frame_bws;
+    int bw = ow, br = orf, bs = os, bb = ob;
+    for (int b : kBws) {
+        c->frame_bws = b;
+        if (frame_blocks(c, width, height, false) * 64 > 0xF0000000ull) continue;
+        float ms = INFINITY;
+        int st = autotune_lists(c, &c->frame_wpc, TuneList{kWpc, 4}, &c->frame_refill,
+                                TuneList{kRefill, 3}, &c->frame_stripes, TuneList{kStripes, 4},
+                                [&](hipStream_t s) {
+                                    return vg_trace_frame(w, camera, width, height,
+                                                          primary_tmax, primary_flags, sun,
+                                                          shadow_tmax, hits, occluded,
+                                                          (void *)s);
+                                },
+                                &ms);
+        if (st) { // leave the options as they were
+            c->frame_wpc = ow;
+            c->frame_refill = orf;
+            c->frame_stripes = os;
+            c->frame_bws = ob;
+            return st;
+        }
+        if (ms < best) {
+            best = ms;
+            bw = c->frame_wpc;
+            br = c->frame_refill;
+            bs = c->frame_stripes;
+            bb = b;
+        }
+    }
+    c->frame_wpc = bw;
+    c->frame_refill = br;
+    c->frame_stripes = bs;
+    c->frame_bws = bb;
+    if (best_ms) *best_ms = best;
+    return VG_OK;
 }
 
 int vg_autotune_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags, vg_hit *hits,

@@ -12,6 +12,7 @@ struct vg_ctx {
     int frame_wpc;       // persistent frame kernel: waves per CU
     int frame_refill;    // ... refill when fewer lanes of a wave are busy
     int frame_stripes;   // ... work stripes, each with its own counter (1..64)
+    int frame_bws;       // ... log2 of the pixel-block width (blocks of 2^bws x 64 / 2^bws)
     int rays_wpc;        // persistent batched kernel: waves per CU
     int rays_refill;
     int probe_wpc;       // persistent probe path kernel: waves per CU
