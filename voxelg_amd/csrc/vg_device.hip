@@ -275,6 +275,7 @@ constexpr size_t kStripeBytes = kMaxStripes * kStripeWords * sizeof(uint32_t);
 struct FrameParams {
     CamParams cam;
     float sun[3];
+    float sun_inv[3]; // safe_inv of each sun component, the shadow rays' shared inverse
     int32_t width, height;
     uint32_t nbx;   // pixel blocks per image row
     uint32_t bws;   // log2 of the block width: blocks of 2^bws x 64 / 2^bws pixels (a wave)
@@ -393,7 +394,6 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
     int kind = 0; // 0 idle, 1 primary ray, 2 shadow ray
     int64_t pix = 0;
-    int32_t px = 0, py = 0;
     int it = 0;
     Walk r;
     WalkCache q;
@@ -415,8 +415,8 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                     const uint32_t nbx = fpp->nbx, bws = fpp->bws;
                     const int32_t width = fpp->width, height = fpp->height;
                     const uint32_t blk = list ? list[id >> 6] : id >> 6, qd = id & 63u;
-                    px = (int32_t)(((blk % nbx) << bws) + (qd & ((1u << bws) - 1u)));
-                    py = (int32_t)((blk / nbx) * (64u >> bws) + (qd >> bws));
+                    const int32_t px = (int32_t)(((blk % nbx) << bws) + (qd & ((1u << bws) - 1u)));
+                    const int32_t py = (int32_t)((blk / nbx) * (64u >> bws) + (qd >> bws));
                     if (px < width && py < height) {
                         pix = (int64_t)py * width + px;
                         const CamParams cam = fpp->cam;
@@ -460,12 +460,17 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                         if (SHADOW) {
                             bool started = false;
                             if (h.hit) {
-                                const CamParams cam = fpp->cam;
-                                float d[3], so[3], sd[3] = {fpp->sun[0], fpp->sun[1], fpp->sun[2]};
-                                primary_dir(cam, px, py, d);
-                                shadow_origin(cam.origin, d, h, so);
+                                // the primary walk still holds its direction, bit for bit
+                                // the pinhole vector primary_dir gave it
+                                const float co[3] = {fpp->cam.origin[0], fpp->cam.origin[1],
+                                                     fpp->cam.origin[2]};
+                                const float sd[3] = {fpp->sun[0], fpp->sun[1], fpp->sun[2]};
+                                const float si[3] = {fpp->sun_inv[0], fpp->sun_inv[1],
+                                                     fpp->sun_inv[2]};
+                                float so[3];
+                                shadow_origin(co, r.dir, h, so);
                                 hit_miss(h);
-                                if (walk_init(r, w, so, sd)) {
+                                if (walk_init(r, w, so, sd, si)) {
                                     cache_reset(q);
                                     it = 0;
                                     kind = 2;
@@ -893,7 +898,11 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     unsigned long long *cd = count ? w->counts_dev : nullptr;
     FrameParams fp;
     fp.cam = cp;
-    for (int k = 0; k < 3; ++k) fp.sun[k] = s[k];
+    for (int k = 0; k < 3; ++k) {
+        fp.sun[k] = s[k];
+        // the device's safe_inv, in host IEEE single precision (both correctly rounded)
+        fp.sun_inv[k] = std::fabs(s[k]) < 1e-8f ? 1e30f : 1.0f / s[k];
+    }
     fp.width = width;
     fp.height = height;
     fp.nbx = nbx;

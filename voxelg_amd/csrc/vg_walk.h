@@ -121,14 +121,16 @@ __device__ __forceinline__ float t_next(const Walk &r, int k) {
 
 // Ray setup: window-local origin, slab entry, 1e-3 bias clamped into the window
 // (shaders/raymarch.wgsl:3369-3411, src/raycast.rs:34-50). dir is already normalized.
+// `inv` holds safe_inv of each component of d (a caller tracing many rays along one
+// direction computes it once).
 __device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const float o[3],
-                                          const float d[3]) {
+                                          const float d[3], const float inv[3]) {
     float tmn[3], tmx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         r.org[k] = o[k] - (float)w.origin[k];
         r.dir[k] = d[k];
-        r.inv[k] = safe_inv(d[k]);
+        r.inv[k] = inv[k];
         float t0 = (0.0f - r.org[k]) * r.inv[k];
         float t1 = ((float)w.dims[k] - r.org[k]) * r.inv[k];
         tmn[k] = fsel_min(t0, t1);
@@ -158,6 +160,12 @@ __device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const floa
     int la = (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) ? 0 : (tmn[1] >= tmn[2] ? 1 : 2);
     r.entry_face = (r.face_pack >> (4 * la)) & 15;
     return true;
+}
+
+__device__ __forceinline__ bool walk_init(Walk &r, const DevWorld &w, const float o[3],
+                                          const float d[3]) {
+    const float inv[3] = {safe_inv(d[0]), safe_inv(d[1]), safe_inv(d[2])};
+    return walk_init(r, w, o, d, inv);
 }
 
 // Advance out of the aligned cell of size S (1 = just the current voxel) that holds the
