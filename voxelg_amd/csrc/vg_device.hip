@@ -107,8 +107,7 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
                     v = v < 0 ? v + w.dims[k] : v;
                     sl[k] = v >= w.dims[k] ? v - w.dims[k] : v;
                 }
-                const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
-                                    (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+                const uint32_t ci = chunk_index(w, sl);
                 occ = (w.chunk[ci] >> cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3)) & 1;
             }
             // the first step (group) that stops the march
@@ -169,8 +168,7 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
                 v = v < 0 ? v + w.dims[k] : v;
                 sl[k] = v >= w.dims[k] ? v - w.dims[k] : v;
             }
-            const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
-                                (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+            const uint32_t ci = chunk_index(w, sl);
             occ = (w.chunk[ci] >> cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3)) & 1;
         }
         if (__ballot(occ) != 0ull) break;

@@ -305,6 +305,18 @@ __device__ __forceinline__ uint64_t group_mask(const int32_t sl[3], int sh) {
 #define LOD_BRICK_T 400.0f
 #define TILE_LOD_T 520.0f
 
+// Linear indices of the L4 cell and the chunk holding slot coordinate sl (x fastest, then
+// y, then z). Worlds are at most 16384 voxels per axis, so every factor and partial sum stays
+// below 2^24 (256 chunks per axis, 2^24 chunks) and full-rate 24-bit multiplies are exact.
+__device__ __forceinline__ uint32_t l4_index(const DevWorld &w, const int32_t sl[3]) {
+    return (uint32_t)(sl[0] >> 8) + __umul24((uint32_t)(sl[1] >> 8), (uint32_t)w.l4dim[0]) +
+           __umul24((uint32_t)(sl[2] >> 8), (uint32_t)(w.l4dim[0] * w.l4dim[1]));
+}
+__device__ __forceinline__ uint32_t chunk_index(const DevWorld &w, const int32_t sl[3]) {
+    return (uint32_t)(sl[0] >> 6) + __umul24((uint32_t)(sl[1] >> 6), (uint32_t)w.cdim[0]) +
+           __umul24((uint32_t)(sl[2] >> 6), (uint32_t)(w.cdim[0] * w.cdim[1]));
+}
+
 // The walk: phase 1 leaves solid voxels the ray starts in (src/raycast.rs:108-115),
 // phase 2 returns the first solid voxel. Each iteration tests the coarsest level and
 // skips the largest empty aligned cell (shaders/raymarch.wgsl:4626-4775). The words of
@@ -354,10 +366,8 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld 
         }
     }
     if (SPEC) {
-        const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
-                             (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
-        const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
-                            (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+        const uint32_t l4i = l4_index(w, sl);
+        const uint32_t ci = chunk_index(w, sl);
         const uint32_t tn = ci * 64u + (uint32_t)cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3,
                                                       (sl[2] >> 4) & 3);
         const uint32_t bn = tn * 64u + (uint32_t)cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3,
@@ -378,8 +388,7 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld 
     }
     int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
     do {
-        const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
-                             (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
+        const uint32_t l4i = l4_index(w, sl);
         if (l4i != q.cur_l4) {
             q.cur_l4 = l4i;
             q.l4w = w.l4[l4i];
@@ -397,8 +406,7 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld 
             S = grp ? 128 : 64;
             break;
         }
-        const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
-                            (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+        const uint32_t ci = chunk_index(w, sl);
         if (ci != q.cur_c) {
             q.cur_c = ci;
             q.cw = w.chunk[ci];
