@@ -7,7 +7,7 @@ is generated on the host, uploaded once, and resident in HBM before the timed re
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
   python bench.py --workload batched   # uneven batched queries (96 batches per step)
-  python bench.py --workload gi        # GI probe update: plain 256-ray vs learned 32-ray
+  python bench.py --workload gi        # GI probe update: plain 256-ray vs learned 16-ray
   python bench.py --workload dynamic   # 4096 sphere edits per frame + retrace + probe update
 
 Prints ONE JSON line on stdout; details go to stderr.
@@ -306,7 +306,7 @@ def main():
     gw.close()
     log(f"device world generation: {gen_out}")
 
-    # the metric's second half: the GI probe update on the same world, learned 32-ray (cache
+    # the metric's second half: the GI probe update on the same world, learned 16-ray (cache
     # trained online) against plain 256-ray probes, errors against the FP64 reference
     gi_out = None
     if not args.no_gi:
@@ -466,13 +466,24 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+# The learned update's configuration: 16 gather rays per probe ending at the first hit in
+# the radiance cache, trained online with one step of 8192 plain-traced bounce samples every
+# second update (measured on this workload: the error is flat across 1..8 updates per step
+# and 8k..64k samples per step, the 16-ray gather quadrature sets it).
+GI_LEARNED_RAYS = 16
+GI_TRAIN_BATCH = 8192
+GI_TRAIN_EVERY = 2
+
+
 def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     """Config-4 GI probes over the world: a 32-voxel probe grid (64x8x64 on 2048x256x2048),
-    sun + sky, 2-bounce diffuse. Plain: 256 rays/probe, every sample traces gather, sun
-    shadow, cosine bounce and its sun shadow. Learned: 32 rays/probe ending at the first hit
-    in the radiance cache, trained online on 65536 plain-traced bounce samples per update
-    (training inside the timed update). Errors: mean relative SH-L1 irradiance error of a
-    fresh update against the FP64 4096-ray 2-bounce estimate of every live probe."""
+    sun + sky, 2-bounce diffuse. Plain N: N rays/probe, every sample traces gather, sun
+    shadow, cosine bounce and its sun shadow (N = 256 is the quality reference; 32 and 16 are
+    the cheaper plain settings). Learned: GI_LEARNED_RAYS rays/probe ending at the first hit
+    in the radiance cache, trained online (GI_TRAIN_BATCH samples per step, one step every
+    GI_TRAIN_EVERY updates, the steps inside the timed updates). Errors: mean relative SH-L1
+    irradiance error of a fresh update against the FP64 4096-ray 2-bounce estimate of every
+    live probe."""
     import numpy as np
 
     import oracle
@@ -483,23 +494,23 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     grid = (dims[0] // 32, dims[1] // 32, dims[2] // 32)
     base = dict(grid=grid, spacing=32, epochs=1, update_div=1, bounces=2, flags=VG_GI_SKIP_BURIED)
     p256 = gi.gi_params(sun, rays_per_probe=256, **base)
-    p32 = gi.gi_params(sun, rays_per_probe=32, **base)
+    pl = gi.gi_params(sun, rays_per_probe=GI_LEARNED_RAYS, **base)
     stream = ctx.stream()
 
     kernel_ms = {}
 
     def timed(fn, p, n, cache=None, grid_obj=None, name=None):
-        # n back-to-back updates between two events on the stream; a learned update's
-        # training step runs on the cache's own stream, so the end event waits for the last
-        # one (vg_nrc_join): the span covers all the work of the n updates. With grid_obj,
-        # the grid also records each launch's time (events on the same stream).
+        # n back-to-back updates (fn(p, i)) between two events on the stream; a learned
+        # update's training step runs on the cache's own stream, so the end event waits for
+        # the last one (vg_nrc_join): the span covers all the work of the n updates. With
+        # grid_obj, the grid also records each launch's time (events on the same stream).
         if grid_obj is not None:
             grid_obj.set_timing(True)
         a, b = ctx.event(), ctx.event()
         a.record(stream)
         for i in range(n):
             p["round"] = 1000 + i
-            fn(p)
+            fn(p, i)
         if cache is not None:
             cache.join(stream)
         b.record(stream)
@@ -516,54 +527,62 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
                 "refill_below": ctx.get_option(L.VG_OPT_PROBE_REFILL),
                 "ids_per_atomic": ctx.get_option(L.VG_OPT_PROBE_CHUNK)}
 
-    g256 = gi.ProbeGrid(dw, p256)
-    rays256, it256 = g256.update_counts(p256)
-    g256.autotune(p256)
-    sched_plain = sched()
-    plain_ms = timed(lambda p: g256.update(p, stream), p256, steps, grid_obj=g256,
-                     name="plain256")
-    log(f"gi plain: {plain_ms:.3f} ms/update ({sched_plain}), rays {rays256}")
+    def train_on(i):
+        return i % GI_TRAIN_EVERY == 0
 
-    cache = nrc.RadianceCache(ctx)
-    g32 = gi.ProbeGrid(dw, p32)
+    # plain settings: 256 (the quality reference) and the cheaper 32 and 16
+    plain = {}
+    for n in (256, 32, 16):
+        p = p256 if n == 256 else gi.gi_params(sun, rays_per_probe=n, **base)
+        gobj = gi.ProbeGrid(dw, p)
+        rays_n, _ = gobj.update_counts(p)
+        gobj.autotune(p)
+        s = sched()
+        ms = timed(lambda q, i: gobj.update(q, stream), p, steps, grid_obj=gobj,
+                   name=f"plain{n}")
+        plain[n] = {"grid": gobj, "params": p, "ms": ms, "sched": s, "rays": rays_n}
+        log(f"gi plain {n} rays: {ms:.3f} ms/update ({s}), rays {rays_n}")
+    plain_ms = plain[256]["ms"]
+
+    cfg = nrc.default_config()
+    cfg["batch"] = GI_TRAIN_BATCH
+    cache = nrc.RadianceCache(ctx, cfg)
+    gl = gi.ProbeGrid(dw, pl)
     t0 = time.perf_counter()
     for r in range(train_updates):
-        p32["round"] = r
-        g32.update_learned(cache, p32, train=True, stream=stream)
+        pl["round"] = r
+        gl.update_learned(cache, pl, train=train_on(r), stream=stream)
     stream.synchronize()
     log(f"gi learned: {train_updates} training updates in {time.perf_counter() - t0:.2f} s, "
         f"loss {cache.stats()[1]:.5f}")
-    g32.autotune(p32, cache, train=True)  # (its repetitions train the cache further)
+    gl.autotune(pl, cache, train=True)  # (its repetitions train the cache further)
     sched_learned = sched()
-    learned_ms = timed(lambda p: g32.update_learned(cache, p, train=True, stream=stream), p32,
-                       steps, cache, g32, "learned32")
-    g32.autotune(p32, cache, train=False)  # the inference-only update has its own schedule
+    # an even number of updates, so the span holds whole training cadences
+    lsteps = max(GI_TRAIN_EVERY, steps - steps % GI_TRAIN_EVERY)
+    learned_ms = timed(lambda p, i: gl.update_learned(cache, p, train=train_on(i),
+                                                       stream=stream),
+                       pl, lsteps, cache, gl, "learned")
+    gl.autotune(pl, cache, train=False)  # the inference-only update has its own schedule
     sched_frozen = sched()
-    frozen_ms = timed(lambda p: g32.update_learned(cache, p, train=False, stream=stream), p32,
-                      steps, cache, g32, "learned32_inference_only")
-    log(f"gi learned: {learned_ms:.3f} ms/update with training ({sched_learned}), "
-        f"{frozen_ms:.3f} without ({sched_frozen})")
-    # the cheapest plain setting beside them: 32 rays per probe, the 2-bounce traced tail
-    gp32 = gi.ProbeGrid(dw, p32)
-    gp32.autotune(p32)
-    sched_plain32 = sched()
-    plain32_ms = timed(lambda p: gp32.update(p, stream), p32, steps, grid_obj=gp32,
-                       name="plain32")
-    log(f"gi plain 32 rays: {plain32_ms:.3f} ms/update ({sched_plain32})")
-    gp32.close()
-    p32["round"] = 5000
-    rays32, nq, it32 = g32.update_learned_counts(cache, p32, train=True, classes=True)
-    log(f"gi learned: {learned_ms:.3f} ms/update, rays {rays32}, cache queries {nq}, "
-        f"walk classes {it32}; launch ms {kernel_ms}")
-    # roofline of the learned update's dominant launch, the path kernel: the pyramid words its
-    # walks read (iteration classes) + per gather sample its probe origin, direction and face
-    # byte (33 B) + per cache query its hit record, radiance and list entry (17 + 32 B) + per
-    # training sample its query, table entry and target (32 + 16 + 48 B)
+    frozen_ms = timed(lambda p, i: gl.update_learned(cache, p, train=False, stream=stream), pl,
+                      steps, cache, gl, "learned_inference_only")
+    log(f"gi learned {GI_LEARNED_RAYS} rays: {learned_ms:.3f} ms/update with training "
+        f"({sched_learned}), {frozen_ms:.3f} without ({sched_frozen})")
+    pl["round"] = 5000
+    raysl, nq, itl = gl.update_learned_counts(cache, pl, train=True, classes=True)
+    log(f"gi learned: rays {raysl}, cache queries {nq}, walk classes {itl}; "
+        f"launch ms {kernel_ms}")
+    # roofline of the learned update's dominant launch, the path kernel, taken on the
+    # inference-only update (a training update's launch also traces the step's targets): the
+    # pyramid words its walks read (iteration classes) + per gather sample its probe origin,
+    # direction, inverse and face byte (49 B) + per cache query its hit record, radiance and
+    # list entry (17 + 32 B)
     from voxelg_amd._lib import algorithmic_bytes
-    path_bytes = (algorithmic_bytes(it32) + 33 * rays32["gather"] + 49 * nq +
-                  96 * rays32["bounce"])
-    path_ms = kernel_ms["learned32"]["paths"]
-    path_gbs = path_bytes / (path_ms / 1e3) / 1e9
+    pl["round"] = 5001
+    raysf, nqf, itf = gl.update_learned_counts(cache, pl, train=False, classes=True)
+    path_bytes_frozen = algorithmic_bytes(itf) + 49 * raysf["gather"] + 49 * nqf
+    path_ms = kernel_ms["learned_inference_only"]["paths"]
+    path_gbs = path_bytes_frozen / (path_ms / 1e3) / 1e9
 
     # the probes' consumer: per-pixel indirect light over the headline 1080p frame's hits
     from voxelg_amd import workloads
@@ -572,11 +591,12 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     fhits, _ = dw.trace_frame(fcam, fw.width, fw.height, fw.primary_tmax, fw.primary_flags,
                               fw.sun(), fw.shadow_tmax)
     find = ctx.alloc(fw.width * fw.height * 16)
-    indirect_ms = timed(lambda p: g32.frame_indirect(p, fcam, fw.width, fw.height, fhits,
-                                                     out=find, stream=stream), p32, steps)
+    indirect_ms = timed(lambda p, i: gl.frame_indirect(p, fcam, fw.width, fw.height, fhits,
+                                                        out=find, stream=stream), pl, steps)
     log(f"gi: per-pixel probe indirect at {fw.width}x{fw.height}: {indirect_ms:.4f} ms")
 
     # errors of fresh updates against the FP64 reference
+    g256 = plain[256]["grid"]
     total = g256.total
     slots = np.arange(total)
 
@@ -586,15 +606,16 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         ctx.synchronize()
         return grid_obj.download()
 
-    p256["round"] = 7000
-    p32["round"] = 7000
-    rec_plain = fresh(g256, lambda: g256.update(p256))
-    rec_learned = fresh(g32, lambda: g32.update_learned(cache, p32, train=True))
-    rec_plain32 = fresh(g32, lambda: g32.update(p32))
-    p1 = p32.copy()
+    recs = {}
+    for n, e in plain.items():
+        e["params"]["round"] = 7000
+        recs[f"plain{n}"] = fresh(e["grid"], lambda: e["grid"].update(e["params"]))
+    pl["round"] = 7000
+    recs["learned"] = fresh(gl, lambda: gl.update_learned(cache, pl, train=True))
+    p1 = pl.copy()
     p1["bounces"] = 1
-    rec_ambient = fresh(g32, lambda: g32.update(p1))
-    live = rec_plain["info"][:, 3] == 1
+    recs["ambient"] = fresh(gl, lambda: gl.update(p1))
+    live = recs["plain256"]["info"][:, 3] == 1
     origins = gi.probe_origins(np.zeros(total, gi.PROBE_DTYPE), p256, slots)[live]
     pref = p256.copy()
     pref["rays_per_probe"] = 4096
@@ -608,12 +629,12 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     def err(rec):
         return float(gi.irradiance_error(gi.sh_coefficients(rec)[live], ref).mean())
 
-    errs = {"plain256": err(rec_plain), "learned32": err(rec_learned),
-            "plain32": err(rec_plain32), "ambient32": err(rec_ambient)}
-    times = {"plain256": plain_ms, "plain32": plain32_ms}
-    under = {k: v for k, v in times.items() if errs[k] <= GI_ERR_LIMIT}
+    errs = {k: err(v) for k, v in recs.items()}
+    under = {f"plain{n}": e["ms"] for n, e in plain.items() if errs[f"plain{n}"] <= GI_ERR_LIMIT}
     fastest = min(under, key=under.get) if under else None
-    ok = errs["learned32"] <= GI_ERR_LIMIT
+    ok = errs["learned"] <= GI_ERR_LIMIT
+    for e in plain.values():
+        e["grid"].close()
     out = {
         "value": round(learned_ms, 4), "unit": "ms/update", "higher_is_better": False,
         "err_limit": GI_ERR_LIMIT, "learned_meets_err_limit": bool(ok),
@@ -623,24 +644,29 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         "speedup_vs_fastest_plain_under_limit":
             round(under[fastest] / learned_ms, 3) if (ok and fastest) else None,
         "probes": int(total), "live_probes": int(live.sum()), "grid": list(grid), "spacing": 32,
+        "learned": {"rays_per_probe": GI_LEARNED_RAYS, "train_batch": GI_TRAIN_BATCH,
+                    "train_every": GI_TRAIN_EVERY, "timed_updates": lsteps},
         "plain_ms": round(plain_ms, 4), "learned_ms": round(learned_ms, 4),
-        "plain32_ms": round(plain32_ms, 4),
+        "plain32_ms": round(plain[32]["ms"], 4), "plain16_ms": round(plain[16]["ms"], 4),
         "speedup": round(plain_ms / learned_ms, 3),
         "learned_inference_only_ms": round(frozen_ms, 4),
         "speedup_inference_only": round(plain_ms / frozen_ms, 3),
-        "schedule_plain": sched_plain, "schedule_plain32": sched_plain32,
-        "schedule_learned": sched_learned,
+        "schedule_plain": plain[256]["sched"], "schedule_plain32": plain[32]["sched"],
+        "schedule_plain16": plain[16]["sched"], "schedule_learned": sched_learned,
         "schedule_inference_only": sched_frozen,
-        "err_plain256": round(err(rec_plain), 5), "err_learned32": round(err(rec_learned), 5),
-        "err_plain32_2bounce": round(err(rec_plain32), 5),
-        "err_ambient_tail32": round(err(rec_ambient), 5),
-        "rays_plain": rays256, "rays_learned": rays32, "cache_queries": nq,
+        "err_plain256": round(errs["plain256"], 5),
+        "err_plain32_2bounce": round(errs["plain32"], 5),
+        "err_plain16_2bounce": round(errs["plain16"], 5),
+        "err_learned": round(errs["learned"], 5),
+        "err_ambient_tail": round(errs["ambient"], 5),
+        "rays_plain": plain[256]["rays"], "rays_learned": raysl, "cache_queries": nq,
         "launch_ms": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv
                           for kk, vv in v.items()} for k, v in kernel_ms.items()},
         "roofline": {"bound": "hbm", "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_probe_paths<false, 3> (learned update's path launch)",
-                     "kernel_ms": round(path_ms, 4), "algorithmic_bytes": int(path_bytes)},
+                     "kernel": "k_probe_paths<false, 3> (the learned update's path launch, "
+                               "inference-only update)",
+                     "kernel_ms": round(path_ms, 4), "algorithmic_bytes": int(path_bytes_frozen)},
         "indirect_1080p_ms": round(indirect_ms, 4),
         "train_steps": cache.stats()[0], "train_batch": cache.batch,
         "reference": f"FP64 4096 rays/probe 2-bounce, oracle on {threads} host threads, "
@@ -670,9 +696,10 @@ def bench_gi_main(args):
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
         "data": "synthetic (seeded fBm terrain with caves, generated on the host)",
         "config": {"workload": "GI probes: 64x8x64 grid at 32-voxel spacing over the "
-                               "2048x256x2048 world, sun + sky, 2-bounce diffuse; learned 32 "
-                               "rays/probe + hash-grid MLP cache trained online (65536 samples "
-                               "per update) vs plain 256 rays/probe",
+                               "2048x256x2048 world, sun + sky, 2-bounce diffuse; learned "
+                               f"{GI_LEARNED_RAYS} rays/probe + hash-grid MLP cache trained "
+                               f"online ({GI_TRAIN_BATCH} samples per step, one step every "
+                               f"{GI_TRAIN_EVERY} updates) vs plain 256 rays/probe",
                    "world": list(wl.dims)},
         "gi": g,
         "cache_mfma_util": {"train": profile_value(COUNTERS_PROFILE, "mfma_util_train"),

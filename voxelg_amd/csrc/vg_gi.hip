@@ -27,8 +27,9 @@ struct vg_probes {
     int32_t rays, epochs;
     int64_t total;          // probes
     vg_probe *probes;       // total records
-    float4 *dirs;           // rays * epochs gather directions
-    float4 *bounce;         // VG_GI_BOUNCE_TABLE cosine-hemisphere directions
+    float4 *dirs;           // rays * epochs gather directions, then as many safe_inv entries
+    float4 *bounce;         // VG_GI_BOUNCE_TABLE cosine-hemisphere directions, then their
+                            // safe_inv entries
     float4 *palette;        // 256 albedo entries
     float4 *pwork;          // per updated probe: ray origin xyz, w = flags bits
     float4 *rad;            // per sample with a gather hit: radiance rgb, visible distance
@@ -62,6 +63,16 @@ constexpr int kCtrHeavy = 4 + 64;
 constexpr int kCtrUpd = kCtrHeavy + 2;
 constexpr int kTimingCap = 64; // updates one timing record holds
 constexpr int kGiCtrs = kCtrUpd + 1;
+// Striped sample queue of the path kernel: kGiStripes counters, one 128-byte line each, after
+// the counters above. Stripe s holds a contiguous range of whole probes (probe slots run x
+// fastest, then y, then z, so a stripe is a slab of the grid); workgroups go round-robin over
+// the 8 XCDs, so the waves of one XCD start on one stripe and keep its slab of the world in
+// their L2.
+constexpr int kGiStripes = 8;
+constexpr int kGiStripe0 = 96;
+constexpr int kGiStripeWords = 32;
+constexpr int kGiCtrWords = kGiStripe0 + kGiStripes * kGiStripeWords;
+static_assert(kGiCtrs <= kGiStripe0, "counters overlap the stripes");
 
 // ----------------------------------------------------------------- helpers --
 __device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
@@ -161,6 +172,18 @@ __device__ __forceinline__ void bounce_dir(const float4 e, uint32_t f, float out
     out[ax == 0 ? 2 : ax - 1] = e.y;
 }
 
+// safe_inv of each component of bounce_dir(e, f): the same permutation and sign applied to
+// the entry's inverses ei, except that a tiny normal component inverts to +1e30 whatever
+// its sign (safe_inv's cut-off), so it keeps that value.
+__device__ __forceinline__ void bounce_inv(const float4 e, const float4 ei, uint32_t f,
+                                           float out[3]) {
+    const int ax = (int)(f >> 1);
+    const float sg = (f & 1u) ? -1.0f : 1.0f;
+    out[ax] = fabsf(e.z) < 1e-8f ? 1e30f : sg * ei.z;
+    out[ax == 2 ? 0 : ax + 1] = ei.x;
+    out[ax == 0 ? 2 : ax - 1] = ei.y;
+}
+
 // Material of a world voxel through the pyramid (0 outside the window).
 __device__ __forceinline__ uint32_t material_at(const DevWorld &w, const int32_t v[3]) {
     int32_t sl[3];
@@ -219,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params 
         ctr[3] = learned ? (par ^ 1u) : par;
         ctr[kCtrHeavy + 1] = hprev;
     }
+    if (j < (uint32_t)kGiStripes) ctr[kGiStripe0 + j * kGiStripeWords] = 0u;
     // this update's heavy-path bits start clear
     uint32_t *hmask_cur = hmask + (size_t)(ctr[kCtrUpd] & 1u) * hwords;
     for (uint32_t k = j; k < hwords; k += gridDim.x * kBlock) hmask_cur[k] = 0u;
@@ -291,10 +315,79 @@ struct PathSlots {
 // Learned mode: finished gather hits wait in a per-wave LDS ring of cache queries and go to
 // the global list 64 at a time (one atomic per run of 64, not per refill).
 constexpr uint32_t kQStage = 128;
+// Work ids of the path kernel, wave-uniform: ids [0, a) (training samples, then the previous
+// update's heavy samples) from the global counter ctr[0], then the natural samples [0, n)
+// as ids a + sample from the stripes. A wave holds a reservation [next, end) of at least
+// `chunk` ids (0: what the refill lacks) and moves to the next stripe when its own is used up.
+struct PathQueue {
+    uint32_t next, end;
+    uint32_t sc;   // current stripe
+    uint32_t left; // stripes not yet found used up
+    bool a_done, drained;
+};
+
+__device__ __forceinline__ bool pq_empty(const PathQueue &pq) {
+    return pq.drained && pq.next >= pq.end;
+}
+
+// Ids for `need` (<= 64) idle lanes: the idle lane of rank `rank` gets the returned id when
+// `want` is set (~0u: none this time). Every lane of the wave calls it together.
+__device__ __forceinline__ uint32_t pq_take(PathQueue &pq, uint32_t *ctr, uint32_t a, uint32_t n,
+                                            uint32_t slen, uint32_t need, uint32_t rank,
+                                            bool want, uint32_t lane, uint32_t chunk) {
+    const uint32_t avail = pq.end - pq.next;
+    uint32_t b = ~0u, e = 0u;
+    if (avail < need && !pq.drained) {
+        const uint32_t g = need - avail > chunk ? need - avail : chunk;
+        if (!pq.a_done) {
+            uint32_t r = 0;
+            if (lane == 0) r = atomicAdd(ctr, g);
+            r = __builtin_amdgcn_readfirstlane(__shfl(r, 0, 64));
+            if (r < a) {
+                b = r;
+                e = a - r < g ? a : r + g;
+            } else {
+                pq.a_done = true;
+            }
+        }
+        while (b == ~0u && pq.a_done && pq.left > 0u) {
+            const uint32_t s0 = pq.sc * slen < n ? pq.sc * slen : n;
+            const uint32_t s1 = n - s0 < slen ? n : s0 + slen;
+            uint32_t r = 0;
+            if (lane == 0) r = atomicAdd(ctr + kGiStripe0 + pq.sc * kGiStripeWords, g);
+            r = __builtin_amdgcn_readfirstlane(__shfl(r, 0, 64));
+            if (r < s1 - s0) {
+                b = a + s0 + r;
+                e = a + (s1 - s0 - r < g ? s1 : s0 + r + g);
+            } else {
+                pq.sc = pq.sc + 1u < (uint32_t)kGiStripes ? pq.sc + 1u : 0u;
+                pq.left--;
+            }
+        }
+        if (b == ~0u && pq.a_done && pq.left == 0u) pq.drained = true;
+    }
+    uint32_t id = ~0u;
+    if (want) {
+        if (rank < avail) id = pq.next + rank;
+        else if (b != ~0u && b + (rank - avail) < e) id = b + (rank - avail);
+    }
+    if (b != ~0u) {
+        const uint32_t nx = b + (need - avail);
+        pq.next = nx < e ? nx : e;
+        pq.end = e;
+    } else {
+        pq.next += avail < need ? avail : need;
+    }
+    return id;
+}
+
 template <uint32_t CAP> struct QueryStage {
     float4 q[kBlock / 64][CAP][2];
 };
 
+#ifndef VG_GI_SPEC
+#define VG_GI_SPEC true
+#endif
 template <bool COUNT, int BOUNCES>
 __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
@@ -307,7 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     __shared__ PathSlots sl;
     __shared__ QueryStage<BOUNCES == 3 ? kQStage : 1u> qs;
     __shared__ float4 spal[256]; // the albedo palette
+    __shared__ float ssinv[3];   // safe_inv of the sun direction, shared by every shadow ray
     for (uint32_t i = threadIdx.x; i < 256u; i += kBlock) spal[i] = pal[i];
+    if (threadIdx.x < 3u) ssinv[threadIdx.x] = safe_inv(p.sun[threadIdx.x]);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t me = threadIdx.x;
@@ -322,7 +417,6 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     uint32_t *cur_ids = ha.ids + (size_t)hp * ha.cap, *cur_mask = ha.mask + (size_t)hp * ha.words;
     const uint32_t *prev_ids = ha.ids + (size_t)(hp ^ 1u) * ha.cap;
     const uint32_t *prev_mask = ha.mask + (size_t)(hp ^ 1u) * ha.words;
-    const uint32_t total_ids = train_n + n_heavy + n_samples;
     uint32_t cnt[VG_IT_COUNT];
     uint32_t nray[3] = {0, 0, 0};
     if (COUNT)
@@ -338,8 +432,13 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     HitRec h;
     hit_miss(h);
     cache_reset(q);
-    IdQueue iq;
-    queue_reset(iq);
+    PathQueue pq;
+    pq.next = pq.end = 0u;
+    pq.sc = blockIdx.x % (uint32_t)kGiStripes; // the stripe of this workgroup's XCD
+    pq.left = (uint32_t)kGiStripes;
+    pq.a_done = pq.drained = false;
+    const uint32_t n_upd_probes = n_samples / N;
+    const uint32_t slen = ((n_upd_probes + kGiStripes - 1) / kGiStripes) * N; // whole probes
     bool emit = false;  // learned: this lane's finished gather hit still has to be listed
     bool heavy = false; // this lane's finished sample path still has to be recorded heavy
     uint32_t qhead = 0, qn = 0; // the wave's staged queries: ring start and count
@@ -363,11 +462,20 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         for (int k = 0; k < 3; ++k)
             out[k] = (&a.x)[k] * (p.sun_color[k] * sy + p.ambient[k] * p.ambient_tail);
     };
-    auto bounce_ray = [&](float d[3]) { // plain 2-bounce: the cosine bounce of sample sid
-        const uint32_t probe = (sid / N) * (uint32_t)p.update_div + phase;
-        bounce_dir(btab[bounce_index(p.seed, p.round, probe, sid % N)], sl.fm[me] & 0xFFu, d);
+    // a cosine bounce off face f from table entry i: direction and its inverses
+    // (iv null: the direction only)
+    auto bounce_from = [&](uint32_t i, uint32_t f, float d[3], float *iv) {
+        const float4 e = btab[i];
+        bounce_dir(e, f, d);
+        if (iv) bounce_inv(e, btab[VG_GI_BOUNCE_TABLE + i], f, iv);
     };
-    auto train_dir = [&](float d[3]) { bounce_dir(btab[sl.tb[me]], sl.fm[me] & 0xFFu, d); };
+    auto bounce_ray = [&](float d[3], float *iv) { // plain 2-bounce: the bounce of sample sid
+        const uint32_t probe = (sid / N) * (uint32_t)p.update_div + phase;
+        bounce_from(bounce_index(p.seed, p.round, probe, sid % N), sl.fm[me] & 0xFFu, d, iv);
+    };
+    auto train_dir = [&](float d[3], float *iv) {
+        bounce_from(sl.tb[me], sl.fm[me] & 0xFFu, d, iv);
+    };
     // a training target: the previous list's query, its bounce-table index, the traced light
     auto train_write = [&](const float t[3]) {
         const float4 *src = reinterpret_cast<const float4 *>(qprev + sl.tq[me]);
@@ -444,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             if (c != ~0u) sl.pc[me] = c + (uint32_t)it;
         }
         for (;;) {
-            float o[3], d[3];
+            float o[3], d[3], iv[3]; // the next ray and the inverses of its direction
             int ns = kIdle;
             bool after_x = false;
             if (ev == kNew) {
@@ -469,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         o[2] = a.z + face_n(f, 2) * 0.03f;
 #pragma unroll
                         for (int k = 0; k < 3; ++k) sl.so[k][me] = o[k];
-                        bounce_dir(btab[tbidx], f, d);
+                        bounce_from(tbidx, f, d, iv);
                         ns = kBounce;
                     }
                 } else {
@@ -484,10 +592,13 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         take = !((prev_mask[sid >> 5] >> (sid & 31u)) & 1u);
                     }
                     sl.pc[me] = 0u;
-                    // the probe's origin and flags and the direction load together
-                    const uint32_t sc = take ? sid : 0u;
+                    // the probe's origin and flags, the direction and its inverses load
+                    // together (and, in the natural order, with the heavy-mask word)
+                    const uint32_t sc = sid < n_samples ? sid : 0u;
                     const float4 po = pwork[sc / N];
-                    const float4 dd = dirs[(sc % N) * (uint32_t)p.epochs + epoch];
+                    const uint32_t di = (sc % N) * (uint32_t)p.epochs + epoch;
+                    const float4 dd = dirs[di];
+                    const float4 dv = dirs[N * (uint32_t)p.epochs + di];
                     if (take && !(__float_as_uint(po.w) & kProbeBuried)) {
                         o[0] = po.x;
                         o[1] = po.y;
@@ -495,6 +606,9 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         d[0] = dd.x;
                         d[1] = dd.y;
                         d[2] = dd.z;
+                        iv[0] = dv.x;
+                        iv[1] = dv.y;
+                        iv[2] = dv.z;
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
                             sl.go[k][me] = o[k];
@@ -520,7 +634,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     for (int k = 0; k < 3; ++k) sl.so[k][me] = o[k];
                     if (ndl > 0.0f) {
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) d[k] = p.sun[k];
+                        for (int k = 0; k < 3; ++k) {
+                            d[k] = p.sun[k];
+                            iv[k] = ssinv[k];
+                        }
                         ns = kShadowX;
                     } else {
                         sl.visX[me] = 0.0f;
@@ -532,8 +649,8 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                 after_x = true;
             } else if (ev == kBounce) {
                 // the bounce of a 2-bounce sample, or a learned lane's training path
-                if (BOUNCES == 3) train_dir(d);
-                else bounce_ray(d);
+                if (BOUNCES == 3) train_dir(d, nullptr);
+                else bounce_ray(d, nullptr);
                 if (!h.hit) {
                     float sky[3], t[3];
                     sky_color(p, d, sky);
@@ -550,7 +667,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                         const float bo[3] = {sl.so[0][me], sl.so[1][me], sl.so[2][me]};
                         gi_offset_origin(bo, d, h.t, f, o);
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) d[k] = p.sun[k];
+                        for (int k = 0; k < 3; ++k) {
+                            d[k] = p.sun[k];
+                            iv[k] = ssinv[k];
+                        }
                         ns = kShadowY;
                     } else {
                         float t[3];
@@ -571,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     finish_hit(lb);
                     emit = true;
                 } else if (BOUNCES == 2) {
-                    bounce_ray(d);
+                    bounce_ray(d, iv);
 #pragma unroll
                     for (int k = 0; k < 3; ++k) o[k] = sl.so[k][me];
                     ns = kBounce;
@@ -589,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                 return;
             }
             hit_miss(h);
-            if (walk_init(r, w, o, d)) {
+            if (walk_init(r, w, o, d, iv)) {
                 cache_reset(q);
                 it = 0;
                 stage = ns;
@@ -600,17 +720,19 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         }
     };
 
-    // Work ids come from ctr[0], `chunk` per atomic (samples of a few probes; 0: what the
-    // refill lacks), so a wave refills its idle lanes from its own reservation; training ids
-    // come first.
+    // Work ids come from the path queue, `chunk` per atomic (samples of a few probes; 0: what
+    // the refill lacks), so a wave refills its idle lanes from its own reservation: training
+    // samples and the previous update's heavy samples first, from one counter, then the
+    // natural order from the stripes.
     for (;;) {
         uint64_t act = __ballot(stage != kIdle);
         int nact = __popcll(act);
-        if (!queue_empty(iq) && nact < threshold) {
+        if (!pq_empty(pq) && nact < threshold) {
             if (BOUNCES == 3) stage_queries(false);
             flush_heavy();
-            const uint32_t id = queue_take(iq, ctr, total_ids, (uint32_t)(64 - nact),
-                                           lanes_below(~act), stage == kIdle, lane, chunk);
+            const uint32_t id = pq_take(pq, ctr, train_n + n_heavy, n_samples, slen,
+                                        (uint32_t)(64 - nact), lanes_below(~act),
+                                        stage == kIdle, lane, chunk);
             if (id != ~0u) {
                 sid = id;
                 stage = kNew;
@@ -618,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             nact = __popcll(__ballot(stage != kIdle));
         }
         if (nact == 0) {
-            if (queue_empty(iq)) {
+            if (pq_empty(pq)) {
                 if (BOUNCES == 3) stage_queries(true);
                 flush_heavy();
                 break;
@@ -629,7 +751,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             bool ended = stage == kNew;
             if (stage >= kGather) {
                 const bool shadow = stage == kShadowX || stage == kShadowY;
-                bool done = walk_iter<COUNT, true>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                bool done = walk_iter<COUNT, VG_GI_SPEC>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
                                                    shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
                                                    false, h, cnt);
                 ended = done || ++it >= max_iter;
@@ -662,7 +784,22 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
                                                         const uint8_t *__restrict__ face,
                                                         vg_probe *probes, uint32_t *ctr) {
     __shared__ float sums[kBlock / 8][20]; // per group: the 20 projected sums
+    // this epoch's gather directions and the sky radiance each of them sees (one evaluation
+    // per direction and block instead of one per sample that misses)
+    __shared__ float4 sdir[kBlock], ssky[kBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[kCtrUpd] += 1u; // the update's last launch
+    const uint32_t N = (uint32_t)p.rays_per_probe;
+    const bool staged = N <= kBlock;
+    if (staged && threadIdx.x < N) {
+        const float4 dd = dirs[threadIdx.x * (uint32_t)p.epochs + epoch];
+        const float d3[3] = {dd.x, dd.y, dd.z};
+        float sky[3];
+        sky_color(p, d3, sky);
+        sdir[threadIdx.x] = dd;
+        ssky[threadIdx.x] = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky, sky[2] * p.gi_sky,
+                                        p.gi_dist);
+    }
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t gl = lane & (L - 1u);                 // lane within the group
     const uint32_t grp = threadIdx.x / L;                // group within the block
@@ -670,7 +807,6 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     const bool live = j < n_upd;
     const uint32_t jj = live ? j : 0u;
     const int64_t idx = (int64_t)jj * p.update_div + phase;
-    const uint32_t N = (uint32_t)p.rays_per_probe;
     const uint32_t fl = __float_as_uint(pwork[jj].w);
     const bool buried = (fl & kProbeBuried) != 0;
     const float wpp = 12.566371f / (float)N; // 4 pi / N (uniform-sphere MC weight)
@@ -682,14 +818,19 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     if (live && !buried) {
         for (uint32_t k = gl; k < N; k += L) {
             const uint32_t s = jj * N + k;
-            const float4 dd = dirs[k * (uint32_t)p.epochs + epoch];
+            const float4 dd = staged ? sdir[k] : dirs[k * (uint32_t)p.epochs + epoch];
             const uint32_t fs = face[s];
             float4 v;
             if (fs == VG_FACE_NONE) { // no gather hit: the sky along the ray, at gi_dist
-                const float d3[3] = {dd.x, dd.y, dd.z};
-                float sky[3];
-                sky_color(p, d3, sky);
-                v = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky, sky[2] * p.gi_sky, p.gi_dist);
+                if (staged) {
+                    v = ssky[k];
+                } else {
+                    const float d3[3] = {dd.x, dd.y, dd.z};
+                    float sky[3];
+                    sky_color(p, d3, sky);
+                    v = make_float4(sky[0] * p.gi_sky, sky[1] * p.gi_sky, sky[2] * p.gi_sky,
+                                    p.gi_dist);
+                }
             } else {
                 v = rad[s];
             }
@@ -1295,14 +1436,15 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
     pr->hwords = (uint32_t)((ns + 31) / 32);
     pr->hcap = (uint32_t)(ns / 32 > 4096 ? ns / 32 : 4096);
     bool ok = hipMalloc((void **)&pr->probes, (size_t)total * sizeof(vg_probe)) == hipSuccess &&
-              hipMalloc((void **)&pr->dirs, nd * sizeof(float4)) == hipSuccess &&
-              hipMalloc((void **)&pr->bounce, VG_GI_BOUNCE_TABLE * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->dirs, 2 * nd * sizeof(float4)) == hipSuccess &&
+              hipMalloc((void **)&pr->bounce, 2 * VG_GI_BOUNCE_TABLE * sizeof(float4)) ==
+                  hipSuccess &&
               hipMalloc((void **)&pr->palette, 256 * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->pwork, (size_t)total * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->rad, ns * sizeof(float4)) == hipSuccess &&
               hipMalloc((void **)&pr->face, ns) == hipSuccess &&
               hipMalloc((void **)&pr->queries, 2 * ns * sizeof(vg_nrc_query)) == hipSuccess &&
-              hipMalloc((void **)&pr->ctr, kGiCtrs * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void **)&pr->ctr, kGiCtrWords * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void **)&pr->heavy, 2 * (size_t)pr->hcap * sizeof(uint32_t)) ==
                   hipSuccess &&
               hipMalloc((void **)&pr->hmask, 2 * (size_t)pr->hwords * sizeof(uint32_t)) ==
@@ -1315,18 +1457,29 @@ int vg_probes_create(vg_world *world, const vg_gi_params *p, vg_probes **out) {
         return fail(VG_E_NOMEM, "probe storage for %lld probes x %d rays", (long long)total,
                     p->rays_per_probe);
     }
-    std::vector<float> tab(nd * 4);
+    // each table is followed by the walk's per-component inverses of its entries (safe_inv,
+    // in host IEEE single precision: the device's correctly rounded divide gives the same)
+    auto add_inverses = [](std::vector<float> &t, size_t n) {
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 4; ++k) {
+                const float x = t[4 * i + k];
+                t[4 * (n + i) + k] = k < 3 ? (std::fabs(x) < 1e-8f ? 1e30f : 1.0f / x) : 0.0f;
+            }
+    };
+    std::vector<float> tab(2 * nd * 4);
     vg_gi_directions((int32_t)nd, tab.data());
-    std::vector<float> bt(VG_GI_BOUNCE_TABLE * 4), pal(256 * 4);
+    add_inverses(tab, nd);
+    std::vector<float> bt(2 * VG_GI_BOUNCE_TABLE * 4), pal(256 * 4);
     vg_gi_bounce_table(bt.data());
+    add_inverses(bt, VG_GI_BOUNCE_TABLE);
     vg_gi_default_palette(pal.data());
-    hipError_t e = hipMemcpy(pr->dirs, tab.data(), nd * sizeof(float4), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(pr->dirs, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(pr->bounce, bt.data(), bt.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(pr->palette, pal.data(), pal.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(pr->probes, 0, (size_t)total * sizeof(vg_probe));
-    if (e == hipSuccess) e = hipMemset(pr->ctr, 0, kGiCtrs * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(pr->ctr, 0, kGiCtrWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(pr->hmask, 0, 2 * (size_t)pr->hwords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {

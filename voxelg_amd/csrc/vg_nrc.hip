@@ -231,10 +231,13 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query 
                                                    const float4 *__restrict__ pal) {
     __shared__ u32x4_t sf[kNrcFrags * 64];
     const vg_nrc_query *q = qb + (sel ? (size_t)sel[0] * q_stride : (size_t)0);
-    load_frags(sf, m.frag_fwd);
-    __syncthreads();
     uint32_t n = n_dev ? *n_dev : n_max;
     n = n < n_max ? n : n_max;
+    // the grid is sized for n_max; a block with no tile leaves before staging the weights
+    // (the whole block together, so no barrier is left waiting)
+    if ((uint64_t)blockIdx.x * 256u >= n) return;
+    load_frags(sf, m.frag_fwd);
+    __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const uint32_t wave = threadIdx.x >> 6;
     for (uint32_t tile = blockIdx.x * 4u + wave; (uint64_t)tile * 64u < n; tile += gridDim.x * 4u) {
