@@ -125,6 +125,8 @@ def parity_report(wl, cam, sun, ow, got, occ, r0, s0, r1, s1):
 
 
 TRAFFIC_PROFILE = "profiles/k_frame_persistent_large_1080p_pmc.md"
+BATCHED_TRAFFIC_PROFILE = "profiles/k_batches_persistent_batched_pmc.md"
+GI_TRAFFIC_PROFILE = "profiles/k_probe_paths_learned_gi_pmc.md"
 
 
 COUNTERS_PROFILE = "profiles/cache_hit_rates_and_mfma_util_pmc.md"
@@ -142,11 +144,11 @@ def profile_value(profile, key):
     return float(m.group(1)) if m else None
 
 
-def measured_traffic():
-    """DRAM-side bytes per frame of k_beam + k_frame_persistent on this workload, from the
-    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (copy-calibrated); None when the
-    summary is absent."""
-    v = profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_frame")
+def measured_traffic(profile=None, key="traffic_bytes_per_frame"):
+    """DRAM-side bytes per launch of a workload's dominant kernel, from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (copy-calibrated); None when the summary is
+    absent. Default: k_beam + k_frame_persistent per headline frame."""
+    v = profile_value(profile or TRAFFIC_PROFILE, key)
     return None if v is None else int(v)
 
 
@@ -456,7 +458,10 @@ def bench_batched(args):
                    "world": list(world.dims), "rays": bs.total, "batches": bs.n,
                    "schedule": dict(sched, kernel="persistent ray-compacting (batches)")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(BATCHED_TRAFFIC_PROFILE,
+                                                 "traffic_bytes_per_step"),
+                     "traffic_source": BATCHED_TRAFFIC_PROFILE,
                      "kernel": "k_batches_persistent",
                      "kernel_ms": round(float(np.mean(kern_ms)), 4),
                      "algorithmic_bytes": int(bytes_per_step)},
@@ -663,7 +668,9 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         "launch_ms": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv
                           for kk, vv in v.items()} for k, v in kernel_ms.items()},
         "roofline": {"bound": "hbm", "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(GI_TRAFFIC_PROFILE, "traffic_bytes_per_update"),
+                     "traffic_source": GI_TRAFFIC_PROFILE,
                      "kernel": "k_probe_paths<false, 3> (the learned update's path launch, "
                                "inference-only update)",
                      "kernel_ms": round(path_ms, 4), "algorithmic_bytes": int(path_bytes_frozen)},
