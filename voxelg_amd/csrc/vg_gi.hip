@@ -396,9 +396,13 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     uint8_t *__restrict__ face_out, LearnedArgs la, HeavyArgs ha, uint32_t *ctr, int threshold,
     uint32_t chunk, unsigned long long *counts) {
     // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail;
-    // the lanes' kBounce / kShadowY stages then trace training targets)
+    // the lanes' kBounce / kShadowY stages then trace training targets), 4 learned without
+    // training targets (an update that takes no training step: no bounce stages compiled in)
+    constexpr bool LEARNED = BOUNCES >= 3;
+    constexpr bool TRAIN = BOUNCES == 3;
+    constexpr bool BOUNCE_STAGES = BOUNCES == 2 || TRAIN;
     __shared__ PathSlots sl;
-    __shared__ QueryStage<BOUNCES == 3 ? kQStage : 1u> qs;
+    __shared__ QueryStage<LEARNED ? kQStage : 1u> qs;
     __shared__ float4 spal[256]; // the albedo palette
     __shared__ float ssinv[3];   // safe_inv of the sun direction, shared by every shadow ray
     for (uint32_t i = threadIdx.x; i < 256u; i += kBlock) spal[i] = pal[i];
@@ -407,11 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t me = threadIdx.x;
     const uint32_t wv = threadIdx.x >> 6;
-    const uint32_t par = BOUNCES == 3 ? ctr[3] : 0u;
+    const uint32_t par = LEARNED ? ctr[3] : 0u;
     vg_nrc_query *qcur = la.queries + (size_t)par * la.half;
     const vg_nrc_query *qprev = la.queries + (size_t)(par ^ 1u) * la.half;
-    const uint32_t nprev = BOUNCES == 3 ? ctr[2] : 0u;
-    const uint32_t train_n = BOUNCES == 3 ? la.train_n : 0u;
+    const uint32_t nprev = TRAIN ? ctr[2] : 0u;
+    const uint32_t train_n = TRAIN ? la.train_n : 0u;
     const uint32_t n_heavy = ctr[kCtrHeavy + 1] < ha.cap ? ctr[kCtrHeavy + 1] : ha.cap;
     const uint32_t hp = ctr[kCtrUpd] & 1u; // this update's half of the heavy lists
     uint32_t *cur_ids = ha.ids + (size_t)hp * ha.cap, *cur_mask = ha.mask + (size_t)hp * ha.words;
@@ -556,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             int ns = kIdle;
             bool after_x = false;
             if (ev == kNew) {
-                if (BOUNCES == 3 && sid < train_n) {
+                if (TRAIN && sid < train_n) {
                     sl.pc[me] = ~0u;
                     if (nprev == 0u) { // no hits listed yet: a zero-weight target
                         float4 *dst = reinterpret_cast<float4 *>(la.tq + sid);
@@ -647,16 +651,16 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             } else if (ev == kShadowX) {
                 sl.visX[me] = h.hit ? 0.0f : 1.0f;
                 after_x = true;
-            } else if (ev == kBounce) {
+            } else if (BOUNCE_STAGES && ev == kBounce) {
                 // the bounce of a 2-bounce sample, or a learned lane's training path
-                if (BOUNCES == 3) train_dir(d, nullptr);
+                if (TRAIN) train_dir(d, nullptr);
                 else bounce_ray(d, nullptr);
                 if (!h.hit) {
                     float sky[3], t[3];
                     sky_color(p, d, sky);
 #pragma unroll
                     for (int k = 0; k < 3; ++k) t[k] = sky[k] * p.gi_sky;
-                    if (BOUNCES == 3) train_write(t);
+                    if (TRAIN) train_write(t);
                     else finish_hit(t);
                 } else {
                     const uint32_t f = h.face;
@@ -675,18 +679,18 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
                     } else {
                         float t[3];
                         bounce_light(0.0f, t);
-                        if (BOUNCES == 3) train_write(t);
+                        if (TRAIN) train_write(t);
                         else finish_hit(t);
                     }
                 }
-            } else { // kShadowY
+            } else if (BOUNCE_STAGES) { // kShadowY
                 float t[3];
                 bounce_light(h.hit ? 0.0f : 1.0f, t);
-                if (BOUNCES == 3) train_write(t);
+                if (TRAIN) train_write(t);
                 else finish_hit(t);
             }
             if (after_x) {
-                if (BOUNCES == 3) { // the cache supplies the bounce: list the hit
+                if (LEARNED) { // the cache supplies the bounce: list the hit
                     const float lb[3] = {0.0f, 0.0f, 0.0f};
                     finish_hit(lb);
                     emit = true;
@@ -728,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         uint64_t act = __ballot(stage != kIdle);
         int nact = __popcll(act);
         if (!pq_empty(pq) && nact < threshold) {
-            if (BOUNCES == 3) stage_queries(false);
+            if (LEARNED) stage_queries(false);
             flush_heavy();
             const uint32_t id = pq_take(pq, ctr, train_n + n_heavy, n_samples, slen,
                                         (uint32_t)(64 - nact), lanes_below(~act),
@@ -741,7 +745,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         }
         if (nact == 0) {
             if (pq_empty(pq)) {
-                if (BOUNCES == 3) stage_queries(true);
+                if (LEARNED) stage_queries(true);
                 flush_heavy();
                 break;
             }
@@ -1290,13 +1294,17 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
                        pr->palette, pr->rad, pr->face, la, ha, pr->ctr, thr,                    \
                        (uint32_t)w->ctx->probe_chunk, cd)
-    const int mode = nrc ? 3 : p->bounces;
+    // a learned update that takes no training step runs the instantiation without the
+    // training stages
+    const int mode = nrc ? (la.train_n ? 3 : 4) : p->bounces;
     if (count) {
-        if (mode == 3) LAUNCH_G(true, 3);
+        if (mode == 4) LAUNCH_G(true, 4);
+        else if (mode == 3) LAUNCH_G(true, 3);
         else if (mode == 2) LAUNCH_G(true, 2);
         else LAUNCH_G(true, 1);
     } else {
-        if (mode == 3) LAUNCH_G(false, 3);
+        if (mode == 4) LAUNCH_G(false, 4);
+        else if (mode == 3) LAUNCH_G(false, 3);
         else if (mode == 2) LAUNCH_G(false, 2);
         else LAUNCH_G(false, 1);
     }
