@@ -124,6 +124,69 @@ def parity_report(wl, cam, sun, ow, got, occ, r0, s0, r1, s1):
                       "shadow": cs}}
 
 
+def parity_configs(ctx, threads):
+    """The other two traversal configurations against the CPU traversals, outside the timed
+    region: config 1 (the 1280x720 frame of the 256x128x256 world, primary tmax 512, LOD off,
+    plus a sun shadow ray per hit, tmax 480) and config 3 (one step of the 96 uneven batches
+    over the same world): the GPU's records bit for bit against R1, and against R0
+    (src/raycast.rs) by the rule of the reference's accel_matches_cpu_raycast test
+    (src/accel.rs:655-718), with the edge-tie and tmax-tie counts."""
+    import numpy as np
+
+    import oracle
+    from voxelg_amd import workloads
+    from voxelg_amd.device import DevicePtr, decode_hits
+
+    def summary(c):
+        return {"compared": c["rays"], "agree": c["agree"], "agree_clean": c["agree_clean"],
+                "ties": c["ties"], "edge_tie_disagreements": c["disagree_ties"],
+                "tmax_ties": c["tmax_ties"], "clean_disagree": c["clean_disagree"],
+                "face_mismatch": c["face_mismatch"], "material_mismatch": c["material_mismatch"],
+                "max_dt": c["max_dt"]}
+
+    wl = workloads.SMALL
+    world = wl.world(threads)
+    dw = ctx.world(world.dims).upload(world)
+    ow = oracle.OracleWorld.view(world)
+    _, cam = wl.camera()
+    sun, stmax = workloads.LARGE.sun(), 480.0
+    W, H = wl.width, wl.height
+    hits, occ = dw.trace_frame(cam, W, H, wl.primary_tmax, 0, sun, stmax)
+    ctx.synchronize()
+    got, occ = decode_hits(hits), occ.numpy(np.uint8)
+    r1, s1, _ = ow.frame(cam, W, H, wl.primary_tmax, 0, sun, stmax, threads=threads)
+    r0, s0 = ow.frame_r0(cam, W, H, wl.primary_tmax, sun, stmax, threads=threads)
+    dirs = oracle.primary_dirs(cam, W, H)
+    c1 = oracle.classify_vs_r0(cam[:3], dirs, got, r0, tmax=wl.primary_tmax)
+    same = (got["hit"] != 0) & (r0["hit"] != 0) & np.all(got["voxel"] == r0["voxel"], axis=1) \
+        & (got["face"] == r0["face"]) & (got["t"] == r0["t"])
+    cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, got, occ, s0, sun, stmax, same)
+    out = {"config1": {"rays": int(len(got)),
+                       "vs_r1_bit_exact": bool(got.tobytes() == r1.tobytes() and
+                                               np.array_equal(occ, s1)),
+                       "vs_r0": dict(summary(c1), shadow=cs)}}
+    rays, sizes = workloads.BATCHED.rays(world)
+    rbuf = ctx.rays(rays)
+    hbuf = ctx.alloc(len(rays) * 20)
+    batches, off = [], 0
+    for n in sizes:
+        batches.append((DevicePtr(rbuf, off * 32), DevicePtr(hbuf, off * 20), n, 0))
+        off += n
+    bs = dw.batch_set(batches)
+    dw.trace_batches(bs)
+    ctx.synchronize()
+    got3 = decode_hits(hbuf)
+    ref3, _ = ow.trace_rays(rays, 0, threads)
+    r03 = ow.raycast_rays(rays, threads)
+    c3 = oracle.classify_vs_r0(rays["o"], oracle.normalize_or_zero_v(rays["d"]), got3, r03,
+                               tmax=rays["tmax"])
+    out["config3"] = {"rays": int(len(rays)), "batches": int(bs.n),
+                      "vs_r1_bit_exact": bool(got3.tobytes() == ref3.tobytes()),
+                      "vs_r0": summary(c3)}
+    dw.close()
+    return out
+
+
 TRAFFIC_PROFILE = "profiles/k_frame_persistent_large_1080p_pmc.md"
 BATCHED_TRAFFIC_PROFILE = "profiles/k_batches_persistent_batched_pmc.md"
 GI_TRAFFIC_PROFILE = "profiles/k_probe_paths_learned_gi_pmc.md"
@@ -319,6 +382,7 @@ def main():
     if not args.no_cpu_baseline:
         base, parity = cpu_baseline(world, wl, cam, sun, hh, occ.numpy(np.uint8))
         log(f"cpu baseline: {base}")
+        parity["other_configs"] = parity_configs(ctx, base["threads"])
         log(f"parity: {parity}")
 
     out = {
