@@ -81,6 +81,16 @@ def test_reference_defaults_one_bounce_matches_oracle(ctx, small, small_world):
     assert np.count_nonzero(ref[live][:, 45:48]) > 0  # some probes relocated
 
 
+@pytest.mark.parametrize("grid,rays,div,bounces", [((2, 1, 2), 8, 1, 2), ((4, 2, 4), 1, 3, 1),
+                                                    ((2, 2, 2), 24, 4, 2), ((3, 1, 3), 5, 2, 1)])
+def test_small_updates_match_oracle(ctx, small, small_world, grid, rays, div, bounces):
+    # updates of fewer probes than the path kernel's 8 work stripes (empty stripes, ragged
+    # last stripe), odd ray counts and partial updates (update_div > 1) over several rounds
+    p = gi.gi_params(SUN, grid=grid, spacing=32, rays_per_probe=rays, epochs=1,
+                     update_div=div, bounces=bounces)
+    run_rounds(ctx, small, small_world, p, range(2 * div))
+
+
 def test_ray_counts_match_oracle(ctx, small, small_world):
     p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=64, epochs=1,
                      update_div=1, bounces=2)
