@@ -98,8 +98,16 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
             bool occ = false;
             const bool tests = kind == 0 && j < total;
             if (tests) {
-                const int32_t tw[3] = {lo[0] + j % n[0], lo[1] + (j / n[0]) % n[1],
-                                       lo[2] + j / (n[0] * n[1])};
+                // j < 8 over a box of n[0] x n[1] x n[2] <= 8 tiles, each n <= 4: the
+                // quotients come from a float reciprocal ((j + 0.5) / m lies at least 1/16
+                // from any integer for m <= 8, far beyond v_rcp_f32's error), not from
+                // integer division
+                const int32_t n01 = n[0] * n[1];
+                const int32_t q0 = (int32_t)(((float)j + 0.5f) * __builtin_amdgcn_rcpf((float)n[0]));
+                const int32_t q1 = (int32_t)(((float)q0 + 0.5f) * __builtin_amdgcn_rcpf((float)n[1]));
+                const int32_t q2 = (int32_t)(((float)j + 0.5f) * __builtin_amdgcn_rcpf((float)n01));
+                const int32_t tw[3] = {lo[0] + (j - q0 * n[0]), lo[1] + (q0 - q1 * n[1]),
+                                       lo[2] + q2};
                 int32_t sl[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
