@@ -215,20 +215,30 @@ __device__ __forceinline__ void walk_advance(Walk &r, const int32_t sl[3], int32
                 e = e > first ? first : e;
                 e = e < last ? last : e;
             }
-            // bounded by the cell width; normally one test each way
-            for (int g = 0; g < 258 && e != first; ++g) {
-                float t = t_plane(r, k, e);
-                if (t < ts || (t == ts && k < ax)) break;
-                e -= st;
-            }
-            float nxt = tf[k]; // the cell face, if e ends on the last in-cell voxel
-            for (int g = 0; g < 258 && e != last; ++g) {
-                float t = t_plane(r, k, e + st);
-                if (!(t < ts || (t == ts && k < ax))) {
-                    nxt = t;
-                    break;
+            // the estimate checked from both sides at once: the crossing into e is before
+            // the exit (or e is the first voxel) and the crossing out of it is not (or e is
+            // the last in-cell voxel, whose crossing out is the cell face tf[k]); then the
+            // corrections below would change nothing and e, with that crossing, is the result
+            const float te = t_plane(r, k, e), tx = t_plane(r, k, e + st);
+            const bool lo_ok = e == first || te < ts || (te == ts && k < ax);
+            const bool hi_ok = e == last || !(tx < ts || (tx == ts && k < ax));
+            float nxt = e == last ? tf[k] : tx;
+            if (!(lo_ok && hi_ok)) {
+                // bounded by the cell width
+                for (int g = 0; g < 258 && e != first; ++g) {
+                    float t = t_plane(r, k, e);
+                    if (t < ts || (t == ts && k < ax)) break;
+                    e -= st;
                 }
-                e += st;
+                nxt = tf[k]; // the cell face, if e ends on the last in-cell voxel
+                for (int g = 0; g < 258 && e != last; ++g) {
+                    float t = t_plane(r, k, e + st);
+                    if (!(t < ts || (t == ts && k < ax))) {
+                        nxt = t;
+                        break;
+                    }
+                    e += st;
+                }
             }
             r.l[k] = e;
             r.tn[k] = nxt;
