@@ -263,11 +263,17 @@ __device__ __forceinline__ void walk_advance_to(Walk &r, float tb) {
             const int32_t nearest = r.l[k] + st;
             int32_t e = (int32_t)floorf(r.org[k] + r.dir[k] * tb);
             if (st > 0 ? e < nearest : e > nearest) e = nearest;
-            for (int g = 0; g < 4096 && e != nearest && !(t_plane(r, k, e) < tb); ++g) e -= st;
-            for (int g = 0; g < 4096 && t_plane(r, k, e + st) < tb; ++g) e += st;
-            const float tk = t_plane(r, k, e);
+            // the estimate checked from both sides at once; the corrections only when it is off
+            float tk = t_plane(r, k, e), tx = t_plane(r, k, e + st);
+            if (!((e == nearest || tk < tb) && !(tx < tb))) {
+                for (int g = 0; g < 4096 && e != nearest && !(t_plane(r, k, e) < tb); ++g)
+                    e -= st;
+                for (int g = 0; g < 4096 && t_plane(r, k, e + st) < tb; ++g) e += st;
+                tk = t_plane(r, k, e);
+                tx = t_plane(r, k, e + st);
+            }
             r.l[k] = e;
-            r.tn[k] = t_plane(r, k, e + st);
+            r.tn[k] = tx;
             if (tk > tl || (tk == tl && k > la)) {
                 tl = tk;
                 la = k;
