@@ -402,10 +402,10 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     int64_t pix = 0;
     int it = 0;
     Walk r;
-    WalkCache q;
+    WalkPhase q;
     HitRec h;
     hit_miss(h);
-    cache_reset(q);
+    phase_reset(q);
     bool exhausted = false;
     for (;;) {
         uint64_t act = __ballot(kind != 0);
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                                 const float tb = beam_t[blk];
                                 if (tb > 0.0f && r.t_cur <= ptmax) walk_advance_to(r, tb);
                             }
-                            cache_reset(q);
+                            phase_reset(q);
                             it = 0;
                             kind = 1;
                         } else {
@@ -456,8 +456,8 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                 // primary and shadow rays at once does not execute two walk bodies
                 const bool prim = kind == 1;
                 bool done = walk_iter<COUNT>(r, q, w, prim ? ptmax : stmax,
-                                             prim ? pflags : (uint32_t)VG_RAY_ANY_HIT, prim && plod,
-                                             h, cnt);
+                                                  prim ? pflags : (uint32_t)VG_RAY_ANY_HIT,
+                                                  prim && plod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     if (kind == 1) {
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                                 shadow_origin(co, r.dir, h, so);
                                 hit_miss(h);
                                 if (walk_init(r, w, so, sd, si)) {
-                                    cache_reset(q);
+                                    phase_reset(q);
                                     it = 0;
                                     kind = 2;
                                     started = true;
@@ -594,15 +594,15 @@ __global__ __launch_bounds__(kBlock) void k_water_persistent(
     int it = 0;
     float fo[3] = {0.0f, 0.0f, 0.0f}, fd[3] = {0.0f, 0.0f, 0.0f}; // the pending refraction ray
     Walk r;
-    WalkCache q;
+    WalkPhase q;
     HitRec h;
     hit_miss(h);
-    cache_reset(q);
+    phase_reset(q);
     auto start = [&](const float o[3], const float draw[3], int k) -> bool {
         float d[3];
         hit_miss(h);
         if (!normalize_or_zero(draw, d) || !walk_init(r, w, o, d)) return false;
-        cache_reset(q);
+        phase_reset(q);
         it = 0;
         kind = k;
         return true;
@@ -646,8 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_water_persistent(
         }
         do {
             if (kind != 0) {
-                // incoherent rays: fetch every level's word of a step at once
-                bool done = walk_iter<false, true>(r, q, w, kSecondaryMaxT,
+                bool done = walk_iter<false>(r, q, w, kSecondaryMaxT,
                                                    VG_RAY_SKIP_TRANSPARENT, false, h, nullptr);
                 done = done || ++it >= max_iter;
                 if (done) {
@@ -684,10 +683,10 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
     bool lod = false;
     int it = 0;
     Walk r;
-    WalkCache q;
+    WalkPhase q;
     HitRec h;
     hit_miss(h);
-    cache_reset(q);
+    phase_reset(q);
     IdQueue iq;
     queue_reset(iq);
     for (;;) {
@@ -708,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
                     lod = (f & VG_RAY_LOD) && !(f & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
                     hit_miss(h);
                     if (normalize_or_zero(din, d) && walk_init(r, w, o, d)) {
-                        cache_reset(q);
+                        phase_reset(q);
                         it = 0;
                         busy = true;
                     } else {
@@ -724,8 +723,7 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
         }
         do {
             if (busy) {
-                // incoherent rays: fetch every level's word of a step at once
-                bool done = walk_iter<COUNT, true>(r, q, w, tmax, f, lod, h, cnt);
+                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     store_hit(hits + id, h);
@@ -761,10 +759,10 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
     bool lod = false;
     int it = 0;
     Walk r;
-    WalkCache q;
+    WalkPhase q;
     HitRec h;
     hit_miss(h);
-    cache_reset(q);
+    phase_reset(q);
     IdQueue iq;
     queue_reset(iq);
     for (;;) {
@@ -794,7 +792,7 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
                     lod = (f & VG_RAY_LOD) && !(f & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
                     hit_miss(h);
                     if (normalize_or_zero(din, d) && walk_init(r, w, o, d)) {
-                        cache_reset(q);
+                        phase_reset(q);
                         it = 0;
                         busy = true;
                     } else {
@@ -811,8 +809,7 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
         }
         do {
             if (busy) {
-                // incoherent rays: fetch every level's word of a step at once
-                bool done = walk_iter<COUNT, true>(r, q, w, tmax, f, lod, h, cnt);
+                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     store_hit(out, h);

@@ -385,9 +385,6 @@ template <uint32_t CAP> struct QueryStage {
     float4 q[kBlock / 64][CAP][2];
 };
 
-#ifndef VG_GI_SPEC
-#define VG_GI_SPEC true
-#endif
 template <bool COUNT, int BOUNCES>
 __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
@@ -432,10 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     uint32_t sid = 0; // sample id (training lane: batch index; kNew: the raw work id)
     int it = 0;
     Walk r;
-    WalkCache q;
+    WalkPhase q;
     HitRec h;
     hit_miss(h);
-    cache_reset(q);
+    phase_reset(q);
     PathQueue pq;
     pq.next = pq.end = 0u;
     pq.sc = blockIdx.x % (uint32_t)kGiStripes; // the stripe of this workgroup's XCD
@@ -714,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             }
             hit_miss(h);
             if (walk_init(r, w, o, d, iv)) {
-                cache_reset(q);
+                phase_reset(q);
                 it = 0;
                 stage = ns;
                 if (COUNT) nray[ns == kGather ? 0 : (ns == kBounce ? 2 : 1)]++;
@@ -755,7 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             bool ended = stage == kNew;
             if (stage >= kGather) {
                 const bool shadow = stage == kShadowX || stage == kShadowY;
-                bool done = walk_iter<COUNT, VG_GI_SPEC>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                bool done = walk_iter<COUNT>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
                                                    shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
                                                    false, h, cnt);
                 ended = done || ++it >= max_iter;
@@ -951,10 +948,10 @@ __global__ __launch_bounds__(kBlock) void k_nrc_targets(
     float ndlY = 0.0f;
     int it = 0;
     Walk r;
-    WalkCache q;
+    WalkPhase q;
     HitRec h;
     hit_miss(h);
-    cache_reset(q);
+    phase_reset(q);
     bool exhausted = false;
     auto write = [&](const float lb[3]) {
         float4 *dst = reinterpret_cast<float4 *>(tq + j);
@@ -982,7 +979,7 @@ __global__ __launch_bounds__(kBlock) void k_nrc_targets(
     auto start = [&](const float o[3], const float d[3], int st) -> bool {
         hit_miss(h);
         if (!walk_init(r, w, o, d)) return false;
-        cache_reset(q);
+        phase_reset(q);
         it = 0;
         stage = st;
         if (COUNT) nray[st == kBounce ? 2 : 1]++;

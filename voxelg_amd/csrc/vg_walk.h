@@ -335,39 +335,28 @@ __device__ __forceinline__ uint32_t chunk_index(const DevWorld &w, const int32_t
 
 // The walk: phase 1 leaves solid voxels the ray starts in (src/raycast.rs:108-115),
 // phase 2 returns the first solid voxel. Each iteration tests the coarsest level and
-// skips the largest empty aligned cell (shaders/raymarch.wgsl:4626-4775). The words of
-// the current L4 cell, chunk, tile and brick stay in registers until the walk leaves
-// them. COUNT tallies iteration classes for algorithmic-byte accounting.
-// Register-cached pyramid words of the cells holding the current voxel, and the phase.
-struct WalkCache {
-    uint32_t cur_l4, cur_c, cur_t, cur_b;
-    uint64_t l4w, cw, tw, ow;
-    uint32_t tu, bu;
+// skips the largest empty aligned cell (shaders/raymarch.wgsl:4626-4775, plus exact 2x2x2
+// groups of empty children). COUNT tallies iteration classes for algorithmic-byte
+// accounting.
+// The phase a ray carries from one walk iteration to the next besides its Walk.
+struct WalkPhase {
     bool phase1;
 };
 
-__device__ __forceinline__ void cache_reset(WalkCache &q) {
-    q.cur_l4 = q.cur_c = q.cur_t = q.cur_b = ~0u;
-    q.l4w = q.cw = q.tw = q.ow = 0;
-    q.tu = q.bu = 0;
-    q.phase1 = true;
-}
+__device__ __forceinline__ void phase_reset(WalkPhase &q) { q.phase1 = true; }
 
-// One iteration of the walk: phase 1 leaves solid voxels the ray starts in
-// (src/raycast.rs:108-115), phase 2 returns the first solid voxel. The iteration tests the
-// coarsest level first and leaves the largest empty aligned cell (the reference's levels,
-// shaders/raymarch.wgsl:4626-4775, plus exact 2x2x2 groups of empty children). The words of
-// the current L4 cell, chunk, tile and brick stay in registers until the walk leaves them.
-// Returns true when the walk has ended (h then holds the hit, or stays a miss). COUNT
-// tallies iteration classes for algorithmic-byte accounting. With SPEC the L4, chunk, tile
-// and brick words of the current voxel are all fetched up front (every level's index is a
-// function of the voxel coordinates alone and every level is stored dense, so each fetch is
-// in bounds): one memory round trip per iteration instead of a chain of dependent ones, for
-// kernels whose rays are too incoherent to hide that chain (batched queries, probe paths,
-// water secondary rays: 1.5-5 % faster; the coherent frame kernel is issue-bound and 9 %
-// slower with it). The decisions are unchanged.
-template <bool COUNT, bool SPEC = false>
-__device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld &w, float tmax,
+// One iteration of the walk. Returns true when the walk has ended (h then holds the hit,
+// or stays a miss). Written for issue and latency: the L4, chunk, tile and brick words of
+// the current voxel (every level's index is a function of the voxel coordinates alone and
+// every level is stored dense, so each fetch is in bounds) are fetched together, one memory
+// round trip per iteration, and the level the walk stops at, the cell it leaves and the
+// phase are picked with selects instead of a chain of divergent branches; only the reads a
+// decision needs (the LOD probes' topmost material, a set voxel's material) stay behind
+// branches. Keeping the words of the cells the walk has not left in registers and fetching
+// only on a change, or testing level by level with branches, is 6-30 % slower on the probe
+// paths and batched queries and no faster on the coherent frame.
+template <bool COUNT>
+__device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld &w, float tmax,
                                           uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
     if (r.t_cur > tmax) return true;
     if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
@@ -381,150 +370,92 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkCache &q, const DevWorld 
             sl[k] = s >= w.dims[k] ? s - w.dims[k] : s;
         }
     }
-    if (SPEC) {
-        const uint32_t l4i = l4_index(w, sl);
-        const uint32_t ci = chunk_index(w, sl);
-        const uint32_t tn = ci * 64u + (uint32_t)cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3,
-                                                      (sl[2] >> 4) & 3);
-        const uint32_t bn = tn * 64u + (uint32_t)cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3,
-                                                      (sl[2] >> 2) & 3);
-        // unconditional loads, all consumed by one empty asm statement: the compiler can
-        // neither sink them into the branches that use them nor wait between them
-        q.l4w = w.l4[l4i];
-        q.cw = w.chunk[ci];
-        q.tw = w.tile[tn];
-        q.tu = w.tile_uni[tn];
-        q.ow = w.occ[bn];
-        q.bu = w.brick_uni[bn];
-        asm volatile("" ::"v"(q.l4w), "v"(q.cw), "v"(q.tw), "v"(q.tu), "v"(q.ow), "v"(q.bu));
-        q.cur_l4 = l4i;
-        q.cur_c = ci;
-        q.cur_t = tn;
-        q.cur_b = bn;
-    }
-    int32_t S = 1; // size of the empty (or phase-1 solid) cell to leave; 1 = one voxel
-    do {
-        const uint32_t l4i = l4_index(w, sl);
-        if (l4i != q.cur_l4) {
-            q.cur_l4 = l4i;
-            q.l4w = w.l4[l4i];
-        }
-        if (q.l4w == 0) {
-            if (COUNT) cnt[VG_IT_L4_SKIP]++;
-            q.phase1 = false;
-            S = 256;
-            break;
-        }
-        if (!((q.l4w >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1)) {
-            q.phase1 = false;
-            const bool grp = !(q.l4w & group_mask(sl, 7));
-            if (COUNT) cnt[grp ? VG_IT_CHUNK_GROUP_SKIP : VG_IT_CHUNK_SKIP]++;
-            S = grp ? 128 : 64;
-            break;
-        }
-        const uint32_t ci = chunk_index(w, sl);
-        if (ci != q.cur_c) {
-            q.cur_c = ci;
-            q.cw = w.chunk[ci];
-        }
-        const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
-        if (!((q.cw >> tb) & 1)) {
-            q.phase1 = false;
-            const bool grp = !(q.cw & group_mask(sl, 5));
-            if (COUNT) cnt[grp ? VG_IT_TILE_GROUP_SKIP : VG_IT_TILE_SKIP]++;
-            S = grp ? 32 : 16;
-            break;
-        }
-        const uint32_t tn = ci * 64u + (uint32_t)tb;
-        if (tn != q.cur_t) {
-            q.cur_t = tn;
-            q.tw = w.tile[tn];
-            q.tu = w.tile_uni[tn];
-        }
-        if (lod && !q.phase1 && r.t_cur > TILE_LOD_T) {
-            if (COUNT) cnt[VG_IT_TILE_LOD]++;
-            if (q.tw) {
-                const uint32_t fb = tn * 64u + (uint32_t)__builtin_ctzll(q.tw);
-                uint32_t lm = topmost_material(w, fb, w.occ[fb]);
-                if (lm != 0u && uniform_optimisable(lm)) {
-                    emit(r, w, lm, h);
-                    return true;
-                }
-            }
-        }
-        if (q.tu != 0u) {
-            if (COUNT) cnt[VG_IT_TILE_UNIFORM]++;
-            if (solid_mat(q.tu, flags)) {
-                if (!q.phase1) {
-                    emit(r, w, q.tu, h);
-                    return true;
-                }
-            } else {
-                q.phase1 = false;
-            }
-            S = 16;
-            break;
-        }
-        const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
-        if (!((q.tw >> bb) & 1)) {
-            q.phase1 = false;
-            // inside an occupied tile a tile-LOD test may fire at any visited voxel past
-            // t 520, so a group skip there must end before t 520: its exit lies within
-            // the 8^3 cell's diagonal (< 14) of the current voxel's entry
-            const bool grp = !(q.tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T);
-            if (COUNT) cnt[grp ? VG_IT_BRICK_GROUP_SKIP : VG_IT_BRICK_SKIP]++;
-            S = grp ? 8 : 4;
-            break;
-        }
-        const uint32_t bn = tn * 64u + (uint32_t)bb;
-        if (bn != q.cur_b) {
-            q.cur_b = bn;
-            q.ow = w.occ[bn];
-            q.bu = w.brick_uni[bn];
-        }
-        if (q.bu != 0u) {
-            if (COUNT) cnt[VG_IT_BRICK_UNIFORM]++;
-            if (solid_mat(q.bu, flags)) {
-                if (!q.phase1) {
-                    emit(r, w, q.bu, h);
-                    return true;
-                }
-            } else {
-                q.phase1 = false;
-            }
-            S = 4;
-            break;
-        }
-        if (lod && !q.phase1 && r.t_cur > LOD_BRICK_T) {
-            if (COUNT) cnt[VG_IT_BRICK_LOD]++;
-            uint32_t m = topmost_material(w, bn, q.ow);
-            if (!is_foliage(m)) {
-                emit(r, w, m, h);
+    const uint32_t l4i = l4_index(w, sl);
+    const uint32_t ci = chunk_index(w, sl);
+    const int cbc = cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3);
+    const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
+    const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
+    const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
+    const uint32_t tn = ci * 64u + (uint32_t)tb;
+    const uint32_t bn = tn * 64u + (uint32_t)bb;
+    // every level is stored dense, so each index is in bounds; one round trip for all six
+    const uint64_t l4w = w.l4[l4i];
+    const uint64_t cw = w.chunk[ci];
+    const uint64_t tw = w.tile[tn];
+    const uint32_t tu = w.tile_uni[tn];
+    const uint64_t ow = w.occ[bn];
+    const uint32_t bu = w.brick_uni[bn];
+    asm volatile("" ::"v"(l4w), "v"(cw), "v"(tw), "v"(tu), "v"(ow), "v"(bu));
+    const bool p1 = q.phase1;
+    // the levels the walk reaches (each test only counts if the ones above it passed)
+    const bool c_on = l4w != 0 && ((l4w >> cbc) & 1);
+    const bool t_on = c_on && ((cw >> tb) & 1);
+    if (t_on && lod && !p1 && r.t_cur > TILE_LOD_T) {
+        if (COUNT) cnt[VG_IT_TILE_LOD]++;
+        if (tw) {
+            const uint32_t fb = tn * 64u + (uint32_t)__builtin_ctzll(tw);
+            const uint32_t lm = topmost_material(w, fb, w.occ[fb]);
+            if (lm != 0u && uniform_optimisable(lm)) {
+                emit(r, w, lm, h);
                 return true;
             }
         }
-        const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
-        if ((q.ow >> vb) & 1) {
-            if (COUNT) cnt[VG_IT_VOXEL]++;
-            uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
-            if (solid_mat(m, flags)) {
-                if (!q.phase1) {
-                    emit(r, w, m, h);
-                    return true;
-                }
-                break; // phase 1: step out of the solid voxel
-            }
-            q.phase1 = false;
-            break;
+    }
+    const bool tu_on = t_on && tu != 0u;
+    const bool b_on = t_on && tu == 0u && ((tw >> bb) & 1);
+    const bool bu_on = b_on && bu != 0u;
+    const bool v_lvl = b_on && bu == 0u;
+    if (v_lvl && lod && !p1 && r.t_cur > LOD_BRICK_T) {
+        if (COUNT) cnt[VG_IT_BRICK_LOD]++;
+        const uint32_t m = topmost_material(w, bn, ow);
+        if (!is_foliage(m)) {
+            emit(r, w, m, h);
+            return true;
         }
-        q.phase1 = false;
-        // an empty 2x2x2 voxel octant is left whole; inside an occupied brick a brick-LOD
-        // test may fire at any visited voxel past t 400, so the octant (diagonal < 4)
-        // must be left before it
-        const bool grp = !(q.ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T);
-        if (COUNT) cnt[grp ? VG_IT_VOXEL_GROUP_SKIP : VG_IT_VOXEL]++;
-        S = grp ? 2 : 1;
-    } while (false);
+    }
+    const bool v_set = v_lvl && ((ow >> vb) & 1);
+    uint32_t vm = 0u;
+    if (v_set) vm = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+    // a uniform tile, a uniform brick or a set voxel decides by its material
+    const uint32_t um = tu_on ? (uint32_t)tu : (bu_on ? (uint32_t)bu : vm);
+    const bool solid = (tu_on || bu_on || v_set) && solid_mat(um, flags);
+    if (COUNT) {
+        int cls;
+        if (l4w == 0) cls = VG_IT_L4_SKIP;
+        else if (!c_on) cls = !(l4w & group_mask(sl, 7)) ? VG_IT_CHUNK_GROUP_SKIP : VG_IT_CHUNK_SKIP;
+        else if (!t_on) cls = !(cw & group_mask(sl, 5)) ? VG_IT_TILE_GROUP_SKIP : VG_IT_TILE_SKIP;
+        else if (tu_on) cls = VG_IT_TILE_UNIFORM;
+        else if (!b_on)
+            cls = (!(tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T))
+                      ? VG_IT_BRICK_GROUP_SKIP
+                      : VG_IT_BRICK_SKIP;
+        else if (bu_on) cls = VG_IT_BRICK_UNIFORM;
+        else if (v_set) cls = VG_IT_VOXEL;
+        else
+            cls = (!(ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T))
+                      ? VG_IT_VOXEL_GROUP_SKIP
+                      : VG_IT_VOXEL;
+        cnt[cls]++;
+    }
+    if (solid && !p1) {
+        emit(r, w, um, h);
+        return true;
+    }
+    // the cell to leave: the empty cell (or 2x2x2 group of empty children) at the level the
+    // walk stopped, the uniform cell, or the voxel (a phase-1 solid keeps the phase)
+    const bool g7 = !(l4w & group_mask(sl, 7));
+    const bool g5 = !(cw & group_mask(sl, 5));
+    const bool g3 = !(tw & group_mask(sl, 3)) && (!lod || r.t_cur + 14.0f <= TILE_LOD_T);
+    const bool g1 = !(ow & group_mask(sl, 1)) && (!lod || r.t_cur + 4.0f <= LOD_BRICK_T);
+    const int32_t S = l4w == 0 ? 256
+                      : !c_on    ? (g7 ? 128 : 64)
+                      : !t_on    ? (g5 ? 32 : 16)
+                      : tu_on    ? 16
+                      : !b_on    ? (g3 ? 8 : 4)
+                      : bu_on    ? 4
+                      : v_set    ? 1
+                                 : (g1 ? 2 : 1);
+    q.phase1 = p1 && solid;
     walk_advance(r, sl, S);
     return false;
 }
@@ -533,8 +464,8 @@ template <bool COUNT>
 __device__ __forceinline__ void walk_run(Walk &r, const DevWorld &w, float tmax, uint32_t flags,
                                          HitRec &h, uint32_t *cnt) {
     const bool lod = (flags & VG_RAY_LOD) && !(flags & (VG_RAY_SKIP_TRANSPARENT | VG_RAY_ANY_HIT));
-    WalkCache q;
-    cache_reset(q);
+    WalkPhase q;
+    phase_reset(q);
     // every iteration crosses at least one voxel plane inside the window
     const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
     for (int it = 0; it < max_iter; ++it)
