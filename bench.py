@@ -594,7 +594,8 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     def sched():
         return {"waves_per_cu": ctx.get_option(L.VG_OPT_PROBE_WAVES_PER_CU),
                 "refill_below": ctx.get_option(L.VG_OPT_PROBE_REFILL),
-                "ids_per_atomic": ctx.get_option(L.VG_OPT_PROBE_CHUNK)}
+                "ids_per_atomic": ctx.get_option(L.VG_OPT_PROBE_CHUNK),
+                "transition_batch": ctx.get_option(L.VG_OPT_PROBE_TBATCH)}
 
     def train_on(i):
         return i % GI_TRAIN_EVERY == 0

@@ -329,9 +329,12 @@ enum {
     VG_OPT_PROBE_CHUNK = 9, /* probe path work ids taken per atomic: 0 (what a refill lacks) or 16..1024 */
     VG_OPT_FRAME_STRIPES = 10, /* persistent frame kernel: contiguous stripes of pixel blocks,
                                   each with its own work counter (1..64) */
-    VG_OPT_FRAME_BLOCK_W = 11  /* persistent frame kernel: pixel-block width 4, 8, 16 or 32 (the
+    VG_OPT_FRAME_BLOCK_W = 11, /* persistent frame kernel: pixel-block width 4, 8, 16 or 32 (the
                                   block is one wave's 64 rays: height 64 / width; the beam
                                   prologue marches one box per block) */
+    VG_OPT_PROBE_TBATCH = 12   /* probe path kernel: a lane whose ray ended waits until this many
+                                  of its wave's lanes have (or too few still walk) and they set
+                                  up their next rays together (1..64; 1: at once) */
 };
 int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
 int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);
@@ -566,10 +569,10 @@ int vg_frame_indirect(vg_probes *probes, const vg_gi_params *params, const vg_ca
 int vg_probes_update_learned(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                              uint32_t flags, void *stream);
 /* Times every schedule of the probe path kernels (waves per CU, refill threshold, work ids
- * per atomic) on this very update (repeated; nrc may be NULL for the plain update, with
- * VG_NRC_TRAIN the cache trains on every repetition, timed as two back-to-back updates so
- * the training step's overlap with the next update counts) and keeps the fastest in the
- * context (VG_OPT_PROBE_*). */
+ * per atomic; then the transition batch at the fastest of those) on this very update
+ * (repeated; nrc may be NULL for the plain update, with VG_NRC_TRAIN the cache trains on
+ * every repetition, timed as two back-to-back updates so the training step's overlap with
+ * the next update counts) and keeps the fastest in the context (VG_OPT_PROBE_*). */
 int vg_autotune_probes(vg_probes *probes, const vg_gi_params *params, vg_nrc *nrc,
                        uint32_t flags, float *best_ms);
 /* The same, reporting rays traced (gather, shadow, training bounce), the cache queries

@@ -73,6 +73,7 @@ VG_OPT_PROBE_REFILL = 8
 VG_OPT_PROBE_CHUNK = 9
 VG_OPT_FRAME_STRIPES = 10
 VG_OPT_FRAME_BLOCK_W = 11
+VG_OPT_PROBE_TBATCH = 12
 VG_RAY_LOD = 1
 VG_RAY_SKIP_TRANSPARENT = 2
 VG_RAY_ANY_HIT = 4

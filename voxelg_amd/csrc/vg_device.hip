@@ -1146,6 +1146,7 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->probe_wpc = 8;
     c->probe_refill = 8;
     c->probe_chunk = 0;
+    c->probe_tbatch = 8;
     NoAllocScratch na; // the no-allocation entry point's scratch address on this device
     int st = noalloc_scratch(device, &na);
     if (st) {
@@ -1639,6 +1640,11 @@ int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
             return fail(VG_E_INVALID, "probe id chunk %d: 0 or 16..1024", value);
         ctx->probe_chunk = value;
         return VG_OK;
+    case VG_OPT_PROBE_TBATCH:
+        if (value < 1 || value > 64)
+            return fail(VG_E_INVALID, "probe transition batch %d not in 1..64", value);
+        ctx->probe_tbatch = value;
+        return VG_OK;
     default:
         return fail(VG_E_INVALID, "option %d cannot be set", option);
     }
@@ -1657,6 +1663,7 @@ int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
     case VG_OPT_PROBE_WAVES_PER_CU: *value = ctx->probe_wpc; return VG_OK;
     case VG_OPT_PROBE_REFILL: *value = ctx->probe_refill; return VG_OK;
     case VG_OPT_PROBE_CHUNK: *value = ctx->probe_chunk; return VG_OK;
+    case VG_OPT_PROBE_TBATCH: *value = ctx->probe_tbatch; return VG_OK;
     case VG_OPT_CU_COUNT: *value = ctx->cus; return VG_OK;
     default: return fail(VG_E_INVALID, "unknown option %d", option);
     }

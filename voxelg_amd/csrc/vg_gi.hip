@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
     const float4 *__restrict__ btab, const float4 *__restrict__ pal, float4 *__restrict__ rad,
     uint8_t *__restrict__ face_out, LearnedArgs la, HeavyArgs ha, uint32_t *ctr, int threshold,
-    uint32_t chunk, unsigned long long *counts) {
+    int tbatch, uint32_t chunk, unsigned long long *counts) {
     // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail;
     // the lanes' kBounce / kShadowY stages then trace training targets), 4 learned without
     // training targets (an update that takes no training step: no bounce stages compiled in)
@@ -748,17 +748,24 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             }
             continue;
         }
+        bool pend = false; // this lane's ray has ended; its transition waits for company
         do {
             bool ended = stage == kNew;
-            if (stage >= kGather) {
+            if (stage >= kGather && !pend) {
                 const bool shadow = stage == kShadowX || stage == kShadowY;
                 bool done = walk_iter<COUNT>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
-                                                   shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
-                                                   false, h, cnt);
+                                             shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
+                                             false, h, cnt);
                 ended = done || ++it >= max_iter;
             }
-            if (ended) transition();
-        } while (__popcll(__ballot(stage != kIdle)) >= threshold);
+            pend = pend || ended;
+            const int npend = __popcll(__ballot(pend));
+            if (npend >= tbatch || nact - npend < threshold) {
+                if (pend) transition();
+                pend = false;
+            }
+            nact = __popcll(__ballot(stage != kIdle));
+        } while (nact >= threshold);
     }
     if (COUNT) {
         flush_counts(cnt, counts);
@@ -791,6 +798,37 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[kCtrUpd] += 1u; // the update's last launch
     const uint32_t N = (uint32_t)p.rays_per_probe;
     const bool staged = N <= kBlock;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gl = lane & (L - 1u);                 // lane within the group
+    const uint32_t grp = threadIdx.x / L;                // group within the block
+    const uint32_t j = blockIdx.x * (kBlock / L) + grp;  // updated probe
+    const bool live = j < n_upd;
+    const uint32_t jj = live ? j : 0u;
+    const int64_t idx = (int64_t)jj * p.update_div + phase;
+    float *rec = reinterpret_cast<float *>(probes + idx);
+    // Every global read of the kernel is issued here, before the direction staging, so they
+    // share one round trip: the probe's flags, the record fields this lane folds (f = gl + i
+    // L), their staging sums (f + 24), the staging count, and the lane's first sample (its
+    // face, and its radiance, which a miss ignores).
+    constexpr int kFields = 48 / 8; // fields per lane at the smallest group (L = 8)
+    float rv[kFields], rs[kFields];
+    float rc = 0.0f;
+    uint32_t fs0 = VG_FACE_NONE;
+    float4 rad0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t fl = __float_as_uint(pwork[jj].w);
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < kFields; ++i) {
+            const uint32_t f = gl + (uint32_t)i * L;
+            rv[i] = f < 48u ? rec[f] : 0.0f;
+            rs[i] = f < 20u ? rec[24u + f] : 0.0f;
+        }
+        rc = rec[44];
+        if (gl < N) {
+            fs0 = face[jj * N + gl];
+            rad0 = rad[jj * N + gl];
+        }
+    }
     if (staged && threadIdx.x < N) {
         const float4 dd = dirs[threadIdx.x * (uint32_t)p.epochs + epoch];
         const float d3[3] = {dd.x, dd.y, dd.z};
@@ -801,26 +839,21 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
                                         p.gi_dist);
     }
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t gl = lane & (L - 1u);                 // lane within the group
-    const uint32_t grp = threadIdx.x / L;                // group within the block
-    const uint32_t j = blockIdx.x * (kBlock / L) + grp;  // updated probe
-    const bool live = j < n_upd;
-    const uint32_t jj = live ? j : 0u;
-    const int64_t idx = (int64_t)jj * p.update_div + phase;
-    const uint32_t fl = __float_as_uint(pwork[jj].w);
     const bool buried = (fl & kProbeBuried) != 0;
     const float wpp = 12.566371f / (float)N; // 4 pi / N (uniform-sphere MC weight)
     float c[20];
 #pragma unroll
     for (int a = 0; a < 20; ++a) c[a] = 0.0f;
     float near_t = 1.0e30f;
-    uint32_t near_k = 0xFFFFFFFFu;
-    if (live && !buried) {
+    uint32_t near_k = 0xFFFFFFFFu, near_f = VG_FACE_NONE;
+    if (live) {
         for (uint32_t k = gl; k < N; k += L) {
             const uint32_t s = jj * N + k;
+            const bool first = k == gl;
+            const uint32_t fs = first ? fs0 : face[s];
+            const float4 rs4 = first ? rad0 : rad[s];
             const float4 dd = staged ? sdir[k] : dirs[k * (uint32_t)p.epochs + epoch];
-            const uint32_t fs = face[s];
+            if (buried) continue;
             float4 v;
             if (fs == VG_FACE_NONE) { // no gather hit: the sky along the ray, at gi_dist
                 if (staged) {
@@ -833,7 +866,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
                                     p.gi_dist);
                 }
             } else {
-                v = rad[s];
+                v = rs4;
             }
             const float bs[4] = {0.282095f, 0.488603f * dd.y, 0.488603f * dd.z, 0.488603f * dd.x};
             const float ws[5] = {wpp * v.x, wpp * v.y, wpp * v.z, wpp * v.w, wpp * v.w * v.w};
@@ -844,6 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
             if (fs != VG_FACE_NONE && v.w < near_t) {
                 near_t = v.w;
                 near_k = k;
+                near_f = fs;
             }
         }
     }
@@ -854,9 +888,11 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
         // nearest hit, ties to the lowest sample (the reference's ascending loop with <)
         const float ot = __shfl_xor(near_t, (int)off, 64);
         const uint32_t ok = __shfl_xor(near_k, (int)off, 64);
+        const uint32_t of = __shfl_xor(near_f, (int)off, 64);
         if (ot < near_t || (ot == near_t && ok < near_k)) {
             near_t = ot;
             near_k = ok;
+            near_f = of;
         }
     }
     if (gl == 0) {
@@ -868,7 +904,6 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     if (!live) return;
     int32_t g[3];
     probe_cell(p, w, idx, g);
-    float *rec = reinterpret_cast<float *>(probes + idx);
     if (buried) {
         for (uint32_t f = gl; f < 48; f += L) rec[f] = f >= 20 && f < 23 ? (float)g[f - 20] : 0.0f;
         return;
@@ -876,14 +911,16 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
     const bool is_new = (fl & kProbeNew) != 0;
     const bool fold = !is_new && epoch == (uint32_t)p.epochs - 1u;
     // staging count after this update (the fold divides by it, then clears it)
-    const float cnt1 = is_new ? 0.0f : rec[44] + 1.0f;
+    const float cnt1 = is_new ? 0.0f : rc + 1.0f;
     float n3[3] = {0.0f, 0.0f, 0.0f};
     if (near_k != 0xFFFFFFFFu) {
-        const uint32_t f = face[jj * N + near_k];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) n3[k] = face_n(f, k);
+        for (int k = 0; k < 3; ++k) n3[k] = face_n(near_f, k);
     }
-    for (uint32_t f = gl; f < 48; f += L) {
+#pragma unroll
+    for (int i = 0; i < kFields; ++i) {
+        const uint32_t f = gl + (uint32_t)i * L;
+        if (f >= 48u) break;
         float out;
         if (f < 20) {
             const float cf = sums[grp][f];
@@ -892,20 +929,20 @@ __global__ __launch_bounds__(kBlock) void k_probe_reduce(DevWorld w, vg_gi_param
             } else if (fold) {
                 const float cnt = cnt1 > 1.0f ? cnt1 : 1.0f;
                 const float h = p.hysteresis;
-                out = ((rec[24 + f] + cf) / cnt) * (1.0f - h) + rec[f] * h;
+                out = ((rs[i] + cf) / cnt) * (1.0f - h) + rv[i] * h;
             } else {
-                out = rec[f];
+                out = rv[i];
             }
         } else if (f < 24) {
             out = f < 23 ? (float)g[f - 20] : 1.0f;
         } else if (f < 44) {
-            out = (is_new || fold) ? 0.0f : rec[f] + sums[grp][f - 24];
+            out = (is_new || fold) ? 0.0f : rv[i] + sums[grp][f - 24];
         } else if (f == 44) {
             out = (is_new || fold) ? 0.0f : cnt1;
         } else {
             // relocation (shaders/gi_probes.wgsl:381-392)
             const int k = (int)f - 45;
-            float o = is_new ? 0.0f : rec[f];
+            float o = is_new ? 0.0f : rv[i];
             if (near_t < 1.2f) {
                 const float m = (1.2f - near_t) < 0.35f ? (1.2f - near_t) : 0.35f;
                 const float v = o + n3[k] * m;
@@ -1290,7 +1327,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
                        pr->palette, pr->rad, pr->face, la, ha, pr->ctr, thr,                    \
-                       (uint32_t)w->ctx->probe_chunk, cd)
+                       w->ctx->probe_tbatch, (uint32_t)w->ctx->probe_chunk, cd)
     // a learned update that takes no training step runs the instantiation without the
     // training stages
     const int mode = nrc ? (la.train_n ? 3 : 4) : p->bounces;
@@ -1594,11 +1631,19 @@ int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32
     // timed on two updates back to back (the steady state) and the span closes on the
     // second one's step; the reported time is per update.
     const bool pair = nrc && (flags & VG_NRC_TRAIN) && nrc->cfg.steps == 1;
-    st = autotune(c, &c->probe_wpc, &c->probe_refill, [&](hipStream_t s) {
+    auto run = [&](hipStream_t s) {
         int r = probes_update(pr, p, nrc, flags, false, s);
         if (r == VG_OK && pair) r = probes_update(pr, p, nrc, flags, false, s);
         return r == VG_OK && nrc ? nrc_join(nrc, s) : r;
-    }, best_ms, &c->probe_chunk);
+    };
+    st = autotune(c, &c->probe_wpc, &c->probe_refill, run, best_ms, &c->probe_chunk);
+    // then the transition batch at the chosen (waves per CU, refill, ids per atomic)
+    static const int kTbatch[] = {1, 4, 8, 16};
+    const int wpc = c->probe_wpc, rf = c->probe_refill;
+    if (st == VG_OK)
+        st = autotune_lists(c, &c->probe_wpc, TuneList{&wpc, 1}, &c->probe_refill,
+                            TuneList{&rf, 1}, &c->probe_tbatch, TuneList{kTbatch, 4}, run,
+                            best_ms);
     if (st == VG_OK && pair && best_ms) *best_ms *= 0.5f;
     return st;
 }

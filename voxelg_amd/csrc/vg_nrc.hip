@@ -193,28 +193,30 @@ __device__ __forceinline__ void load_frags(u32x4_t *dst, const u32x4_t *src) {
     for (int i = threadIdx.x; i < kNrcFrags * 64; i += blockDim.x) dst[i] = src[i];
 }
 
-// acc[ob][sb] = sum over k-steps of fragment (lyr, ob, ks) x input[ks][sb].
-template <int NOB>
-__device__ __forceinline__ void mlp_layer(const u32x4_t *sf, int lyr, const u32x4_t x[2][4],
-                                          f32x4_t acc[4][4], int lane) {
+// acc[ob][sb] = sum over k-steps of fragment (lyr, ob, ks) x input[ks][sb], for the wave's
+// SB column blocks of 16 samples.
+template <int NOB, int SB = 4>
+__device__ __forceinline__ void mlp_layer(const u32x4_t *sf, int lyr, const u32x4_t x[2][SB],
+                                          f32x4_t acc[4][SB], int lane) {
 #pragma unroll
     for (int ob = 0; ob < NOB; ++ob) {
 #pragma unroll
-        for (int sb = 0; sb < 4; ++sb) acc[ob][sb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int sb = 0; sb < SB; ++sb) acc[ob][sb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const u32x4_t a = sf[((lyr * 4 + ob) * 2 + ks) * 64 + lane];
 #pragma unroll
-            for (int sb = 0; sb < 4; ++sb) acc[ob][sb] = mfma(a, x[ks][sb], acc[ob][sb]);
+            for (int sb = 0; sb < SB; ++sb) acc[ob][sb] = mfma(a, x[ks][sb], acc[ob][sb]);
         }
     }
 }
 
-__device__ __forceinline__ void relu_pack(const f32x4_t acc[4][4], u32x4_t x[2][4]) {
+template <int SB = 4>
+__device__ __forceinline__ void relu_pack(const f32x4_t acc[4][SB], u32x4_t x[2][SB]) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int sb = 0; sb < 4; ++sb) {
+        for (int sb = 0; sb < SB; ++sb) {
             const f32x4_t a = acc[2 * ks][sb], b = acc[2 * ks + 1][sb];
             x[ks][sb] = u32x4_t{pack2(relu(a[0]), relu(a[1])), pack2(relu(a[2]), relu(a[3])),
                                 pack2(relu(b[0]), relu(b[1])), pack2(relu(b[2]), relu(b[3]))};
@@ -222,7 +224,13 @@ __device__ __forceinline__ void relu_pack(const f32x4_t acc[4][4], u32x4_t x[2][
 }
 
 // ---------------------------------------------------------------- inference --
-template <bool PROBE>
+#ifndef VG_NRC_INFER_SB
+#define VG_NRC_INFER_SB 1
+#endif
+constexpr int kInferSB = VG_NRC_INFER_SB;
+// A wave evaluates SB column blocks of 16 queries (16 SB per tile): small tiles give the
+// gathers of a launch of a few ten thousand queries enough waves to overlap.
+template <bool PROBE, int SB>
 __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query *__restrict__ qb,
                                                    size_t q_stride, const uint32_t *sel,
                                                    const uint32_t *n_dev, uint32_t n_max,
@@ -235,33 +243,34 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query 
     n = n < n_max ? n : n_max;
     // the grid is sized for n_max; a block with no tile leaves before staging the weights
     // (the whole block together, so no barrier is left waiting)
-    if ((uint64_t)blockIdx.x * 256u >= n) return;
+    if ((uint64_t)blockIdx.x * 4u * (16u * SB) >= n) return;
     load_frags(sf, m.frag_fwd);
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const uint32_t wave = threadIdx.x >> 6;
-    for (uint32_t tile = blockIdx.x * 4u + wave; (uint64_t)tile * 64u < n; tile += gridDim.x * 4u) {
-        u32x4_t x[2][4];
-        uint32_t face[4], aux[4];
+    constexpr uint32_t kTile = 16u * SB;
+    for (uint32_t tile = blockIdx.x * 4u + wave; (uint64_t)tile * kTile < n; tile += gridDim.x * 4u) {
+        u32x4_t x[2][SB];
+        uint32_t face[SB], aux[SB];
 #pragma unroll
-        for (int sb = 0; sb < 4; ++sb) {
-            const uint32_t s = tile * 64u + (uint32_t)(sb * 16 + c);
+        for (int sb = 0; sb < SB; ++sb) {
+            const uint32_t s = tile * kTile + (uint32_t)(sb * 16 + c);
             float p[3], d[3];
             load_query(q, s, s < n, p, d, face[sb], aux[sb]);
             build_input(m, g, p, d, face[sb] & 0xFFu, x[0][sb], x[1][sb]);
         }
-        f32x4_t acc[4][4];
-        mlp_layer<4>(sf, 0, x, acc, lane);
-        relu_pack(acc, x);
-        mlp_layer<4>(sf, 1, x, acc, lane);
-        relu_pack(acc, x);
-        mlp_layer<4>(sf, 2, x, acc, lane);
-        relu_pack(acc, x);
-        mlp_layer<1>(sf, 3, x, acc, lane);
+        f32x4_t acc[4][SB];
+        mlp_layer<4, SB>(sf, 0, x, acc, lane);
+        relu_pack<SB>(acc, x);
+        mlp_layer<4, SB>(sf, 1, x, acc, lane);
+        relu_pack<SB>(acc, x);
+        mlp_layer<4, SB>(sf, 2, x, acc, lane);
+        relu_pack<SB>(acc, x);
+        mlp_layer<1, SB>(sf, 3, x, acc, lane);
         if (g == 0) {
 #pragma unroll
-            for (int sb = 0; sb < 4; ++sb) {
-                const uint32_t s = tile * 64u + (uint32_t)(sb * 16 + c);
+            for (int sb = 0; sb < SB; ++sb) {
+                const uint32_t s = tile * kTile + (uint32_t)(sb * 16 + c);
                 if (s >= n) continue;
                 const f32x4_t y = acc[0][sb];
                 if (PROBE) {
@@ -800,15 +809,15 @@ int nrc_infer_launch(vg_nrc *m, const vg_nrc_query *q, size_t q_stride, const ui
     if (n_max == 0) return VG_OK;
     int st = nrc_join(m, stream);
     if (st) return st;
-    const uint64_t tiles = ((uint64_t)n_max + 63) / 64;
+    const uint64_t tiles = ((uint64_t)n_max + 16 * kInferSB - 1) / (16 * kInferSB);
     uint64_t blocks = (tiles + 3) / 4;
     const uint64_t cap = (uint64_t)m->ctx->cus * 8;
     if (blocks > cap) blocks = cap;
     if (rad)
-        hipLaunchKernelGGL(k_nrc_infer<true>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+        hipLaunchKernelGGL((k_nrc_infer<true, kInferSB>), dim3((uint32_t)blocks), dim3(256), 0, stream,
                            dev_view(m), q, q_stride, sel, n_dev, n_max, out4, rad, pal);
     else
-        hipLaunchKernelGGL(k_nrc_infer<false>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+        hipLaunchKernelGGL((k_nrc_infer<false, kInferSB>), dim3((uint32_t)blocks), dim3(256), 0, stream,
                            dev_view(m), q, q_stride, sel, n_dev, n_max, out4, rad, pal);
     HIP_TRY(hipGetLastError());
     return VG_OK;

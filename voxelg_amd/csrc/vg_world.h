@@ -18,6 +18,7 @@ struct vg_ctx {
     int probe_wpc;       // persistent probe path kernel: waves per CU
     int probe_refill;
     int probe_chunk;     // ... work ids per atomic (0: exactly what a refill lacks)
+    int probe_tbatch;    // ... lanes whose rays ended that set up their next rays together
 };
 
 struct vg_edit_state; // sphere-edit buffers (vg_edit.hip), created on first use
