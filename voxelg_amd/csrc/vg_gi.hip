@@ -184,7 +184,9 @@ __device__ __forceinline__ void bounce_inv(const float4 e, const float4 ei, uint
     out[ax == 0 ? 2 : ax - 1] = ei.y;
 }
 
-// Material of a world voxel through the pyramid (0 outside the window).
+// Material of a world voxel through the pyramid (0 outside the window). Every level is a
+// dense array indexed by the voxel's coordinates, so the seven words are loaded together (one
+// round trip) and the test chain then runs on registers.
 __device__ __forceinline__ uint32_t material_at(const DevWorld &w, const int32_t v[3]) {
     int32_t sl[3];
 #pragma unroll
@@ -196,21 +198,22 @@ __device__ __forceinline__ uint32_t material_at(const DevWorld &w, const int32_t
     }
     const uint32_t l4i = (uint32_t)(sl[0] >> 8) + (uint32_t)(sl[1] >> 8) * w.l4dim[0] +
                          (uint32_t)(sl[2] >> 8) * w.l4dim[0] * w.l4dim[1];
-    if (!((w.l4[l4i] >> cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3)) & 1))
-        return 0u;
     const uint32_t ci = (uint32_t)(sl[0] >> 6) + (uint32_t)(sl[1] >> 6) * w.cdim[0] +
                         (uint32_t)(sl[2] >> 6) * w.cdim[0] * w.cdim[1];
+    const int cb = cbit((sl[0] >> 6) & 3, (sl[1] >> 6) & 3, (sl[2] >> 6) & 3);
     const int tb = cbit((sl[0] >> 4) & 3, (sl[1] >> 4) & 3, (sl[2] >> 4) & 3);
-    if (!((w.chunk[ci] >> tb) & 1)) return 0u;
-    const uint32_t tn = ci * 64u + (uint32_t)tb;
-    if (w.tile_uni[tn]) return w.tile_uni[tn];
     const int bb = cbit((sl[0] >> 2) & 3, (sl[1] >> 2) & 3, (sl[2] >> 2) & 3);
-    if (!((w.tile[tn] >> bb) & 1)) return 0u;
-    const uint32_t bn = tn * 64u + (uint32_t)bb;
-    if (w.brick_uni[bn]) return w.brick_uni[bn];
     const int vb = cbit(sl[0] & 3, sl[1] & 3, sl[2] & 3);
-    if (!((w.occ[bn] >> vb) & 1)) return 0u;
-    return w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+    const uint32_t tn = ci * 64u + (uint32_t)tb;
+    const uint32_t bn = tn * 64u + (uint32_t)bb;
+    const uint64_t l4w = w.l4[l4i], cw = w.chunk[ci], tw = w.tile[tn], ow = w.occ[bn];
+    const uint32_t tu = w.tile_uni[tn], bu = w.brick_uni[bn];
+    const uint32_t m = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+    if (!((l4w >> cb) & 1) || !((cw >> tb) & 1)) return 0u;
+    if (tu) return tu;
+    if (!((tw >> bb) & 1)) return 0u;
+    if (bu) return bu;
+    return ((ow >> vb) & 1) ? m : 0u;
 }
 
 // World grid cell of probe slot idx for the current window (shaders/gi_probes.wgsl:281-297).

@@ -244,21 +244,40 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query 
     // the grid is sized for n_max; a block with no tile leaves before staging the weights
     // (the whole block together, so no barrier is left waiting)
     if ((uint64_t)blockIdx.x * 4u * (16u * SB) >= n) return;
-    load_frags(sf, m.frag_fwd);
-    __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const uint32_t wave = threadIdx.x >> 6;
     constexpr uint32_t kTile = 16u * SB;
-    for (uint32_t tile = blockIdx.x * 4u + wave; (uint64_t)tile * kTile < n; tile += gridDim.x * 4u) {
-        u32x4_t x[2][SB];
-        uint32_t face[SB], aux[SB];
+    const uint32_t stride = gridDim.x * 4u;
+    float p[SB][3], d[SB][3];
+    uint32_t face[SB], aux[SB];
+    auto load_tile = [&](uint32_t t) {
 #pragma unroll
         for (int sb = 0; sb < SB; ++sb) {
-            const uint32_t s = tile * kTile + (uint32_t)(sb * 16 + c);
-            float p[3], d[3];
-            load_query(q, s, s < n, p, d, face[sb], aux[sb]);
-            build_input(m, g, p, d, face[sb] & 0xFFu, x[0][sb], x[1][sb]);
+            const uint32_t s = t * kTile + (uint32_t)(sb * 16 + c);
+            load_query(q, s, s < n, p[sb], d[sb], face[sb], aux[sb]);
         }
+    };
+    // the wave's first queries load together with the weight staging
+    uint32_t tile = blockIdx.x * 4u + wave;
+    if ((uint64_t)tile * kTile < n) load_tile(tile);
+    load_frags(sf, m.frag_fwd);
+    __syncthreads();
+    for (; (uint64_t)tile * kTile < n; tile += stride) {
+        // probe epilogue: the sample's radiance and the hit's albedo load with the gathers
+        float4 r0[SB], a0[SB];
+#pragma unroll
+        for (int sb = 0; sb < SB; ++sb) {
+            r0[sb] = a0[sb] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const uint32_t s = tile * kTile + (uint32_t)(sb * 16 + c);
+            if (PROBE && g == 0 && s < n) {
+                a0[sb] = pal[(face[sb] >> 8) & 0xFFu];
+                r0[sb] = rad[aux[sb]];
+            }
+        }
+        u32x4_t x[2][SB];
+#pragma unroll
+        for (int sb = 0; sb < SB; ++sb)
+            build_input(m, g, p[sb], d[sb], face[sb] & 0xFFu, x[0][sb], x[1][sb]);
         f32x4_t acc[4][SB];
         mlp_layer<4, SB>(sf, 0, x, acc, lane);
         relu_pack<SB>(acc, x);
@@ -274,17 +293,17 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDev m, const vg_nrc_query 
                 if (s >= n) continue;
                 const f32x4_t y = acc[0][sb];
                 if (PROBE) {
-                    const float4 a = pal[(face[sb] >> 8) & 0xFFu];
-                    float4 r = rad[aux[sb]];
-                    r.x += a.x * relu(y[0]);
-                    r.y += a.y * relu(y[1]);
-                    r.z += a.z * relu(y[2]);
+                    float4 r = r0[sb];
+                    r.x += a0[sb].x * relu(y[0]);
+                    r.y += a0[sb].y * relu(y[1]);
+                    r.z += a0[sb].z * relu(y[2]);
                     rad[aux[sb]] = r;
                 } else {
                     out[s] = make_float4(y[0], y[1], y[2], 0.0f);
                 }
             }
         }
+        if ((uint64_t)(tile + stride) * kTile < n) load_tile(tile + stride);
     }
 }
 
