@@ -24,6 +24,7 @@ HIT_DTYPE = np.dtype(
 )
 LOD, SKIP_TRANSPARENT, ANY_HIT, BEAM = 1, 2, 4, 8
 NO_GROUPS = 0x100  # oracle only: the reference's levels without 2x2x2 group skips
+NO_YTOP = 0x200  # oracle only: no y-bound exit (orc_world.ytop)
 FACE_NONE = 6
 IT_COUNT = 14
 IT_NAMES = ("l4_skip", "chunk_skip", "tile_skip", "tile_lod", "tile_uniform", "brick_skip",
@@ -66,7 +67,23 @@ PROBE_FIELDS = 48
 class OrcWorld(C.Structure):
     _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_int32 * 3), ("bricks", C.c_void_p),
                 ("tile_mask", C.c_void_p), ("chunk_mask", C.c_void_p), ("l4_mask", C.c_void_p),
-                ("brick_uniform", C.c_void_p), ("tile_uniform", C.c_void_p)]
+                ("brick_uniform", C.c_void_p), ("tile_uniform", C.c_void_p), ("ytop", C.c_int32)]
+
+
+def world_ytop(dims, origin, tile_mask, tile_uniform) -> int:
+    """orc_world.ytop of a world's arrays: 4 x (the highest window-local brick row holding a
+    set tile bit + 1), a uniform tile counting as all its rows; dims[1] when the window folds
+    y. Tiles are in the reference order (x fastest, then y, then z)."""
+    if int(origin[1]) % int(dims[1]) != 0:
+        return int(dims[1])
+    t0, t1 = int(dims[0]) // 16, int(dims[1]) // 16
+    m = np.asarray(tile_mask, np.uint64)
+    ty = (np.arange(len(m), dtype=np.int64) // t0) % t1
+    by = np.where(m >> np.uint64(48), 3, np.where(m >> np.uint64(32), 2,
+                                                   np.where(m >> np.uint64(16), 1, 0)))
+    top = np.where(np.asarray(tile_uniform) != 0, ty * 16 + 16,
+                   np.where(m != 0, ty * 16 + by * 4 + 4, 0))
+    return int(top.max()) if len(top) else 0
 
 
 _lib = None
@@ -284,6 +301,7 @@ class OracleWorld:
         s.l4_mask = _p(self.l4_mask)
         s.brick_uniform = _p(self.brick_uniform)
         s.tile_uniform = _p(self.tile_uniform)
+        s.ytop = world_ytop(self.dims, self.origin, self.tile_mask, self.tile_uniform)
         self._c = s  # keep alive
         return s
 

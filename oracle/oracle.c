@@ -65,6 +65,7 @@ int orc_world_alloc(orc_world *w, const int32_t dims[3]) {
     w->tile_uniform = (uint8_t *)calloc((size_t)n[1], 1);
     w->chunk_mask = (uint64_t *)calloc((size_t)n[2], 8);
     w->l4_mask = (uint64_t *)calloc((size_t)n[3], 8);
+    w->ytop = dims[1];
     if (!w->bricks || !w->brick_uniform || !w->tile_mask || !w->tile_uniform || !w->chunk_mask ||
         !w->l4_mask) {
         orc_world_free(w);
@@ -586,6 +587,7 @@ typedef struct r1_t {
     dims_t d;
     float org[3], dir[3], inv[3], tmin3[3];
     int32_t step[3], l[3], ro_mod[3];
+    int32_t ytop;
     float t_cur;
     int last_axis;
 } r1_t;
@@ -626,6 +628,7 @@ static int r1_init(r1_t *s, const orc_world *w, const float o[3], const float di
     }
     s->t_cur = t_enter;
     s->last_axis = -1;
+    s->ytop = w->ytop;
     return 1;
 }
 
@@ -766,12 +769,15 @@ static void r1_walk(r1_t *s, float tmax, uint32_t flags, orc_hit *out, uint64_t 
     const dims_t *d = &s->d;
     const int lod = (flags & ORC_LOD) && !(flags & (ORC_SKIP_TRANSPARENT | ORC_ANY_HIT));
     const int groups = !(flags & ORC_NO_GROUPS);
+    const int32_t ytop = (flags & ORC_NO_YTOP) ? w->dims[1] : s->ytop;
     int phase1 = 1;
     for (;;) {
         if (s->t_cur > tmax) return;
         if (s->l[0] < 0 || s->l[1] < 0 || s->l[2] < 0 || s->l[0] >= w->dims[0] ||
             s->l[1] >= w->dims[1] || s->l[2] >= w->dims[2])
             return;
+        /* above every occupied voxel and not moving down: nothing left to hit */
+        if (s->step[1] >= 0 && s->l[1] >= ytop) return;
         int32_t sl[3];
         for (int k = 0; k < 3; ++k) {
             sl[k] = s->l[k] + s->ro_mod[k];
@@ -1262,7 +1268,7 @@ static void frame_rows(job_t *j) {
                 orc_shadow_origin(cam, dir, h, so);
                 orc_hit sh;
                 if (j->kind == 2)
-                    trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT | (j->flags & ORC_NO_GROUPS),
+                    trace_core(j->w, so, j->sun, j->stmax, ORC_ANY_HIT | (j->flags & (ORC_NO_GROUPS | ORC_NO_YTOP)),
                                &sh, c);
                 else orc_raycast_r0(j->w, so, j->sun_in, j->stmax, &sh); /* normalizes it */
                 occ = sh.hit;

@@ -54,6 +54,10 @@ typedef struct orc_world {
     uint64_t *l4_mask;    /* one u64 per 256^3 cell, bit per child chunk */
     uint8_t *brick_uniform;
     uint8_t *tile_uniform;
+    /* Every occupied voxel has a window-local y below ytop (4 x (the highest brick row that
+     * holds a set tile bit + 1), a uniform tile counting as all its rows; dims[1] when the
+     * window folds y): R1 ends a walk at y >= ytop that does not move down, with a miss. */
+    int32_t ytop;
 } orc_world;
 
 /* Ray and hit records: the same bytes as vg_ray / vg_hit in include/vg.h. */
@@ -101,6 +105,8 @@ enum {
 
 /* Oracle-only flag: walk the reference's levels only (no 2x2x2 group skips). */
 #define ORC_NO_GROUPS 0x100u
+/* Oracle-only flag: walk without the y-bound exit (orc_world.ytop). */
+#define ORC_NO_YTOP 0x200u
 
 /* dims / counts helpers */
 void orc_counts(const int32_t dims[3], int64_t out[5]); /* bricks, tiles, chunks, l4, voxels */

@@ -246,12 +246,36 @@ def test_group_skips_keep_results(small_world, flags):
     _, cam = wl.camera()
     sun = workloads.LARGE.sun()
     ow = oracle.OracleWorld.view(small_world)
-    a, sa, ca = ow.frame(cam, wl.width, wl.height, 700.0, flags, sun, 480.0)
-    b, sb, cb = ow.frame(cam, wl.width, wl.height, 700.0, flags | oracle.NO_GROUPS, sun, 480.0)
+    # (the y-bound exit off, so the walks above the terrain skip groups too)
+    a, sa, ca = ow.frame(cam, wl.width, wl.height, 700.0, flags | oracle.NO_YTOP, sun, 480.0)
+    b, sb, cb = ow.frame(cam, wl.width, wl.height, 700.0,
+                         flags | oracle.NO_GROUPS | oracle.NO_YTOP, sun, 480.0)
     assert a.tobytes() == b.tobytes() and np.array_equal(sa, sb)
     assert cb["brick_group_skip"] == 0 and cb["tile_group_skip"] == 0
     assert ca["brick_group_skip"] > 0 and ca["tile_group_skip"] > 0
     assert sum(ca.values()) < sum(cb.values())
+
+
+@pytest.mark.parametrize("wl", [workloads.SMALL, workloads.LARGE], ids=["small", "large"])
+def test_ytop_exit_keeps_results(wl):
+    # a walk at y >= ytop that does not move down ends with a miss: the same frame (primary
+    # LOD rays and their shadow rays) as without the exit, in fewer iterations; the bound is
+    # the highest occupied brick row's top
+    world = workloads.SMALL.world() if wl is workloads.SMALL else wl.world()
+    _, cam = wl.camera()
+    sun = workloads.LARGE.sun()
+    ow = oracle.OracleWorld.view(world)
+    top = oracle.world_ytop(world.dims, world.origin, world.tile_mask, world.tile_uniform)
+    occ = world.bricks["occupancy"].reshape(world.brick_dims[::-1]) != 0
+    rows = np.nonzero(occ.any(axis=(0, 2)))[0]
+    assert top == 4 * (int(rows.max()) + 1) < world.dims[1]
+    rows_traced = None if wl is workloads.SMALL else (0, 1080, 4)
+    a, sa, ca = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, oracle.LOD, sun, 480.0,
+                         rows_traced)
+    b, sb, cb = ow.frame(cam, wl.width, wl.height, wl.primary_tmax, oracle.LOD | oracle.NO_YTOP,
+                         sun, 480.0, rows_traced)
+    assert a.tobytes() == b.tobytes() and np.array_equal(sa, sb)
+    assert sum(ca.values()) < 0.95 * sum(cb.values())
 
 
 def test_group_skips_keep_lod_results_far():

@@ -192,6 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevWorld w, const vg_ray 
                                                        int64_t n, uint32_t flags,
                                                        vg_hit *__restrict__ hits,
                                                        unsigned long long *counts) {
+    world_ready(w);
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
@@ -222,6 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_frame(DevWorld w, CamParams ca
                                                         vg_hit *__restrict__ hits,
                                                         uint8_t *__restrict__ occluded,
                                                         unsigned long long *counts) {
+    world_ready(w);
     const uint32_t nb = gridDim.x;
     const uint32_t b = xcd_block(blockIdx.x, nb);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -379,6 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     const float *__restrict__ beam_t, vg_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
     uint32_t *work, int threshold, uint32_t stripes, const uint32_t *__restrict__ list,
     const uint32_t *list_n, unsigned long long *counts) {
+    world_ready(w);
     const FrameParams *__restrict__ fpp = frame_params(work);
     // with a block list, ray ids run over the listed 8x8 blocks only
     const uint32_t lane = threadIdx.x & 63;
@@ -586,6 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_water_persistent(
     DevWorld w, CamParams cam, int32_t width, const uint32_t *__restrict__ list,
     const uint32_t *list_n, const vg_hit *__restrict__ prim, vg_hit *__restrict__ refl,
     vg_hit *__restrict__ refr, uint32_t *work, int threshold) {
+    world_ready(w);
     const uint32_t total_ids = *list_n;
     const uint32_t lane = threadIdx.x & 63;
     const int max_iter = w.dims[0] + w.dims[1] + w.dims[2] + 8;
@@ -671,6 +675,7 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
                                                             vg_hit *__restrict__ hits,
                                                             uint32_t *work, int threshold,
                                                             unsigned long long *counts) {
+    world_ready(w);
     const uint32_t lane = threadIdx.x & 63;
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
@@ -746,6 +751,7 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
                                                                uint32_t nbat, uint32_t total,
                                                                uint32_t *work, int threshold,
                                                                unsigned long long *counts) {
+    world_ready(w);
     const uint32_t lane = threadIdx.x & 63;
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
@@ -1108,9 +1114,25 @@ static int launch_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags
     return VG_OK;
 }
 
+// The y bound of an uploaded world (DevWorld::ytop): 4 x (the highest window-local brick row
+// that holds a set tile bit, + 1), a uniform tile counting as all its rows; tiles in the
+// reference order (x fastest, then y, then z). A window that folds y gets no bound.
+static int32_t world_ytop(const Dims &d, const uint64_t *tile_mask, const uint8_t *tile_uniform,
+                          int32_t ro_mod_y) {
+    if (ro_mod_y != 0) return d.v[1];
+    int32_t top = 0;
+    for (int64_t t = 0; t < d.n_tiles; ++t) {
+        const int32_t ty = (int32_t)((t / d.t[0]) % d.t[1]);
+        if (tile_uniform[t]) top = std::max(top, ty * 16 + 16);
+        else if (tile_mask[t])
+            top = std::max(top, ty * 16 + ((63 - __builtin_clzll(tile_mask[t])) >> 4) * 4 + 4);
+    }
+    return top;
+}
+
 static void free_world(vg_world *w) {
-    void *ptrs[] = {w->l4,      w->chunk,   w->tile,       w->occ,  w->tile_uni, w->brick_uni,
-                    w->mat,     w->scratch, w->counts_dev, w->work, w->beam_t};
+    void *ptrs[] = {w->l4,     w->chunk,   w->tile,       w->occ,  w->tile_uni, w->brick_uni,
+                    w->mat,    w->scratch, w->counts_dev, w->work, w->beam_t,   w->ytop_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (w->wlist) (void)hipFree(w->wlist);
@@ -1188,6 +1210,7 @@ int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
         {(void **)&w->counts_dev, VG_IT_COUNT * sizeof(unsigned long long)},
         {(void **)&w->work, kWorkBytes},
         {(void **)&w->beam_t, (size_t)kBeamDefaultBlocks * sizeof(float)},
+        {(void **)&w->ytop_dev, 256},
     };
     for (auto &a : allocs) {
         if (hipMalloc(a.p, a.bytes) != hipSuccess) {
@@ -1213,6 +1236,14 @@ int vg_world_create(vg_ctx *ctx, const int32_t dims[3], vg_world **out) {
         dw.ro_mod[k] = 0;
     }
     dw.fold = 0;
+    // no bound until the world is uploaded or generated (a walk never ends early on it)
+    dw.ytop = d.v[1];
+    dw.ytop_p = w->ytop_dev;
+    if (hipMemcpy(w->ytop_dev, &dw.ytop, sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        free_world(w);
+        delete w;
+        return fail(VG_E_DEVICE, "world y bound");
+    }
     *out = w;
     return VG_OK;
 }
@@ -1295,6 +1326,8 @@ int vg_world_upload(vg_world *w, const vg_brick *bricks, const uint64_t *tile_ma
         w->dw.ro_mod[k] = pos_mod(origin[k], d.v[k]);
         if (w->dw.ro_mod[k]) w->dw.fold = 1;
     }
+    const int32_t ytop = world_ytop(d, tile_mask, tile_uniform, w->dw.ro_mod[1]);
+    HIP_TRY(hipMemcpy(w->ytop_dev, &ytop, sizeof(ytop), hipMemcpyHostToDevice));
     w->uploaded = true;
     return VG_OK;
 }

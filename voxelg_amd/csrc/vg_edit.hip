@@ -253,8 +253,26 @@ __global__ __launch_bounds__(1024) void k_edit_bricks(DevWorld w, const vg_edit 
     }
 }
 
+// 1 + the highest window-local voxel row a tile can hold (0: an empty tile): nested tile tn
+// (chunk tn / 64 in the reference order, tile bit tn % 64), its word, whether it is uniform.
+// Its row in a window that folds y is a slot row, but such a window's bound is dims[1].
+__device__ __forceinline__ int32_t tile_ytop(uint32_t tn, uint64_t word, bool uni, uint32_t cdim0,
+                                             uint32_t cdim1) {
+    const uint32_t ci = tn >> 6;
+    const int32_t ty = (int32_t)(((ci / cdim0) % cdim1) * 4u + ((tn & 63u) >> 4));
+    if (uni) return ty * 16 + 16;
+    if (!word) return 0;
+    return ty * 16 + ((63 - __clzll((long long)word)) >> 4) * 4 + 4;
+}
+
+// Raise the world's y bound to `top` (it only grows; most tiles leave it as it is).
+__device__ __forceinline__ void raise_ytop(int32_t *ytop, int32_t top) {
+    if (top > __hip_atomic_load(ytop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(ytop, top);
+}
+
 // Waves walk the touched-tile list: a dirty tile recomputes its word and uniform byte from
-// its 64 bricks and marks its chunk.
+// its 64 bricks, marks its chunk and raises the world's y bound (never lowered by an edit).
 __global__ __launch_bounds__(256) void k_edit_tiles(const uint32_t *__restrict__ tile_dirty,
                                                     const uint32_t *__restrict__ tile_list,
                                                     const uint32_t *__restrict__ tile_n,
@@ -262,7 +280,8 @@ __global__ __launch_bounds__(256) void k_edit_tiles(const uint32_t *__restrict__
                                                     const uint8_t *__restrict__ bu,
                                                     uint64_t *__restrict__ tile,
                                                     uint8_t *__restrict__ tu,
-                                                    uint32_t *__restrict__ chunk_dirty) {
+                                                    uint32_t *__restrict__ chunk_dirty,
+                                                    uint32_t cdim0, uint32_t cdim1, int32_t *ytop) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nt = *tile_n;
     for (uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6); t < nt; t += gridDim.x * 4u) {
@@ -281,6 +300,7 @@ __global__ __launch_bounds__(256) void k_edit_tiles(const uint32_t *__restrict__
                 const uint32_t ci = tn >> 6;
                 atomicOr(chunk_dirty + (ci >> 5), 1u << (ci & 31u));
             }
+            raise_ytop(ytop, tile_ytop(tn, word, uni, cdim0, cdim1));
         }
     }
 }
@@ -620,7 +640,8 @@ __global__ __launch_bounds__(256) void k_gen_bricks(DevWorld w, float sx, float 
 // per chunk, one thread per L4 cell.
 __global__ __launch_bounds__(256) void k_gen_tiles(uint32_t n_tiles, const uint64_t *__restrict__ occ,
                                                    const uint8_t *__restrict__ bu,
-                                                   uint64_t *__restrict__ tile, uint8_t *__restrict__ tu) {
+                                                   uint64_t *__restrict__ tile, uint8_t *__restrict__ tu,
+                                                   uint32_t cdim0, uint32_t cdim1, int32_t *ytop) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t tn = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (tn >= n_tiles) return; // wave-uniform
@@ -632,8 +653,13 @@ __global__ __launch_bounds__(256) void k_gen_tiles(uint32_t n_tiles, const uint6
     if (lane == 0) {
         tile[tn] = word;
         tu[tn] = uni ? (uint8_t)b0 : (uint8_t)0;
+        raise_ytop(ytop, tile_ytop(tn, word, uni, cdim0, cdim1));
     }
 }
+
+// The generated world's y bound starts at 0 (dims[1] for a window that folds y); the tile
+// pass raises it.
+__global__ void k_set_ytop(int32_t *ytop, int32_t v) { *ytop = v; }
 
 __global__ __launch_bounds__(256) void k_gen_chunks(uint32_t n_chunks, const uint64_t *__restrict__ tile,
                                                     uint64_t *__restrict__ chunk) {
@@ -843,7 +869,8 @@ int vg_world_apply_edits(vg_world *w, const vg_edit *edits, uint32_t n, void *st
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_edit_tiles, dim3((uint32_t)std::min<int64_t>((d.n_tiles + 3) / 4, 1024)),
                        dim3(256), 0, hs, s->tile_dirty, s->tile_list, s->tile_n, w->occ,
-                       w->brick_uni, w->tile, w->tile_uni, s->chunk_dirty);
+                       w->brick_uni, w->tile, w->tile_uni, s->chunk_dirty, (uint32_t)d.c[0],
+                       (uint32_t)d.c[1], w->ytop_dev);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_edit_chunks, dim3((uint32_t)((d.n_chunks + 3) / 4)), dim3(256), 0, hs,
                        s->chunk_dirty, w->dw, (uint32_t)d.n_chunks, w->tile, w->chunk, s->l4_dirty);
@@ -962,8 +989,12 @@ int vg_world_generate(vg_world *w, uint64_t seed, const int32_t origin[3], void 
     hipLaunchKernelGGL(k_gen_bricks, dim3((ncols + 3) / 4), dim3(256), 0, hs, w->dw, sx, sz, ncols,
                        w->occ, w->mat, w->brick_uni);
     HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_set_ytop, dim3(1), dim3(1), 0, hs, w->ytop_dev,
+                       w->dw.ro_mod[1] ? d.v[1] : 0);
+    HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_gen_tiles, dim3((uint32_t)((d.n_tiles + 3) / 4)), dim3(256), 0, hs,
-                       (uint32_t)d.n_tiles, w->occ, w->brick_uni, w->tile, w->tile_uni);
+                       (uint32_t)d.n_tiles, w->occ, w->brick_uni, w->tile, w->tile_uni,
+                       (uint32_t)d.c[0], (uint32_t)d.c[1], w->ytop_dev);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_gen_chunks, dim3((uint32_t)((d.n_chunks + 3) / 4)), dim3(256), 0, hs,
                        (uint32_t)d.n_chunks, w->tile, w->chunk);

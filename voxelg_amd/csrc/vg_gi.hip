@@ -395,6 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     const float4 *__restrict__ btab, const float4 *__restrict__ pal, float4 *__restrict__ rad,
     uint8_t *__restrict__ face_out, LearnedArgs la, HeavyArgs ha, uint32_t *ctr, int threshold,
     int tbatch, uint32_t chunk, unsigned long long *counts) {
+    world_ready(w);
     // BOUNCES: 1 ambient tail, 2 traced cosine bounce, 3 learned (the cache adds the tail;
     // the lanes' kBounce / kShadowY stages then trace training targets), 4 learned without
     // training targets (an update that takes no training step: no bounce stages compiled in)
@@ -972,6 +973,7 @@ __global__ __launch_bounds__(kBlock) void k_nrc_targets(
     const float4 *__restrict__ btab, const float4 *__restrict__ pal,
     vg_nrc_query *__restrict__ tq, float4 *__restrict__ tt, uint32_t *ctr, int threshold,
     unsigned long long *counts) {
+    world_ready(w);
     const vg_nrc_query *queries = qbase + (size_t)(gctr[3] ^ 1u) * half;
     const uint32_t lane = threadIdx.x & 63;
     uint32_t cnt[VG_IT_COUNT];

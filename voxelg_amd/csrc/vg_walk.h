@@ -48,7 +48,16 @@ struct DevWorld {
     int32_t cdim[3];
     int32_t l4dim[3];
     int32_t fold; // 1 when the window origin folds any axis (toroidal slots)
+    // Every occupied voxel has a window-local y below ytop, so a ray at y >= ytop that does
+    // not move down can hit nothing: the walk ends there with a miss. Kernels load the
+    // world's current bound from ytop_p (kept by upload, generation and edits) with
+    // world_ready; the host-side value is dims[1], which never ends a walk early.
+    int32_t ytop;
+    const int32_t *ytop_p;
 };
+
+// The world's y bound for this launch (wave-uniform: one scalar load per kernel).
+__device__ __forceinline__ void world_ready(DevWorld &w) { w.ytop = *w.ytop_p; }
 
 struct CamParams {
     float origin[3];
@@ -359,7 +368,9 @@ template <bool COUNT>
 __device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld &w, float tmax,
                                           uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
     if (r.t_cur > tmax) return true;
-    if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)w.dims[1] ||
+    // out of the window, or above every occupied voxel and not moving down (a miss either way)
+    const int32_t ylim = r.step[1] >= 0 ? w.ytop : w.dims[1];
+    if ((uint32_t)r.l[0] >= (uint32_t)w.dims[0] || (uint32_t)r.l[1] >= (uint32_t)ylim ||
         (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
         return true;
     int32_t sl[3] = {r.l[0], r.l[1], r.l[2]};

@@ -34,6 +34,7 @@ struct vg_world {
     unsigned long long *counts_dev;  // VG_IT_COUNT totals for the *_counts entry points
     uint32_t *work;                  // persistent kernels' ray counter
     float *beam_t;                   // beam start t per 8x8 pixel block
+    int32_t *ytop_dev;               // DevWorld::ytop_p: the y bound of the occupied voxels
     size_t beam_cap;                 // entries of beam_t
     vg_edit_state *edit;
     uint32_t *wlist;                 // water pixels of the last vg_trace_water_secondary
