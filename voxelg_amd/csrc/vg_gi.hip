@@ -74,6 +74,27 @@ constexpr int kGiStripeWords = 32;
 constexpr int kGiCtrWords = kGiStripe0 + kGiStripes * kGiStripeWords;
 static_assert(kGiCtrs <= kGiStripe0, "counters overlap the stripes");
 
+// Static first reservations of the probe path launch (its waves 4 per workgroup, workgroup b
+// on stripe b % kGiStripes): wave gw starts with the ids [64 gw, 64 gw + 64) of the front
+// region (training samples, then the previous update's heavy samples) below lim = min(front,
+// 64 waves); each wave past those (gw >= nfw = ceil(lim / 64)) starts with the next 64 natural
+// samples of its stripe, in the order of gw. k_probe_prep starts the shared counters past the
+// static ids, so no wave waits on an atomic for its first 64 samples (at the launch's start
+// thousands of waves on one counter take ~20 us, and hold up the memory requests behind them).
+// The waves of stripe sc with an index below x:
+__device__ __forceinline__ uint32_t stripe_waves_below(uint32_t x, uint32_t sc) {
+    const uint32_t b = x >> 2, rem = x & 3u; // whole workgroups below x, waves of the next one
+    const uint32_t full = b > sc ? (b - sc + kGiStripes - 1u) / kGiStripes : 0u;
+    return full * 4u + (b % kGiStripes == sc ? rem : 0u);
+}
+
+// Natural samples [s0, s1) of stripe sc (whole probes, slen samples per stripe).
+__device__ __forceinline__ void stripe_range(uint32_t sc, uint32_t slen, uint32_t n, uint32_t &s0,
+                                             uint32_t &s1) {
+    s0 = sc * slen < n ? sc * slen : n;
+    s1 = n - s0 < slen ? n : s0 + slen;
+}
+
 // ----------------------------------------------------------------- helpers --
 __device__ __forceinline__ float gi_smooth(float e0, float e1, float x) {
     float t = (x - e0) / (e1 - e0);
@@ -236,16 +257,31 @@ __device__ __forceinline__ void probe_cell(const vg_gi_params &p, const DevWorld
 __global__ __launch_bounds__(kBlock) void k_probe_prep(DevWorld w, vg_gi_params p, uint32_t n_upd,
                                                       uint32_t phase, const vg_probe *probes,
                                                       float4 *pwork, uint32_t *ctr, int learned,
-                                                      uint32_t *hmask, uint32_t hwords) {
+                                                      uint32_t *hmask, uint32_t hwords,
+                                                      uint32_t train_n, uint32_t hcap,
+                                                      uint32_t static_ids) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j == 0) { // the kernels enqueued next consume the counters
         const uint32_t prev = ctr[1], par = ctr[3], hprev = ctr[kCtrHeavy];
         for (int k = 0; k < kCtrUpd; ++k) ctr[k] = 0u;
+        // the shared counters start past the path launch's static reservations
+        const uint32_t front = train_n + (hprev < hcap ? hprev : hcap);
+        const uint32_t lim = front < static_ids ? front : static_ids;
+        ctr[0] = lim;
+        const uint32_t nw = static_ids / 64u, nfw = (lim + 63u) / 64u;
+        const uint32_t N = (uint32_t)p.rays_per_probe, n = n_upd * N;
+        const uint32_t slen = ((n_upd + kGiStripes - 1u) / kGiStripes) * N;
+        for (uint32_t sc = 0; sc < (uint32_t)kGiStripes; ++sc) {
+            uint32_t s0, s1;
+            stripe_range(sc, slen, n, s0, s1);
+            const uint32_t taken =
+                (stripe_waves_below(nw, sc) - stripe_waves_below(nfw, sc)) * 64u;
+            ctr[kGiStripe0 + sc * kGiStripeWords] = taken < s1 - s0 ? taken : s1 - s0;
+        }
         ctr[2] = learned ? prev : 0u;
         ctr[3] = learned ? (par ^ 1u) : par;
         ctr[kCtrHeavy + 1] = hprev;
     }
-    if (j < (uint32_t)kGiStripes) ctr[kGiStripe0 + j * kGiStripeWords] = 0u;
     // this update's heavy-path bits start clear
     uint32_t *hmask_cur = hmask + (size_t)(ctr[kCtrUpd] & 1u) * hwords;
     for (uint32_t k = j; k < hwords; k += gridDim.x * kBlock) hmask_cur[k] = 0u;
@@ -354,8 +390,8 @@ __device__ __forceinline__ uint32_t pq_take(PathQueue &pq, uint32_t *ctr, uint32
             }
         }
         while (b == ~0u && pq.a_done && pq.left > 0u) {
-            const uint32_t s0 = pq.sc * slen < n ? pq.sc * slen : n;
-            const uint32_t s1 = n - s0 < slen ? n : s0 + slen;
+            uint32_t s0, s1;
+            stripe_range(pq.sc, slen, n, s0, s1);
             uint32_t r = 0;
             if (lane == 0) r = atomicAdd(ctr + kGiStripe0 + pq.sc * kGiStripeWords, g);
             r = __builtin_amdgcn_readfirstlane(__shfl(r, 0, 64));
@@ -437,13 +473,34 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     HitRec h;
     hit_miss(h);
     phase_reset(q);
+    const uint32_t n_upd_probes = n_samples / N;
+    const uint32_t slen = ((n_upd_probes + kGiStripes - 1) / kGiStripes) * N; // whole probes
     PathQueue pq;
-    pq.next = pq.end = 0u;
     pq.sc = blockIdx.x % (uint32_t)kGiStripes; // the stripe of this workgroup's XCD
     pq.left = (uint32_t)kGiStripes;
     pq.a_done = pq.drained = false;
-    const uint32_t n_upd_probes = n_samples / N;
-    const uint32_t slen = ((n_upd_probes + kGiStripes - 1) / kGiStripes) * N; // whole probes
+    {
+        // the wave's static first reservation (stripe_waves_below)
+        const uint32_t front = train_n + n_heavy;
+        const uint32_t nw = gridDim.x * (kBlock / 64);
+        const uint32_t lim = front < nw * 64u ? front : nw * 64u;
+        const uint32_t nfw = (lim + 63u) / 64u;
+        const uint32_t gw = blockIdx.x * (kBlock / 64) + wv;
+        if (gw < nfw) {
+            pq.next = gw * 64u;
+            pq.end = gw * 64u + 64u < lim ? gw * 64u + 64u : lim;
+        } else {
+            // the front region is all reserved statically: natural samples of the stripe
+            uint32_t s0, s1;
+            stripe_range(pq.sc, slen, n_samples, s0, s1);
+            const uint32_t rank =
+                stripe_waves_below(gw, pq.sc) - stripe_waves_below(nfw, pq.sc);
+            const uint32_t b = s0 + rank * 64u;
+            pq.next = front + (b < s1 ? b : s1);
+            pq.end = front + (b + 64u < s1 ? b + 64u : s1);
+            pq.a_done = true;
+        }
+    }
     bool emit = false;  // learned: this lane's finished gather hit still has to be listed
     bool heavy = false; // this lane's finished sample path still has to be recorded heavy
     uint32_t qhead = 0, qn = 0; // the wave's staged queries: ring start and count
@@ -1310,21 +1367,23 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
         pr->tn++;
         HIP_TRY(hipEventRecord(tev[0], stream));
     }
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
+    const bool train = nrc && (nflags & VG_NRC_TRAIN) && nrc->cfg.steps > 0;
+    const uint32_t train_n = train ? (uint32_t)nrc->tcap : 0u;
     hipLaunchKernelGGL(k_probe_prep, dim3((n_upd + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, phase, pr->probes, pr->pwork, pr->ctr, nrc ? 1 : 0,
-                       pr->hmask, pr->hwords);
+                       pr->hmask, pr->hwords, train_n, ha.cap,
+                       grid.x * (uint32_t)(kBlock / 64) * 64u);
     HIP_TRY(hipGetLastError());
     if (tev) HIP_TRY(hipEventRecord(tev[1], stream));
-    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->probe_wpc)), block(kBlock);
     const int thr = w->ctx->probe_refill;
-    const bool train = nrc && (nflags & VG_NRC_TRAIN) && nrc->cfg.steps > 0;
     // the path launch writes this update's training batch into the half the previous
     // update's training step (possibly still running) does not read
     const int tb = nrc ? nrc->tpar ^ 1 : 0;
     LearnedArgs la;
     la.queries = pr->queries;
     la.half = (uint32_t)pr->ns;
-    la.train_n = train ? (uint32_t)nrc->tcap : 0u;
+    la.train_n = train_n;
     la.tseed = nrc ? nrc->cfg.seed ^ 0x9E3779B9u : 0u;
     la.tq = nrc ? nrc->tq_base + (size_t)tb * nrc->tcap : nullptr;
     la.tt = nrc ? nrc->tt_base + (size_t)tb * nrc->tcap : nullptr;
