@@ -605,6 +605,12 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
                 "ids_per_atomic": ctx.get_option(L.VG_OPT_PROBE_CHUNK),
                 "transition_batch": ctx.get_option(L.VG_OPT_PROBE_TBATCH)}
 
+    def set_sched(sc):
+        ctx.set_option(L.VG_OPT_PROBE_WAVES_PER_CU, sc["waves_per_cu"])
+        ctx.set_option(L.VG_OPT_PROBE_REFILL, sc["refill_below"])
+        ctx.set_option(L.VG_OPT_PROBE_CHUNK, sc["ids_per_atomic"])
+        ctx.set_option(L.VG_OPT_PROBE_TBATCH, sc["transition_batch"])
+
     def train_on(i):
         return i % GI_TRAIN_EVERY == 0
 
@@ -633,19 +639,32 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
     stream.synchronize()
     log(f"gi learned: {train_updates} training updates in {time.perf_counter() - t0:.2f} s, "
         f"loss {cache.stats()[1]:.5f}")
+    # two candidate schedules: tuned on updates that each take a training step (the tuner's
+    # pairs), and tuned on inference-only updates; the learned update (a step every
+    # GI_TRAIN_EVERY updates) is timed under both and keeps the faster
     gl.autotune(pl, cache, train=True)  # (its repetitions train the cache further)
-    sched_learned = sched()
-    # an even number of updates, so the span holds whole training cadences
-    lsteps = max(GI_TRAIN_EVERY, steps - steps % GI_TRAIN_EVERY)
-    learned_ms = timed(lambda p, i: gl.update_learned(cache, p, train=train_on(i),
-                                                       stream=stream),
-                       pl, lsteps, cache, gl, "learned")
+    sched_train = sched()
     gl.autotune(pl, cache, train=False)  # the inference-only update has its own schedule
     sched_frozen = sched()
     frozen_ms = timed(lambda p, i: gl.update_learned(cache, p, train=False, stream=stream), pl,
                       steps, cache, gl, "learned_inference_only")
+    # an even number of updates, so the span holds whole training cadences
+    lsteps = max(GI_TRAIN_EVERY, steps - steps % GI_TRAIN_EVERY)
+
+    def learned_run(p, i):
+        gl.update_learned(cache, p, train=train_on(i), stream=stream)
+
+    cand = {}
+    for name, sc in (("tuned_with_training", sched_train), ("tuned_inference_only", sched_frozen)):
+        set_sched(sc)
+        cand[name] = span(learned_run, pl, lsteps, cache)
+    best = min(cand, key=cand.get)
+    sched_learned = dict(sched_train if best == "tuned_with_training" else sched_frozen,
+                         picked=best)
+    set_sched(sched_learned)
+    learned_ms = timed(learned_run, pl, lsteps, cache, gl, "learned")
     log(f"gi learned {GI_LEARNED_RAYS} rays: {learned_ms:.3f} ms/update with training "
-        f"({sched_learned}), {frozen_ms:.3f} without ({sched_frozen})")
+        f"({sched_learned}; candidates {cand}), {frozen_ms:.3f} without ({sched_frozen})")
     pl["round"] = 5000
     raysl, nq, itl = gl.update_learned_counts(cache, pl, train=True, classes=True)
     log(f"gi learned: rays {raysl}, cache queries {nq}, walk classes {itl}; "
@@ -733,6 +752,7 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         "schedule_plain": plain[256]["sched"], "schedule_plain32": plain[32]["sched"],
         "schedule_plain16": plain[16]["sched"], "schedule_learned": sched_learned,
         "schedule_inference_only": sched_frozen,
+        "learned_schedule_candidates_ms": {k: round(v, 4) for k, v in cand.items()},
         "err_plain256": round(errs["plain256"], 5),
         "err_plain32_2bounce": round(errs["plain32"], 5),
         "err_plain16_2bounce": round(errs["plain16"], 5),

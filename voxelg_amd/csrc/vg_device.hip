@@ -287,6 +287,9 @@ struct FrameParams {
     int32_t width, height;
     uint32_t nbx;   // pixel blocks per image row
     uint32_t bws;   // log2 of the block width: blocks of 2^bws x 64 / 2^bws pixels (a wave)
+    // the persistent launch's static first reservations (frame_static): its workgroups, its
+    // stripes, the frame's pixel blocks (0 for a block list: no static reservations)
+    uint32_t grid_wg, stripes, nblk;
 };
 constexpr size_t kWorkBytes = kStripeBytes + 256; // counters, then the FrameParams
 static_assert(sizeof(FrameParams) <= 256, "FrameParams must fit its slot");
@@ -323,8 +326,49 @@ __device__ __forceinline__ void stripe_take(uint32_t *work, uint32_t stripes, ui
 }
 
 // Zero the striped counters (one thread per stripe).
-__device__ __forceinline__ void stripe_reset(uint32_t *work, uint32_t t) {
-    if (t < kMaxStripes) work[t * kStripeWords] = 0u;
+// Static first reservations of the persistent frame kernel. Workgroup b starts on stripe
+// frame_stripe(b) (round-robin over the XCDs, as the kernel picks it); the waves of the
+// workgroups of one stripe, in the order of b, take its first pixel blocks one each, so no
+// wave waits on the stripe's atomic for its first 64 rays (thousands of waves on a few
+// counters at the launch's start take ~20 us); the counters start past those blocks.
+__device__ __forceinline__ uint32_t frame_stripe(uint32_t b, uint32_t stripes) {
+    const uint32_t per_xcd = stripes >= 8u ? stripes / 8u : 1u;
+    return stripes >= 8u ? (b % 8u) * per_xcd + (b / 8u) % per_xcd : b % stripes;
+}
+// the workgroups before b on b's stripe
+__device__ __forceinline__ uint32_t frame_stripe_rank(uint32_t b, uint32_t stripes) {
+    return stripes >= 8u ? (b / 8u) / (stripes / 8u) : b / stripes;
+}
+// the workgroups of a grid_wg-workgroup launch on stripe sc
+__device__ __forceinline__ uint32_t frame_stripe_wgs(uint32_t sc, uint32_t stripes,
+                                                     uint32_t grid_wg) {
+    if (stripes >= 8u) {
+        const uint32_t per_xcd = stripes / 8u;
+        if (sc >= 8u * per_xcd) return 0u;
+        const uint32_t first = sc / per_xcd + 8u * (sc % per_xcd); // its lowest workgroup
+        return first < grid_wg ? (grid_wg - 1u - first) / (8u * per_xcd) + 1u : 0u;
+    }
+    return sc < grid_wg ? (grid_wg - 1u - sc) / stripes + 1u : 0u;
+}
+// pixel blocks [b0, b1) of stripe sc: sblk = ceil(nblk / stripes) each
+__device__ __forceinline__ void frame_stripe_blocks(uint32_t sc, uint32_t sblk, uint32_t nblk,
+                                                    uint32_t &b0, uint32_t &b1) {
+    b0 = sc * sblk < nblk ? sc * sblk : nblk;
+    b1 = b0 + sblk < nblk ? b0 + sblk : nblk;
+}
+
+// Each stripe counter (one thread per stripe) starts past the blocks its waves take statically.
+__device__ __forceinline__ void stripe_reset(uint32_t *work, uint32_t t, const FrameParams &fp) {
+    if (t >= kMaxStripes) return;
+    uint32_t v = 0u;
+    if (fp.nblk && t < fp.stripes) {
+        const uint32_t sblk = (fp.nblk + fp.stripes - 1u) / fp.stripes;
+        uint32_t b0, b1;
+        frame_stripe_blocks(t, sblk, fp.nblk, b0, b1);
+        const uint32_t waves = frame_stripe_wgs(t, fp.stripes, fp.grid_wg) * (kBlock / 64);
+        v = (waves < b1 - b0 ? waves : b1 - b0) * 64u;
+    }
+    work[t * kStripeWords] = v;
 }
 
 // Beam pre-pass of the persistent frame: one wave per 8x8 pixel block writes the block's
@@ -339,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, FrameParams fp, uin
                                                  const uint32_t *list_n,
                                                  unsigned long long *counts) {
     if (blockIdx.x == 0) {
-        stripe_reset(work, threadIdx.x);
+        stripe_reset(work, threadIdx.x, fp);
         if (threadIdx.x == 0) *frame_params(work) = fp;
     }
     const CamParams &cam = fp.cam;
@@ -365,7 +409,7 @@ __global__ void k_reset_work(uint32_t *work) { *work = 0u; }
 
 // The frame launch's first kernel without the beam: counters and FrameParams.
 __global__ void k_frame_setup(uint32_t *work, FrameParams fp) {
-    stripe_reset(work, threadIdx.x);
+    stripe_reset(work, threadIdx.x, fp);
     if (threadIdx.x == 0) *frame_params(work) = fp;
 }
 
@@ -394,6 +438,18 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     const uint32_t per_xcd = stripes >= 8u ? stripes / 8u : 1u;
     uint32_t sc = stripes >= 8u ? (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u) % per_xcd
                                 : blockIdx.x % stripes;
+    // the wave's static first reservation: one pixel block of its stripe (frame_stripe)
+    uint32_t st_base = 0u, st_lim = 0u;
+    if (!list) {
+        uint32_t b0, b1;
+        frame_stripe_blocks(sc, sblk, nblk, b0, b1);
+        const uint32_t blk = b0 + frame_stripe_rank(blockIdx.x, stripes) * (kBlock / 64) +
+                             (threadIdx.x >> 6);
+        if (blk < b1) {
+            st_base = blk * 64u;
+            st_lim = st_base + 64u;
+        }
+    }
     uint32_t cnt[VG_IT_COUNT];
     if (COUNT)
 #pragma unroll
@@ -415,7 +471,14 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
         int nact = __popcll(act);
         if (!exhausted && nact < threshold) {
             uint32_t base = ~0u, lim = 0u; // ids [base, lim) for the idle lanes
-            stripe_take(work, stripes, nblk, sblk, 64u - (uint32_t)nact, lane, sc, base, lim);
+            if (st_lim) { // the first refill (every lane idle): the static reservation
+                base = st_base;
+                lim = st_lim;
+                st_lim = 0u;
+            } else {
+                stripe_take(work, stripes, nblk, sblk, 64u - (uint32_t)nact, lane, sc, base,
+                            lim);
+            }
             if (base == ~0u) {
                 exhausted = true;
             } else if (kind == 0) {
@@ -916,6 +979,11 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     fp.height = height;
     fp.nbx = nbx;
     fp.bws = bws;
+    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
+    const uint32_t stripes = (uint32_t)w->ctx->frame_stripes;
+    fp.grid_wg = grid.x;
+    fp.stripes = stripes;
+    fp.nblk = list ? 0u : (uint32_t)nblocks;
     if (beam) {
         dim3 g((uint32_t)((nblocks + 3) / 4));
         if (count)
@@ -928,10 +996,8 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
         hipLaunchKernelGGL(k_frame_setup, dim3(1), dim3(kMaxStripes), 0, stream, work, fp);
     }
     HIP_TRY(hipGetLastError());
-    const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->frame_wpc)), block(kBlock);
     const uint32_t total = (uint32_t)(nblocks * 64);
     const int thr = w->ctx->frame_refill;
-    const uint32_t stripes = (uint32_t)w->ctx->frame_stripes;
 #define LAUNCH_P(C, SH, B)                                                                     \
     hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, total,     \
                        ptmax, pflags, stmax, beam_t, hits, occluded, work, thr, stripes, list,  \

@@ -479,6 +479,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     pq.sc = blockIdx.x % (uint32_t)kGiStripes; // the stripe of this workgroup's XCD
     pq.left = (uint32_t)kGiStripes;
     pq.a_done = pq.drained = false;
+    // A wave whose static reservation holds front samples (the longest paths) takes no other
+    // work when the whole front region is reserved statically and other waves are left for
+    // the natural samples: its loop trips then carry only its own paths' walks and transitions.
+    bool front_only = false;
     {
         // the wave's static first reservation (stripe_waves_below)
         const uint32_t front = train_n + n_heavy;
@@ -489,6 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
         if (gw < nfw) {
             pq.next = gw * 64u;
             pq.end = gw * 64u + 64u < lim ? gw * 64u + 64u : lim;
+            front_only = lim == front && nfw < nw;
         } else {
             // the front region is all reserved statically: natural samples of the stripe
             uint32_t s0, s1;
@@ -789,7 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
     for (;;) {
         uint64_t act = __ballot(stage != kIdle);
         int nact = __popcll(act);
-        if (!pq_empty(pq) && nact < threshold) {
+        if (!pq_empty(pq) && nact < threshold && !(front_only && pq.next >= pq.end)) {
             if (LEARNED) stage_queries(false);
             flush_heavy();
             const uint32_t id = pq_take(pq, ctr, train_n + n_heavy, n_samples, slen,
@@ -802,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             nact = __popcll(__ballot(stage != kIdle));
         }
         if (nact == 0) {
-            if (pq_empty(pq)) {
+            if (pq_empty(pq) || (front_only && pq.next >= pq.end)) {
                 if (LEARNED) stage_queries(true);
                 flush_heavy();
                 break;
