@@ -226,7 +226,7 @@ def main():
                     help="dynamic: bit-exact oracle check of the frame every N frames")
     ap.add_argument("--gi-train-updates", type=int, default=150,
                     help="learned GI: online-training updates before the timed ones")
-    ap.add_argument("--gi-steps", type=int, default=10, help="GI: timed probe updates per mode")
+    ap.add_argument("--gi-steps", type=int, default=20, help="GI: timed probe updates per mode")
     ap.add_argument("--no-gi", action="store_true", help="frame: skip the GI probe half")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (a plain copy kernel "
@@ -537,13 +537,13 @@ def bench_batched(args):
 
 # The learned update's configuration: 16 gather rays per probe ending at the first hit in
 # the radiance cache (16 hash-grid levels of 2^16 entries), trained online with one step of
-# 4096 plain-traced bounce samples every second update. Measured on this workload: the error
+# 4096 plain-traced bounce samples every fourth update. Measured on this workload: the error
 # is flat across 1..8 updates per step, 2k..64k samples per step and 2^15..2^19 entries per
-# level (1.80-1.86 %): the 16-ray gather quadrature sets it; the smaller table and batch
-# make the step and the inference cheaper.
+# level (1.80-1.86 %): the 16-ray gather quadrature sets it; the smaller table, batch and
+# cadence make the training step's share of the chip smaller.
 GI_LEARNED_RAYS = 16
 GI_TRAIN_BATCH = 4096
-GI_TRAIN_EVERY = 2
+GI_TRAIN_EVERY = 4
 GI_CACHE_LOG2_TABLE = 16
 
 
