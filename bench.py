@@ -765,8 +765,8 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
                      "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(GI_TRAFFIC_PROFILE, "traffic_bytes_per_update"),
                      "traffic_source": GI_TRAFFIC_PROFILE,
-                     "kernel": "k_probe_paths<false, 3> (the learned update's path launch, "
-                               "inference-only update)",
+                     "kernel": "k_probe_paths<false, 4> (the learned update's path launch, "
+                               "an update without a training step)",
                      "kernel_ms": round(path_ms, 4), "algorithmic_bytes": int(path_bytes_frozen)},
         "indirect_1080p_ms": round(indirect_ms, 4),
         "train_steps": cache.stats()[0], "train_batch": cache.batch,
@@ -778,8 +778,8 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
 
 
 def bench_gi_main(args):
-    """One step = one learned GI probe update (32 rays/probe + online training) on the
-    2048x256x2048 world; the plain 256-ray update is timed beside it."""
+    """One step = one learned GI probe update (GI_LEARNED_RAYS rays/probe, the cache trained
+    online) on the 2048x256x2048 world; the plain 256-ray update is timed beside it."""
     from voxelg_amd import workloads
     from voxelg_amd.device import Context, device_count
 
@@ -827,7 +827,7 @@ def bench_dynamic(args):
     import voxelg_amd as vg
     from voxelg_amd import gi, nrc, workloads
     from voxelg_amd._lib import VG_GI_SKIP_BURIED
-    from voxelg_amd.device import Context, decode_hits, device_count
+    from voxelg_amd.device import Context, decode_hits, device_count, mask_words
 
     assert device_count() > 0, "bench.py needs a HIP device"
     wl = workloads.DYNAMIC
@@ -846,11 +846,15 @@ def bench_dynamic(args):
     p = gi.gi_params(sun, grid=pgrid, spacing=32, rays_per_probe=32, epochs=1, update_div=1,
                      bounces=2, flags=VG_GI_SKIP_BURIED)
     probes = gi.ProbeGrid(dw, p)
-    cache = nrc.RadianceCache(ctx)
+    # the GI bench's cache size and batch (the world changes every frame: a step every frame)
+    cfg = nrc.default_config(log2_table=GI_CACHE_LOG2_TABLE)
+    cfg["batch"] = GI_TRAIN_BATCH
+    cache = nrc.RadianceCache(ctx, cfg)
     n_frames = args.warmup + args.steps
     edits = [wl.edits(f, world) for f in range(n_frames)]
     phases = ("edits", "dirty", "retrace", "probes", "indirect")
     ind = ctx.alloc(W * H * 16)
+    mask_buf = ctx.alloc(4 * mask_words(W, H))  # one block mask, reused every frame
     times = {k: [] for k in phases}
     dirty_frac = []
     checks = []
@@ -862,7 +866,7 @@ def bench_dynamic(args):
         ev[0].record(stream)
         dw.apply_edits(edits[f], stream)
         ev[1].record(stream)
-        mask = dw.dirty_blocks(cam, W, H, sun, 480.0, stream=stream)
+        mask = dw.dirty_blocks(cam, W, H, sun, 480.0, mask=mask_buf, stream=stream)
         ev[2].record(stream)
         dw.trace_frame_blocks(cam, W, H, 512.0, 0, sun, 480.0, mask, hits, occ, stream)
         ev[3].record(stream)
@@ -906,7 +910,9 @@ def bench_dynamic(args):
         "config": {"workload": "dynamic: 256x128x256 world, 4096 sphere edits per frame (r 2..12, "
                                "50% carve / 50% fill), 1920x1080 primary + sun shadow retrace of "
                                "the dirty 8x8 blocks, learned probe update (8x4x8 probes, 32 "
-                               "rays, online training), per-pixel probe indirect light",
+                               f"rays, online training: {GI_TRAIN_BATCH} samples per frame, "
+                               f"2^{GI_CACHE_LOG2_TABLE} entries per level), per-pixel probe "
+                               "indirect light",
                    "world": list(wl.dims), "resolution": [W, H], "edits_per_frame": wl.n_edits},
         "phases_ms": ms, "dirty_block_fraction": round(float(np.mean(dirty_frac)), 4),
         "checks": checks,

@@ -147,6 +147,13 @@ def _stream(stream) -> int | None:
     return h if h is not None else int(stream)
 
 
+def mask_words(width: int, height: int) -> int:
+    """u32 words of a frame's 8x8-block mask (vg_frame_mask_words)."""
+    n = C.c_uint32()
+    check(lib().vg_frame_mask_words(width, height, C.byref(n)), "vg_frame_mask_words")
+    return int(n.value)
+
+
 def decode_hits(hits, count: int | None = None) -> np.ndarray:
     """Device hits (DeviceBuffer, torch tensor or numpy) -> numpy structured HIT_DTYPE."""
     if isinstance(hits, DeviceBuffer):
@@ -329,12 +336,10 @@ class DeviceWorld:
     def dirty_blocks(self, camera: np.ndarray, width: int, height: int, sun=None,
                      shadow_tmax: float = 480.0, mask=None, stream=None):
         """Device block mask of the 8x8 blocks the last edits can change (sun None: primary
-        rays only)."""
+        rays only); into `mask` (mask_words(width, height) u32) when given."""
         cam = np.ascontiguousarray(camera, np.float32)
-        n = C.c_uint32()
-        check(lib().vg_frame_mask_words(width, height, C.byref(n)), "vg_frame_mask_words")
         if mask is None:
-            mask = self._buf(int(n.value) * 4)
+            mask = self._buf(mask_words(width, height) * 4)
         check(lib().vg_frame_dirty_blocks(self._h, ptr(cam), width, height,
                                           None if sun is None else f32x3(sun), shadow_tmax,
                                           mask.data_ptr(), _stream(stream)),
