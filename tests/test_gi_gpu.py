@@ -61,6 +61,20 @@ def run_rounds(ctx, dw, world, params, rounds, check_every=1):
     return grid, ref
 
 
+def test_plain_update_on_a_folded_window(ctx):
+    # a window whose origin folds x and z into toroidal slots runs the path kernel's folding
+    # instantiation: the records against the oracle's on the same window
+    import voxelg_amd as vg
+    from voxelg_amd import workloads
+    world = vg.World(workloads.SMALL.dims, origin=(96, 0, 160)).generate(workloads.SMALL.seed)
+    dw = ctx.world(world.dims).upload(world)
+    p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=32, epochs=1,
+                     update_div=1, bounces=2)
+    _, ref = run_rounds(ctx, dw, world, p, range(2))
+    assert (ref[:, 23] == 1.0).sum() > 50
+    dw.close()
+
+
 def test_plain_two_bounce_matches_oracle(ctx, small, small_world):
     p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=128, epochs=2,
                      update_div=1, bounces=2)

@@ -157,6 +157,29 @@ def test_kat_floor_pillars(ctx, origin):
     dw.close()
 
 
+@pytest.mark.parametrize("origin", [(0, 0, 0), (96, 0, 160)])
+def test_frame_on_pillars_folded_or_not(ctx, origin):
+    # the frame kernel has one instantiation for windows whose origin folds an axis (toroidal
+    # slots) and one for windows that fold none: both against R1's frame, bit for bit, with
+    # LOD, the beam pre-pass and the sun shadow rays
+    ow = scenes.floor_pillars(origin)
+    cam = vg.Camera(pos=(origin[0] + 4.0, 110.0, origin[2] + 4.0), yaw=0.8, pitch=-0.6,
+                    fov_y=np.radians(70.0)).uniform(320, 180)
+    sun = workloads.LARGE.sun()
+    flags = vg.VG_RAY_LOD | vg.VG_RAY_BEAM
+    print(f"case: pillars frame origin {origin}", flush=True)
+    dw = ctx.world(ow.dims).upload(scenes.to_world(ow))
+    hits, occ = dw.trace_frame(cam, 320, 180, 700.0, flags, sun, 480.0)
+    ctx.synchronize()
+    got, occ = gpu_hits(hits), occ.numpy(np.uint8)
+    ref, rsh, cnt = ow.frame(cam, 320, 180, 700.0, flags, sun, 480.0, threads=THREADS)
+    assert_same(got, ref, f"pillars frame origin {origin} vs R1")
+    assert occ.tobytes() == rsh.tobytes()
+    assert ref["hit"].sum() > 10000 and 0 < rsh.sum() < ref["hit"].sum()
+    assert cnt["beam_test"] > 0
+    dw.close()
+
+
 def test_kat_roof_shadow_and_empty(ctx):
     ow = scenes.roof_scene()
     origins, sun, under, open_ = scenes.roof_samples()

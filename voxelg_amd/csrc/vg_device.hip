@@ -418,8 +418,9 @@ __global__ void k_frame_setup(uint32_t *work, FrameParams fp) {
 // it has idle lanes from one global counter (one atomic per wave) and hands them out by
 // lane rank. Ray ids run over 8x8 pixel blocks, so a refill stays spatially coherent. A
 // primary hit becomes that lane's shadow ray at once. The grid is sized to the machine
-// (waves per CU chosen by the autotuner), not to the image.
-template <bool COUNT, bool SHADOW, bool BEAM>
+// (waves per CU chosen by the autotuner), not to the image. FOLD = false: the world's window
+// folds no axis (the host picks the instantiation).
+template <bool COUNT, bool SHADOW, bool BEAM, bool FOLD>
 __global__ __launch_bounds__(kBlock) void k_frame_persistent(
     DevWorld w, uint32_t total_ids, float ptmax, uint32_t pflags, float stmax,
     const float *__restrict__ beam_t, vg_hit *__restrict__ hits, uint8_t *__restrict__ occluded,
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_persistent(
                 // one inlined walk for both kinds (per-lane parameters), so a wave holding
                 // primary and shadow rays at once does not execute two walk bodies
                 const bool prim = kind == 1;
-                bool done = walk_iter<COUNT>(r, q, w, prim ? ptmax : stmax,
+                bool done = walk_iter<COUNT, FOLD>(r, q, w, prim ? ptmax : stmax,
                                                   prim ? pflags : (uint32_t)VG_RAY_ANY_HIT,
                                                   prim && plod, h, cnt);
                 done = done || ++it >= max_iter;
@@ -731,7 +732,8 @@ __global__ __launch_bounds__(kBlock) void k_water_persistent(
 }
 
 // Persistent, ray-compacting batched query: the same refill discipline over a vg_ray array.
-template <bool COUNT>
+// FOLD as for k_frame_persistent.
+template <bool COUNT, bool FOLD>
 __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
                                                             const vg_ray *__restrict__ rays,
                                                             uint32_t n, uint32_t flags,
@@ -791,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
         }
         do {
             if (busy) {
-                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
+                bool done = walk_iter<COUNT, FOLD>(r, q, w, tmax, f, lod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     store_hit(hits + id, h);
@@ -807,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void k_rays_persistent(DevWorld w,
 // concatenation of all batches (prefix[b] = first id of batch b, prefix[nb] = total), each
 // lane finds its batch by binary search, and the refill discipline is that of
 // k_rays_persistent, so short and long batches share the machine without tail waste.
-template <bool COUNT>
+template <bool COUNT, bool FOLD>
 __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
                                                                const vg_batch *__restrict__ bat,
                                                                const uint32_t *__restrict__ prefix,
@@ -878,7 +880,7 @@ __global__ __launch_bounds__(kBlock) void k_batches_persistent(DevWorld w,
         }
         do {
             if (busy) {
-                bool done = walk_iter<COUNT>(r, q, w, tmax, f, lod, h, cnt);
+                bool done = walk_iter<COUNT, FOLD>(r, q, w, tmax, f, lod, h, cnt);
                 done = done || ++it >= max_iter;
                 if (done) {
                     store_hit(out, h);
@@ -999,9 +1001,16 @@ static int launch_frame_persistent_on(vg_world *w, const CamParams &cp, int32_t 
     const uint32_t total = (uint32_t)(nblocks * 64);
     const int thr = w->ctx->frame_refill;
 #define LAUNCH_P(C, SH, B)                                                                     \
-    hipLaunchKernelGGL((k_frame_persistent<C, SH, B>), grid, block, 0, stream, w->dw, total,     \
-                       ptmax, pflags, stmax, beam_t, hits, occluded, work, thr, stripes, list,  \
-                       list_n, cd)
+    do {                                                                                       \
+        if (w->dw.fold)                                                                        \
+            hipLaunchKernelGGL((k_frame_persistent<C, SH, B, true>), grid, block, 0, stream,   \
+                               w->dw, total, ptmax, pflags, stmax, beam_t, hits, occluded,     \
+                               work, thr, stripes, list, list_n, cd);                          \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_frame_persistent<C, SH, B, false>), grid, block, 0, stream,  \
+                               w->dw, total, ptmax, pflags, stmax, beam_t, hits, occluded,     \
+                               work, thr, stripes, list, list_n, cd);                          \
+    } while (0)
     if (count) {
         if (shadow) {
             if (beam) LAUNCH_P(true, true, true);
@@ -1159,13 +1168,18 @@ static int launch_rays(vg_world *w, const vg_ray *rays, size_t n, uint32_t flags
     if (w->ctx->schedule == VG_SCHEDULE_PERSISTENT) {
         hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, stream, w->work);
         const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
-        if (count)
-            hipLaunchKernelGGL((k_rays_persistent<true>), grid, block, 0, stream, w->dw, rays,
-                               (uint32_t)n, flags, hits, w->work, w->ctx->rays_refill,
-                               w->counts_dev);
-        else
-            hipLaunchKernelGGL((k_rays_persistent<false>), grid, block, 0, stream, w->dw, rays,
-                               (uint32_t)n, flags, hits, w->work, w->ctx->rays_refill, nullptr);
+#define LAUNCH_R(C, F)                                                                         \
+    hipLaunchKernelGGL((k_rays_persistent<C, F>), grid, block, 0, stream, w->dw, rays,           \
+                       (uint32_t)n, flags, hits, w->work, w->ctx->rays_refill,                  \
+                       C ? w->counts_dev : nullptr)
+        if (count) {
+            if (w->dw.fold) LAUNCH_R(true, true);
+            else LAUNCH_R(true, false);
+        } else {
+            if (w->dw.fold) LAUNCH_R(false, true);
+            else LAUNCH_R(false, false);
+        }
+#undef LAUNCH_R
         HIP_TRY(hipGetLastError());
         return VG_OK;
     }
@@ -1840,8 +1854,12 @@ int vg_trace_batches(vg_world *w, const vg_batch *batches, const uint32_t *prefi
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, w->work);
     const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
-    hipLaunchKernelGGL((k_batches_persistent<false>), grid, block, 0, s, w->dw, batches, prefix,
-                       n_batches, total, w->work, w->ctx->rays_refill, nullptr);
+    if (w->dw.fold)
+        hipLaunchKernelGGL((k_batches_persistent<false, true>), grid, block, 0, s, w->dw, batches,
+                           prefix, n_batches, total, w->work, w->ctx->rays_refill, nullptr);
+    else
+        hipLaunchKernelGGL((k_batches_persistent<false, false>), grid, block, 0, s, w->dw, batches,
+                           prefix, n_batches, total, w->work, w->ctx->rays_refill, nullptr);
     HIP_TRY(hipGetLastError());
     return VG_OK;
 }
@@ -1856,8 +1874,14 @@ int vg_trace_batches_counts(vg_world *w, const vg_batch *batches, const uint32_t
         if (!batches || !prefix) return fail(VG_E_INVALID, "null batches/prefix");
         hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, 0, w->work);
         const dim3 grid((uint32_t)persistent_grid(w->ctx, w->ctx->rays_wpc)), block(kBlock);
-        hipLaunchKernelGGL((k_batches_persistent<true>), grid, block, 0, 0, w->dw, batches, prefix,
-                           n_batches, total, w->work, w->ctx->rays_refill, w->counts_dev);
+        if (w->dw.fold)
+            hipLaunchKernelGGL((k_batches_persistent<true, true>), grid, block, 0, 0, w->dw,
+                               batches, prefix, n_batches, total, w->work, w->ctx->rays_refill,
+                               w->counts_dev);
+        else
+            hipLaunchKernelGGL((k_batches_persistent<true, false>), grid, block, 0, 0, w->dw,
+                               batches, prefix, n_batches, total, w->work, w->ctx->rays_refill,
+                               w->counts_dev);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpy(counts_host, w->counts_dev, VG_IT_COUNT * 8, hipMemcpyDeviceToHost));

@@ -424,7 +424,7 @@ template <uint32_t CAP> struct QueryStage {
     float4 q[kBlock / 64][CAP][2];
 };
 
-template <bool COUNT, int BOUNCES>
+template <bool COUNT, int BOUNCES, bool FOLD>
 __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
     const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
@@ -819,9 +819,10 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             bool ended = stage == kNew;
             if (stage >= kGather && !pend) {
                 const bool shadow = stage == kShadowX || stage == kShadowY;
-                bool done = walk_iter<COUNT>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
-                                             shadow ? VG_RAY_ANY_HIT : VG_RAY_SKIP_TRANSPARENT,
-                                             false, h, cnt);
+                bool done = walk_iter<COUNT, FOLD>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                                                   shadow ? VG_RAY_ANY_HIT
+                                                          : VG_RAY_SKIP_TRANSPARENT,
+                                                   false, h, cnt);
                 ended = done || ++it >= max_iter;
             }
             pend = pend || ended;
@@ -1392,11 +1393,16 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     la.tseed = nrc ? nrc->cfg.seed ^ 0x9E3779B9u : 0u;
     la.tq = nrc ? nrc->tq_base + (size_t)tb * nrc->tcap : nullptr;
     la.tt = nrc ? nrc->tt_base + (size_t)tb * nrc->tcap : nullptr;
-#define LAUNCH_G(C, B)                                                                         \
-    hipLaunchKernelGGL((k_probe_paths<C, B>), grid, block, 0, stream, w->dw, *p,                 \
+#define LAUNCH_GF(C, B, F)                                                                     \
+    hipLaunchKernelGGL((k_probe_paths<C, B, F>), grid, block, 0, stream, w->dw, *p,              \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
                        pr->palette, pr->rad, pr->face, la, ha, pr->ctr, thr,                    \
                        w->ctx->probe_tbatch, (uint32_t)w->ctx->probe_chunk, cd)
+#define LAUNCH_G(C, B)                                                                         \
+    do {                                                                                       \
+        if (w->dw.fold) LAUNCH_GF(C, B, true);                                                 \
+        else LAUNCH_GF(C, B, false);                                                           \
+    } while (0)
     // a learned update that takes no training step runs the instantiation without the
     // training stages
     const int mode = nrc ? (la.train_n ? 3 : 4) : p->bounces;
@@ -1412,6 +1418,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
         else LAUNCH_G(false, 1);
     }
 #undef LAUNCH_G
+#undef LAUNCH_GF
     HIP_TRY(hipGetLastError());
     if (tev) HIP_TRY(hipEventRecord(tev[2], stream));
     if (nrc) {

@@ -363,8 +363,9 @@ __device__ __forceinline__ void phase_reset(WalkPhase &q) { q.phase1 = true; }
 // decision needs (the LOD probes' topmost material, a set voxel's material) stay behind
 // branches. Keeping the words of the cells the walk has not left in registers and fetching
 // only on a change, or testing level by level with branches, is 6-30 % slower on the probe
-// paths and batched queries and no faster on the coherent frame.
-template <bool COUNT>
+// paths and batched queries and no faster on the coherent frame. FOLD = false compiles the
+// slot folding out, for worlds whose window origin folds no axis (fewer registers).
+template <bool COUNT, bool FOLD = true>
 __device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld &w, float tmax,
                                           uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
     if (r.t_cur > tmax) return true;
@@ -374,7 +375,7 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld 
         (uint32_t)r.l[2] >= (uint32_t)w.dims[2])
         return true;
     int32_t sl[3] = {r.l[0], r.l[1], r.l[2]};
-    if (w.fold) {
+    if (FOLD && w.fold) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             int32_t s = r.l[k] + w.ro_mod[k];
