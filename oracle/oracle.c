@@ -1190,16 +1190,22 @@ float orc_beam_t(const orc_world *w, const float *cam, int32_t bx, int32_t by, f
                 dmax[k] = fmax_(dmax[k], dd[k]);
             }
         }
+    /* every block ray rises (or stays level) and a step's box lies at or above the world's
+     * occupied y bound: nothing is left to hit, as when the box leaves the window */
+    const int rising = dmin[1] >= 0.0f;
+    const float ybound = (float)(w->origin[1] + w->ytop);
     float t0 = 0.0f;
     for (int st = 0; st < BEAM_MAX_STEPS && t0 < tmax; ++st) {
         float t1 = (float)(st + 1) * BEAM_STEP;
         int32_t lo[3], hi[3];
         int64_t n = 1;
+        float mny = 0.0f;
         for (int k = 0; k < 3; ++k) {
             float a = cam[k] + dmin[k] * t0, b = cam[k] + dmin[k] * t1;
             float c = cam[k] + dmax[k] * t0, e = cam[k] + dmax[k] * t1;
             float mn = fmin_(fmin_(a, b), fmin_(c, e)) - 1.0f;
             float mx = fmax_(fmax_(a, b), fmax_(c, e)) + 1.0f;
+            if (k == 1) mny = mn;
             int32_t tl = (int32_t)floorf(mn * 0.0625f), th = (int32_t)floorf(mx * 0.0625f);
             /* world tiles of the window */
             int32_t wl = w->origin[k] >> 4, wh = (w->origin[k] + w->dims[k] - 1) >> 4;
@@ -1209,7 +1215,9 @@ float orc_beam_t(const orc_world *w, const float *cam, int32_t bx, int32_t by, f
             hi[k] = th;
             n *= th >= tl ? (int64_t)(th - tl + 1) : 0;
         }
-        if (n == 0) { /* the box left the window: every block ray has exited it for good */
+        if (n == 0 || (rising && mny >= ybound)) {
+            /* the box left the window (every block ray has exited it for good), or every
+             * block ray is above every occupied voxel from here on */
             t0 = t1;
             break;
         }

@@ -59,6 +59,10 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
         dmax[k] = hi;
     }
     const int lane = threadIdx.x & 63;
+    // every block ray moves up (or level) and the step's box lies at or above the world's
+    // occupied y bound: nothing is left to hit, as when the box leaves the window
+    const bool rising = dmin[1] >= 0.0f;
+    const float ybound = (float)(w.origin[1] + w.ytop);
     // Packed rounds: 8 consecutive steps at once, 8 lanes each, while every box fits 8 tiles
     // (the common case: a block's box is 1-2 tiles wide per axis); the first step that stops
     // the march decides, exactly as the one-step loop below would. A step whose box needs
@@ -77,12 +81,14 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
             if (s >= kBeamMaxSteps || !(ts0 < tmax)) {
                 kind = 1;
             } else {
+                float mny = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     float a = c.origin[k] + dmin[k] * ts0, b = c.origin[k] + dmin[k] * ts1;
                     float cc = c.origin[k] + dmax[k] * ts0, e = c.origin[k] + dmax[k] * ts1;
                     float mn = fsel_min(fsel_min(a, b), fsel_min(cc, e)) - 1.0f;
                     float mx = fsel_max(fsel_max(a, b), fsel_max(cc, e)) + 1.0f;
+                    if (k == 1) mny = mn;
                     int32_t tl = (int32_t)floorf(mn * 0.0625f), th = (int32_t)floorf(mx * 0.0625f);
                     const int32_t wl = w.origin[k] >> 4, wh = (w.origin[k] + w.dims[k] - 1) >> 4;
                     tl = tl < wl ? wl : tl;
@@ -91,7 +97,7 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
                     n[k] = th >= tl ? th - tl + 1 : 0;
                     total *= n[k];
                 }
-                if (total == 0) kind = 2;
+                if (total == 0 || (rising && mny >= ybound)) kind = 2;
                 else if (n[0] > 4 || n[1] > 4 || n[2] > 4) kind = 3;
                 else if (total > 8) kind = 4;
             }
@@ -141,12 +147,14 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
         const float t1 = (float)(st + 1) * kBeamStep;
         int32_t lo[3], n[3];
         int32_t total = 1;
+        float mny = 0.0f;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             float a = c.origin[k] + dmin[k] * t0, b = c.origin[k] + dmin[k] * t1;
             float cc = c.origin[k] + dmax[k] * t0, e = c.origin[k] + dmax[k] * t1;
             float mn = fsel_min(fsel_min(a, b), fsel_min(cc, e)) - 1.0f;
             float mx = fsel_max(fsel_max(a, b), fsel_max(cc, e)) + 1.0f;
+            if (k == 1) mny = mn;
             int32_t tl = (int32_t)floorf(mn * 0.0625f), th = (int32_t)floorf(mx * 0.0625f);
             const int32_t wl = w.origin[k] >> 4, wh = (w.origin[k] + w.dims[k] - 1) >> 4;
             tl = tl < wl ? wl : tl;
@@ -155,9 +163,10 @@ __device__ __forceinline__ float beam_start(const DevWorld &w, const CamParams &
             n[k] = th >= tl ? th - tl + 1 : 0;
             total *= n[k];
         }
-        if (total == 0) {
+        if (total == 0 || (rising && mny >= ybound)) {
             // the box left the window: every block ray has exited it for good (the window is
-            // convex), so nothing of the window remains to visit
+            // convex), so nothing of the window remains to visit; or every block ray rises
+            // above every occupied voxel from here on
             t0 = t1;
             break;
         }
@@ -382,6 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_beam(DevWorld w, FrameParams fp, uin
                                                  const uint32_t *__restrict__ list,
                                                  const uint32_t *list_n,
                                                  unsigned long long *counts) {
+    world_ready(w);
     if (blockIdx.x == 0) {
         stripe_reset(work, threadIdx.x, fp);
         if (threadIdx.x == 0) *frame_params(work) = fp;
