@@ -603,13 +603,15 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
         return {"waves_per_cu": ctx.get_option(L.VG_OPT_PROBE_WAVES_PER_CU),
                 "refill_below": ctx.get_option(L.VG_OPT_PROBE_REFILL),
                 "ids_per_atomic": ctx.get_option(L.VG_OPT_PROBE_CHUNK),
-                "transition_batch": ctx.get_option(L.VG_OPT_PROBE_TBATCH)}
+                "transition_batch": ctx.get_option(L.VG_OPT_PROBE_TBATCH),
+                "material_prefetch": ctx.get_option(L.VG_OPT_PROBE_MATSPEC)}
 
     def set_sched(sc):
         ctx.set_option(L.VG_OPT_PROBE_WAVES_PER_CU, sc["waves_per_cu"])
         ctx.set_option(L.VG_OPT_PROBE_REFILL, sc["refill_below"])
         ctx.set_option(L.VG_OPT_PROBE_CHUNK, sc["ids_per_atomic"])
         ctx.set_option(L.VG_OPT_PROBE_TBATCH, sc["transition_batch"])
+        ctx.set_option(L.VG_OPT_PROBE_MATSPEC, sc["material_prefetch"])
 
     def train_on(i):
         return i % GI_TRAIN_EVERY == 0

@@ -332,9 +332,13 @@ enum {
     VG_OPT_FRAME_BLOCK_W = 11, /* persistent frame kernel: pixel-block width 4, 8, 16 or 32 (the
                                   block is one wave's 64 rays: height 64 / width; the beam
                                   prologue marches one box per block) */
-    VG_OPT_PROBE_TBATCH = 12   /* probe path kernel: a lane whose ray ended waits until this many
+    VG_OPT_PROBE_TBATCH = 12,  /* probe path kernel: a lane whose ray ended waits until this many
                                   of its wave's lanes have (or too few still walk) and they set
                                   up their next rays together (1..64; 1: at once) */
+    VG_OPT_PROBE_MATSPEC = 13  /* probe path kernel: 1 loads each walk step's voxel material
+                                  together with its pyramid words (one memory round trip per
+                                  step, where a set voxel otherwise takes two; the extra load
+                                  is wasted on every other step); 0 loads it when needed */
 };
 int vg_ctx_set_option(vg_ctx *ctx, int option, int value);
 int vg_ctx_get_option(vg_ctx *ctx, int option, int *value);

@@ -156,20 +156,20 @@ def test_param_errors(ctx, small):
 @pytest.mark.parametrize("bounces", [1, 2])
 def test_schedules_give_the_same_records(ctx, small, small_world, bounces):
     """The probe path kernel's schedule (waves per CU, refill threshold, work ids per atomic,
-    transition batch) and its longest-first order (each update traces the previous update's heavy paths first
+    transition batch, material prefetch) and its longest-first order (each update traces the previous update's heavy paths first
     and skips them in the natural order) change only the order of the work: every schedule
     gives the oracle's records, round after round."""
     from voxelg_amd import _lib as L
     opts = (L.VG_OPT_PROBE_WAVES_PER_CU, L.VG_OPT_PROBE_REFILL, L.VG_OPT_PROBE_CHUNK,
-            L.VG_OPT_PROBE_TBATCH)
+            L.VG_OPT_PROBE_TBATCH, L.VG_OPT_PROBE_MATSPEC)
     keep = [ctx.get_option(o) for o in opts]
     try:
-        for wpc, refill, chunk, tb in ((8, 4, 0, 1), (16, 48, 16, 8), (4, 60, 128, 64),
-                                       (24, 32, 1024, 16), (8, 1, 32, 64)):
-            for o, v in zip(opts, (wpc, refill, chunk, tb)):
+        for wpc, refill, chunk, tb, ms in ((8, 4, 0, 1, 0), (16, 48, 16, 8, 1), (4, 60, 128, 64, 0),
+                                           (24, 32, 1024, 16, 1), (8, 1, 32, 64, 1)):
+            for o, v in zip(opts, (wpc, refill, chunk, tb, ms)):
                 ctx.set_option(o, v)
             print(f"case: schedule {wpc} waves/CU, refill {refill}, chunk {chunk}, "
-                  f"transition batch {tb}", flush=True)
+                  f"transition batch {tb}, material prefetch {ms}", flush=True)
             p = gi.gi_params(SUN, grid=(8, 4, 8), spacing=32, rays_per_probe=64, epochs=1,
                              update_div=1, bounces=bounces)
             run_rounds(ctx, small, small_world, p, range(3))
@@ -186,6 +186,9 @@ def test_schedule_option_errors(ctx):
     for bad in (0, -3, 65):
         with pytest.raises(VGError):
             ctx.set_option(L.VG_OPT_PROBE_TBATCH, bad)
+    for bad in (-1, 2):
+        with pytest.raises(VGError):
+            ctx.set_option(L.VG_OPT_PROBE_MATSPEC, bad)
     keep = ctx.get_option(L.VG_OPT_PROBE_CHUNK)
     ctx.set_option(L.VG_OPT_PROBE_CHUNK, 256)
     assert ctx.get_option(L.VG_OPT_PROBE_CHUNK) == 256

@@ -186,6 +186,34 @@ def test_learned_update_matches_oracle(ctx, small, small_world):
     assert cache.stats()[0] == 0  # no training without the flag
 
 
+def test_learned_update_with_material_prefetch(ctx, small, small_world):
+    """The path launch's material prefetch (a schedule option) changes only when each walk
+    step's material is loaded: the learned update gives the same records bit for bit with it
+    on and off, round after round (counting runs never prefetch, so this uses the timed
+    entry point)."""
+    from voxelg_amd import _lib as L
+    cfg = nrc.default_config(log2_table=14, batch=2048)
+    a, b = nrc.RadianceCache(ctx, cfg), nrc.RadianceCache(ctx, cfg)
+    mlp, grid = rand_params(np.random.default_rng(22), a, scale_mlp=0.15, scale_grid=0.3)
+    a.set_params(mlp, grid)
+    b.set_params(mlp, grid)
+    p, _, _, _ = probe_setup(small)
+    ga, gb = gi.ProbeGrid(small, p), gi.ProbeGrid(small, p)
+    keep = ctx.get_option(L.VG_OPT_PROBE_MATSPEC)
+    try:
+        for r in range(3):
+            p["round"] = r
+            case(f"learned round {r}: material prefetch off / on")
+            ctx.set_option(L.VG_OPT_PROBE_MATSPEC, 0)
+            ga.update_learned(a, p, train=False)
+            ctx.set_option(L.VG_OPT_PROBE_MATSPEC, 1)
+            gb.update_learned(b, p, train=False)
+            ctx.synchronize()
+            assert ga.download().tobytes() == gb.download().tobytes(), f"round {r}"
+    finally:
+        ctx.set_option(L.VG_OPT_PROBE_MATSPEC, keep)
+
+
 def test_training_targets_match_oracle(ctx, small, small_world):
     """A learned update trains on the previous learned update's hits: the first has none
     (every target weight 0), the second traces a bounce for each of its batch."""

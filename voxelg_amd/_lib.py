@@ -74,6 +74,7 @@ VG_OPT_PROBE_CHUNK = 9
 VG_OPT_FRAME_STRIPES = 10
 VG_OPT_FRAME_BLOCK_W = 11
 VG_OPT_PROBE_TBATCH = 12
+VG_OPT_PROBE_MATSPEC = 13
 VG_RAY_LOD = 1
 VG_RAY_SKIP_TRANSPARENT = 2
 VG_RAY_ANY_HIT = 4

@@ -1259,6 +1259,7 @@ int vg_ctx_create(int device, vg_ctx **out) {
     c->probe_refill = 8;
     c->probe_chunk = 0;
     c->probe_tbatch = 8;
+    c->probe_matspec = 0;
     NoAllocScratch na; // the no-allocation entry point's scratch address on this device
     int st = noalloc_scratch(device, &na);
     if (st) {
@@ -1768,6 +1769,11 @@ int vg_ctx_set_option(vg_ctx *ctx, int option, int value) {
             return fail(VG_E_INVALID, "probe transition batch %d not in 1..64", value);
         ctx->probe_tbatch = value;
         return VG_OK;
+    case VG_OPT_PROBE_MATSPEC:
+        if (value != 0 && value != 1)
+            return fail(VG_E_INVALID, "probe material prefetch %d: 0 or 1", value);
+        ctx->probe_matspec = value;
+        return VG_OK;
     default:
         return fail(VG_E_INVALID, "option %d cannot be set", option);
     }
@@ -1787,6 +1793,7 @@ int vg_ctx_get_option(vg_ctx *ctx, int option, int *value) {
     case VG_OPT_PROBE_REFILL: *value = ctx->probe_refill; return VG_OK;
     case VG_OPT_PROBE_CHUNK: *value = ctx->probe_chunk; return VG_OK;
     case VG_OPT_PROBE_TBATCH: *value = ctx->probe_tbatch; return VG_OK;
+    case VG_OPT_PROBE_MATSPEC: *value = ctx->probe_matspec; return VG_OK;
     case VG_OPT_CU_COUNT: *value = ctx->cus; return VG_OK;
     default: return fail(VG_E_INVALID, "unknown option %d", option);
     }

@@ -424,7 +424,7 @@ template <uint32_t CAP> struct QueryStage {
     float4 q[kBlock / 64][CAP][2];
 };
 
-template <bool COUNT, int BOUNCES, bool FOLD>
+template <bool COUNT, int BOUNCES, bool FOLD, bool MATSPEC>
 __global__ __launch_bounds__(kBlock) void k_probe_paths(
     DevWorld w, vg_gi_params p, uint32_t n_samples, uint32_t epoch, uint32_t phase,
     const float4 *__restrict__ pwork, const float4 *__restrict__ dirs,
@@ -819,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_paths(
             bool ended = stage == kNew;
             if (stage >= kGather && !pend) {
                 const bool shadow = stage == kShadowX || stage == kShadowY;
-                bool done = walk_iter<COUNT, FOLD>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
+                bool done = walk_iter<COUNT, FOLD, MATSPEC>(r, q, w, shadow ? p.shadow_tmax : p.gi_dist,
                                                    shadow ? VG_RAY_ANY_HIT
                                                           : VG_RAY_SKIP_TRANSPARENT,
                                                    false, h, cnt);
@@ -1393,11 +1393,17 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     la.tseed = nrc ? nrc->cfg.seed ^ 0x9E3779B9u : 0u;
     la.tq = nrc ? nrc->tq_base + (size_t)tb * nrc->tcap : nullptr;
     la.tt = nrc ? nrc->tt_base + (size_t)tb * nrc->tcap : nullptr;
-#define LAUNCH_GF(C, B, F)                                                                     \
-    hipLaunchKernelGGL((k_probe_paths<C, B, F>), grid, block, 0, stream, w->dw, *p,              \
+#define LAUNCH_GS(C, B, F, S)                                                                  \
+    hipLaunchKernelGGL((k_probe_paths<C, B, F, S>), grid, block, 0, stream, w->dw, *p,           \
                        (uint32_t)n_samples, epoch, phase, pr->pwork, pr->dirs, pr->bounce,      \
                        pr->palette, pr->rad, pr->face, la, ha, pr->ctr, thr,                    \
                        w->ctx->probe_tbatch, (uint32_t)w->ctx->probe_chunk, cd)
+    // counting runs (one per update at most, outside timed regions) never prefetch
+#define LAUNCH_GF(C, B, F)                                                                     \
+    do {                                                                                       \
+        if (!C && w->ctx->probe_matspec) LAUNCH_GS(C, B, F, !C);                               \
+        else LAUNCH_GS(C, B, F, false);                                                        \
+    } while (0)
 #define LAUNCH_G(C, B)                                                                         \
     do {                                                                                       \
         if (w->dw.fold) LAUNCH_GF(C, B, true);                                                 \
@@ -1419,6 +1425,7 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     }
 #undef LAUNCH_G
 #undef LAUNCH_GF
+#undef LAUNCH_GS
     HIP_TRY(hipGetLastError());
     if (tev) HIP_TRY(hipEventRecord(tev[2], stream));
     if (nrc) {
@@ -1712,15 +1719,35 @@ int vg_autotune_probes(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint32
         if (r == VG_OK && pair) r = probes_update(pr, p, nrc, flags, false, s);
         return r == VG_OK && nrc ? nrc_join(nrc, s) : r;
     };
-    st = autotune(c, &c->probe_wpc, &c->probe_refill, run, best_ms, &c->probe_chunk);
-    // then the transition batch at the chosen (waves per CU, refill, ids per atomic)
+    // for each material prefetch setting: waves per CU x refill x ids per atomic, then the
+    // transition batch at the chosen point; the faster of the two settings is kept
     static const int kTbatch[] = {1, 4, 8, 16};
-    const int wpc = c->probe_wpc, rf = c->probe_refill;
-    if (st == VG_OK)
-        st = autotune_lists(c, &c->probe_wpc, TuneList{&wpc, 1}, &c->probe_refill,
-                            TuneList{&rf, 1}, &c->probe_tbatch, TuneList{kTbatch, 4}, run,
-                            best_ms);
-    if (st == VG_OK && pair && best_ms) *best_ms *= 0.5f;
+    float best_all = INFINITY;
+    int keep[5] = {c->probe_wpc, c->probe_refill, c->probe_chunk, c->probe_tbatch, 0};
+    for (int ms = 0; ms < 2 && st == VG_OK; ++ms) {
+        c->probe_matspec = ms;
+        float b = INFINITY;
+        st = autotune(c, &c->probe_wpc, &c->probe_refill, run, &b, &c->probe_chunk);
+        const int wpc = c->probe_wpc, rf = c->probe_refill;
+        if (st == VG_OK)
+            st = autotune_lists(c, &c->probe_wpc, TuneList{&wpc, 1}, &c->probe_refill,
+                                TuneList{&rf, 1}, &c->probe_tbatch, TuneList{kTbatch, 4}, run,
+                                &b);
+        if (st == VG_OK && b < best_all) {
+            best_all = b;
+            keep[0] = c->probe_wpc;
+            keep[1] = c->probe_refill;
+            keep[2] = c->probe_chunk;
+            keep[3] = c->probe_tbatch;
+            keep[4] = ms;
+        }
+    }
+    c->probe_wpc = keep[0];
+    c->probe_refill = keep[1];
+    c->probe_chunk = keep[2];
+    c->probe_tbatch = keep[3];
+    c->probe_matspec = keep[4];
+    if (best_ms) *best_ms = pair ? 0.5f * best_all : best_all;
     return st;
 }
 

@@ -365,7 +365,10 @@ __device__ __forceinline__ void phase_reset(WalkPhase &q) { q.phase1 = true; }
 // only on a change, or testing level by level with branches, is 6-30 % slower on the probe
 // paths and batched queries and no faster on the coherent frame. FOLD = false compiles the
 // slot folding out, for worlds whose window origin folds no axis (fewer registers).
-template <bool COUNT, bool FOLD = true>
+// MATSPEC = true also loads the current voxel's material with those words, so a set voxel
+// costs one round trip instead of two (for latency-bound launches; the load is wasted on
+// every other step).
+template <bool COUNT, bool FOLD = true, bool MATSPEC = false>
 __device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld &w, float tmax,
                                           uint32_t flags, bool lod, HitRec &h, uint32_t *cnt) {
     if (r.t_cur > tmax) return true;
@@ -397,7 +400,8 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld 
     const uint32_t tu = w.tile_uni[tn];
     const uint64_t ow = w.occ[bn];
     const uint32_t bu = w.brick_uni[bn];
-    asm volatile("" ::"v"(l4w), "v"(cw), "v"(tw), "v"(tu), "v"(ow), "v"(bu));
+    const uint32_t vms = MATSPEC ? (uint32_t)w.mat[(uint64_t)bn * 64u + (uint32_t)vb] : 0u;
+    asm volatile("" ::"v"(l4w), "v"(cw), "v"(tw), "v"(tu), "v"(ow), "v"(bu), "v"(vms));
     const bool p1 = q.phase1;
     // the levels the walk reaches (each test only counts if the ones above it passed)
     const bool c_on = l4w != 0 && ((l4w >> cbc) & 1);
@@ -427,7 +431,7 @@ __device__ __forceinline__ bool walk_iter(Walk &r, WalkPhase &q, const DevWorld 
     }
     const bool v_set = v_lvl && ((ow >> vb) & 1);
     uint32_t vm = 0u;
-    if (v_set) vm = w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
+    if (v_set) vm = MATSPEC ? vms : (uint32_t)w.mat[(uint64_t)bn * 64u + (uint32_t)vb];
     // a uniform tile, a uniform brick or a set voxel decides by its material
     const uint32_t um = tu_on ? (uint32_t)tu : (bu_on ? (uint32_t)bu : vm);
     const bool solid = (tu_on || bu_on || v_set) && solid_mat(um, flags);

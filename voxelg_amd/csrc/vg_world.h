@@ -19,6 +19,7 @@ struct vg_ctx {
     int probe_refill;
     int probe_chunk;     // ... work ids per atomic (0: exactly what a refill lacks)
     int probe_tbatch;    // ... lanes whose rays ended that set up their next rays together
+    int probe_matspec;   // ... 1: each walk step loads its voxel's material with the pyramid words
 };
 
 struct vg_edit_state; // sphere-edit buffers (vg_edit.hip), created on first use
