@@ -852,6 +852,11 @@ def bench_dynamic(args):
     cfg = nrc.default_config(log2_table=GI_CACHE_LOG2_TABLE)
     cfg["batch"] = GI_TRAIN_BATCH
     cache = nrc.RadianceCache(ctx, cfg)
+    # schedules tuned on the unedited world before the timed frames: the probe update's with
+    # its training step (the tuning's updates train the cache too), the frame kernel's on the
+    # full frame (the dirty-block retrace keeps 8x8 blocks and takes the rest)
+    probes.autotune(p, cache, train=True)
+    dw.autotune_frame(cam, W, H, 512.0, 0, sun, 480.0)
     n_frames = args.warmup + args.steps
     edits = [wl.edits(f, world) for f in range(n_frames)]
     phases = ("edits", "dirty", "retrace", "probes", "indirect")
