@@ -767,8 +767,8 @@ def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
                      "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(GI_TRAFFIC_PROFILE, "traffic_bytes_per_update"),
                      "traffic_source": GI_TRAFFIC_PROFILE,
-                     "kernel": "k_probe_paths<false, 4> (the learned update's path launch, "
-                               "an update without a training step)",
+                     "kernel": "k_probe_paths<false, 4, *, *> (the learned update's path "
+                               "launch, an update without a training step)",
                      "kernel_ms": round(path_ms, 4), "algorithmic_bytes": int(path_bytes_frozen)},
         "indirect_1080p_ms": round(indirect_ms, 4),
         "train_steps": cache.stats()[0], "train_batch": cache.batch,
