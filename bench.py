@@ -125,9 +125,9 @@ def parity_report(wl, cam, sun, ow, got, occ, r0, s0, r1, s1):
 
 
 def parity_configs(ctx, threads):
-    """The other two traversal configurations against the CPU traversals, outside the timed
-    region: config 1 (the 1280x720 frame of the 256x128x256 world, primary tmax 512, LOD off,
-    plus a sun shadow ray per hit, tmax 480) and config 3 (one step of the 96 uneven batches
+    """The other two traversal workloads against the CPU traversals, outside the timed
+    region: the small frame (1280x720 over the 256x128x256 world, primary tmax 512, LOD off,
+    plus a sun shadow ray per hit, tmax 480) and the batched step (the 96 uneven batches
     over the same world): the GPU's records bit for bit against R1, and against R0
     (src/raycast.rs) by the rule of the reference's accel_matches_cpu_raycast test
     (src/accel.rs:655-718), with the edge-tie and tmax-tie counts."""
@@ -161,7 +161,7 @@ def parity_configs(ctx, threads):
     same = (got["hit"] != 0) & (r0["hit"] != 0) & np.all(got["voxel"] == r0["voxel"], axis=1) \
         & (got["face"] == r0["face"]) & (got["t"] == r0["t"])
     cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, got, occ, s0, sun, stmax, same)
-    out = {"config1": {"rays": int(len(got)),
+    out = {"small_frame_1280x720": {"rays": int(len(got)),
                        "vs_r1_bit_exact": bool(got.tobytes() == r1.tobytes() and
                                                np.array_equal(occ, s1)),
                        "vs_r0": dict(summary(c1), shadow=cs)}}
@@ -180,7 +180,7 @@ def parity_configs(ctx, threads):
     r03 = ow.raycast_rays(rays, threads)
     c3 = oracle.classify_vs_r0(rays["o"], oracle.normalize_or_zero_v(rays["d"]), got3, r03,
                                tmax=rays["tmax"])
-    out["config3"] = {"rays": int(len(rays)), "batches": int(bs.n),
+    out["batched_96"] = {"rays": int(len(rays)), "batches": int(bs.n),
                       "vs_r1_bit_exact": bool(got3.tobytes() == ref3.tobytes()),
                       "vs_r0": summary(c3)}
     dw.close()
@@ -548,7 +548,7 @@ GI_CACHE_LOG2_TABLE = 16
 
 
 def gi_probe_bench(ctx, dw, world, sun, steps, train_updates, threads):
-    """Config-4 GI probes over the world: a 32-voxel probe grid (64x8x64 on 2048x256x2048),
+    """GI probes over the world: a 32-voxel probe grid (64x8x64 on 2048x256x2048),
     sun + sky, 2-bounce diffuse. Plain N: N rays/probe, every sample traces gather, sun
     shadow, cosine bounce and its sun shadow (N = 256 is the quality reference; 32 and 16 are
     the cheaper plain settings). Learned: GI_LEARNED_RAYS rays/probe ending at the first hit
@@ -813,7 +813,7 @@ def bench_gi_main(args):
 
 
 def bench_dynamic(args):
-    """Config 5: the 256x128x256 world plus 4096 seeded sphere edits per frame (radius 2..12,
+    """Dynamic world: the 256x128x256 world plus 4096 seeded sphere edits per frame (radius 2..12,
     half carve / half fill). One step = one frame: apply the edits on the device (incremental
     pyramid / uniform rebuild), mark the dirtied 8x8 blocks (primary and sun-shadow rays),
     retrace them at 1920x1080 (primary tmax 512 + shadow 480), a learned probe update

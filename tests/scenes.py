@@ -154,8 +154,8 @@ def roof_samples():
     return np.array(origins, np.float32), sun, np.array(under), np.array(open_)
 
 
-def assert_north_star(c, what):
-    """A classify_vs_r0 result meets the north-star criterion: hits match R0 on voxel, face
+def assert_agrees_with_r0(c, what):
+    """A classify_vs_r0 result meets the R0 agreement criterion: hits match R0 on voxel, face
     and material, t within 1e-4 voxels, and the only disagreements are edge ties
     (lattice_touch >= 2) or tmax-boundary ties, rare ones."""
     assert c["clean_disagree"] == 0, (what, c)

@@ -345,7 +345,7 @@ def test_edge_cases_and_errors(ctx, small, small_world):
 
 
 def test_small_frame_vs_r0_full(ctx, small, small_world):
-    # the north-star criterion on the 1280x720 frame of the 256x128x256 world (primary tmax
+    # the R0 agreement criterion on the 1280x720 frame of the 256x128x256 world (primary tmax
     # 512, LOD off) plus its sun shadow rays: the GPU's hits against the reference CPU
     # traversal R0 (src/raycast.rs:21-141) traced over the same rays, every pixel
     wl = workloads.SMALL
@@ -359,7 +359,7 @@ def test_small_frame_vs_r0_full(ctx, small, small_world):
     r0, s0 = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, sun, 480.0, threads=THREADS)
     dirs = oracle.primary_dirs(cam, wl.width, wl.height)
     c = oracle.classify_vs_r0(cam[:3], dirs, got, r0, tmax=wl.primary_tmax)
-    scenes.assert_north_star(c, "small frame GPU vs R0")
+    scenes.assert_agrees_with_r0(c, "small frame GPU vs R0")
     assert c["agree_clean"] > 0.85 * r0["hit"].sum()
     alln = np.ones(len(got), bool)
     cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, got, occ, s0, sun, 480.0,
@@ -379,14 +379,14 @@ def test_uneven_batches_vs_r0_full(ctx, small, small_world):
     r0 = oracle.OracleWorld.view(small_world).raycast_rays(rays, THREADS)
     c = oracle.classify_vs_r0(rays["o"], oracle.normalize_or_zero_v(rays["d"]), got, r0,
                               tmax=rays["tmax"])
-    scenes.assert_north_star(c, "uneven batches GPU vs R0")
+    scenes.assert_agrees_with_r0(c, "uneven batches GPU vs R0")
     assert c["agree_clean"] > 0.8 * r0["hit"].sum()
 
 
 def test_large_frame_bit_exact_and_vs_r0(ctx):
     # the headline frame (2048x256x2048 world, 1920x1080, LOD 400 / 520, beam, sun shadow per
     # hit): every pixel equals the oracle's R1 bit for bit, and against R0 the hits before
-    # t 400 (where LOD cannot have fired) and their shadow rays meet the north-star criterion
+    # t 400 (where LOD cannot have fired) and their shadow rays meet the R0 agreement criterion
     wl = workloads.LARGE
     print("case: large world generate", flush=True)
     world = wl.world()
@@ -423,7 +423,7 @@ def test_large_frame_bit_exact_and_vs_r0(ctx):
     dirs = oracle.primary_dirs(cam, wl.width, wl.height)
     sel = (flat["hit"] == 0) | (flat["t"] < 400.0)
     c = oracle.classify_vs_r0(cam[:3], dirs[sel], flat[sel], r0[sel], tmax=wl.primary_tmax)
-    scenes.assert_north_star(c, "large frame GPU vs R0 (t < 400)")
+    scenes.assert_agrees_with_r0(c, "large frame GPU vs R0 (t < 400)")
     assert c["agree_clean"] > 0.5 * sel.sum()
     cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, flat, occ, s0, sun, wl.shadow_tmax,
                                        scenes.same_shadow_ray(flat, r0, cam[:3], dirs, sel))

@@ -285,8 +285,8 @@ def test_online_training_gains_over_no_cache(ctx, small, small_world):
     assert e_trained < 0.75 * e_untrained
 
 
-def test_learned_update_error_config1_world(ctx, small, small_world):
-    """Config 4's update on the config-1 world (8x4x8 probes at spacing 32, 32 gather rays per
+def test_learned_update_error_small_world(ctx, small, small_world):
+    """The learned update on the 256x128x256 world (8x4x8 probes at spacing 32, 32 gather rays per
     probe, the cache trained online on 65 536 samples per update), against the FP64 4096-ray
     2-bounce estimate. The 2 % limit is a property of the ray count here: the plain update
     tracing both bounces with the same 32 directions is 2.9 % off (64 rays: 1.9 %), so the
@@ -330,7 +330,7 @@ def test_learned_update_error_config1_world(ctx, small, small_world):
         return err(g)
 
     e32, e_ambient, e256 = plain(32, 2), plain(32, 1), plain(256, 2)
-    print(f"config-1 world: learned 32-ray {e_learned:.4f}, plain 32-ray 2-bounce {e32:.4f}, "
+    print(f"256x128x256 world: learned 32-ray {e_learned:.4f}, plain 32-ray 2-bounce {e32:.4f}, "
           f"32-ray ambient tail {e_ambient:.4f}, plain 256-ray {e256:.4f}", flush=True)
     assert e_learned <= 1.02 * e32, (e_learned, e32)
     assert e_learned < 0.5 * e_ambient

@@ -132,7 +132,7 @@ def test_r1_vs_r0_small_frame_full(small_world):
     r0, _ = ow.frame_r0(cam, wl.width, wl.height, wl.primary_tmax, None, 0.0)
     dirs = oracle.primary_dirs(cam, wl.width, wl.height)
     c = oracle.classify_vs_r0(cam[:3], dirs, r1, r0, tmax=wl.primary_tmax)
-    scenes.assert_north_star(c, "small frame R1 vs R0")
+    scenes.assert_agrees_with_r0(c, "small frame R1 vs R0")
     assert c["agree_clean"] > 0.85 * r0["hit"].sum()
 
 
@@ -145,7 +145,7 @@ def test_r1_vs_r0_batched_step_full(small_world):
     r0 = ow.raycast_rays(rays)
     d = oracle.normalize_or_zero_v(rays["d"])
     c = oracle.classify_vs_r0(rays["o"], d, r1, r0, tmax=rays["tmax"])
-    scenes.assert_north_star(c, "batched step R1 vs R0")
+    scenes.assert_agrees_with_r0(c, "batched step R1 vs R0")
     assert c["agree_clean"] > 0.8 * r0["hit"].sum()
 
 
@@ -165,7 +165,7 @@ def test_r1_vs_r0_large_frame_rows():
     dirs = oracle.primary_dirs(cam, wl.width, wl.height)[rows[0] * wl.width:rows[1] * wl.width]
     sel = (r1["hit"] == 0) | (r1["t"] < 400.0)
     c = oracle.classify_vs_r0(cam[:3], dirs[sel], r1[sel], r0[sel], tmax=wl.primary_tmax)
-    scenes.assert_north_star(c, "large frame rows R1 vs R0")
+    scenes.assert_agrees_with_r0(c, "large frame rows R1 vs R0")
     assert c["agree_clean"] > 1000 and (~sel).sum() > 1000
     same_ray = scenes.same_shadow_ray(r1, r0, cam[:3], dirs, sel)
     cs = oracle.classify_shadows_vs_r0(ow, cam[:3], dirs, r1, s1, s0, sun, wl.shadow_tmax,
