@@ -1443,8 +1443,10 @@ static int probes_update(vg_probes *pr, const vg_gi_params *p, vg_nrc *nrc, uint
     const bool overlap = train && nrc->cfg.steps == 1 && !count;
     if (overlap) HIP_TRY(hipEventRecord(nrc->ev_ready, stream));
     if (tev) HIP_TRY(hipEventRecord(tev[3], stream));
-    uint32_t lpp = 8; // lanes per probe in the reduction: the sample count's power of two
-    while (lpp < 64 && lpp < (uint32_t)pr->rays) lpp <<= 1;
+    // lanes per probe in the reduction: half the sample count's power of two (two samples per
+    // lane before the butterfly, which halves it), 8..64
+    uint32_t lpp = 8;
+    while (lpp < 64 && 2u * lpp < (uint32_t)pr->rays) lpp <<= 1;
     const uint32_t ppb = kBlock / lpp;
     hipLaunchKernelGGL(k_probe_reduce, dim3((n_upd + ppb - 1) / ppb), dim3(kBlock), 0, stream,
                        w->dw, *p, n_upd, epoch, phase, lpp, pr->pwork, pr->dirs, pr->rad,
